@@ -1,0 +1,340 @@
+"""ctypes binding of libcmtj_amd.so (the C ABI of include/cmtj_amd.h).
+
+The numpy dtypes below mirror the C structs field by field; their sizes and
+key offsets are checked against the library's own ``cmtj_abi`` at load time.
+The library must exist: nothing here falls back to another implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libcmtj_amd.so"
+
+MAX_LAYERS = 8
+NSLOT = 30
+ENERGY_CHANNELS = 23
+
+# enum cmtj_update_type
+CONSTANT, PULSE, SINE, STEP, POSINE, HALFSINE, TRAPEZOID, GAUSSIMPULSE, GAUSSSTEP = range(9)
+TABLE = 101
+
+# enum cmtj_slot
+SLOT_HEXT, SLOT_HOE, SLOT_HDMI, SLOT_HRES = 0, 3, 6, 9
+SLOT_K, SLOT_K2 = 12, 13
+SLOT_J_TOP, SLOT_J_BOTTOM, SLOT_J2_TOP, SLOT_J2_BOTTOM = 14, 15, 16, 17
+SLOT_CURRENT, SLOT_FL, SLOT_DL, SLOT_FL2, SLOT_DL2 = 18, 19, 20, 21, 22
+SLOT_TEMPERATURE = 23
+SLOT_IDMI_TOP, SLOT_IDMI_BOTTOM = 24, 27
+
+# enum cmtj_reference / cmtj_solver / cmtj_mr_mode
+REF_NONE, REF_FIXED, REF_TOP, REF_BOTTOM = range(4)
+EULER_HEUN, RK4, DORMAND_PRINCE, HEUN = range(4)
+MR_NONE, MR_CLASSIC, MR_STRIP = range(3)
+
+# enum cmtj_log_bits
+LOG_MX, LOG_MYZ, LOG_MR, LOG_ENERGY = 1, 2, 4, 8
+LOG_FULL = LOG_MX | LOG_MYZ | LOG_MR
+
+FP64, FP32 = 0, 1
+LOG_F64, LOG_F32 = 0, 1
+
+OK, ERR_INVALID, ERR_HIP, ERR_NAN = 0, 1, 2, 3
+
+DRIVER_DTYPE = np.dtype(
+    [
+        ("type", "<i4"),
+        ("table_index", "<i4"),
+        ("constant", "<f8"),
+        ("amplitude", "<f8"),
+        ("frequency", "<f8"),
+        ("phase", "<f8"),
+        ("period", "<f8"),
+        ("cycle", "<f8"),
+        ("t_start", "<f8"),
+        ("t_stop", "<f8"),
+        ("edge_time", "<f8"),
+        ("steady_time", "<f8"),
+    ],
+    align=True,
+)
+
+LAYER_DTYPE = np.dtype(
+    [
+        ("mag", "<f8", (3,)),
+        ("anis", "<f8", (3,)),
+        ("Ms", "<f8"),
+        ("thickness", "<f8"),
+        ("cell_surface", "<f8"),
+        ("damping", "<f8"),
+        ("demag", "<f8", (9,)),
+        ("dipole_top", "<f8", (9,)),
+        ("dipole_bottom", "<f8", (9,)),
+        ("ref", "<f8", (3,)),
+        ("ref2", "<f8", (3,)),
+        ("kappa", "<f8"),
+        ("beta", "<f8"),
+        ("slonczewski", "<f8"),
+        ("polarisation", "<f8"),
+        ("fl_torque", "<f8"),
+        ("dl_torque", "<f8"),
+        ("ref_type", "<i4"),
+        ("include_stt", "<i4"),
+        ("include_sot", "<i4"),
+        ("dynamic_sot", "<i4"),
+        ("alternative_stt", "<i4"),
+        ("temperature_set", "<i4"),
+        ("drv", DRIVER_DTYPE, (NSLOT,)),
+    ],
+    align=True,
+)
+
+JUNCTION_DTYPE = np.dtype(
+    [
+        ("n_layers", "<i4"),
+        ("mr_mode", "<i4"),
+        ("solver", "<i4"),
+        ("log_mask", "<i4"),
+        ("n_steps", "<u4"),
+        ("write_every", "<u4"),
+        ("n_entries", "<u4"),
+        ("layer_offset", "<u4"),
+        ("seed", "<u8"),
+        ("stream", "<u8"),
+        ("log_offset", "<u8"),
+        ("dt", "<f8"),
+        ("Rp", "<f8"),
+        ("Rap", "<f8"),
+        ("Rx0", "<f8", (MAX_LAYERS,)),
+        ("Ry0", "<f8", (MAX_LAYERS,)),
+        ("AMR_X", "<f8", (MAX_LAYERS,)),
+        ("AMR_Y", "<f8", (MAX_LAYERS,)),
+        ("SMR_X", "<f8", (MAX_LAYERS,)),
+        ("SMR_Y", "<f8", (MAX_LAYERS,)),
+        ("AHE", "<f8", (MAX_LAYERS,)),
+    ],
+    align=True,
+)
+
+CONSTANTS_DTYPE = np.dtype(
+    [("mu0", "<f8"), ("gyro", "<f8"), ("TtoAm", "<f8"), ("hbar", "<f8"), ("echarge", "<f8"), ("kB", "<f8")],
+    align=True,
+)
+
+STATUS_DTYPE = np.dtype(
+    [("code", "<i4"), ("steps_done", "<u4"), ("entries_done", "<u4"), ("nan_layer", "<i4")], align=True
+)
+
+
+class _AbiInfo(ctypes.Structure):
+    _fields_ = [
+        (n, ctypes.c_int32)
+        for n in (
+            "version",
+            "max_layers",
+            "nslot",
+            "sizeof_driver",
+            "sizeof_layer",
+            "sizeof_junction",
+            "sizeof_constants",
+            "sizeof_status",
+            "offsetof_layer_drv",
+            "offsetof_layer_ref_type",
+            "offsetof_junction_dt",
+            "offsetof_junction_AHE",
+        )
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_instances", ctypes.c_uint64),
+        ("log_elements", ctypes.c_uint64),
+        ("total_steps", ctypes.c_uint64),
+        ("max_steps", ctypes.c_uint64),
+        ("precision", ctypes.c_int32),
+        ("log_dtype", ctypes.c_int32),
+        ("max_layers", ctypes.c_int32),
+        ("n_kernels", ctypes.c_int32),
+        ("lanes_per_instance", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("kernel_name", ctypes.c_char * 128),
+    ]
+
+
+# every entry point declared in include/cmtj_amd.h
+EXPORTS = (
+    "cmtj_abi",
+    "cmtj_last_error",
+    "cmtj_constants_get",
+    "cmtj_constants_set",
+    "cmtj_constants_reset",
+    "cmtj_driver_eval",
+    "cmtj_run",
+    "cmtj_plan_create",
+    "cmtj_plan_execute",
+    "cmtj_plan_query",
+    "cmtj_plan_destroy",
+)
+
+_P = ctypes.c_void_p
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP engine first "
+            "(python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH), mode=ctypes.RTLD_GLOBAL)
+    lib.cmtj_abi.argtypes = [ctypes.POINTER(_AbiInfo)]
+    lib.cmtj_last_error.restype = ctypes.c_char_p
+    lib.cmtj_constants_get.argtypes = [_P]
+    lib.cmtj_constants_set.argtypes = [_P]
+    lib.cmtj_driver_eval.argtypes = [_P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    lib.cmtj_run.argtypes = [_P, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P]
+    lib.cmtj_plan_create.argtypes = [
+        _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t,
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+    ]
+    lib.cmtj_plan_create.restype = _P
+    lib.cmtj_plan_execute.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P]
+    lib.cmtj_plan_query.argtypes = [_P, ctypes.POINTER(PlanInfo)]
+    lib.cmtj_plan_destroy.argtypes = [_P]
+    info = _AbiInfo()
+    lib.cmtj_abi(ctypes.byref(info))
+    checks = {
+        "version": (info.version, 1),
+        "max_layers": (info.max_layers, MAX_LAYERS),
+        "nslot": (info.nslot, NSLOT),
+        "driver": (info.sizeof_driver, DRIVER_DTYPE.itemsize),
+        "layer": (info.sizeof_layer, LAYER_DTYPE.itemsize),
+        "junction": (info.sizeof_junction, JUNCTION_DTYPE.itemsize),
+        "constants": (info.sizeof_constants, CONSTANTS_DTYPE.itemsize),
+        "status": (info.sizeof_status, STATUS_DTYPE.itemsize),
+        "layer.drv": (info.offsetof_layer_drv, LAYER_DTYPE.fields["drv"][1]),
+        "layer.ref_type": (info.offsetof_layer_ref_type, LAYER_DTYPE.fields["ref_type"][1]),
+        "junction.dt": (info.offsetof_junction_dt, JUNCTION_DTYPE.fields["dt"][1]),
+        "junction.AHE": (info.offsetof_junction_AHE, JUNCTION_DTYPE.fields["AHE"][1]),
+    }
+    for k, (got, want) in checks.items():
+        if got != want:
+            raise ImportError(f"ABI mismatch for {k}: library {got}, binding {want}")
+    _lib = lib
+    return lib
+
+
+def lib():
+    return _load()
+
+
+def last_error() -> str:
+    return (lib().cmtj_last_error() or b"").decode()
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    if a is None:
+        return None
+    return a.ctypes.data
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what}: {last_error()}")
+
+
+def n_channels(n_layers: int, mr_mode: int, log_mask: int) -> int:
+    n = 0
+    if log_mask & LOG_MX:
+        n += n_layers
+    if log_mask & LOG_MYZ:
+        n += 2 * n_layers
+    if log_mask & LOG_MR:
+        n += 1 if mr_mode == MR_CLASSIC else (2 if mr_mode == MR_STRIP else 0)
+    if log_mask & LOG_ENERGY:
+        n += ENERGY_CHANNELS * n_layers
+    return n
+
+
+def constants_get() -> np.ndarray:
+    c = np.zeros((), dtype=CONSTANTS_DTYPE)
+    check(lib().cmtj_constants_get(ptr(c)), "cmtj_constants_get")
+    return c
+
+
+def constants_set(c: np.ndarray) -> None:
+    c = np.ascontiguousarray(c, dtype=CONSTANTS_DTYPE)
+    check(lib().cmtj_constants_set(ptr(c)), "cmtj_constants_set")
+
+
+def constants_reset() -> None:
+    check(lib().cmtj_constants_reset(), "cmtj_constants_reset")
+
+
+def driver_eval(rec: np.ndarray, t: float) -> float:
+    out = ctypes.c_double(0.0)
+    check(lib().cmtj_driver_eval(ptr(rec), float(t), ctypes.byref(out)), "cmtj_driver_eval")
+    return out.value
+
+
+def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: np.ndarray | None):
+    """One junction through cmtj_run; returns (log [ch, entries], mag [L,3], status)."""
+    j = np.ascontiguousarray(j, dtype=JUNCTION_DTYPE).reshape(())
+    layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
+    nch = n_channels(int(j["n_layers"]), int(j["mr_mode"]), int(j["log_mask"]))
+    ne = int(j["n_entries"])
+    log = np.empty((nch, ne), dtype=np.float64)
+    mag = np.zeros((int(j["n_layers"]), 3), dtype=np.float64)
+    st = np.zeros((), dtype=STATUS_DTYPE)
+    tabs = None if tables is None or len(tables) == 0 else np.ascontiguousarray(tables, dtype=np.float64)
+    rc = lib().cmtj_run(
+        ptr(j), ptr(layers), ptr(consts), ptr(tabs), 0 if tabs is None else tabs.size,
+        ptr(log) if log.size else None, log.size, ptr(mag), ptr(st),
+    )
+    check(rc, "cmtj_run")
+    return log, mag, st
+
+
+class Plan:
+    """Owner of a cmtj_plan (device-resident descriptors of a batch)."""
+
+    def __init__(self, js, layers, consts, tables, precision: int, log_dtype: int, lanes_per_instance: int = 0):
+        self._js = np.ascontiguousarray(js, dtype=JUNCTION_DTYPE)
+        self._layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
+        tabs = None if tables is None or len(tables) == 0 else np.ascontiguousarray(tables, dtype=np.float64)
+        self._h = lib().cmtj_plan_create(
+            ptr(self._js), self._js.size, ptr(self._layers), self._layers.size, ptr(consts),
+            ptr(tabs), 0 if tabs is None else tabs.size, precision, log_dtype, lanes_per_instance,
+        )
+        if not self._h:
+            raise RuntimeError(f"cmtj_plan_create: {last_error()}")
+        self.info = PlanInfo()
+        check(lib().cmtj_plan_query(self._h, ctypes.byref(self.info)), "cmtj_plan_query")
+
+    def execute(self, d_log: int | None, log_capacity: int, d_mag: int | None, d_status: int | None, stream: int | None):
+        check(
+            lib().cmtj_plan_execute(self._h, d_log, log_capacity, d_mag, d_status, stream),
+            "cmtj_plan_execute",
+        )
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cmtj_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+if os.environ.get("CMTJ_AMD_EAGER_LOAD", "1") == "1":
+    _load()

@@ -1,0 +1,559 @@
+// engine.hip -- host side of the MI355X macrospin engine: kernel registry,
+// plan building (grouping, cost ordering, driver records), the drop-in
+// single-junction run and the C ABI of include/cmtj_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using namespace cmtj;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(CMTJ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// constants store (reference core/constants.hpp:39-53)
+
+static const cmtj_constants kDefaults = {12.566e-7,           220880.0,      795774.715459,
+                                         6.62607015e-34 / (2. * M_PI), 1.60217662e-19,
+                                         1.380649e-23};
+static cmtj_constants g_consts = kDefaults;
+static std::mutex g_consts_mu;
+
+// ---------------------------------------------------------------------------
+// kernel registry
+
+#include "registry.h"
+
+static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, bool loge) {
+  static const KTable kTables[] = {kernels_f64(), kernels_f64e(), kernels_f32(), kernels_spec()};
+  const KEntry* best = nullptr;
+  for (const KTable& t : kTables)
+  for (size_t q = 0; q < t.n; ++q) {
+    const KEntry& k = t.e[q];
+    if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
+    if ((k.fm & flags) != flags) continue;
+    if (!best || k.lm < best->lm ||
+        (k.lm == best->lm && __builtin_popcount(k.fm) < __builtin_popcount(best->fm)))
+      best = &k;
+  }
+  return best;
+}
+
+// ---------------------------------------------------------------------------
+// descriptor analysis
+
+static int n_channels(int nl, int mr, int mask) {
+  int n = 0;
+  if (mask & CMTJ_LOG_MX) n += nl;
+  if (mask & CMTJ_LOG_MYZ) n += 2 * nl;
+  if (mask & CMTJ_LOG_MR) n += (mr == CMTJ_MR_CLASSIC) ? 1 : (mr == CMTJ_MR_STRIP ? 2 : 0);
+  if (mask & CMTJ_LOG_ENERGY) n += CMTJ_ENERGY_CHANNELS * nl;
+  return n;
+}
+
+static bool live(const cmtj_driver& d) { return d.type != CMTJ_CONSTANT || d.constant != 0.0; }
+
+static uint32_t junction_flags(const cmtj_junction& j, const cmtj_layer* L) {
+  uint32_t f = 0;
+  const int nl = j.n_layers;
+  for (int l = 0; l < nl; ++l) {
+    const cmtj_layer& y = L[l];
+    const double* N = y.demag;
+    if (N[1] != 0 || N[2] != 0 || N[3] != 0 || N[5] != 0 || N[6] != 0 || N[7] != 0) f |= F_DEMAG_OFF;
+    if (nl > 1)
+      for (int q = 0; q < 9; ++q)
+        if (y.dipole_top[q] != 0 || y.dipole_bottom[q] != 0) f |= F_DIPOLE;
+    if (live(y.drv[CMTJ_SLOT_K])) f |= F_ANIS;
+    if (live(y.drv[CMTJ_SLOT_K2])) f |= F_ANIS2;
+    if (nl > 1 && (live(y.drv[CMTJ_SLOT_J_TOP]) || live(y.drv[CMTJ_SLOT_J_BOTTOM]))) f |= F_IEC;
+    if (nl > 1 && (live(y.drv[CMTJ_SLOT_J2_TOP]) || live(y.drv[CMTJ_SLOT_J2_BOTTOM]))) f |= F_IEC2;
+    if (nl > 1)
+      for (int q = 0; q < 3; ++q)
+        if (live(y.drv[CMTJ_SLOT_IDMI_TOP + q]) || live(y.drv[CMTJ_SLOT_IDMI_BOTTOM + q]))
+          f |= F_IDMI;
+    if (y.include_stt) f |= F_STT;
+    if (!y.include_stt && y.include_sot) {
+      f |= F_SOT;
+      if (y.dynamic_sot && (live(y.drv[CMTJ_SLOT_FL2]) || live(y.drv[CMTJ_SLOT_DL2]))) f |= F_SOT2;
+    }
+    if (y.temperature_set && j.solver != CMTJ_RK4) f |= F_THERMAL;
+    for (int s = 0; s < CMTJ_NSLOT; ++s) {
+      if (y.drv[s].type != CMTJ_CONSTANT) f |= F_DYN;
+      if (y.drv[s].type == CMTJ_TABLE) f |= F_TABLE;
+    }
+  }
+  return f;
+}
+
+static int validate(const cmtj_junction& j, size_t idx, const cmtj_layer* layers, size_t rows,
+                    int prec, size_t n_table) {
+  const std::string at = " (junction " + std::to_string(idx) + ")";
+  if (j.n_layers < 1 || j.n_layers > CMTJ_MAX_LAYERS)
+    return fail(CMTJ_ERR_INVALID, "n_layers must be in [1, 8]" + at);
+  if ((size_t)j.layer_offset + (size_t)j.n_layers > rows)
+    return fail(CMTJ_ERR_INVALID, "layer rows out of range" + at);
+  if (j.solver == CMTJ_DORMAND_PRINCE)
+    return fail(CMTJ_ERR_INVALID, "DormandPrince adaptive stepping is not provided by the engine" + at);
+  if (j.solver != CMTJ_RK4 && j.solver != CMTJ_HEUN && j.solver != CMTJ_EULER_HEUN)
+    return fail(CMTJ_ERR_INVALID, "unknown solver" + at);
+  if (!(j.dt > 0.0) || !std::isfinite(j.dt)) return fail(CMTJ_ERR_INVALID, "time step must be > 0" + at);
+  if (j.write_every < 1) return fail(CMTJ_ERR_INVALID, "write_every must be >= 1" + at);
+  const uint64_t ne = j.n_steps == 0 ? 0 : ((uint64_t)j.n_steps + j.write_every - 1) / j.write_every;
+  if (ne != j.n_entries) return fail(CMTJ_ERR_INVALID, "n_entries != ceil(n_steps / write_every)" + at);
+  if ((j.log_mask & CMTJ_LOG_ENERGY) && prec != CMTJ_FP64)
+    return fail(CMTJ_ERR_INVALID, "energy channels need an FP64 run" + at);
+  if (j.mr_mode == CMTJ_MR_CLASSIC && j.n_layers > 2)
+    return fail(CMTJ_ERR_INVALID, "CLASSIC magnetoresistance supports 1 or 2 layers" + at);
+  const int sps = j.solver == CMTJ_RK4 ? 3 : (j.solver == CMTJ_HEUN ? 2 : 1);
+  for (int l = 0; l < j.n_layers; ++l) {
+    const cmtj_layer& y = layers[j.layer_offset + l];
+    if (y.Ms * y.thickness == 0.0) return fail(CMTJ_ERR_INVALID, "Failed to divide vector by zero!");
+    if (y.temperature_set && j.solver != CMTJ_RK4 && y.cell_surface * y.thickness == 0.0)
+      return fail(CMTJ_ERR_INVALID, "Cell surface cannot be 0 during temp. calculations!");
+    for (int s = 0; s < CMTJ_NSLOT; ++s) {
+      const cmtj_driver& d = y.drv[s];
+      if (d.type == CMTJ_TABLE) {
+        if (d.table_index < 0 || (uint64_t)d.table_index + (uint64_t)j.n_steps * sps > n_table)
+          return fail(CMTJ_ERR_INVALID, "driver table out of range" + at);
+      } else if (d.type < 0 || d.type > CMTJ_GAUSSSTEP) {
+        return fail(CMTJ_ERR_INVALID, "unknown driver type" + at);
+      }
+    }
+  }
+  return 0;
+}
+
+// Time-dependent records of one junction; identical records on several
+// layers merge into one with a layer mask.
+struct Rec {
+  int32_t meta;
+  DynRec num;
+};
+
+static void junction_records(const cmtj_junction& j, const cmtj_layer* L, std::vector<Rec>& out) {
+  out.clear();
+  for (int l = 0; l < j.n_layers; ++l) {
+    for (int s = 0; s < CMTJ_NSLOT; ++s) {
+      const cmtj_driver& d = L[l].drv[s];
+      if (d.type == CMTJ_CONSTANT) continue;
+      Rec r;
+      std::memset(&r, 0, sizeof(r));
+      double A, p1, p2, p3;
+      wave_params(d, A, p1, p2, p3);
+      r.meta = (d.type & 0xff) | (s << 8) | ((1 << l) << 16);
+      r.num.tidx = d.type == CMTJ_TABLE ? d.table_index : 0;
+      r.num.A = A;
+      r.num.p1 = p1;
+      r.num.p2 = p2;
+      r.num.p3 = p3;
+      bool merged = false;
+      for (Rec& o : out) {
+        if ((o.meta & 0xffff) == (r.meta & 0xffff) && o.num.tidx == r.num.tidx && o.num.A == r.num.A &&
+            o.num.p1 == r.num.p1 && o.num.p2 == r.num.p2 && o.num.p3 == r.num.p3) {
+          o.meta |= r.meta & 0xff0000;
+          merged = true;
+          break;
+        }
+      }
+      if (!merged) out.push_back(r);
+    }
+  }
+}
+
+static int32_t layer_bits(const cmtj_layer& y) {
+  return (y.ref_type & 3) | ((y.include_stt ? 1 : (y.include_sot ? 2 : 0)) << 2) |
+         ((y.dynamic_sot ? 1 : 0) << 4) | ((y.alternative_stt ? 1 : 0) << 5) |
+         ((y.temperature_set ? 1 : 0) << 6);
+}
+
+// ---------------------------------------------------------------------------
+// plans
+
+struct Group {
+  const KEntry* k;
+  GroupArgs a;
+  uint32_t blocks;
+};
+
+struct cmtj_plan {
+  void* arena = nullptr;
+  bool owns = true;
+  std::vector<Group> groups;
+  cmtj_plan_info info;
+  uint64_t log_extent = 0;
+};
+
+typedef void* (*ArenaFn)(size_t bytes, void* ctx);
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Builds every group of a plan into one device arena obtained from `get`.
+// A group holds the junctions of one structure signature (layer count,
+// solver, log layout, per-layer modes, driver kinds and slots), so every
+// branch on structure inside the kernel is wave-uniform.
+static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmtj_layer* layers,
+                      size_t rows, const cmtj_constants* c, const double* tables, size_t n_table,
+                      int prec, int log_dtype, ArenaFn get, void* gctx, hipStream_t stream) {
+  if (n == 0) return fail(CMTJ_ERR_INVALID, "empty batch");
+  if (n > 0xFFFFFFFFull) return fail(CMTJ_ERR_INVALID, "too many instances");
+  if (prec != CMTJ_FP64 && prec != CMTJ_FP32) return fail(CMTJ_ERR_INVALID, "unknown precision");
+  if (log_dtype != CMTJ_LOG_F64 && log_dtype != CMTJ_LOG_F32) return fail(CMTJ_ERR_INVALID, "unknown log dtype");
+  std::memset(&P->info, 0, sizeof(P->info));
+  uint64_t total_steps = 0, max_steps = 0, extent = 0;
+  int max_layers = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const int r = validate(js[i], i, layers, rows, prec, n_table);
+    if (r) return r;
+    total_steps += js[i].n_steps;
+    max_steps = std::max<uint64_t>(max_steps, js[i].n_steps);
+    max_layers = std::max(max_layers, js[i].n_layers);
+    const uint64_t e = js[i].log_offset +
+                       (uint64_t)n_channels(js[i].n_layers, js[i].mr_mode, js[i].log_mask) * js[i].n_entries;
+    extent = std::max(extent, e);
+  }
+  // signatures
+  std::vector<std::vector<int32_t>> sigs;
+  std::vector<std::vector<uint32_t>> members;
+  std::vector<std::vector<Rec>> recs_of(n);
+  std::vector<int32_t> sig;
+  for (size_t i = 0; i < n; ++i) {
+    const cmtj_junction& j = js[i];
+    const cmtj_layer* L = layers + j.layer_offset;
+    junction_records(j, L, recs_of[i]);
+    if (recs_of[i].size() > (size_t)MAX_DYN)
+      return fail(CMTJ_ERR_INVALID, "more than 16 time-dependent drivers in one junction");
+    sig.assign({j.n_layers, j.solver, j.log_mask, j.mr_mode});
+    for (int l = 0; l < j.n_layers; ++l) sig.push_back(layer_bits(L[l]));
+    for (const Rec& r : recs_of[i]) sig.push_back(r.meta);
+    size_t g = 0;
+    for (; g < sigs.size(); ++g)
+      if (sigs[g] == sig) break;
+    if (g == sigs.size()) {
+      sigs.push_back(sig);
+      members.emplace_back();
+    }
+    members[g].push_back((uint32_t)i);
+  }
+  const size_t G = sigs.size();
+  std::vector<std::vector<uint32_t>> orders(G);
+  std::vector<std::vector<DynRec>> nums(G);
+  std::vector<uint32_t> flags(G);
+  for (size_t g = 0; g < G; ++g) {
+    std::vector<uint32_t>& ord = orders[g];
+    ord = members[g];
+    // longest first: the waves of a group finish together and the hardware
+    // dispatcher starts the long ones first
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](uint32_t a, uint32_t b) { return js[a].n_steps > js[b].n_steps; });
+    uint32_t f = 0;
+    const size_t nd = recs_of[ord[0]].size();
+    nums[g].resize(std::max<size_t>(nd, 1) * ord.size());
+    for (size_t p = 0; p < ord.size(); ++p) {
+      const cmtj_junction& j = js[ord[p]];
+      f |= junction_flags(j, layers + j.layer_offset);
+      for (size_t k = 0; k < nd; ++k) nums[g][k * ord.size() + p] = recs_of[ord[p]][k].num;
+    }
+    flags[g] = f;
+  }
+  // arena layout
+  size_t off = 0;
+  const size_t o_js = off;
+  off = align_up(off + n * sizeof(cmtj_junction));
+  const size_t o_layers = off;
+  off = align_up(off + rows * sizeof(cmtj_layer));
+  const size_t o_tables = off;
+  off = align_up(off + std::max<size_t>(n_table, 1) * sizeof(double));
+  std::vector<size_t> o_order(G), o_dyn(G);
+  for (size_t g = 0; g < G; ++g) {
+    o_order[g] = off;
+    off = align_up(off + orders[g].size() * sizeof(uint32_t));
+    o_dyn[g] = off;
+    off = align_up(off + nums[g].size() * sizeof(DynRec));
+  }
+  char* arena = (char*)get(off, gctx);
+  if (!arena) return CMTJ_ERR_HIP;
+  P->arena = arena;
+  HIP_TRY(hipMemcpyAsync(arena + o_js, js, n * sizeof(cmtj_junction), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(arena + o_layers, layers, rows * sizeof(cmtj_layer), hipMemcpyHostToDevice, stream));
+  if (n_table)
+    HIP_TRY(hipMemcpyAsync(arena + o_tables, tables, n_table * sizeof(double), hipMemcpyHostToDevice, stream));
+  P->groups.clear();
+  std::string names;
+  for (size_t g = 0; g < G; ++g) {
+    HIP_TRY(hipMemcpyAsync(arena + o_order[g], orders[g].data(), orders[g].size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(arena + o_dyn[g], nums[g].data(), nums[g].size() * sizeof(DynRec),
+                           hipMemcpyHostToDevice, stream));
+    const cmtj_junction& j0 = js[orders[g][0]];
+    const cmtj_layer* L0 = layers + j0.layer_offset;
+    const bool loge = (j0.log_mask & CMTJ_LOG_ENERGY) != 0;
+    const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], loge);
+    if (!k) return fail(CMTJ_ERR_INVALID, "no kernel for this junction configuration");
+    Group Gr;
+    Gr.k = k;
+    std::memset(&Gr.a, 0, sizeof(Gr.a));
+    Gr.a.js = (const cmtj_junction*)(arena + o_js);
+    Gr.a.layers = (const cmtj_layer*)(arena + o_layers);
+    Gr.a.order = (const uint32_t*)(arena + o_order[g]);
+    Gr.a.dyn = (const DynRec*)(arena + o_dyn[g]);
+    Gr.a.tables = (const double*)(arena + o_tables);
+    Gr.a.n = (uint32_t)orders[g].size();
+    Gr.a.flags = flags[g];
+    Gr.a.log_f32 = log_dtype == CMTJ_LOG_F32;
+    Gr.a.nl = j0.n_layers;
+    Gr.a.log_mask = j0.log_mask;
+    Gr.a.mr_mode = j0.mr_mode;
+    const std::vector<Rec>& r0 = recs_of[orders[g][0]];
+    Gr.a.ndyn = (int32_t)r0.size();
+    for (int l = 0; l < j0.n_layers; ++l) Gr.a.lbits[l] = layer_bits(L0[l]);
+    for (size_t q = 0; q < r0.size(); ++q) Gr.a.dmeta[q] = r0[q].meta;
+    Gr.a.c = *c;
+    Gr.blocks = (Gr.a.n + 63) / 64;
+    P->groups.push_back(Gr);
+    if (names.find(k->name) == std::string::npos) {
+      if (!names.empty()) names += ";";
+      names += k->name;
+    }
+  }
+  // host staging vectors die with this function: finish the copies first
+  HIP_TRY(hipStreamSynchronize(stream));
+  P->info.n_instances = n;
+  P->info.log_elements = extent;
+  P->info.total_steps = total_steps;
+  P->info.max_steps = max_steps;
+  P->info.precision = prec;
+  P->info.log_dtype = log_dtype;
+  P->info.max_layers = max_layers;
+  P->info.n_kernels = (int32_t)P->groups.size();
+  P->info.lanes_per_instance = 1;
+  std::strncpy(P->info.kernel_name, names.c_str(), sizeof(P->info.kernel_name) - 1);
+  P->log_extent = extent;
+  return 0;
+}
+
+static int launch_plan(cmtj_plan* P, void* d_log, double* d_mag, cmtj_status* d_status, hipStream_t s) {
+  for (Group& G : P->groups) {
+    GroupArgs a = G.a;
+    a.log = d_log;
+    a.mag_out = d_mag;
+    a.status = d_status;
+    void* args[] = {&a};
+    HIP_TRY(hipLaunchKernel(G.k->fn, dim3(G.blocks), dim3(64), args, 0, s));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the drop-in context: one stream, grow-only device buffers
+
+namespace {
+struct PlainCtx {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  int device = -1;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  void* out = nullptr;
+  size_t out_bytes = 0;
+};
+PlainCtx g_plain;
+
+void* grow(void*& p, size_t& have, size_t need) {
+  if (need <= have) return p;
+  const size_t want = std::max(need, have * 2);
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  have = 0;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    g_err = "hipMalloc failed for " + std::to_string(want) + " bytes";
+    p = nullptr;
+    return nullptr;
+  }
+  have = want;
+  return p;
+}
+
+void* plain_arena(size_t bytes, void* ctx) {
+  PlainCtx* c = (PlainCtx*)ctx;
+  return grow(c->arena, c->arena_bytes, bytes);
+}
+
+void* malloc_arena(size_t bytes, void*) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    g_err = "hipMalloc failed for " + std::to_string(bytes) + " bytes";
+    return nullptr;
+  }
+  return p;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int cmtj_abi(cmtj_abi_info* out) {
+  if (!out) return fail(CMTJ_ERR_INVALID, "null pointer");
+  out->version = CMTJ_ABI_VERSION;
+  out->max_layers = CMTJ_MAX_LAYERS;
+  out->nslot = CMTJ_NSLOT;
+  out->sizeof_driver = (int32_t)sizeof(cmtj_driver);
+  out->sizeof_layer = (int32_t)sizeof(cmtj_layer);
+  out->sizeof_junction = (int32_t)sizeof(cmtj_junction);
+  out->sizeof_constants = (int32_t)sizeof(cmtj_constants);
+  out->sizeof_status = (int32_t)sizeof(cmtj_status);
+  out->offsetof_layer_drv = (int32_t)offsetof(cmtj_layer, drv);
+  out->offsetof_layer_ref_type = (int32_t)offsetof(cmtj_layer, ref_type);
+  out->offsetof_junction_dt = (int32_t)offsetof(cmtj_junction, dt);
+  out->offsetof_junction_AHE = (int32_t)offsetof(cmtj_junction, AHE);
+  return 0;
+}
+
+const char* cmtj_last_error(void) { return g_err.c_str(); }
+
+int cmtj_constants_get(cmtj_constants* out) {
+  if (!out) return fail(CMTJ_ERR_INVALID, "null pointer");
+  std::lock_guard<std::mutex> lk(g_consts_mu);
+  *out = g_consts;
+  return 0;
+}
+
+int cmtj_constants_set(const cmtj_constants* in) {
+  if (!in) return fail(CMTJ_ERR_INVALID, "null pointer");
+  std::lock_guard<std::mutex> lk(g_consts_mu);
+  g_consts = *in;
+  return 0;
+}
+
+int cmtj_constants_reset(void) {
+  std::lock_guard<std::mutex> lk(g_consts_mu);
+  g_consts = kDefaults;
+  return 0;
+}
+
+int cmtj_driver_eval(const cmtj_driver* d, double t, double* out) {
+  if (!d || !out) return fail(CMTJ_ERR_INVALID, "null pointer");
+  if (d->type == CMTJ_TABLE) return fail(CMTJ_ERR_INVALID, "tabulated drivers have no closed form");
+  if (d->type < 0 || d->type > CMTJ_GAUSSSTEP) return fail(CMTJ_ERR_INVALID, "unknown driver type");
+  *out = driver_value(*d, t);
+  return 0;
+}
+
+int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
+             const double* tables, size_t n_table, double* log_host, size_t log_cap,
+             double* mag_out_host, cmtj_status* st) {
+  if (!j || !layers || !c || !mag_out_host || !st) return fail(CMTJ_ERR_INVALID, "null pointer");
+  std::lock_guard<std::mutex> lk(g_plain.mu);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (g_plain.stream == nullptr || g_plain.device != dev) {
+    if (g_plain.stream) (void)hipStreamDestroy(g_plain.stream);
+    if (g_plain.arena) (void)hipFree(g_plain.arena);
+    if (g_plain.out) (void)hipFree(g_plain.out);
+    g_plain.arena = g_plain.out = nullptr;
+    g_plain.arena_bytes = g_plain.out_bytes = 0;
+    HIP_TRY(hipStreamCreateWithFlags(&g_plain.stream, hipStreamNonBlocking));
+    g_plain.device = dev;
+  }
+  cmtj_junction jj = *j;
+  jj.layer_offset = 0;
+  jj.log_offset = 0;
+  const size_t nch = (size_t)n_channels(jj.n_layers, jj.mr_mode, jj.log_mask);
+  const size_t nlog = nch * jj.n_entries;
+  if (nlog > 0 && (!log_host || log_cap < nlog)) return fail(CMTJ_ERR_INVALID, "log buffer too small");
+  cmtj_plan P;
+  P.owns = false;
+  int r = build_plan(&P, &jj, 1, layers, (size_t)jj.n_layers, c, tables, n_table, CMTJ_FP64, CMTJ_LOG_F64,
+                     plain_arena, &g_plain, g_plain.stream);
+  if (r) return r;
+  const size_t o_mag = align_up(nlog * sizeof(double));
+  const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS);
+  const size_t out_bytes = o_st + sizeof(cmtj_status);
+  char* out = (char*)grow(g_plain.out, g_plain.out_bytes, out_bytes);
+  if (!out) return CMTJ_ERR_HIP;
+  r = launch_plan(&P, nlog ? out : nullptr, (double*)(out + o_mag), (cmtj_status*)(out + o_st), g_plain.stream);
+  if (r) return r;
+  if (nlog)
+    HIP_TRY(hipMemcpyAsync(log_host, out, nlog * sizeof(double), hipMemcpyDeviceToHost, g_plain.stream));
+  double mag[3 * CMTJ_MAX_LAYERS];
+  HIP_TRY(hipMemcpyAsync(mag, out + o_mag, sizeof(mag), hipMemcpyDeviceToHost, g_plain.stream));
+  HIP_TRY(hipMemcpyAsync(st, out + o_st, sizeof(cmtj_status), hipMemcpyDeviceToHost, g_plain.stream));
+  HIP_TRY(hipStreamSynchronize(g_plain.stream));
+  std::memcpy(mag_out_host, mag, sizeof(double) * 3 * (size_t)jj.n_layers);
+  return 0;
+}
+
+cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,
+                            const cmtj_constants* c, const double* tables, size_t n_table, int32_t precision,
+                            int32_t log_dtype, int32_t lanes_per_instance) {
+  if (!js || !layers || !c) {
+    fail(CMTJ_ERR_INVALID, "null pointer");
+    return nullptr;
+  }
+  if (lanes_per_instance != 0 && lanes_per_instance != 1) {
+    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 or 1");
+    return nullptr;
+  }
+  cmtj_plan* P = new cmtj_plan();
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    fail(CMTJ_ERR_HIP, "hipStreamCreate failed");
+    delete P;
+    return nullptr;
+  }
+  const int r = build_plan(P, js, n, layers, n_layer_rows, c, tables, n_table, precision, log_dtype,
+                           malloc_arena, nullptr, s);
+  (void)hipStreamDestroy(s);
+  if (r) {
+    if (P->arena) (void)hipFree(P->arena);
+    delete P;
+    return nullptr;
+  }
+  return P;
+}
+
+int cmtj_plan_execute(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_mag_out,
+                      cmtj_status* d_status, void* stream) {
+  if (!p) return fail(CMTJ_ERR_INVALID, "null plan");
+  if (d_log && log_capacity < p->log_extent)
+    return fail(CMTJ_ERR_INVALID, "log buffer smaller than the plan's log extent (" +
+                                      std::to_string(p->log_extent) + " elements)");
+  return launch_plan(p, d_log, d_mag_out, d_status, (hipStream_t)stream);
+}
+
+int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out) {
+  if (!p || !out) return fail(CMTJ_ERR_INVALID, "null pointer");
+  *out = p->info;
+  return 0;
+}
+
+int cmtj_plan_destroy(cmtj_plan* p) {
+  if (!p) return 0;
+  if (p->owns && p->arena) (void)hipFree(p->arena);
+  delete p;
+  return 0;
+}
+
+}  // extern "C"

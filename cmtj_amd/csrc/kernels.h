@@ -1,0 +1,650 @@
+// kernels.h -- the junction integration kernel family.
+//
+// One junction instance per lane.  All layers of an instance live in the
+// lane's registers; per step the kernel runs the reference's integrator
+// (RK4 with start-of-step neighbours, Heun, Euler-Heun: core/junction.hpp:
+// 1017-1041, 1682-1707, 1709-1815) and writes a decimated log entry every
+// write_every steps (core/junction.hpp:1973-1981).
+//
+// Template parameters: R (double: the FP64 drop-in; float: field/torque
+// stages in binary32 with state m, clock and driver phase in binary64), LM
+// (layer capacity), SOLVER, FM (compiled-in features; the batch-uniform
+// runtime flags select among them), LOGE (energy channels).
+#pragma once
+
+#include "physics.h"
+
+namespace cmtj {
+
+constexpr int MAX_DYN = 16;
+
+// Numbers of one time-dependent driver of one lane, stored [k][lane] so that
+// a wave reads record k of its 64 lanes from contiguous memory.  The kind,
+// slot and layer mask of record k are the same for every lane of a group and
+// travel in the kernel arguments.
+struct DynRec {
+  double A, p1, p2, p3;
+  int32_t tidx;  // table base (CMTJ_TABLE)
+  int32_t pad;
+};
+
+// Kernel arguments of one group.  A group shares its structure (layer count,
+// per-layer modes, driver kinds and slots, log layout): branches on these
+// are wave-uniform.  Numbers differ per lane.
+struct GroupArgs {
+  const cmtj_junction* js;
+  const cmtj_layer* layers;
+  const uint32_t* order;  // lane -> junction index
+  const DynRec* dyn;
+  const double* tables;
+  void* log;
+  double* mag_out;  // [n][CMTJ_MAX_LAYERS][3] in junction order
+  cmtj_status* status;
+  uint32_t n;       // lanes of this group
+  uint32_t flags;   // feature bits of the group
+  int32_t log_f32;
+  int32_t nl;       // layers
+  int32_t log_mask;
+  int32_t mr_mode;
+  int32_t ndyn;     // time-dependent driver records
+  int32_t pad;
+  int32_t lbits[CMTJ_MAX_LAYERS];  // ref_type | mode << 2 | dyn_sot << 4 | alt_stt << 5 | temp << 6
+  int32_t dmeta[MAX_DYN];          // kind | slot << 8 | layer mask << 16
+  cmtj_constants c;
+};
+
+// Driver values of one layer at one time.
+template <class R>
+struct DV {
+  V3<R> H;     // Hext + Hoe + Hdmi + Hres
+  V3<R> Hext;  // kept apart for the energy channels
+  R K, K2, Jt, Jb, J2t, J2b, I, FL, DL, FL2, DL2, T;
+  V3<R> Dt, Db;
+};
+
+template <class R>
+__device__ __forceinline__ void dv_zero(DV<R>& d) {
+  d.H = mk<R>(0, 0, 0);
+  d.Hext = d.H;
+  d.Dt = d.H;
+  d.Db = d.H;
+  d.K = d.K2 = d.Jt = d.Jb = d.J2t = d.J2b = d.I = d.FL = d.DL = d.FL2 = d.DL2 = d.T = R(0);
+}
+
+// Adds a driver value to the field of its slot.  The slot is wave-uniform, so
+// every condition below is a scalar select; adds into fields the kernel never
+// reads are removed by the compiler.
+template <class R, bool LOGE>
+__device__ __forceinline__ void add_slot(DV<R>& d, int slot, R v) {
+  const bool isH = slot < 12;  // Hext, Hoe, Hdmi, Hres
+  const int axis = slot % 3;
+  d.H.x += (isH && axis == 0) ? v : R(0);
+  d.H.y += (isH && axis == 1) ? v : R(0);
+  d.H.z += (isH && axis == 2) ? v : R(0);
+  if (LOGE) {
+    d.Hext.x += (slot == CMTJ_SLOT_HEXT + 0) ? v : R(0);
+    d.Hext.y += (slot == CMTJ_SLOT_HEXT + 1) ? v : R(0);
+    d.Hext.z += (slot == CMTJ_SLOT_HEXT + 2) ? v : R(0);
+  }
+  d.K += (slot == CMTJ_SLOT_K) ? v : R(0);
+  d.K2 += (slot == CMTJ_SLOT_K2) ? v : R(0);
+  d.Jt += (slot == CMTJ_SLOT_J_TOP) ? v : R(0);
+  d.Jb += (slot == CMTJ_SLOT_J_BOTTOM) ? v : R(0);
+  d.J2t += (slot == CMTJ_SLOT_J2_TOP) ? v : R(0);
+  d.J2b += (slot == CMTJ_SLOT_J2_BOTTOM) ? v : R(0);
+  d.I += (slot == CMTJ_SLOT_CURRENT) ? v : R(0);
+  d.FL += (slot == CMTJ_SLOT_FL) ? v : R(0);
+  d.DL += (slot == CMTJ_SLOT_DL) ? v : R(0);
+  d.FL2 += (slot == CMTJ_SLOT_FL2) ? v : R(0);
+  d.DL2 += (slot == CMTJ_SLOT_DL2) ? v : R(0);
+  d.T += (slot == CMTJ_SLOT_TEMPERATURE) ? v : R(0);
+  const bool isDt = slot >= CMTJ_SLOT_IDMI_TOP && slot < CMTJ_SLOT_IDMI_BOTTOM;
+  const bool isDb = slot >= CMTJ_SLOT_IDMI_BOTTOM && slot < CMTJ_NSLOT;
+  d.Dt.x += (isDt && axis == 0) ? v : R(0);
+  d.Dt.y += (isDt && axis == 1) ? v : R(0);
+  d.Dt.z += (isDt && axis == 2) ? v : R(0);
+  d.Db.x += (isDb && axis == 0) ? v : R(0);
+  d.Db.y += (isDb && axis == 1) ? v : R(0);
+  d.Db.z += (isDb && axis == 2) ? v : R(0);
+}
+
+// Derived per-layer parameters (in the stage type R).
+template <class R>
+struct LP {
+  R ms_mu0;   // Ms / mu0
+  R inv_ms;   // 1 / Ms
+  R inv_mst;  // 1 / (Ms thickness)
+  R alpha;
+  R neg_gamma;
+  R conv;     // 1 / (1 + alpha^2)
+  R gconv;    // -gamma / (1 + alpha^2)
+  V3<R> anis;
+  R N[9];
+  R Nb[9], Nt[9];  // dipole tensors (bottom, top)
+  V3<R> ref, ref2;
+  R stt_pref;  // gamma hbar / (e Ms t)
+  R P, lam, lam2, kappa, beta, fl_t, dl_t;
+  R sig2;  // 2 alpha kB / (Ms V gamma)
+};
+
+// Values of the last field/torque evaluation (energy channels).
+template <class R>
+struct ERec {
+  R K, K2, Jb, Jt, I;
+  V3<R> Hext, Hiec, Hanis, Hdemag, Hfl, Hdl;
+};
+
+#define CMTJ_HAS(b) ((FM & (b)) && (fl & (b)))
+
+template <class R>
+__device__ __forceinline__ V3<R> tensor3(const R* T, V3<R> m) {
+  return mk<R>(T[0] * m.x + T[1] * m.y + T[2] * m.z, T[3] * m.x + T[4] * m.y + T[5] * m.z,
+               T[6] * m.x + T[7] * m.y + T[8] * m.z);
+}
+
+// dm/dt of one layer: effective field (calculateHeffDipoleInjection,
+// core/junction.hpp:590-622) and solveLLG (:716-815).
+template <class R, uint32_t FM, bool LOGE>
+__device__ __forceinline__ V3<R> rhs(const LP<R>& p, int bits, uint32_t fl, V3<R> m,
+                                     const DV<R>& d, V3<R> bot, V3<R> top, ERec<R>& er) {
+  V3<R> Nm;
+  if (CMTJ_HAS(F_DEMAG_OFF))
+    Nm = tensor3(p.N, m);
+  else
+    Nm = mk<R>(p.N[0] * m.x, p.N[4] * m.y, p.N[8] * m.z);
+  const V3<R> Hdemag = Nm * p.ms_mu0;
+  V3<R> H = d.H;
+  if (CMTJ_HAS(F_ANIS)) {
+    const V3<R> Ha = p.anis * ((R(2) * d.K) * dot(p.anis, m) * p.inv_ms);
+    H = H + Ha;
+    if (LOGE) er.Hanis = Ha;
+  } else if (LOGE) {
+    er.Hanis = mk<R>(0, 0, 0);
+  }
+  if (CMTJ_HAS(F_ANIS2)) {
+    const R am = dot(p.anis, m);
+    H = H + p.anis * ((R(4) * d.K2) * (am * am * am) * p.inv_ms);
+  }
+  if (CMTJ_HAS(F_IEC) || CMTJ_HAS(F_IEC2)) {
+    R sb = d.Jb, st = d.Jt;
+    if (CMTJ_HAS(F_IEC2)) {
+      sb += R(2) * d.J2b * dot(bot, m);
+      st += R(2) * d.J2t * dot(top, m);
+    }
+    const V3<R> Hiec = (bot * sb + top * st) * p.inv_mst;
+    H = H + Hiec;
+    if (LOGE) er.Hiec = Hiec;
+  } else if (LOGE) {
+    er.Hiec = mk<R>(0, 0, 0);
+  }
+  if (CMTJ_HAS(F_IDMI)) H = H - (cross(d.Db, bot) + cross(d.Dt, top)) * p.inv_mst;
+  H = H - Hdemag;
+  if (CMTJ_HAS(F_DIPOLE)) H = H - (tensor3(p.Nb, bot) + tensor3(p.Nt, top)) * p.ms_mu0;
+  if (LOGE) {
+    er.K = d.K;
+    er.K2 = d.K2;
+    er.Jb = d.Jb;
+    er.Jt = d.Jt;
+    er.Hext = d.Hext;
+    er.Hdemag = Hdemag;
+    er.I = R(0);
+    er.Hfl = mk<R>(0, 0, 0);
+    er.Hdl = er.Hfl;
+  }
+  const V3<R> prod = cross(m, H);
+  const V3<R> prod2 = cross(m, prod);
+  const V3<R> dmdt = prod + prod2 * p.alpha;
+  V3<R> ref = mk<R>(0, 0, 0);
+  if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT)) {
+    const int rt = bits & 3;
+    if (rt == CMTJ_REF_FIXED)
+      ref = p.ref;
+    else if (rt == CMTJ_REF_TOP)
+      ref = top;
+    else if (rt == CMTJ_REF_BOTTOM)
+      ref = bot;
+  }
+  if (CMTJ_HAS(F_STT) && ((bits >> 2) & 3) == 1) {
+    const R mp = dot(m, ref);
+    R eta;
+    if ((bits >> 5) & 1)
+      eta = p.P / (R(1) + p.lam * mp);
+    else
+      eta = (p.P * p.lam2) / (p.lam2 + R(1) + (p.lam2 - R(1)) * mp);
+    const R stt = p.stt_pref * d.I * eta;
+    const V3<R> fieldLike = cross(m, ref);
+    const V3<R> dampingLike = cross(m, fieldLike);
+    if (LOGE) er.I = d.I;
+    return (dmdt * p.neg_gamma + dampingLike * (-stt * p.kappa) + fieldLike * (stt * p.beta)) *
+           p.conv;
+  }
+  if (CMTJ_HAS(F_SOT) && ((bits >> 2) & 3) == 2) {
+    R Hdl, Hfl, Hdl2 = R(0), Hfl2 = R(0);
+    if ((bits >> 4) & 1) {
+      Hdl = d.DL;
+      Hfl = d.FL;
+      Hdl2 = d.DL2;
+      Hfl2 = d.FL2;
+    } else {
+      Hdl = p.dl_t * d.I;
+      Hfl = p.fl_t * d.I;
+      if (LOGE) er.I = d.I;
+    }
+    const R a1 = Hfl - p.alpha * Hdl;
+    const R b1 = Hdl + p.alpha * Hfl;
+    const V3<R> cm = cross(m, ref);
+    const V3<R> ccm = cross(m, cm);
+    V3<R> r = dmdt + cm * a1 + ccm * b1;
+    if (CMTJ_HAS(F_SOT2)) {
+      const V3<R> cm2 = cross(m, p.ref2);
+      const V3<R> ccm2 = cross(m, cm2);
+      r = r + cm2 * (Hfl2 - p.alpha * Hdl2) + ccm2 * (Hdl2 + p.alpha * Hfl2);
+    }
+    if (LOGE) {
+      er.Hfl = ref * a1;
+      er.Hdl = ref * b1;
+    }
+    return r * p.gconv;
+  }
+  return dmdt * p.gconv;
+}
+
+// stochasticTorque, core/junction.hpp:824-831.
+template <class R>
+__device__ __forceinline__ V3<R> stoch(const LP<R>& p, V3<R> m, V3<R> dW) {
+  const V3<R> c1 = cross(m, dW);
+  const V3<R> c2 = cross(m, c1);
+  return (c1 + c2 * p.alpha) * p.gconv;
+}
+
+template <class R>
+__device__ __forceinline__ void load_layer(const cmtj_layer& L, const cmtj_constants& c, LP<R>& p,
+                                           DV<R>& base, V3<double>& m) {
+  p.ms_mu0 = (R)(L.Ms / c.mu0);
+  p.inv_ms = (R)(1.0 / L.Ms);
+  p.inv_mst = (R)(1.0 / (L.Ms * L.thickness));
+  p.alpha = (R)L.damping;
+  p.neg_gamma = (R)(-c.gyro);
+  p.conv = (R)(1.0 / (1.0 + L.damping * L.damping));
+  p.gconv = (R)(-c.gyro / (1.0 + L.damping * L.damping));
+  p.anis = mk<R>((R)L.anis[0], (R)L.anis[1], (R)L.anis[2]);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    p.N[q] = (R)L.demag[q];
+    p.Nb[q] = (R)L.dipole_bottom[q];
+    p.Nt[q] = (R)L.dipole_top[q];
+  }
+  p.ref = mk<R>((R)L.ref[0], (R)L.ref[1], (R)L.ref[2]);
+  p.ref2 = mk<R>((R)L.ref2[0], (R)L.ref2[1], (R)L.ref2[2]);
+  p.stt_pref = (R)(c.gyro * c.hbar / (c.echarge * L.Ms * L.thickness));
+  p.P = (R)L.polarisation;
+  p.lam = (R)L.slonczewski;
+  p.lam2 = (R)(L.slonczewski * L.slonczewski);
+  p.kappa = (R)L.kappa;
+  p.beta = (R)L.beta;
+  p.fl_t = (R)L.fl_torque;
+  p.dl_t = (R)L.dl_torque;
+  const double V = L.cell_surface * L.thickness;
+  p.sig2 = (R)((2.0 * L.damping * c.kB) / (L.Ms * V * c.gyro));
+  dv_zero(base);
+  for (int s = 0; s < CMTJ_NSLOT; ++s) {
+    const int ty = L.drv[s].type;
+    const double cv = (ty == CMTJ_TABLE) ? 0.0 : L.drv[s].constant;
+    if (cv != 0.0) add_slot<R, true>(base, s, (R)cv);
+  }
+  m = mk<double>(L.mag[0], L.mag[1], L.mag[2]);
+}
+
+// Numbers of the first DREG records stay in registers for the whole run.
+constexpr int DREG = 2;
+
+// Adds the time-dependent driver values at time t to dv.  The kind, slot and
+// layer mask of every record are wave-uniform (kernel arguments), so the
+// switches below are scalar branches.
+template <class R, int LM, uint32_t FM, bool LOGE>
+__device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base)[LM],
+                                             const DynRec (&dr)[DREG], const GroupArgs& g,
+                                             uint32_t lane, uint32_t fl, double t, uint32_t step,
+                                             int stage, int sps) {
+#pragma unroll
+  for (int l = 0; l < LM; ++l) dv[l] = base[l];
+  if (!CMTJ_HAS(F_DYN)) return;
+  for (int k = 0; k < g.ndyn; ++k) {
+    const int meta = g.dmeta[k];
+    const int type = meta & 0xff;
+    const int slot = (meta >> 8) & 0xff;
+    const int lmask = (meta >> 16) & 0xff;
+    DynRec r;
+    if (k < DREG) {
+#pragma unroll
+      for (int q = 0; q < DREG; ++q)
+        if (q == k) r = dr[q];
+    } else {
+      r = g.dyn[(size_t)k * g.n + lane];
+    }
+    R v;
+    if (CMTJ_HAS(F_TABLE) && type == CMTJ_TABLE) {
+      v = (R)g.tables[(size_t)r.tidx + (size_t)step * sps + stage];
+    } else if (sizeof(R) == 4) {
+      v = (R)wave_part_f(type, r.A, r.p1, r.p2, r.p3, t);
+    } else {
+      v = (R)wave_part(type, r.A, r.p1, r.p2, r.p3, t);
+    }
+#pragma unroll
+    for (int l = 0; l < LM; ++l)
+      if ((lmask >> l) & 1) add_slot<R, LOGE>(dv[l], slot, v);
+  }
+}
+
+// CVector::normalize (core/cvector.hpp:185-192): divide by the length.
+__device__ __forceinline__ V3<double> normalize_exact(V3<double> a) {
+  const double mag = sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+  if (mag != 0.0) {
+    a.x = a.x / mag;
+    a.y = a.y / mag;
+    a.z = a.z / mag;
+  }
+  return a;
+}
+
+// Same result to ~1 ulp: binary32 reciprocal square root refined by two
+// Newton steps in binary64 (used by the FP32-bulk kernels).
+__device__ __forceinline__ V3<double> normalize_fast(V3<double> a) {
+  const double s = a.x * a.x + a.y * a.y + a.z * a.z;
+  if (s == 0.0) return a;
+  double y = (double)__frsqrt_rn((float)s);
+  y = y * (1.5 - 0.5 * s * y * y);
+  y = y * (1.5 - 0.5 * s * y * y);
+  return mk<double>(a.x * y, a.y * y, a.z * y);
+}
+
+template <class R>
+__device__ __forceinline__ V3<double> normalize_m(V3<double> a) {
+  if (sizeof(R) == 8) return normalize_exact(a);
+  return normalize_fast(a);
+}
+
+__device__ __forceinline__ void log_put(const GroupArgs& g, size_t idx, double v) {
+  if (g.log_f32)
+    ((float*)g.log)[idx] = (float)v;
+  else
+    ((double*)g.log)[idx] = v;
+}
+
+// logLayerParams (core/junction.hpp:1580-1636) for entry k.
+template <class R, int LM, uint32_t FM, bool LOGE>
+__device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junction* J,
+                                            const V3<double> (&m)[LM], const ERec<R> (&er)[LM],
+                                            uint32_t k, uint64_t base, uint32_t NE, double Rp,
+                                            double Rap) {
+  const int nl = g.nl, lmask = g.log_mask, mr = g.mr_mode;
+  uint32_t ch = 0;
+#pragma unroll
+  for (int l = 0; l < LM; ++l) {
+    if (l < nl) {
+      if (lmask & CMTJ_LOG_MX) log_put(g, base + (size_t)(ch++) * NE + k, m[l].x);
+      if (lmask & CMTJ_LOG_MYZ) {
+        log_put(g, base + (size_t)(ch++) * NE + k, m[l].y);
+        log_put(g, base + (size_t)(ch++) * NE + k, m[l].z);
+      }
+    }
+  }
+  if (lmask & CMTJ_LOG_MR) {
+    if (mr == CMTJ_MR_CLASSIC) {
+      // calculateMagnetoresistance (:1865-1867); 1 layer against the fixed
+      // reference vector, 2 layers against each other (:1623-1629)
+      double cosT;
+      if (nl == 1) {
+        const double* rf = g.layers[J->layer_offset].ref;
+        cosT = m[0].x * rf[0] + m[0].y * rf[1] + m[0].z * rf[2];
+      } else {
+        const V3<double>& m1 = m[LM > 1 ? 1 : 0];
+        cosT = m[0].x * m1.x + m[0].y * m1.y + m[0].z * m1.z;
+      }
+      log_put(g, base + (size_t)(ch++) * NE + k, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
+    } else if (mr == CMTJ_MR_STRIP) {
+      // stripMagnetoResistance (:1835-1857), parallel sum of the layers
+      double rx = 0.0, ry = 0.0;
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const double Rx = J->Rx0[l] + J->AMR_X[l] * (m[l].x * m[l].x) + J->SMR_X[l] * (m[l].y * m[l].y);
+          const double Ry = J->Ry0[l] + 0.5 * J->AHE[l] * m[l].z +
+                            (J->AMR_Y[l] + J->SMR_Y[l]) * m[l].x * m[l].y;
+          rx += 1. / Rx;
+          ry += 1. / Ry;
+        }
+      }
+      log_put(g, base + (size_t)(ch++) * NE + k, 1 / rx);
+      log_put(g, base + (size_t)(ch++) * NE + k, 1 / ry);
+    }
+  }
+  if (LOGE && (lmask & CMTJ_LOG_ENERGY)) {
+#pragma unroll
+    for (int l = 0; l < LM; ++l) {
+      if (l < nl) {
+        const ERec<R>& e = er[l];
+        const double v[CMTJ_ENERGY_CHANNELS] = {
+            e.K, e.K2, e.Jb, e.Jt, e.I, e.Hext.x, e.Hext.y, e.Hext.z,
+            e.Hiec.x, e.Hiec.y, e.Hiec.z, e.Hanis.x, e.Hanis.y, e.Hanis.z,
+            e.Hdemag.x, e.Hdemag.y, e.Hdemag.z, e.Hfl.x, e.Hfl.y, e.Hfl.z,
+            e.Hdl.x, e.Hdl.y, e.Hdl.z};
+#pragma unroll
+        for (int q = 0; q < CMTJ_ENERGY_CHANNELS; ++q) log_put(g, base + (size_t)(ch++) * NE + k, v[q]);
+      }
+    }
+  }
+}
+
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
+__global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
+  const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= g.n) return;
+  const uint32_t fl = g.flags;
+  const int nl = g.nl;
+  const uint32_t jid = g.order[lane];
+  const cmtj_junction* J = g.js + jid;
+  const uint32_t nsteps = J->n_steps;
+  const uint32_t W = J->write_every;
+  const uint32_t NE = J->n_entries;
+  const double dt = J->dt;
+  const uint64_t seed = J->seed, stream = J->stream;
+  const uint64_t lbase = J->log_offset;
+  const bool logging = (g.log != nullptr) && g.log_mask != 0;
+  const double Rp = J->Rp, Rap = J->Rap;
+  const int sps = (SOLVER == CMTJ_RK4) ? 3 : (SOLVER == CMTJ_HEUN ? 2 : 1);
+
+  LP<R> P[LM];
+  DV<R> base[LM];
+  V3<double> m[LM];
+#pragma unroll
+  for (int l = 0; l < LM; ++l) {
+    if (l < nl) {
+      load_layer<R>(g.layers[J->layer_offset + l], g.c, P[l], base[l], m[l]);
+    } else {
+      dv_zero(base[l]);
+      m[l] = mk<double>(0, 0, 0);
+      P[l] = P[0];
+    }
+  }
+  DynRec dr[DREG];
+#pragma unroll
+  for (int k = 0; k < DREG; ++k) {
+    if (k < g.ndyn)
+      dr[k] = g.dyn[(size_t)k * g.n + lane];
+    else
+      dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
+  }
+
+  ERec<R> er[LM];
+  DV<R> dv[LM];
+  const R dtR = (R)dt;
+  const R sdt = (R)sqrt(dt);
+  const R sixth_inv = R(1.0 / 6.0);
+  const V3<R> zero = mk<R>(0, 0, 0);
+  uint64_t next_log = 0;
+  uint32_t k = 0;
+  int code = CMTJ_OK, nan_layer = -1;
+  uint32_t i = 0;
+  for (; i < nsteps; ++i) {
+    const double t = (double)i * dt;
+    if (SOLVER == CMTJ_RK4) {
+      // rk4_step (core/junction.hpp:1017-1041) for every layer, neighbours
+      // frozen at the start of the step (runMultiLayerSolver, :1682-1707).
+      // Stages run outermost so the driver values of one time serve all
+      // layers; layers are independent within a step.
+      V3<R> nbB[LM], nbT[LM], ms[LM], acc[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) ms[l] = cvt<R>(m[l]);
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        nbB[l] = (l > 0) ? ms[l > 0 ? l - 1 : 0] : zero;
+        nbT[l] = (l + 1 < nl) ? ms[l + 1 < LM ? l + 1 : l] : zero;
+      }
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t, i, 0, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const V3<R> k1 = rhs<R, FM, LOGE>(P[l], g.lbits[l], fl, ms[l], dv[l], nbB[l], nbT[l], er[l]) * dtR;
+          acc[l] = k1;
+          ms[l] = cvt<R>(m[l]) + k1 * R(0.5);
+        }
+      }
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t + 0.5 * dt, i, 1, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const V3<R> k2 = rhs<R, FM, LOGE>(P[l], g.lbits[l], fl, ms[l], dv[l], nbB[l], nbT[l], er[l]) * dtR;
+          acc[l] = acc[l] + k2 * R(2.0);
+          ms[l] = cvt<R>(m[l]) + k2 * R(0.5);
+        }
+      }
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const V3<R> k3 = rhs<R, FM, LOGE>(P[l], g.lbits[l], fl, ms[l], dv[l], nbB[l], nbT[l], er[l]) * dtR;
+          acc[l] = acc[l] + k3 * R(2.0);
+          ms[l] = cvt<R>(m[l]) + k3;
+        }
+      }
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t + dt, i, 2, sps);
+      V3<double> mn[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const V3<R> k4 = rhs<R, FM, LOGE>(P[l], g.lbits[l], fl, ms[l], dv[l], nbB[l], nbT[l], er[l]) * dtR;
+          acc[l] = acc[l] + k4;
+          V3<R> inc;
+          if (sizeof(R) == 8)
+            inc = mk<R>(acc[l].x / R(6.0), acc[l].y / R(6.0), acc[l].z / R(6.0));
+          else
+            inc = acc[l] * sixth_inv;
+          mn[l] = normalize_m<R>(m[l] + cvt<double>(inc));
+        }
+      }
+      // the reference updates layers in turn; the first NaN stops the run
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl && nan_layer < 0) {
+          m[l] = mn[l];
+          if (isnan(mn[l].x)) nan_layer = l;
+        }
+      }
+      if (nan_layer >= 0) {
+        code = CMTJ_ERR_NAN;
+        break;
+      }
+    } else if (SOLVER == CMTJ_EULER_HEUN) {
+      // eulerHeunSolverStep, core/junction.hpp:1709-1754
+      V3<R> mr_[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) mr_[l] = cvt<R>(m[l]);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t, i, 0, sps);
+      V3<double> mn[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const V3<R> bot = (l > 0) ? mr_[l > 0 ? l - 1 : 0] : zero;
+          const V3<R> top = (l + 1 < nl) ? mr_[l + 1 < LM ? l + 1 : l] : zero;
+          const V3<R> f = rhs<R, FM, LOGE>(P[l], g.lbits[l], fl, mr_[l], dv[l], bot, top, er[l]);
+          V3<double> acc = m[l] + cvt<double>(f * dtR);
+          if (CMTJ_HAS(F_THERMAL) && ((g.lbits[l] >> 6) & 1)) {
+            const R sigma = sqrt(P[l].sig2 * dv[l].T);
+            const V3<R> dW = thermal_normals<R>(seed, stream, i, (uint32_t)l) * sigma;
+            const V3<R> gn = stoch<R>(P[l], mr_[l], dW);
+            const V3<R> mN = mr_[l] + gn * sdt;
+            const V3<R> gp = stoch<R>(P[l], mN, dW);
+            acc = acc + cvt<double>((gn + gp) * R(0.5) * sdt);
+          }
+          mn[l] = acc;
+        }
+      }
+#pragma unroll
+      for (int l = 0; l < LM; ++l)
+        if (l < nl) m[l] = normalize_m<R>(mn[l]);
+    } else {
+      // heunSolverStep, core/junction.hpp:1756-1815
+      V3<R> mr_[LM], fn[LM], gn[LM], dW[LM], mN[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) mr_[l] = cvt<R>(m[l]);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t, i, 0, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const V3<R> bot = (l > 0) ? mr_[l > 0 ? l - 1 : 0] : zero;
+          const V3<R> top = (l + 1 < nl) ? mr_[l + 1 < LM ? l + 1 : l] : zero;
+          fn[l] = rhs<R, FM, LOGE>(P[l], g.lbits[l], fl, mr_[l], dv[l], bot, top, er[l]);
+          dW[l] = zero;
+          gn[l] = zero;
+          if (CMTJ_HAS(F_THERMAL) && ((g.lbits[l] >> 6) & 1)) {
+            const R sigma = sqrt(P[l].sig2 * dv[l].T);
+            dW[l] = thermal_normals<R>(seed, stream, i, (uint32_t)l) * sigma;
+            gn[l] = stoch<R>(P[l], mr_[l], dW[l]);
+          }
+          mN[l] = mr_[l] + fn[l] * dtR + gn[l] * sdt;
+        }
+      }
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t + dt, i, 1, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const V3<R> bot = (l > 0) ? mN[l > 0 ? l - 1 : 0] : zero;
+          const V3<R> top = (l + 1 < nl) ? mN[l + 1 < LM ? l + 1 : l] : zero;
+          const V3<R> f2 = rhs<R, FM, LOGE>(P[l], g.lbits[l], fl, mN[l], dv[l], bot, top, er[l]);
+          V3<double> acc = m[l] + cvt<double>((fn[l] + f2) * (R(0.5) * dtR));
+          if (CMTJ_HAS(F_THERMAL) && ((g.lbits[l] >> 6) & 1)) {
+            const V3<R> g2 = stoch<R>(P[l], mN[l], dW[l]);
+            acc = acc + cvt<double>((gn[l] + g2) * R(0.5) * sdt);
+          }
+          m[l] = normalize_m<R>(acc);
+        }
+      }
+    }
+    if ((uint64_t)i == next_log) {
+      if (logging) write_entry<R, LM, FM, LOGE>(g, J, m, er, k, lbase, NE, Rp, Rap);
+      ++k;
+      next_log += W;
+    }
+  }
+  if (g.mag_out) {
+#pragma unroll
+    for (int l = 0; l < LM; ++l) {
+      if (l < nl) {
+        double* o = g.mag_out + ((size_t)jid * CMTJ_MAX_LAYERS + l) * 3;
+        o[0] = m[l].x;
+        o[1] = m[l].y;
+        o[2] = m[l].z;
+      }
+    }
+  }
+  if (g.status) {
+    cmtj_status s;
+    s.code = code;
+    s.steps_done = i;
+    s.entries_done = k;
+    s.nan_layer = nan_layer;
+    g.status[jid] = s;
+  }
+}
+
+}  // namespace cmtj

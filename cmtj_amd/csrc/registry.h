@@ -1,0 +1,49 @@
+// registry.h -- kernel registry shared by the instantiation units.
+#pragma once
+
+#include "kernels.h"
+
+namespace cmtj {
+
+struct KEntry {
+  int prec;
+  int lm;
+  int solver;
+  uint32_t fm;
+  bool loge;
+  const void* fn;
+  const char* name;
+};
+
+struct KTable {
+  const KEntry* e;
+  size_t n;
+};
+
+constexpr uint32_t FM_LEAN = F_ANIS | F_IEC | F_DYN;
+constexpr uint32_t FM_STT = F_ANIS | F_IEC | F_DYN | F_STT;
+constexpr uint32_t FM_SOTT = F_ANIS | F_IEC | F_DYN | F_SOT | F_THERMAL;
+
+// one table per instantiation unit
+KTable kernels_f64();
+KTable kernels_f64e();
+KTable kernels_f32();
+KTable kernels_spec();
+
+}  // namespace cmtj
+
+#define CMTJ_KE(R, PREC, LM, S, FM, E)                                  \
+  {PREC, LM, S, FM, E, (const void*)&cmtj::k_junction<R, LM, S, FM, E>, \
+   "k_junction<" #R "," #LM "," #S "," #FM "," #E ">"}
+
+#define CMTJ_KE_SOLVERS(R, PREC, LM, FM, E)                                     \
+  CMTJ_KE(R, PREC, LM, CMTJ_RK4, FM, E), CMTJ_KE(R, PREC, LM, CMTJ_HEUN, FM, E), \
+      CMTJ_KE(R, PREC, LM, CMTJ_EULER_HEUN, FM, E)
+
+#define CMTJ_KE_LMS(R, PREC, FM, E)                                                          \
+  CMTJ_KE_SOLVERS(R, PREC, 1, FM, E), CMTJ_KE_SOLVERS(R, PREC, 2, FM, E),                   \
+      CMTJ_KE_SOLVERS(R, PREC, 4, FM, E), CMTJ_KE_SOLVERS(R, PREC, 8, FM, E)
+
+#define CMTJ_KE_LMS124(R, PREC, FM, E)                                       \
+  CMTJ_KE_SOLVERS(R, PREC, 1, FM, E), CMTJ_KE_SOLVERS(R, PREC, 2, FM, E), \
+      CMTJ_KE_SOLVERS(R, PREC, 4, FM, E)

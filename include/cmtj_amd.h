@@ -1,0 +1,293 @@
+/*
+ * cmtj_amd.h -- C ABI of the MI355X macrospin (s)LLGS engine.
+ *
+ * The library integrates macrospin junctions (one junction instance per GPU
+ * lane) with the semantics of cmtj's Junction::runSimulation fixed-step
+ * branch (reference: core/junction.hpp:1948-1981).  Everything crossing this
+ * boundary is plain C: POD descriptors, raw pointers and sizes, int status
+ * codes plus a thread-local message.  No torch or C++ types appear here.
+ *
+ * Units follow the reference: Ms in tesla, fields in A/m, K in J/m^3,
+ * J in J/m^2, gamma in m/(A s).
+ */
+#ifndef CMTJ_AMD_H_
+#define CMTJ_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMTJ_ABI_VERSION 1
+#define CMTJ_MAX_LAYERS 8
+
+/* ScalarDriver waveform kinds; numbering of reference UpdateType
+ * (core/drivers.hpp:18-29).  CMTJ_TABLE is this library's tabulated form of
+ * the reference's Python-callback driver (custom = 100). */
+enum cmtj_update_type {
+  CMTJ_CONSTANT = 0,
+  CMTJ_PULSE = 1,
+  CMTJ_SINE = 2,
+  CMTJ_STEP = 3,
+  CMTJ_POSINE = 4,
+  CMTJ_HALFSINE = 5,
+  CMTJ_TRAPEZOID = 6,
+  CMTJ_GAUSSIMPULSE = 7,
+  CMTJ_GAUSSSTEP = 8,
+  CMTJ_TABLE = 101
+};
+
+/* One ScalarDriver (reference Driver<T> fields, core/drivers.hpp:32-35, plus
+ * ScalarDriver edgeTime/steadyTime, :67-68).  value(t) = constant + f(t). */
+typedef struct cmtj_driver {
+  int32_t type;        /* enum cmtj_update_type */
+  int32_t table_index; /* CMTJ_TABLE: index into the run's driver tables */
+  double constant;
+  double amplitude;
+  double frequency;
+  double phase;
+  double period;
+  double cycle;
+  double t_start;
+  double t_stop;
+  double edge_time;   /* trapezoid edge; sigma of the gaussian kinds */
+  double steady_time;
+} cmtj_driver;
+
+/* Driver slots of one layer.  Axial drivers occupy three consecutive slots
+ * (x, y, z).  Reference members: core/junction.hpp:148-165. */
+enum cmtj_slot {
+  CMTJ_SLOT_HEXT = 0,       /* externalFieldDriver            (3) */
+  CMTJ_SLOT_HOE = 3,        /* HoeDriver                      (3) */
+  CMTJ_SLOT_HDMI = 6,       /* HdmiDriver                     (3) */
+  CMTJ_SLOT_HRES = 9,       /* HreservedInteractionFieldDriver(3) */
+  CMTJ_SLOT_K = 12,         /* anisotropyDriver */
+  CMTJ_SLOT_K2 = 13,        /* secondOrderAnisotropyDriver */
+  CMTJ_SLOT_J_TOP = 14,     /* IECDriverTop */
+  CMTJ_SLOT_J_BOTTOM = 15,  /* IECDriverBottom */
+  CMTJ_SLOT_J2_TOP = 16,    /* IECQuadDriverTop */
+  CMTJ_SLOT_J2_BOTTOM = 17, /* IECQuadDriverBottom */
+  CMTJ_SLOT_CURRENT = 18,   /* currentDriver */
+  CMTJ_SLOT_FL = 19,        /* fieldLikeTorqueDriver */
+  CMTJ_SLOT_DL = 20,        /* dampingLikeTorqueDriver */
+  CMTJ_SLOT_FL2 = 21,       /* fieldLikeTorqueDriver2 */
+  CMTJ_SLOT_DL2 = 22,       /* dampingLikeTorqueDriver2 */
+  CMTJ_SLOT_TEMPERATURE = 23,
+  CMTJ_SLOT_IDMI_TOP = 24,    /* IDMIDriverTop    (3) */
+  CMTJ_SLOT_IDMI_BOTTOM = 27, /* IDMIDriverBottom (3) */
+  CMTJ_NSLOT = 30
+};
+
+/* Reference enum Reference (core/junction.hpp:138). */
+enum cmtj_reference { CMTJ_REF_NONE = 0, CMTJ_REF_FIXED = 1, CMTJ_REF_TOP = 2, CMTJ_REF_BOTTOM = 3 };
+
+/* Reference enum SolverMode (core/junction.hpp:140). */
+enum cmtj_solver { CMTJ_EULER_HEUN = 0, CMTJ_RK4 = 1, CMTJ_DORMAND_PRINCE = 2, CMTJ_HEUN = 3 };
+
+/* Reference Junction::MRmode (core/junction.hpp:1144). */
+enum cmtj_mr_mode { CMTJ_MR_NONE = 0, CMTJ_MR_CLASSIC = 1, CMTJ_MR_STRIP = 2 };
+
+/* Log channel selection (bit mask).  Channel order inside one instance:
+ *   for each layer l: [mx] [my mz]          (LOG_MX, LOG_MYZ)
+ *   MR: CLASSIC -> R ; STRIP -> Rx, Ry      (LOG_MR; Rz is identically 0)
+ *   for each layer l: 23 energy channels    (LOG_ENERGY, FP64 runs only):
+ *     K K2 Jbottom Jtop I Hext(xyz) Hiec(xyz) Hanis(xyz) Hdemag(xyz)
+ *     Hfl(xyz) Hdl(xyz)
+ * Entries are written every write_every steps (entry k after step k*W,
+ * reference core/junction.hpp:1973-1981). */
+enum cmtj_log_bits { CMTJ_LOG_MX = 1, CMTJ_LOG_MYZ = 2, CMTJ_LOG_MR = 4, CMTJ_LOG_ENERGY = 8 };
+#define CMTJ_LOG_FULL (CMTJ_LOG_MX | CMTJ_LOG_MYZ | CMTJ_LOG_MR)
+#define CMTJ_ENERGY_CHANNELS 23
+
+/* Arithmetic of a run. */
+enum cmtj_precision {
+  CMTJ_FP64 = 0, /* everything in binary64 (the drop-in) */
+  CMTJ_FP32 = 1  /* field/torque stages in binary32; state m, clock, driver
+                    phase and resistance in binary64 */
+};
+enum cmtj_log_dtype { CMTJ_LOG_F64 = 0, CMTJ_LOG_F32 = 1 };
+
+/* One layer (reference Layer<T>, core/junction.hpp:145-259).  `mag` is the
+ * stored magnetisation as the reference keeps it (the constructor does not
+ * normalise it, core/junction.hpp:195; setMagnetisation does, :491-498). */
+typedef struct cmtj_layer {
+  double mag[3];
+  double anis[3];
+  double Ms;
+  double thickness;
+  double cell_surface;
+  double damping;
+  double demag[9];         /* row-major 3x3 */
+  double dipole_top[9];
+  double dipole_bottom[9];
+  double ref[3];           /* referenceLayer */
+  double ref2[3];          /* secondaryReferenceLayer */
+  double kappa;            /* STT damping-like scale, default 1 */
+  double beta;             /* STT field-like scale */
+  double slonczewski;      /* STT spacer parameter Lambda */
+  double polarisation;     /* STT spin polarisation */
+  double fl_torque;        /* static SOT field-like coefficient */
+  double dl_torque;        /* static SOT damping-like coefficient */
+  int32_t ref_type;        /* enum cmtj_reference */
+  int32_t include_stt;
+  int32_t include_sot;
+  int32_t dynamic_sot;
+  int32_t alternative_stt;
+  int32_t temperature_set; /* thermal (stochastic) field active */
+  cmtj_driver drv[CMTJ_NSLOT];
+} cmtj_layer;
+
+/* One junction instance of a run.  n_steps and write_every are the FP64
+ * truncations (unsigned)(totalTime/timeStep), (unsigned)(writeFrequency/
+ * timeStep) of core/junction.hpp:1956-1959; n_entries = ceil(n_steps /
+ * write_every).  The instance's log block starts at element log_offset of the
+ * log buffer and is laid out [channel][entry]. */
+typedef struct cmtj_junction {
+  int32_t n_layers;
+  int32_t mr_mode;       /* enum cmtj_mr_mode */
+  int32_t solver;        /* enum cmtj_solver, after the temperature switch */
+  int32_t log_mask;      /* enum cmtj_log_bits */
+  uint32_t n_steps;
+  uint32_t write_every;
+  uint32_t n_entries;
+  uint32_t layer_offset; /* first row of this junction in the layer array */
+  uint64_t seed;         /* Philox key of the thermal field */
+  uint64_t stream;       /* Philox counter word (e.g. run index of the junction) */
+  uint64_t log_offset;   /* in log elements */
+  double dt;
+  double Rp;
+  double Rap;
+  double Rx0[CMTJ_MAX_LAYERS];
+  double Ry0[CMTJ_MAX_LAYERS];
+  double AMR_X[CMTJ_MAX_LAYERS];
+  double AMR_Y[CMTJ_MAX_LAYERS];
+  double SMR_X[CMTJ_MAX_LAYERS];
+  double SMR_Y[CMTJ_MAX_LAYERS];
+  double AHE[CMTJ_MAX_LAYERS];
+} cmtj_junction;
+
+/* Snapshot of reference PhysicalConstants (core/constants.hpp:6-53). */
+typedef struct cmtj_constants {
+  double mu0;
+  double gyro;
+  double TtoAm;
+  double hbar;
+  double echarge;
+  double kB;
+} cmtj_constants;
+
+/* Per-instance outcome. */
+enum cmtj_status_code {
+  CMTJ_OK = 0,
+  CMTJ_ERR_INVALID = 1,
+  CMTJ_ERR_HIP = 2,
+  CMTJ_ERR_NAN = 3 /* reference: runtime_error("NAN magnetisation"), core/junction.hpp:1037-1039 */
+};
+typedef struct cmtj_status {
+  int32_t code;
+  uint32_t steps_done;    /* steps completed (the failing step excluded) */
+  uint32_t entries_done;  /* log entries written */
+  int32_t nan_layer;      /* first layer with a NaN, else -1 */
+} cmtj_status;
+
+/* Sizes and field offsets, for host bindings that mirror these structs. */
+typedef struct cmtj_abi_info {
+  int32_t version;
+  int32_t max_layers;
+  int32_t nslot;
+  int32_t sizeof_driver;
+  int32_t sizeof_layer;
+  int32_t sizeof_junction;
+  int32_t sizeof_constants;
+  int32_t sizeof_status;
+  int32_t offsetof_layer_drv;
+  int32_t offsetof_layer_ref_type;
+  int32_t offsetof_junction_dt;
+  int32_t offsetof_junction_AHE;
+} cmtj_abi_info;
+
+/* ---- library-level ------------------------------------------------------ */
+
+/* Fills *out; returns 0.  No reference counterpart (ABI self-description). */
+int cmtj_abi(cmtj_abi_info* out);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* cmtj_last_error(void);
+
+/* Process-wide constants store behind the Python PhysicalConstants class.
+ * Replaces the static members and setters/getters of
+ * core/constants.hpp:6-53 (bound at python/cmtj.cpp:496-511). */
+int cmtj_constants_get(cmtj_constants* out);
+int cmtj_constants_set(const cmtj_constants* in);
+int cmtj_constants_reset(void);
+
+/* Host evaluation of one driver at time t, the same code the kernels run.
+ * Replaces ScalarDriver<T>::getCurrentScalarValue (core/drivers.hpp:262-305)
+ * for the non-callback kinds; CMTJ_TABLE is rejected here. */
+int cmtj_driver_eval(const cmtj_driver* d, double t, double* out);
+
+/* ---- the drop-in: one junction, FP64, host buffers -------------------- */
+
+/* Runs one junction (FP64) on the current GPU and copies results back.
+ * Replaces Junction<T>::runSimulation's fixed-step branch
+ * (core/junction.hpp:1948-1981) with the log of logLayerParams (:1580-1636)
+ * written to `log_host` as [channel][entry] doubles (capacity log_cap
+ * elements), the final magnetisation of every layer to mag_out_host[3*L].
+ * tables/n_table: samples of CMTJ_TABLE drivers (may be NULL/0).
+ * Returns 0 on success (st->code tells NaN failures), nonzero on error. */
+int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
+             const cmtj_constants* c, const double* tables, size_t n_table,
+             double* log_host, size_t log_cap, double* mag_out_host,
+             cmtj_status* st);
+
+/* ---- batched plans: many uneven instances, device buffers -------------- */
+
+typedef struct cmtj_plan cmtj_plan;
+
+typedef struct cmtj_plan_info {
+  uint64_t n_instances;
+  uint64_t log_elements;    /* sum over instances of channels*entries */
+  uint64_t total_steps;     /* sum of n_steps (junction-steps per execute) */
+  uint64_t max_steps;
+  int32_t precision;
+  int32_t log_dtype;
+  int32_t max_layers;       /* largest layer count in the batch */
+  int32_t n_kernels;        /* kernel launches per execute */
+  int32_t lanes_per_instance;
+  int32_t reserved;
+  char kernel_name[128];
+} cmtj_plan_info;
+
+/* Builds a plan: uploads descriptors to HBM, orders instances by cost and
+ * selects kernels.  The n junctions reference rows of `layers`
+ * (n_layer_rows rows) through layer_offset; log_offset of every junction
+ * must already place its block in the caller's log buffer.  `tables` are
+ * CMTJ_TABLE samples (may be NULL).  Batches the reference's per-point
+ * runSimulation loop (cmtj/utils/procedures.py:141, cmtj/utils/parallel.py:47)
+ * which has no single-call counterpart in the reference.
+ * lanes_per_instance: 1 (all layers in one lane) or 0 (library picks).
+ * Returns NULL on error (see cmtj_last_error). */
+cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n,
+                            const cmtj_layer* layers, size_t n_layer_rows,
+                            const cmtj_constants* c, const double* tables,
+                            size_t n_table, int32_t precision,
+                            int32_t log_dtype, int32_t lanes_per_instance);
+
+/* Runs every instance of the plan once from its initial magnetisation.
+ * d_log: device log buffer (f64 or f32 per the plan) of log_capacity
+ * elements (checked against the plan's log extent), d_mag_out: device
+ * [n][CMTJ_MAX_LAYERS][3] doubles in the caller's junction order (may be
+ * NULL), d_status: device [n] cmtj_status (may be NULL).  stream: a
+ * hipStream_t (NULL = default stream).  Asynchronous; allocates nothing. */
+int cmtj_plan_execute(cmtj_plan* p, void* d_log, size_t log_capacity,
+                      double* d_mag_out, cmtj_status* d_status, void* stream);
+
+int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out);
+int cmtj_plan_destroy(cmtj_plan* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMTJ_AMD_H_ */

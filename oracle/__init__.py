@@ -1,0 +1,123 @@
+"""CPU FP64 oracle of the (s)LLGS path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+this package.  It wraps liboracle.so (oracle.c, a plain-C restatement of the
+reference algorithm) and the optional _ref/ref_drivers harness built from the
+reference's own core/drivers.hpp.
+
+Pinning (what fixes each oracle function independently of its own formula):
+  * driver waveforms: reference known answers (tests/test_drivers.py:51-65)
+    and the compiled reference drivers.hpp (_ref/ref_drivers) on a time grid;
+  * field/torque/RK4 on one layer: the exact damped-precession trajectory in a
+    uniform field and RK4's 4th-order convergence; test_sto.json,
+    test_harmonic_hall.json, test_cims_hysteresis.json golden fixtures;
+  * multi-layer stepping: test_fmr_basic.json, test_ahe_loops.json fixtures and
+    the first-order convergence that start-of-step neighbours imply;
+  * Heun / Euler-Heun: convergence orders 2 / 1 against the exact trajectory;
+  * thermal field: Philox4x32-10 known-answer vectors (Random123) and the
+    Boltzmann (Langevin-function) equilibrium of an ensemble;
+  * energy channels: key sets of tests/test_basic.py and closed forms of the
+    logged fields.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_DRIVERS = HERE / "_ref" / "ref_drivers"
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise ImportError(f"{LIB} missing: run `make -C oracle`")
+        from cmtj_amd import _lib as abi  # struct dtypes only
+
+        lib = ctypes.CDLL(str(LIB))
+        P = ctypes.c_void_p
+        lib.oracle_run.argtypes = [P, P, P, P, ctypes.c_size_t, P, P, P]
+        lib.oracle_run_batch.argtypes = [P, ctypes.c_size_t, P, P, P, ctypes.c_size_t, P, P, P, ctypes.c_int]
+        lib.oracle_driver_value.argtypes = [P, ctypes.c_double, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P]
+        lib.oracle_driver_value.restype = ctypes.c_double
+        lib.oracle_philox4x32_10.argtypes = [P, P, P]
+        lib.oracle_thermal_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                               ctypes.c_uint32, P]
+        lib.oracle_n_channels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        _lib = (lib, abi)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def run(j, layers, consts, tables=None):
+    """One junction; returns (log [ch, entries], mag [L, 3], status) like cmtj_run."""
+    lib, abi = _load()
+    j = np.ascontiguousarray(j, dtype=abi.JUNCTION_DTYPE).reshape(())
+    layers = np.ascontiguousarray(layers, dtype=abi.LAYER_DTYPE)
+    nch = lib.oracle_n_channels(int(j["n_layers"]), int(j["mr_mode"]), int(j["log_mask"]))
+    log = np.zeros((nch, int(j["n_entries"])), dtype=np.float64)
+    mag = np.zeros((int(j["n_layers"]), 3), dtype=np.float64)
+    st = np.zeros((), dtype=abi.STATUS_DTYPE)
+    tabs = None if tables is None else np.ascontiguousarray(tables, dtype=np.float64)
+    lib.oracle_run(_p(j), _p(layers), _p(np.ascontiguousarray(consts)), _p(tabs),
+                   0 if tabs is None else tabs.size, _p(log), _p(mag), _p(st))
+    return log, mag, st
+
+
+def run_batch(js, layers, consts, tables=None, log_elements=0, n_threads=None):
+    """Many junctions over host threads; log blocks at each log_offset (FP64)."""
+    lib, abi = _load()
+    js = np.ascontiguousarray(js, dtype=abi.JUNCTION_DTYPE)
+    layers = np.ascontiguousarray(layers, dtype=abi.LAYER_DTYPE)
+    log = np.zeros(max(int(log_elements), 1), dtype=np.float64)
+    mag = np.zeros((js.size, abi.MAX_LAYERS, 3), dtype=np.float64)
+    st = np.zeros(js.size, dtype=abi.STATUS_DTYPE)
+    tabs = None if tables is None else np.ascontiguousarray(tables, dtype=np.float64)
+    n_threads = n_threads or os.cpu_count() or 1
+    lib.oracle_run_batch(_p(js), js.size, _p(layers), _p(np.ascontiguousarray(consts)), _p(tabs),
+                         0 if tabs is None else tabs.size, _p(log), _p(mag), _p(st), int(n_threads))
+    return log, mag, st
+
+
+def driver_value(rec, t: float) -> float:
+    lib, abi = _load()
+    rec = np.ascontiguousarray(rec, dtype=abi.DRIVER_DTYPE).reshape(())
+    return lib.oracle_driver_value(_p(rec), float(t), 0, 0, 1, None)
+
+
+def philox4x32_10(ctr, key):
+    lib, _ = _load()
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib.oracle_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def thermal_normals(seed: int, stream: int, step: int, layer: int):
+    lib, _ = _load()
+    xi = np.zeros(3, dtype=np.float64)
+    lib.oracle_thermal_normals(seed, stream, step, layer, _p(xi))
+    return xi
+
+
+def ref_drivers_available() -> bool:
+    return REF_DRIVERS.exists()
+
+
+def ref_driver_values(lines: list[str]) -> list[float]:
+    """Evaluates reference-built drivers (see ref_drivers_harness.cpp)."""
+    out = subprocess.run([str(REF_DRIVERS)], input="\n".join(lines) + "\n", capture_output=True, text=True,
+                         check=True)
+    return [float.fromhex(x) for x in out.stdout.split()]

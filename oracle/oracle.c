@@ -1,0 +1,558 @@
+/*
+ * oracle.c -- CPU FP64 restatement of cmtj's (s)LLGS time integration.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; never part of the product path).
+ *
+ * Plain C, scalar binary64, compiled without FP contraction so every product
+ * and sum rounds separately, as the reference's x86-64 build does.  Each
+ * function restates one reference function; the reference line is cited.
+ * Thermal noise: the reference draws std::normal_distribution over an
+ * unseedable mt19937 (core/junction.hpp:174-175); this oracle and the engine
+ * share a counter-based Philox4x32-10 + Box-Muller stream instead, so a
+ * stochastic run is reproducible and comparable trajectory by trajectory.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+  double x, y, z;
+} vec3;
+
+static vec3 v3(double x, double y, double z) {
+  vec3 r = {x, y, z};
+  return r;
+}
+static vec3 vadd(vec3 a, vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static vec3 vsub(vec3 a, vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static vec3 vmul(vec3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
+static vec3 smul(double s, vec3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+static vec3 vdiv(vec3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+/* c_dot, core/junction.hpp:82-84 */
+static double vdot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+/* c_cross, core/junction.hpp:74-80 */
+static vec3 vcross(vec3 a, vec3 b) {
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static vec3 vload(const double* p) { return v3(p[0], p[1], p[2]); }
+/* CVector::length / normalize, core/cvector.hpp:179-192 */
+static double vlength(vec3 a) { return sqrt(pow(a.x, 2) + pow(a.y, 2) + pow(a.z, 2)); }
+static vec3 vnormalize(vec3 a) {
+  const double mag = vlength(a);
+  if (mag != 0) {
+    a.x = a.x / mag;
+    a.y = a.y / mag;
+    a.z = a.z / mag;
+  }
+  return a;
+}
+
+/* calculate_tensor_interaction, core/junction.hpp:54-62 */
+static vec3 tensor_interaction(vec3 m, const double* t, double Ms, double mu0) {
+  vec3 r = v3(t[0] * m.x + t[1] * m.y + t[2] * m.z, t[3] * m.x + t[4] * m.y + t[5] * m.z,
+              t[6] * m.x + t[7] * m.y + t[8] * m.z);
+  return vmul(r, Ms / mu0);
+}
+
+/* ---- drivers ------------------------------------------------------------ */
+
+/* ScalarDriver::getCurrentScalarValue, core/drivers.hpp:262-305, with the
+ * helpers stepUpdate (:73-78), pulseTrain (:79-88), trapezoidalUpdate
+ * (:90-108).  CMTJ_TABLE returns the pre-sampled callback value. */
+double oracle_driver_value(const cmtj_driver* d, double time, uint32_t step, int stage,
+                           int samples_per_step, const double* tables) {
+  double v = d->constant;
+  switch (d->type) {
+    case CMTJ_PULSE: {
+      const int n = (int)(time / d->period);
+      const double dT = d->cycle * d->period;
+      const double nT = n * d->period;
+      if (nT <= time && time <= (nT + dT)) v += d->amplitude;
+      break;
+    }
+    case CMTJ_SINE:
+      v += d->amplitude * sin(2 * M_PI * time * d->frequency + d->phase);
+      break;
+    case CMTJ_POSINE:
+      v += fabs(d->amplitude * sin(2 * M_PI * time * d->frequency + d->phase));
+      break;
+    case CMTJ_HALFSINE: {
+      const double tamp = d->amplitude * sin(2 * M_PI * time * d->frequency + d->phase);
+      if (tamp <= 0) v += tamp;
+      break;
+    }
+    case CMTJ_STEP:
+      if (time >= d->t_start && time <= d->t_stop) v += d->amplitude;
+      break;
+    case CMTJ_TRAPEZOID: {
+      const double a = d->amplitude, t0 = d->t_start, e = d->edge_time, s = d->steady_time;
+      double add = 0;
+      if (time < t0)
+        add = 0;
+      else if (time <= t0 + e)
+        add = (a / e) * (time - t0);
+      else if (time <= t0 + e + s)
+        add = a;
+      else if (time <= t0 + 2 * e + s)
+        add = a - (a / e) * (time - (t0 + e + s));
+      v += add;
+      break;
+    }
+    case CMTJ_GAUSSIMPULSE:
+      v += d->amplitude * exp(-pow(time - d->t_start, 2) / (2 * pow(d->edge_time, 2)));
+      break;
+    case CMTJ_GAUSSSTEP:
+      v += 0.5 * d->amplitude * (1 + erf((time - d->t_start) / (sqrt(2) * d->edge_time)));
+      break;
+    case CMTJ_TABLE:
+      /* the reference returns the callback value alone (drivers.hpp:294-302) */
+      return tables[(size_t)d->table_index + (size_t)step * (size_t)samples_per_step + (size_t)stage];
+    default:
+      break;
+  }
+  return v;
+}
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) + Box-Muller ------------------ */
+
+static void philox_round(uint32_t c[4], const uint32_t k[2]) {
+  const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+  const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+  const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+  const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+  const uint32_t n0 = hi1 ^ c[1] ^ k[0];
+  const uint32_t n1 = lo1;
+  const uint32_t n2 = hi0 ^ c[3] ^ k[1];
+  const uint32_t n3 = lo0;
+  c[0] = n0;
+  c[1] = n1;
+  c[2] = n2;
+  c[3] = n3;
+}
+
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  uint32_t k[2] = {key[0], key[1]};
+  for (int r = 0; r < 10; ++r) {
+    philox_round(c, k);
+    if (r < 9) {
+      k[0] += 0x9E3779B9u;
+      k[1] += 0xBB67AE85u;
+    }
+  }
+  memcpy(out, c, sizeof(c));
+}
+
+/* Counter {step, layer, stream lo, stream hi}, key {seed lo, seed hi};
+ * u = (x + 0.5) 2^-32; xi = sqrt(-2 ln u0) (cos 2pi u1, sin 2pi u1),
+ * sqrt(-2 ln u2) cos 2pi u3. */
+void oracle_thermal_normals(uint64_t seed, uint64_t stream, uint32_t step, uint32_t layer,
+                            double xi[3]) {
+  const uint32_t ctr[4] = {step, layer, (uint32_t)stream, (uint32_t)(stream >> 32)};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  oracle_philox4x32_10(ctr, key, o);
+  const double s = 2.3283064365386963e-10; /* 2^-32 */
+  const double u0 = ((double)o[0] + 0.5) * s, u1 = ((double)o[1] + 0.5) * s;
+  const double u2 = ((double)o[2] + 0.5) * s, u3 = ((double)o[3] + 0.5) * s;
+  const double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+  xi[0] = r0 * cos(2.0 * M_PI * u1);
+  xi[1] = r0 * sin(2.0 * M_PI * u1);
+  xi[2] = r1 * cos(2.0 * M_PI * u3);
+}
+
+/* ---- one layer ------------------------------------------------------------ */
+
+typedef struct {
+  const cmtj_layer* p;
+  const cmtj_constants* c;
+  const double* tables;
+  int sps; /* table samples per step */
+  vec3 mag;
+  double cellVolume;
+  /* values of the last field/torque evaluation (logged with energies) */
+  double K_log, K2_log, Jbottom_log, Jtop_log, I_log;
+  vec3 Hext, HIEC, HAnis, Hdemag, Hfl_v, Hdl_v;
+} olayer;
+
+static double drv(const olayer* L, int slot, double t, uint32_t step, int stage) {
+  return oracle_driver_value(&L->p->drv[slot], t, step, stage, L->sps, L->tables);
+}
+/* AxialDriver::getCurrentAxialDrivers, core/drivers.hpp:429-433 */
+static vec3 axial(const olayer* L, int slot, double t, uint32_t step, int stage) {
+  return v3(drv(L, slot, t, step, stage), drv(L, slot + 1, t, step, stage),
+            drv(L, slot + 2, t, step, stage));
+}
+
+/* calculateIEC_, core/junction.hpp:654-662 */
+static vec3 iec_(const olayer* L, double J, double J2, vec3 m, vec3 coupled) {
+  return vdiv(vmul(coupled, J + 2 * J2 * vdot(coupled, m)), L->p->Ms * L->p->thickness);
+}
+/* calculateIDMI_, core/junction.hpp:677-693 */
+static vec3 idmi_(const olayer* L, vec3 D, vec3 coupled) {
+  return vdiv(smul(-1.0, vcross(D, coupled)), L->p->Ms * L->p->thickness);
+}
+
+/* calculateHeff + calculateHeffDipoleInjection, core/junction.hpp:579-622,
+ * with the term functions :624-701.  Stores the logged values. */
+static vec3 heff(olayer* L, double t, uint32_t step, int stage, vec3 m, vec3 bottom, vec3 top) {
+  const cmtj_layer* p = L->p;
+  const double mu0 = L->c->mu0;
+  const vec3 Hdipole = vadd(tensor_interaction(bottom, p->dipole_bottom, p->Ms, mu0),
+                            tensor_interaction(top, p->dipole_top, p->Ms, mu0));
+  L->Hext = axial(L, CMTJ_SLOT_HEXT, t, step, stage);
+  const vec3 Hoe = axial(L, CMTJ_SLOT_HOE, t, step, stage);
+  L->Hdemag = tensor_interaction(m, p->demag, p->Ms, mu0);
+  /* calculateIEC, :664-675 */
+  L->Jbottom_log = drv(L, CMTJ_SLOT_J_BOTTOM, t, step, stage);
+  L->Jtop_log = drv(L, CMTJ_SLOT_J_TOP, t, step, stage);
+  const double J2b = drv(L, CMTJ_SLOT_J2_BOTTOM, t, step, stage);
+  const double J2t = drv(L, CMTJ_SLOT_J2_TOP, t, step, stage);
+  L->HIEC = vadd(iec_(L, L->Jbottom_log, J2b, m, bottom), iec_(L, L->Jtop_log, J2t, m, top));
+  /* calculateIDMI, :695-701 */
+  const vec3 Hidmi = vadd(idmi_(L, axial(L, CMTJ_SLOT_IDMI_BOTTOM, t, step, stage), bottom),
+                          idmi_(L, axial(L, CMTJ_SLOT_IDMI_TOP, t, step, stage), top));
+  /* calculateAnisotropy, :638-643 */
+  L->K_log = drv(L, CMTJ_SLOT_K, t, step, stage);
+  const vec3 anis = vload(p->anis);
+  L->HAnis = vmul(anis, (2 * L->K_log) * vdot(anis, m) / (p->Ms));
+  const vec3 Hdmi = axial(L, CMTJ_SLOT_HDMI, t, step, stage);
+  const vec3 Hres = axial(L, CMTJ_SLOT_HRES, t, step, stage);
+  /* calculateSecondOrderAnisotropy, :645-652 */
+  L->K2_log = drv(L, CMTJ_SLOT_K2, t, step, stage);
+  const vec3 HAnis2 = vmul(anis, (4 * L->K2_log) * pow(vdot(anis, m), 3) / (p->Ms));
+  const vec3 Hfluct = v3(0, 0, 0);
+  /* summation order of :607-620 */
+  vec3 H = vadd(L->Hext, L->HAnis);
+  H = vadd(H, HAnis2);
+  H = vadd(H, L->HIEC);
+  H = vadd(H, Hidmi);
+  H = vadd(H, Hoe);
+  H = vadd(H, Hdmi);
+  H = vadd(H, Hfluct);
+  H = vsub(H, L->Hdemag);
+  H = vsub(H, Hdipole);
+  H = vadd(H, Hres);
+  return H;
+}
+
+/* solveLLG, core/junction.hpp:716-815 */
+static vec3 solve_llg(olayer* L, double t, uint32_t step, int stage, vec3 m, vec3 bottom, vec3 top,
+                      vec3 H) {
+  const cmtj_layer* p = L->p;
+  const double GYRO = L->c->gyro;
+  const vec3 prod = vcross(m, H);
+  const vec3 prod2 = vcross(m, prod);
+  const double convTerm = 1 / (1 + pow(p->damping, 2));
+  const vec3 dmdt = vadd(prod, vmul(prod2, p->damping));
+  vec3 reference = v3(0, 0, 0);
+  switch (p->ref_type) {
+    case CMTJ_REF_FIXED:
+      reference = vload(p->ref);
+      break;
+    case CMTJ_REF_TOP:
+      reference = top;
+      break;
+    case CMTJ_REF_BOTTOM:
+      reference = bottom;
+      break;
+    default:
+      break;
+  }
+  if (p->include_stt) {
+    L->I_log = drv(L, CMTJ_SLOT_CURRENT, t, step, stage);
+    const double aJ = L->c->hbar * L->I_log / (L->c->echarge * p->Ms * p->thickness);
+    double eta;
+    if (p->alternative_stt) {
+      eta = p->polarisation / (1 + p->slonczewski * vdot(m, reference));
+    } else {
+      const double slonSq = pow(p->slonczewski, 2);
+      eta = (p->polarisation * slonSq) / (slonSq + 1 + (slonSq - 1) * vdot(m, reference));
+    }
+    const double sttTerm = GYRO * aJ * eta;
+    const vec3 fieldLike = vcross(m, reference);
+    const vec3 dampingLike = vcross(m, fieldLike);
+    vec3 r = vadd(vmul(dmdt, -GYRO), vmul(vmul(dampingLike, -sttTerm), p->kappa));
+    r = vadd(r, vmul(vmul(fieldLike, sttTerm), p->beta));
+    return vmul(r, convTerm);
+  } else if (p->include_sot) {
+    double Hdl = 0, Hfl = 0, Hdl2 = 0, Hfl2 = 0;
+    if (p->dynamic_sot) {
+      Hdl = drv(L, CMTJ_SLOT_DL, t, step, stage);
+      Hfl = drv(L, CMTJ_SLOT_FL, t, step, stage);
+      Hdl2 = drv(L, CMTJ_SLOT_DL2, t, step, stage);
+      Hfl2 = drv(L, CMTJ_SLOT_FL2, t, step, stage);
+    } else {
+      L->I_log = drv(L, CMTJ_SLOT_CURRENT, t, step, stage);
+      Hdl = p->dl_torque * L->I_log;
+      Hfl = p->fl_torque * L->I_log;
+    }
+    const vec3 ref2 = vload(p->ref2);
+    L->Hfl_v = vmul(reference, Hfl - p->damping * Hdl);
+    L->Hdl_v = vmul(reference, Hdl + p->damping * Hfl);
+    const vec3 cm = vcross(m, reference);
+    const vec3 ccm = vcross(m, cm);
+    const vec3 cm2 = vcross(m, ref2);
+    const vec3 ccm2 = vcross(m, cm2);
+    const vec3 flT = vmul(cm, Hfl - p->damping * Hdl);
+    const vec3 dlT = vmul(ccm, Hdl + p->damping * Hfl);
+    const vec3 flT2 = vmul(cm2, Hfl2 - p->damping * Hdl2);
+    const vec3 dlT2 = vmul(ccm2, Hdl2 + p->damping * Hfl2);
+    vec3 r = vadd(vadd(vadd(vadd(dmdt, flT), dlT), flT2), dlT2);
+    return vmul(vmul(r, -GYRO), convTerm);
+  }
+  return vmul(vmul(dmdt, -GYRO), convTerm);
+}
+
+/* calculateLLGWithFieldTorque, core/junction.hpp:855-864 */
+static vec3 llg(olayer* L, double t, uint32_t step, int stage, vec3 m, vec3 bottom, vec3 top) {
+  const vec3 H = heff(L, t, step, stage, m, bottom, top);
+  return solve_llg(L, t, step, stage, m, bottom, top, H);
+}
+
+/* stochasticTorque, core/junction.hpp:824-831 */
+static vec3 stochastic_torque(const olayer* L, vec3 m, vec3 dW) {
+  const double convTerm = -L->c->gyro / (1. + pow(L->p->damping, 2));
+  const vec3 thcross = vcross(m, dW);
+  const vec3 thcross2 = vcross(m, thcross);
+  return vmul(vadd(thcross, vmul(thcross2, L->p->damping)), convTerm);
+}
+
+/* getLangevinStochasticStandardDeviation + getStochasticLangevinVector,
+ * core/junction.hpp:1111-1128 (the draw uses the shared Philox stream). */
+static vec3 langevin_vector(const olayer* L, const cmtj_junction* j, double t, uint32_t step,
+                            uint32_t layer_index) {
+  if (!L->p->temperature_set) return v3(0, 0, 0);
+  const double T = drv(L, CMTJ_SLOT_TEMPERATURE, t, step, 0);
+  const double mainFactor =
+      (2 * L->p->damping * L->c->kB * T) / (L->p->Ms * L->cellVolume * L->c->gyro);
+  const double sigma = sqrt(mainFactor);
+  double xi[3];
+  oracle_thermal_normals(j->seed, j->stream, step, layer_index, xi);
+  return vmul(v3(xi[0], xi[1], xi[2]), sigma);
+}
+
+/* rk4_step, core/junction.hpp:1017-1041.  Returns 1 on NaN. */
+static int rk4_step(olayer* L, double time, double dt, uint32_t step, vec3 bottom, vec3 top) {
+  vec3 m_t = L->mag;
+  const vec3 k1 = vmul(llg(L, time, step, 0, m_t, bottom, top), dt);
+  const vec3 k2 = vmul(llg(L, time + 0.5 * dt, step, 1, vadd(m_t, vmul(k1, 0.5)), bottom, top), dt);
+  const vec3 k3 = vmul(llg(L, time + 0.5 * dt, step, 1, vadd(m_t, vmul(k2, 0.5)), bottom, top), dt);
+  const vec3 k4 = vmul(llg(L, time + dt, step, 2, vadd(m_t, k3), bottom, top), dt);
+  m_t = vadd(m_t, vdiv(vadd(vadd(vadd(k1, vmul(k2, 2.0)), vmul(k3, 2.0)), k4), 6.0));
+  m_t = vnormalize(m_t);
+  L->mag = m_t;
+  return isnan(L->mag.x) ? 1 : 0;
+}
+
+int oracle_n_channels(int n_layers, int mr_mode, int log_mask) {
+  int n = 0;
+  if (log_mask & CMTJ_LOG_MX) n += n_layers;
+  if (log_mask & CMTJ_LOG_MYZ) n += 2 * n_layers;
+  if (log_mask & CMTJ_LOG_MR) n += (mr_mode == CMTJ_MR_CLASSIC) ? 1 : (mr_mode == CMTJ_MR_STRIP ? 2 : 0);
+  if (log_mask & CMTJ_LOG_ENERGY) n += CMTJ_ENERGY_CHANNELS * n_layers;
+  return n;
+}
+
+/* logLayerParams (core/junction.hpp:1580-1636) into channel rows, with
+ * calculateMagnetoresistance (:1865-1867) and stripMagnetoResistance
+ * (:1835-1857). */
+static void log_entry(const cmtj_junction* j, const olayer* Ls, double* log, uint32_t k) {
+  const int nl = j->n_layers;
+  const size_t E = j->n_entries;
+  int ch = 0;
+  for (int l = 0; l < nl; ++l) {
+    if (j->log_mask & CMTJ_LOG_MX) log[(size_t)(ch++) * E + k] = Ls[l].mag.x;
+    if (j->log_mask & CMTJ_LOG_MYZ) {
+      log[(size_t)(ch++) * E + k] = Ls[l].mag.y;
+      log[(size_t)(ch++) * E + k] = Ls[l].mag.z;
+    }
+  }
+  if (j->log_mask & CMTJ_LOG_MR) {
+    if (j->mr_mode == CMTJ_MR_CLASSIC) {
+      const double cosTheta = (nl == 1) ? vdot(Ls[0].mag, vload(Ls[0].p->ref))
+                                        : vdot(Ls[0].mag, Ls[1].mag);
+      log[(size_t)(ch++) * E + k] = j->Rp + (((j->Rap - j->Rp) / 2.0) * (1.0 - cosTheta));
+    } else if (j->mr_mode == CMTJ_MR_STRIP) {
+      double Rx_acc = 0.0, Ry_acc = 0.0;
+      for (int l = 0; l < nl; ++l) {
+        const vec3 m = Ls[l].mag;
+        const double Rx = j->Rx0[l] + j->AMR_X[l] * (m.x * m.x) + j->SMR_X[l] * (m.y * m.y);
+        const double Ry = j->Ry0[l] + 0.5 * j->AHE[l] * m.z + (j->AMR_Y[l] + j->SMR_Y[l]) * m.x * m.y;
+        Rx_acc += 1. / Rx;
+        Ry_acc += 1. / Ry;
+      }
+      log[(size_t)(ch++) * E + k] = 1 / Rx_acc;
+      log[(size_t)(ch++) * E + k] = 1 / Ry_acc;
+    }
+  }
+  if (j->log_mask & CMTJ_LOG_ENERGY) {
+    for (int l = 0; l < nl; ++l) {
+      const olayer* L = &Ls[l];
+      const int sot = L->p->include_sot;
+      const double vals[CMTJ_ENERGY_CHANNELS] = {
+          L->K_log,  L->K2_log,  L->Jbottom_log, L->Jtop_log, L->I_log,
+          L->Hext.x, L->Hext.y,  L->Hext.z,      L->HIEC.x,   L->HIEC.y,
+          L->HIEC.z, L->HAnis.x, L->HAnis.y,     L->HAnis.z,  L->Hdemag.x,
+          L->Hdemag.y, L->Hdemag.z,
+          sot ? L->Hfl_v.x : 0, sot ? L->Hfl_v.y : 0, sot ? L->Hfl_v.z : 0,
+          sot ? L->Hdl_v.x : 0, sot ? L->Hdl_v.y : 0, sot ? L->Hdl_v.z : 0,
+      };
+      for (int q = 0; q < CMTJ_ENERGY_CHANNELS; ++q) log[(size_t)(ch++) * E + k] = vals[q];
+    }
+  }
+}
+
+int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
+               const double* tables, size_t n_table, double* log, double* mag_out, cmtj_status* st) {
+  (void)n_table;
+  const int nl = j->n_layers;
+  olayer Ls[CMTJ_MAX_LAYERS];
+  memset(Ls, 0, sizeof(Ls));
+  const int sps = (j->solver == CMTJ_RK4) ? 3 : (j->solver == CMTJ_HEUN ? 2 : 1);
+  for (int l = 0; l < nl; ++l) {
+    Ls[l].p = &layers[l];
+    Ls[l].c = c;
+    Ls[l].tables = tables;
+    Ls[l].sps = sps;
+    Ls[l].mag = vload(layers[l].mag);
+    Ls[l].cellVolume = layers[l].cell_surface * layers[l].thickness;
+  }
+  st->code = CMTJ_OK;
+  st->steps_done = 0;
+  st->entries_done = 0;
+  st->nan_layer = -1;
+  const double dt = j->dt;
+  const vec3 zero = v3(0, 0, 0);
+  uint32_t k = 0;
+  for (uint32_t i = 0; i < j->n_steps; ++i) {
+    const double t = i * dt; /* core/junction.hpp:1974 */
+    if (j->solver == CMTJ_RK4) {
+      /* runSingleLayerSolver / runMultiLayerSolver, :1668-1707: neighbours
+       * frozen at the start of the step for every stage of every layer */
+      vec3 copies[CMTJ_MAX_LAYERS + 2];
+      copies[0] = zero;
+      copies[nl + 1] = zero;
+      for (int l = 0; l < nl; ++l) copies[l + 1] = Ls[l].mag;
+      for (int l = 0; l < nl; ++l) {
+        if (rk4_step(&Ls[l], t, dt, i, copies[l], copies[l + 2])) {
+          st->code = CMTJ_ERR_NAN;
+          st->nan_layer = l;
+          goto done;
+        }
+      }
+    } else if (j->solver == CMTJ_EULER_HEUN) {
+      /* eulerHeunSolverStep, :1709-1754 */
+      vec3 mPrime[CMTJ_MAX_LAYERS];
+      for (int l = 0; l < nl; ++l) {
+        olayer* L = &Ls[l];
+        const vec3 dW = langevin_vector(L, j, t, i, (uint32_t)l);
+        const vec3 bottom = (l == 0) ? zero : Ls[l - 1].mag;
+        const vec3 top = (l == nl - 1) ? zero : Ls[l + 1].mag;
+        const vec3 fn = llg(L, t, i, 0, L->mag, bottom, top);
+        const vec3 gn = stochastic_torque(L, L->mag, dW);
+        const vec3 mNext = vadd(L->mag, vmul(gn, sqrt(dt)));
+        const vec3 gnp = stochastic_torque(L, mNext, dW);
+        mPrime[l] = vadd(vadd(L->mag, vmul(fn, dt)), vmul(smul(0.5, vadd(gn, gnp)), sqrt(dt)));
+      }
+      for (int l = 0; l < nl; ++l) Ls[l].mag = vnormalize(mPrime[l]);
+    } else if (j->solver == CMTJ_HEUN) {
+      /* heunSolverStep, :1756-1815 */
+      vec3 fn[CMTJ_MAX_LAYERS], gn[CMTJ_MAX_LAYERS], dW[CMTJ_MAX_LAYERS], mNext[CMTJ_MAX_LAYERS];
+      for (int l = 0; l < nl; ++l) {
+        olayer* L = &Ls[l];
+        const vec3 bottom = (l == 0) ? zero : Ls[l - 1].mag;
+        const vec3 top = (l == nl - 1) ? zero : Ls[l + 1].mag;
+        fn[l] = llg(L, t, i, 0, L->mag, bottom, top);
+        dW[l] = langevin_vector(L, j, t, i, (uint32_t)l);
+        gn[l] = stochastic_torque(L, L->mag, dW[l]);
+        mNext[l] = vadd(vadd(L->mag, vmul(fn[l], dt)), vmul(gn[l], sqrt(dt)));
+      }
+      for (int l = 0; l < nl; ++l) {
+        olayer* L = &Ls[l];
+        const vec3 bottom = (l == 0) ? zero : mNext[l - 1];
+        const vec3 top = (l == nl - 1) ? zero : mNext[l + 1];
+        const vec3 f2 = llg(L, t + dt, i, 1, mNext[l], bottom, top);
+        const vec3 g2 = stochastic_torque(L, mNext[l], dW[l]);
+        vec3 m = vadd(L->mag, smul(0.5 * dt, vadd(fn[l], f2)));
+        m = vadd(m, vmul(smul(0.5, vadd(gn[l], g2)), sqrt(dt)));
+        L->mag = vnormalize(m);
+      }
+    } else {
+      st->code = CMTJ_ERR_INVALID;
+      goto done;
+    }
+    st->steps_done = i + 1;
+    if (i % j->write_every == 0) { /* :1978-1980 */
+      log_entry(j, Ls, log, k);
+      ++k;
+      st->entries_done = k;
+    }
+  }
+done:
+  for (int l = 0; l < nl; ++l) {
+    mag_out[3 * l + 0] = Ls[l].mag.x;
+    mag_out[3 * l + 1] = Ls[l].mag.y;
+    mag_out[3 * l + 2] = Ls[l].mag.z;
+  }
+  return 0;
+}
+
+/* ---- threaded batch (CPU baseline) ------------------------------------ */
+
+typedef struct {
+  const cmtj_junction* js;
+  size_t n;
+  const cmtj_layer* layers;
+  const cmtj_constants* c;
+  const double* tables;
+  size_t n_table;
+  double* log;
+  double* mag_out;
+  cmtj_status* st;
+  size_t next;
+  pthread_mutex_t mu;
+} batch_ctx;
+
+static void* batch_worker(void* arg) {
+  batch_ctx* b = (batch_ctx*)arg;
+  for (;;) {
+    pthread_mutex_lock(&b->mu);
+    const size_t i = b->next++;
+    pthread_mutex_unlock(&b->mu);
+    if (i >= b->n) break;
+    const cmtj_junction* j = &b->js[i];
+    double* lg = b->log ? b->log + j->log_offset : NULL;
+    double scratch_mag[3 * CMTJ_MAX_LAYERS];
+    oracle_run(j, b->layers + j->layer_offset, b->c, b->tables, b->n_table, lg,
+               b->mag_out ? b->mag_out + (size_t)i * 3 * CMTJ_MAX_LAYERS : scratch_mag, &b->st[i]);
+  }
+  return NULL;
+}
+
+int oracle_run_batch(const cmtj_junction* js, size_t n, const cmtj_layer* layers,
+                     const cmtj_constants* c, const double* tables, size_t n_table, double* log,
+                     double* mag_out, cmtj_status* st, int n_threads) {
+  batch_ctx b;
+  b.js = js;
+  b.n = n;
+  b.layers = layers;
+  b.c = c;
+  b.tables = tables;
+  b.n_table = n_table;
+  b.log = log;
+  b.mag_out = mag_out;
+  b.st = st;
+  b.next = 0;
+  pthread_mutex_init(&b.mu, NULL);
+  if (n_threads < 1) n_threads = 1;
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)n_threads);
+  for (int t = 0; t < n_threads; ++t) pthread_create(&th[t], NULL, batch_worker, &b);
+  for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+  free(th);
+  pthread_mutex_destroy(&b.mu);
+  return 0;
+}

@@ -1,0 +1,12 @@
+"""A Junction whose runs go through the CPU oracle instead of the GPU --
+test infrastructure for checking the oracle against the reference's golden
+vectors with the same user-level scripts."""
+from __future__ import annotations
+
+import oracle
+from cmtj_amd import Junction
+
+
+class OracleJunction(Junction):
+    def _execute(self, j, L, consts, tables):
+        return oracle.run(j, L, consts, tables)
