@@ -1,0 +1,145 @@
+"""Synthetic sweep workloads of the benchmark, as batches.
+
+  r_h_loop      1-layer R(H) loop, 101 fields, STRIP Rx, RK4 dt=1e-13, 10 ns
+  pimm          2-layer MTJ pulse-induced spectra, fields 0..600 kA/m at 30 deg
+                in-plane, per-point duration U(2, 20) ns, 1 kA/m Hoe_y step
+  cims_pbit     2-layer static-SOT switching ensemble at 300 K, Euler-Heun
+  sto           2-layer STT oscillator, DC current sweep, RK4
+
+Parameters the workload descriptions leave open are fixed here and listed in
+each builder's docstring.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .batch import JunctionBatch
+from .core import AxialDriver, CVector, Junction, Layer, NullDriver, ScalarDriver, SolverMode, constantDriver
+
+THIN = [CVector(0, 0, 0), CVector(0, 0, 0), CVector(0, 0, 1)]
+
+
+def r_h_loop(n_fields: int = 101, total_time: float = 10e-9, dt: float = 1e-13, wf: float = 1e-12,
+             precision: str = "fp64", log: str = "full") -> JunctionBatch:
+    """Free layer Ms 1.2 T, t 2 nm, S 1e-16 m^2, alpha 0.01, Ku 5e3 J/m^3
+    along x, N = diag(0.00024, 0.0022, 0.99756), m0 = (1,0,0); STRIP with
+    Rx0 = 100, AMR_X = 1; Hx swept -800..800 kA/m.  Each field point starts
+    from m0 (independent points)."""
+    l = Layer("free", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=1.2, thickness=2e-9, cellSurface=1e-16,
+              demagTensor=[CVector(0.00024, 0, 0), CVector(0, 0.0022, 0), CVector(0, 0, 0.99756)], damping=0.01)
+    j = Junction([l], [100.0], [0.0], [1.0], [0.0], [0.0], [0.0], [0.0])
+    j.setLayerAnisotropyDriver("free", constantDriver(5e3))
+    b = JunctionBatch.replicate(j, n_fields, total_time, dt, wf, SolverMode.RK4, precision=precision, log=log)
+    H = np.zeros((n_fields, 3))
+    H[:, 0] = np.linspace(-800e3, 800e3, n_fields)
+    b.set_external_field("all", H)
+    return b
+
+
+def pimm_junction() -> Junction:
+    """Free: Ms 1.0 T, t 1 nm, Ku 1.5e5 J/m^3 along z, alpha 0.01; reference:
+    Ms 1.6 T, t 3 nm, Ku 1e4 J/m^3 along x, alpha 0.01; thin-film demag
+    diag(0,0,1); S = 1e-16 m^2; J = -1e-4 J/m^2; Rp 100, Rap 200 Ohm; Hoe_y
+    step of 1 kA/m over the first 3 steps (the PIMM_procedure default,
+    cmtj/utils/procedures.py:96-108)."""
+    free = Layer("free", mag=CVector(math.cos(math.pi / 6), math.sin(math.pi / 6), 0.05), anis=CVector(0, 0, 1),
+                 Ms=1.0, thickness=1e-9, cellSurface=1e-16, demagTensor=THIN, damping=0.01)
+    ref = Layer("ref", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=1.6, thickness=3e-9, cellSurface=1e-16,
+                demagTensor=THIN, damping=0.01)
+    j = Junction([free, ref], 100, 200)
+    j.setLayerAnisotropyDriver("free", constantDriver(1.5e5))
+    j.setLayerAnisotropyDriver("ref", constantDriver(1e4))
+    j.setIECDriver("free", "ref", constantDriver(-1e-4))
+    dt = 1e-13
+    j.setLayerOerstedFieldDriver("all", AxialDriver(NullDriver(), ScalarDriver.getStepDriver(0, 1e3, 0, 3 * dt),
+                                                    NullDriver()))
+    return j
+
+
+def pimm(n_points: int = 4096, seed: int = 0, t_min: float = 2e-9, t_max: float = 20e-9, dt: float = 1e-13,
+         precision: str = "fp32", log: str = "mx", log_dtype=None) -> JunctionBatch:
+    """PIMM sweep: bias 0..600 kA/m at 30 deg in-plane, per-point duration
+    drawn U(t_min, t_max) from `seed`, logging every step (the FFT input)."""
+    rng = np.random.default_rng(seed)
+    durations = rng.uniform(t_min, t_max, n_points)
+    j = pimm_junction()
+    b = JunctionBatch.replicate(j, n_points, durations, dt, dt, SolverMode.RK4, precision=precision, log=log,
+                                log_dtype=log_dtype)
+    Hmag = np.linspace(0, 600e3, n_points)
+    phi = math.radians(30)
+    H = np.stack([Hmag * math.cos(phi), Hmag * math.sin(phi), np.zeros(n_points)], axis=1)
+    b.set_external_field("all", H)
+    b.set_times(durations, dt, dt)
+    return b
+
+
+def cims_junction(T: float = 300.0) -> Junction:
+    """Free: Ms 1.1 T, t 1.2 nm, Ku 3e5 J/m^3 along z, alpha 0.02, static SOT
+    with tau_DL/j = 1e-12 T per A/m^2 (x TtoAm -> A/m) and FL/DL = 0.3,
+    polarisation along y; pinned layer: Ms 1.2 T, t 3 nm, Ku 1e5 J/m^3 along
+    x, alpha 0.02; S = (50 nm)^2 pi; thermal field on both layers."""
+    TtoAm = 795774.715459
+    hdl = 1e-12 * TtoAm
+    S = (50e-9) ** 2 * math.pi
+    free = Layer.createSOTLayer("free", mag=CVector(0.01, 0.0, 1.0), anis=CVector(0, 0, 1), Ms=1.1,
+                                thickness=1.2e-9, cellSurface=S, demagTensor=THIN, damping=0.02,
+                                fieldLikeTorque=0.3 * hdl, dampingLikeTorque=hdl)
+    free.setReferenceLayer(CVector(0, 1, 0))
+    pinned = Layer("pinned", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=1.2, thickness=3e-9, cellSurface=S,
+                   demagTensor=THIN, damping=0.02)
+    j = Junction([free, pinned], 100, 200)
+    j.setLayerAnisotropyDriver("free", constantDriver(3e5))
+    j.setLayerAnisotropyDriver("pinned", constantDriver(1e5))
+    j.setLayerTemperatureDriver("all", constantDriver(T))
+    j.setLayerExternalFieldDriver("all", AxialDriver(5e3, 0, 0))
+    return j
+
+
+def cims_pbit(n_currents: int = 256, n_seeds: int = 4096, total_time: float = 5e-9, dt: float = 1e-14,
+              pulse: float = 5e-9, precision: str = "fp32", seed: int = 1) -> JunctionBatch:
+    """Current pulses 1e10..1e12 A/m^2 (log-spaced) x seeds; final state only."""
+    j = cims_junction()
+    n = n_currents * n_seeds
+    b = JunctionBatch.replicate(j, n, total_time, dt, total_time, SolverMode.EulerHeun, precision=precision,
+                                log="none")
+    jcur = np.repeat(np.logspace(10, 12, n_currents), n_seeds)
+    b.set_driver("free", "current", "type", np.full(n, 3, dtype=np.int32))  # step
+    b.set_driver("free", "current", "amplitude", jcur)
+    b.set_driver("free", "current", "t_start", np.zeros(n))
+    b.set_driver("free", "current", "t_stop", np.full(n, pulse))
+    rng = np.random.default_rng(seed)
+    b.set_seeds(rng.integers(0, 2**63, n, dtype=np.uint64))
+    return b
+
+
+def sto_junction() -> Junction:
+    """Free: STT layer Ms 0.8 T, t 2 nm, in-plane Ku 2e3 J/m^3 along x,
+    alpha 0.005, P 0.6, Lambda 1, polariser = the pinned layer below (TOP/
+    BOTTOM reference); pinned: Ms 1.2 T, t 3 nm, Ku 1e5 along x; out-of-plane
+    bias 200 kA/m; S 1e-16 m^2."""
+    free = Layer.createSTTLayer("free", mag=CVector(1, 0.1, 0.1), anis=CVector(1, 0, 0), Ms=0.8, thickness=2e-9,
+                                cellSurface=1e-16, demagTensor=THIN, damping=0.005,
+                                SlonczewskiSpacerLayerParameter=1.0, beta=0.0, spinPolarisation=0.6)
+    pinned = Layer("pinned", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=1.2, thickness=3e-9, cellSurface=1e-16,
+                   demagTensor=THIN, damping=0.02)
+    j = Junction([pinned, free], 100, 200)
+    j.setLayerReferenceType("free", 3)  # BOTTOM: the pinned layer
+    j.setLayerAnisotropyDriver("free", constantDriver(2e3))
+    j.setLayerAnisotropyDriver("pinned", constantDriver(1e5))
+    j.setLayerExternalFieldDriver("all", AxialDriver(0, 0, 200e3))
+    return j
+
+
+def sto(n_currents: int = 512, total_time: float = 100e-9, dt: float = 1e-13, wf: float = 1e-11,
+        precision: str = "fp32", log: str = "full") -> JunctionBatch:
+    """DC current sweep (A/m^2) of the STT oscillator, decimated log."""
+    j = sto_junction()
+    b = JunctionBatch.replicate(j, n_currents, total_time, dt, wf, SolverMode.RK4, precision=precision, log=log)
+    I = np.linspace(-2e11, -2e10, n_currents)
+    b.set_driver("free", "current", "constant", I)
+    return b
+
+
+__all__ = ["r_h_loop", "pimm", "pimm_junction", "cims_pbit", "cims_junction", "sto", "sto_junction"]

@@ -1,0 +1,54 @@
+"""The C ABI: the library loads without a GPU, exports every entry point
+declared in include/*.h, and its structs match the host binding."""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+from cmtj_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared() -> set:
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(cmtj_[a-z_0-9]+)\s*\(", text))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    declared = _declared()
+    assert declared, "no declarations found"
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert declared == set(_lib.EXPORTS)
+
+
+def test_struct_layout_matches_binding():
+    _lib.lib()  # raises ImportError on any mismatch
+    assert _lib.LAYER_DTYPE.itemsize == 3056
+    assert _lib.JUNCTION_DTYPE.itemsize == 528
+
+
+def test_host_entry_points_need_no_gpu():
+    c = _lib.constants_get()
+    assert float(c["gyro"]) == 220880.0
+    rec = _lib.ScalarDriverRecord() if hasattr(_lib, "ScalarDriverRecord") else None
+    import numpy as np
+
+    r = np.zeros((), dtype=_lib.DRIVER_DTYPE)
+    r["type"] = _lib.STEP
+    r["amplitude"] = 2.0
+    r["t_start"] = 1.0
+    r["t_stop"] = 2.0
+    assert _lib.driver_eval(r, 1.0) == 2.0 and _lib.driver_eval(r, 2.5) == 0.0
+    r["type"] = _lib.TABLE
+    try:
+        _lib.driver_eval(r, 0.0)
+        raise AssertionError("tabulated drivers must be rejected")
+    except RuntimeError as e:
+        assert "tabulated" in str(e)
