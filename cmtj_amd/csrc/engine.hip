@@ -99,7 +99,7 @@ static uint32_t junction_flags(const cmtj_junction& j, const cmtj_layer* L) {
     }
     if (y.temperature_set && j.solver != CMTJ_RK4) f |= F_THERMAL;
     for (int s = 0; s < CMTJ_NSLOT; ++s) {
-      if (y.drv[s].type != CMTJ_CONSTANT) f |= F_DYN;
+      if (y.drv[s].type != CMTJ_CONSTANT) f |= (s < CMTJ_SLOT_K) ? F_DYNH : F_DYN;
       if (y.drv[s].type == CMTJ_TABLE) f |= F_TABLE;
     }
   }

@@ -136,6 +136,16 @@ struct ERec {
 
 #define CMTJ_HAS(b) ((FM & (b)) && (fl & (b)))
 
+// Quantities of the binary64 anchor evaluation of one layer at (m0, t),
+// kept in binary32 for the increment stages of the anchored RK4.
+struct Anchor {
+  V3<float> m0, H0, P0, Q0;  // m0, total field, m0 x H0, m0 x (m0 x H0)
+  float u0;                  // anis . m0
+  V3<float> F0, G0;          // STT: m0 x p, m0 x (m0 x p)
+  float D0, eta0, stt0;      // STT: eta denominator, eta, gamma aJ eta
+  V3<float> C0, CC0, C20, CC20;  // SOT: m0 x p, m0 x (m0 x p), secondary
+};
+
 template <class R>
 __device__ __forceinline__ V3<R> tensor3(const R* T, V3<R> m) {
   return mk<R>(T[0] * m.x + T[1] * m.y + T[2] * m.z, T[3] * m.x + T[4] * m.y + T[5] * m.z,
@@ -144,9 +154,10 @@ __device__ __forceinline__ V3<R> tensor3(const R* T, V3<R> m) {
 
 // dm/dt of one layer: effective field (calculateHeffDipoleInjection,
 // core/junction.hpp:590-622) and solveLLG (:716-815).
-template <class R, uint32_t FM, bool LOGE>
+template <class R, uint32_t FM, bool LOGE, bool ANCH = false>
 __device__ __forceinline__ V3<R> rhs(const LP<R>& p, int bits, uint32_t fl, V3<R> m,
-                                     const DV<R>& d, V3<R> bot, V3<R> top, ERec<R>& er) {
+                                     const DV<R>& d, V3<R> bot, V3<R> top, ERec<R>& er,
+                                     Anchor* an = nullptr) {
   V3<R> Nm;
   if (CMTJ_HAS(F_DEMAG_OFF))
     Nm = tensor3(p.N, m);
@@ -154,6 +165,10 @@ __device__ __forceinline__ V3<R> rhs(const LP<R>& p, int bits, uint32_t fl, V3<R
     Nm = mk<R>(p.N[0] * m.x, p.N[4] * m.y, p.N[8] * m.z);
   const V3<R> Hdemag = Nm * p.ms_mu0;
   V3<R> H = d.H;
+  if (ANCH) {
+    an->m0 = cvt<float>(m);
+    an->u0 = (float)dot(p.anis, m);
+  }
   if (CMTJ_HAS(F_ANIS)) {
     const V3<R> Ha = p.anis * ((R(2) * d.K) * dot(p.anis, m) * p.inv_ms);
     H = H + Ha;
@@ -194,6 +209,11 @@ __device__ __forceinline__ V3<R> rhs(const LP<R>& p, int bits, uint32_t fl, V3<R
   const V3<R> prod = cross(m, H);
   const V3<R> prod2 = cross(m, prod);
   const V3<R> dmdt = prod + prod2 * p.alpha;
+  if (ANCH) {
+    an->H0 = cvt<float>(H);
+    an->P0 = cvt<float>(prod);
+    an->Q0 = cvt<float>(prod2);
+  }
   V3<R> ref = mk<R>(0, 0, 0);
   if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT)) {
     const int rt = bits & 3;
@@ -206,14 +226,24 @@ __device__ __forceinline__ V3<R> rhs(const LP<R>& p, int bits, uint32_t fl, V3<R
   }
   if (CMTJ_HAS(F_STT) && ((bits >> 2) & 3) == 1) {
     const R mp = dot(m, ref);
-    R eta;
-    if ((bits >> 5) & 1)
-      eta = p.P / (R(1) + p.lam * mp);
-    else
-      eta = (p.P * p.lam2) / (p.lam2 + R(1) + (p.lam2 - R(1)) * mp);
+    R eta, den;
+    if ((bits >> 5) & 1) {
+      den = R(1) + p.lam * mp;
+      eta = p.P / den;
+    } else {
+      den = p.lam2 + R(1) + (p.lam2 - R(1)) * mp;
+      eta = (p.P * p.lam2) / den;
+    }
     const R stt = p.stt_pref * d.I * eta;
     const V3<R> fieldLike = cross(m, ref);
     const V3<R> dampingLike = cross(m, fieldLike);
+    if (ANCH) {
+      an->D0 = (float)den;
+      an->eta0 = (float)eta;
+      an->stt0 = (float)stt;
+      an->F0 = cvt<float>(fieldLike);
+      an->G0 = cvt<float>(dampingLike);
+    }
     if (LOGE) er.I = d.I;
     return (dmdt * p.neg_gamma + dampingLike * (-stt * p.kappa) + fieldLike * (stt * p.beta)) *
            p.conv;
@@ -235,10 +265,18 @@ __device__ __forceinline__ V3<R> rhs(const LP<R>& p, int bits, uint32_t fl, V3<R
     const V3<R> cm = cross(m, ref);
     const V3<R> ccm = cross(m, cm);
     V3<R> r = dmdt + cm * a1 + ccm * b1;
+    if (ANCH) {
+      an->C0 = cvt<float>(cm);
+      an->CC0 = cvt<float>(ccm);
+    }
     if (CMTJ_HAS(F_SOT2)) {
       const V3<R> cm2 = cross(m, p.ref2);
       const V3<R> ccm2 = cross(m, cm2);
       r = r + cm2 * (Hfl2 - p.alpha * Hdl2) + ccm2 * (Hdl2 + p.alpha * Hfl2);
+      if (ANCH) {
+        an->C20 = cvt<float>(cm2);
+        an->CC20 = cvt<float>(ccm2);
+      }
     }
     if (LOGE) {
       er.Hfl = ref * a1;
@@ -255,6 +293,95 @@ __device__ __forceinline__ V3<R> stoch(const LP<R>& p, V3<R> m, V3<R> dW) {
   const V3<R> c1 = cross(m, dW);
   const V3<R> c2 = cross(m, c1);
   return (c1 + c2 * p.alpha) * p.gconv;
+}
+
+// Increment of dm/dt between the anchor (m0, t) and a stage (m0 + d, t_s):
+// f(m0 + d, t_s) - f(m0, t), written so that binary32 rounding acts on
+// small quantities only (d, field and torque increments), never on the
+// full field or torque.  dd = driver values at t_s minus at t, ds = driver
+// values at t_s.  Same terms as rhs(), expanded: products x y become
+// dx y_s + x0 dy.
+template <uint32_t FM>
+__device__ __forceinline__ V3<float> rhs_delta(const LP<float>& p, int bits, uint32_t fl,
+                                               const Anchor& A, V3<float> d, const DV<float>& dd,
+                                               const DV<float>& ds, V3<float> bot, V3<float> top) {
+  V3<float> Nd;
+  if (CMTJ_HAS(F_DEMAG_OFF))
+    Nd = tensor3(p.N, d);
+  else
+    Nd = mk<float>(p.N[0] * d.x, p.N[4] * d.y, p.N[8] * d.z);
+  V3<float> dH = dd.H - Nd * p.ms_mu0;
+  const float du = dot(p.anis, d);
+  if (CMTJ_HAS(F_ANIS)) dH = dH + p.anis * (2.f * p.inv_ms * (dd.K * A.u0 + ds.K * du));
+  if (CMTJ_HAS(F_ANIS2)) {
+    const float us = A.u0 + du;
+    const float dcube = du * (us * us + us * A.u0 + A.u0 * A.u0);
+    dH = dH + p.anis * (4.f * p.inv_ms * (dd.K2 * (A.u0 * A.u0 * A.u0) + ds.K2 * dcube));
+  }
+  if (CMTJ_HAS(F_IEC) || CMTJ_HAS(F_IEC2)) {
+    float sb = dd.Jb, st = dd.Jt;
+    if (CMTJ_HAS(F_IEC2)) {
+      sb += 2.f * (dd.J2b * dot(bot, A.m0) + ds.J2b * dot(bot, d));
+      st += 2.f * (dd.J2t * dot(top, A.m0) + ds.J2t * dot(top, d));
+    }
+    dH = dH + (bot * sb + top * st) * p.inv_mst;
+  }
+  if (CMTJ_HAS(F_IDMI)) dH = dH - (cross(dd.Db, bot) + cross(dd.Dt, top)) * p.inv_mst;
+  const V3<float> Hs = A.H0 + dH;
+  const V3<float> dP = cross(A.m0, dH) + cross(d, Hs);
+  const V3<float> dQ = cross(A.m0, dP) + cross(d, A.P0 + dP);
+  const V3<float> ddm = dP + dQ * p.alpha;
+  V3<float> ref = mk<float>(0, 0, 0);
+  if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT)) {
+    const int rt = bits & 3;
+    if (rt == CMTJ_REF_FIXED)
+      ref = p.ref;
+    else if (rt == CMTJ_REF_TOP)
+      ref = top;
+    else if (rt == CMTJ_REF_BOTTOM)
+      ref = bot;
+  }
+  if (CMTJ_HAS(F_STT) && ((bits >> 2) & 3) == 1) {
+    const float dw = dot(d, ref);
+    const float c = ((bits >> 5) & 1) ? p.lam : (p.lam2 - 1.f);
+    const float deta = -A.eta0 * (c * dw) / (A.D0 + c * dw);
+    const float dstt = p.stt_pref * (dd.I * A.eta0 + ds.I * deta);
+    const float stts = A.stt0 + dstt;
+    const V3<float> dF = cross(d, ref);
+    const V3<float> dG = cross(A.m0, dF) + cross(d, A.F0 + dF);
+    return (ddm * p.neg_gamma - (dG * stts + A.G0 * dstt) * p.kappa + (dF * stts + A.F0 * dstt) * p.beta) *
+           p.conv;
+  }
+  if (CMTJ_HAS(F_SOT) && ((bits >> 2) & 3) == 2) {
+    float Hdl, Hfl, dHdl, dHfl, Hdl2 = 0.f, Hfl2 = 0.f, dHdl2 = 0.f, dHfl2 = 0.f;
+    if ((bits >> 4) & 1) {
+      Hdl = ds.DL;
+      Hfl = ds.FL;
+      dHdl = dd.DL;
+      dHfl = dd.FL;
+      Hdl2 = ds.DL2;
+      Hfl2 = ds.FL2;
+      dHdl2 = dd.DL2;
+      dHfl2 = dd.FL2;
+    } else {
+      Hdl = p.dl_t * ds.I;
+      Hfl = p.fl_t * ds.I;
+      dHdl = p.dl_t * dd.I;
+      dHfl = p.fl_t * dd.I;
+    }
+    const V3<float> dC = cross(d, ref);
+    const V3<float> dCC = cross(A.m0, dC) + cross(d, A.C0 + dC);
+    V3<float> r = ddm + dC * (Hfl - p.alpha * Hdl) + A.C0 * (dHfl - p.alpha * dHdl) +
+                  dCC * (Hdl + p.alpha * Hfl) + A.CC0 * (dHdl + p.alpha * dHfl);
+    if (CMTJ_HAS(F_SOT2)) {
+      const V3<float> dC2 = cross(d, p.ref2);
+      const V3<float> dCC2 = cross(A.m0, dC2) + cross(d, A.C20 + dC2);
+      r = r + dC2 * (Hfl2 - p.alpha * Hdl2) + A.C20 * (dHfl2 - p.alpha * dHdl2) +
+          dCC2 * (Hdl2 + p.alpha * Hfl2) + A.CC20 * (dHdl2 + p.alpha * dHfl2);
+    }
+    return r * p.gconv;
+  }
+  return ddm * p.gconv;
 }
 
 template <class R>
@@ -295,45 +422,207 @@ __device__ __forceinline__ void load_layer(const cmtj_layer& L, const cmtj_const
   m = mk<double>(L.mag[0], L.mag[1], L.mag[2]);
 }
 
+template <class A, class B>
+__device__ __forceinline__ DV<A> dv_cvt(const DV<B>& s) {
+  DV<A> d;
+  d.H = cvt<A>(s.H);
+  d.Hext = cvt<A>(s.Hext);
+  d.Dt = cvt<A>(s.Dt);
+  d.Db = cvt<A>(s.Db);
+  d.K = (A)s.K;
+  d.K2 = (A)s.K2;
+  d.Jt = (A)s.Jt;
+  d.Jb = (A)s.Jb;
+  d.J2t = (A)s.J2t;
+  d.J2b = (A)s.J2b;
+  d.I = (A)s.I;
+  d.FL = (A)s.FL;
+  d.DL = (A)s.DL;
+  d.FL2 = (A)s.FL2;
+  d.DL2 = (A)s.DL2;
+  d.T = (A)s.T;
+  return d;
+}
+
+// (a - b) in binary64, rounded once to binary32
+__device__ __forceinline__ DV<float> dv_diff(const DV<double>& a, const DV<double>& b) {
+  DV<float> d;
+  d.H = cvt<float>(a.H - b.H);
+  d.Hext = cvt<float>(a.Hext - b.Hext);
+  d.Dt = cvt<float>(a.Dt - b.Dt);
+  d.Db = cvt<float>(a.Db - b.Db);
+  d.K = (float)(a.K - b.K);
+  d.K2 = (float)(a.K2 - b.K2);
+  d.Jt = (float)(a.Jt - b.Jt);
+  d.Jb = (float)(a.Jb - b.Jb);
+  d.J2t = (float)(a.J2t - b.J2t);
+  d.J2b = (float)(a.J2b - b.J2b);
+  d.I = (float)(a.I - b.I);
+  d.FL = (float)(a.FL - b.FL);
+  d.DL = (float)(a.DL - b.DL);
+  d.FL2 = (float)(a.FL2 - b.FL2);
+  d.DL2 = (float)(a.DL2 - b.DL2);
+  d.T = (float)(a.T - b.T);
+  return d;
+}
+
 // Numbers of the first DREG records stay in registers for the whole run.
 constexpr int DREG = 2;
 
-// Adds the time-dependent driver values at time t to dv.  The kind, slot and
-// layer mask of every record are wave-uniform (kernel arguments), so the
-// switches below are scalar branches.
-template <class R, int LM, uint32_t FM, bool LOGE>
-__device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base)[LM],
-                                             const DynRec (&dr)[DREG], const GroupArgs& g,
-                                             uint32_t lane, uint32_t fl, double t, uint32_t step,
-                                             int stage, int sps) {
+// Field weights of the first DREG records: w[k][l] is the unit vector of
+// the axis that record k feeds in layer l (zero if it feeds none).  Built
+// once per lane from the wave-uniform record metadata.
+template <class R, int LM>
+struct FieldWeights {
+  V3<R> w[DREG][LM];
+};
+
+template <class R, int LM>
+__device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const GroupArgs& g) {
 #pragma unroll
-  for (int l = 0; l < LM; ++l) dv[l] = base[l];
-  if (!CMTJ_HAS(F_DYN)) return;
-  for (int k = 0; k < g.ndyn; ++k) {
-    const int meta = g.dmeta[k];
+  for (int k = 0; k < DREG; ++k) {
+    const int meta = (k < g.ndyn) ? g.dmeta[k] : 0;
     const int type = meta & 0xff;
     const int slot = (meta >> 8) & 0xff;
     const int lmask = (meta >> 16) & 0xff;
-    DynRec r;
-    if (k < DREG) {
+    const bool isH = type != CMTJ_CONSTANT && slot < CMTJ_SLOT_K;
+    const int axis = slot % 3;
 #pragma unroll
-      for (int q = 0; q < DREG; ++q)
-        if (q == k) r = dr[q];
-    } else {
-      r = g.dyn[(size_t)k * g.n + lane];
+    for (int l = 0; l < LM; ++l) {
+      const bool on = isH && ((lmask >> l) & 1);
+      fw.w[k][l] = mk<R>((on && axis == 0) ? R(1) : R(0), (on && axis == 1) ? R(1) : R(0),
+                         (on && axis == 2) ? R(1) : R(0));
     }
-    R v;
-    if (CMTJ_HAS(F_TABLE) && type == CMTJ_TABLE) {
-      v = (R)g.tables[(size_t)r.tidx + (size_t)step * sps + stage];
-    } else if (sizeof(R) == 4) {
-      v = (R)wave_part_f(type, r.A, r.p1, r.p2, r.p3, t);
-    } else {
-      v = (R)wave_part(type, r.A, r.p1, r.p2, r.p3, t);
-    }
-#pragma unroll
-    for (int l = 0; l < LM; ++l)
-      if ((lmask >> l) & 1) add_slot<R, LOGE>(dv[l], slot, v);
   }
+}
+
+__device__ __forceinline__ DynRec pick_rec(const DynRec (&dr)[DREG], const GroupArgs& g, int k, uint32_t lane) {
+  DynRec r;
+  if (k < DREG) {
+#pragma unroll
+    for (int q = 0; q < DREG; ++q)
+      if (q == k) r = dr[q];
+  } else {
+    r = g.dyn[(size_t)k * g.n + lane];
+  }
+  return r;
+}
+
+template <class R>
+__device__ __forceinline__ R wave_value(int type, const DynRec& r, const GroupArgs& g, double t, uint32_t step,
+                                        int stage, int sps) {
+  if (type == CMTJ_TABLE) return (R)g.tables[(size_t)r.tidx + (size_t)step * sps + stage];
+  if (sizeof(R) == 4) return (R)wave_part_f(type, r.A, r.p1, r.p2, r.p3, t);
+  return (R)wave_part(type, r.A, r.p1, r.p2, r.p3, t);
+}
+
+// Driver values at time t: the per-layer base (constant parts) plus every
+// time-dependent record.  Kinds, slots and layer masks are wave-uniform.
+// Records that feed only field slots take the fast path (one FMA per axis
+// with a uniform weight); other slots go through add_slot.
+template <class R, int LM, uint32_t FM, bool LOGE>
+__device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base)[LM],
+                                             const DynRec (&dr)[DREG], const FieldWeights<R, LM>& fw,
+                                             const GroupArgs& g, uint32_t lane, uint32_t fl, double t,
+                                             uint32_t step, int stage, int sps) {
+#pragma unroll
+  for (int l = 0; l < LM; ++l) dv[l] = base[l];
+  if (CMTJ_HAS(F_DYN) || (LOGE && CMTJ_HAS(F_DYNH))) {
+    for (int k = 0; k < g.ndyn; ++k) {
+      const int meta = g.dmeta[k];
+      const int type = meta & 0xff;
+      const int slot = (meta >> 8) & 0xff;
+      const int lmask = (meta >> 16) & 0xff;
+      const DynRec r = pick_rec(dr, g, k, lane);
+      const R v = wave_value<R>(type, r, g, t, step, stage, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l)
+        if ((lmask >> l) & 1) add_slot<R, LOGE>(dv[l], slot, v);
+    }
+  } else if (CMTJ_HAS(F_DYNH)) {
+#pragma unroll
+    for (int k = 0; k < DREG; ++k) {
+      if (k < g.ndyn) {
+        const int type = g.dmeta[k] & 0xff;
+        const R v = wave_value<R>(type, dr[k], g, t, step, stage, sps);
+#pragma unroll
+        for (int l = 0; l < LM; ++l) dv[l].H = dv[l].H + fw.w[k][l] * v;
+      }
+    }
+    for (int k = DREG; k < g.ndyn; ++k) {
+      const int meta = g.dmeta[k];
+      const DynRec r = pick_rec(dr, g, k, lane);
+      const R v = wave_value<R>(meta & 0xff, r, g, t, step, stage, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l)
+        if (((meta >> 16) >> l) & 1) add_slot<R, false>(dv[l], (meta >> 8) & 0xff, v);
+    }
+  }
+}
+
+// Driver increments dd = dv(ts) - dv(t0) of the time-dependent records
+// (binary64 difference, one rounding to binary32); constant parts cancel.
+template <int LM, uint32_t FM>
+__device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const DynRec (&dr)[DREG],
+                                                   const FieldWeights<float, LM>& fw, const GroupArgs& g,
+                                                   uint32_t lane, uint32_t fl, double t0, double ts,
+                                                   uint32_t step, int stage, int sps) {
+#pragma unroll
+  for (int l = 0; l < LM; ++l) dv_zero(dd[l]);
+  if (CMTJ_HAS(F_DYN)) {
+    for (int k = 0; k < g.ndyn; ++k) {
+      const int meta = g.dmeta[k];
+      const int type = meta & 0xff;
+      const DynRec r = pick_rec(dr, g, k, lane);
+      const double v = wave_value<double>(type, r, g, ts, step, stage, sps) -
+                       wave_value<double>(type, r, g, t0, step, 0, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l)
+        if (((meta >> 16) >> l) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
+    }
+  } else if (CMTJ_HAS(F_DYNH)) {
+#pragma unroll
+    for (int k = 0; k < DREG; ++k) {
+      if (k < g.ndyn) {
+        const int type = g.dmeta[k] & 0xff;
+        const float v = (float)(wave_value<double>(type, dr[k], g, ts, step, stage, sps) -
+                                wave_value<double>(type, dr[k], g, t0, step, 0, sps));
+#pragma unroll
+        for (int l = 0; l < LM; ++l) dd[l].H = dd[l].H + fw.w[k][l] * v;
+      }
+    }
+    for (int k = DREG; k < g.ndyn; ++k) {
+      const int meta = g.dmeta[k];
+      const DynRec r = pick_rec(dr, g, k, lane);
+      const double v = wave_value<double>(meta & 0xff, r, g, ts, step, stage, sps) -
+                       wave_value<double>(meta & 0xff, r, g, t0, step, 0, sps);
+#pragma unroll
+      for (int l = 0; l < LM; ++l)
+        if (((meta >> 16) >> l) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
+    }
+  }
+}
+
+template <class R>
+__device__ __forceinline__ DV<R> dv_add(const DV<R>& a, const DV<R>& b) {
+  DV<R> d;
+  d.H = a.H + b.H;
+  d.Hext = a.Hext + b.Hext;
+  d.Dt = a.Dt + b.Dt;
+  d.Db = a.Db + b.Db;
+  d.K = a.K + b.K;
+  d.K2 = a.K2 + b.K2;
+  d.Jt = a.Jt + b.Jt;
+  d.Jb = a.Jb + b.Jb;
+  d.J2t = a.J2t + b.J2t;
+  d.J2b = a.J2b + b.J2b;
+  d.I = a.I + b.I;
+  d.FL = a.FL + b.FL;
+  d.DL = a.DL + b.DL;
+  d.FL2 = a.FL2 + b.FL2;
+  d.DL2 = a.DL2 + b.DL2;
+  d.T = a.T + b.T;
+  return d;
 }
 
 // CVector::normalize (core/cvector.hpp:185-192): divide by the length.
@@ -364,28 +653,34 @@ __device__ __forceinline__ V3<double> normalize_m(V3<double> a) {
   return normalize_fast(a);
 }
 
-__device__ __forceinline__ void log_put(const GroupArgs& g, size_t idx, double v) {
-  if (g.log_f32)
-    ((float*)g.log)[idx] = (float)v;
+// One log value at byte address p (binary32 or binary64 log).
+__device__ __forceinline__ void log_put(bool f32, char* p, double v) {
+  if (f32)
+    *(float*)p = (float)v;
   else
-    ((double*)g.log)[idx] = v;
+    *(double*)p = v;
 }
 
-// logLayerParams (core/junction.hpp:1580-1636) for entry k.
+// logLayerParams (core/junction.hpp:1580-1636) for entry k.  p points at
+// channel 0 of entry k; consecutive channels are cs bytes apart.
 template <class R, int LM, uint32_t FM, bool LOGE>
 __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junction* J,
-                                            const V3<double> (&m)[LM], const ERec<R> (&er)[LM],
-                                            uint32_t k, uint64_t base, uint32_t NE, double Rp,
-                                            double Rap) {
+                                            const V3<double> (&m)[LM], const ERec<R> (&er)[LM], char* p,
+                                            uint64_t cs, double Rp, double Rap) {
   const int nl = g.nl, lmask = g.log_mask, mr = g.mr_mode;
-  uint32_t ch = 0;
+  const bool f32 = g.log_f32;
 #pragma unroll
   for (int l = 0; l < LM; ++l) {
     if (l < nl) {
-      if (lmask & CMTJ_LOG_MX) log_put(g, base + (size_t)(ch++) * NE + k, m[l].x);
+      if (lmask & CMTJ_LOG_MX) {
+        log_put(f32, p, m[l].x);
+        p += cs;
+      }
       if (lmask & CMTJ_LOG_MYZ) {
-        log_put(g, base + (size_t)(ch++) * NE + k, m[l].y);
-        log_put(g, base + (size_t)(ch++) * NE + k, m[l].z);
+        log_put(f32, p, m[l].y);
+        p += cs;
+        log_put(f32, p, m[l].z);
+        p += cs;
       }
     }
   }
@@ -401,7 +696,8 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
         const V3<double>& m1 = m[LM > 1 ? 1 : 0];
         cosT = m[0].x * m1.x + m[0].y * m1.y + m[0].z * m1.z;
       }
-      log_put(g, base + (size_t)(ch++) * NE + k, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
+      log_put(f32, p, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
+      p += cs;
     } else if (mr == CMTJ_MR_STRIP) {
       // stripMagnetoResistance (:1835-1857), parallel sum of the layers
       double rx = 0.0, ry = 0.0;
@@ -415,8 +711,10 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
           ry += 1. / Ry;
         }
       }
-      log_put(g, base + (size_t)(ch++) * NE + k, 1 / rx);
-      log_put(g, base + (size_t)(ch++) * NE + k, 1 / ry);
+      log_put(f32, p, 1 / rx);
+      p += cs;
+      log_put(f32, p, 1 / ry);
+      p += cs;
     }
   }
   if (LOGE && (lmask & CMTJ_LOG_ENERGY)) {
@@ -430,7 +728,10 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
             e.Hdemag.x, e.Hdemag.y, e.Hdemag.z, e.Hfl.x, e.Hfl.y, e.Hfl.z,
             e.Hdl.x, e.Hdl.y, e.Hdl.z};
 #pragma unroll
-        for (int q = 0; q < CMTJ_ENERGY_CHANNELS; ++q) log_put(g, base + (size_t)(ch++) * NE + k, v[q]);
+        for (int q = 0; q < CMTJ_ENERGY_CHANNELS; ++q) {
+          log_put(f32, p, v[q]);
+          p += cs;
+        }
       }
     }
   }
@@ -451,20 +752,31 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
   const uint64_t seed = J->seed, stream = J->stream;
   const uint64_t lbase = J->log_offset;
   const bool logging = (g.log != nullptr) && g.log_mask != 0;
+  const uint32_t esize = g.log_f32 ? 4u : 8u;
+  char* const lp = logging ? (char*)g.log + lbase * esize : nullptr;
+  const uint64_t cs = (uint64_t)NE * esize;
   const double Rp = J->Rp, Rap = J->Rap;
   const int sps = (SOLVER == CMTJ_RK4) ? 3 : (SOLVER == CMTJ_HEUN ? 2 : 1);
 
+  constexpr bool ANCHORED = (sizeof(R) == 4) && (SOLVER == CMTJ_RK4);
   LP<R> P[LM];
   DV<R> base[LM];
+  LP<double> Pd[ANCHORED ? LM : 1];
+  DV<double> based[ANCHORED ? LM : 1];
   V3<double> m[LM];
 #pragma unroll
   for (int l = 0; l < LM; ++l) {
     if (l < nl) {
       load_layer<R>(g.layers[J->layer_offset + l], g.c, P[l], base[l], m[l]);
+      if constexpr (ANCHORED) load_layer<double>(g.layers[J->layer_offset + l], g.c, Pd[l], based[l], m[l]);
     } else {
       dv_zero(base[l]);
       m[l] = mk<double>(0, 0, 0);
       P[l] = P[0];
+      if constexpr (ANCHORED) {
+        dv_zero(based[l]);
+        Pd[l] = Pd[0];
+      }
     }
   }
   DynRec dr[DREG];
@@ -476,6 +788,12 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
   }
 
+  FieldWeights<R, LM> fw;
+  field_weights<R, LM>(fw, g);
+  FieldWeights<float, LM> fwf;
+  if constexpr (ANCHORED) field_weights<float, LM>(fwf, g);
+  FieldWeights<double, LM> fwd;
+  if constexpr (ANCHORED) field_weights<double, LM>(fwd, g);
   ERec<R> er[LM];
   DV<R> dv[LM];
   const R dtR = (R)dt;
@@ -488,7 +806,72 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
   uint32_t i = 0;
   for (; i < nsteps; ++i) {
     const double t = (double)i * dt;
-    if (SOLVER == CMTJ_RK4) {
+    if constexpr (ANCHORED) {
+      // RK4 of the reference (core/junction.hpp:1017-1041, neighbours frozen
+      // at the start of the step) in anchored binary32: k1 and the anchor
+      // quantities in binary64, stages 2-4 as binary32 increments against
+      // the anchor (rhs_delta), combination m + k1 + (2dk2 + 2dk3 + dk4)/6.
+      V3<double> nbBd[LM], nbTd[LM];
+      V3<float> nbB[LM], nbT[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        nbBd[l] = (l > 0) ? m[l > 0 ? l - 1 : 0] : mk<double>(0, 0, 0);
+        nbTd[l] = (l + 1 < nl) ? m[l + 1 < LM ? l + 1 : l] : mk<double>(0, 0, 0);
+        nbB[l] = cvt<float>(nbBd[l]);
+        nbT[l] = cvt<float>(nbTd[l]);
+      }
+      DV<double> dv0[LM];
+      eval_drivers<double, LM, FM, false>(dv0, based, dr, fwd, g, lane, fl, t, i, 0, sps);
+      V3<double> k1[LM];
+      Anchor A[LM];
+      ERec<double> erd;
+      DV<float> ds0[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          k1[l] = rhs<double, FM, false, true>(Pd[l], g.lbits[l], fl, m[l], dv0[l], nbBd[l], nbTd[l], erd, &A[l]) * dt;
+          ds0[l] = dv_cvt<float>(dv0[l]);
+        }
+      }
+      DV<float> dd[LM];
+      eval_driver_deltas<LM, FM>(dd, dr, fwf, g, lane, fl, t, t + 0.5 * dt, i, 1, sps);
+      V3<float> acc[LM], dk[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const DV<float> ds = dv_add(ds0[l], dd[l]);
+          const V3<float> d2 = cvt<float>(k1[l] * 0.5);
+          dk[l] = rhs_delta<FM>(P[l], g.lbits[l], fl, A[l], d2, dd[l], ds, nbB[l], nbT[l]) * dtR;
+          acc[l] = dk[l] * 2.f;
+          const V3<float> d3 = d2 + dk[l] * 0.5f;
+          dk[l] = rhs_delta<FM>(P[l], g.lbits[l], fl, A[l], d3, dd[l], ds, nbB[l], nbT[l]) * dtR;
+          acc[l] = acc[l] + dk[l] * 2.f;
+        }
+      }
+      eval_driver_deltas<LM, FM>(dd, dr, fwf, g, lane, fl, t, t + dt, i, 2, sps);
+      V3<double> mn[LM];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl) {
+          const DV<float> ds = dv_add(ds0[l], dd[l]);
+          const V3<float> d4 = cvt<float>(k1[l]) + dk[l];
+          const V3<float> dk4 = rhs_delta<FM>(P[l], g.lbits[l], fl, A[l], d4, dd[l], ds, nbB[l], nbT[l]) * dtR;
+          acc[l] = acc[l] + dk4;
+          mn[l] = normalize_m<R>(m[l] + k1[l] + cvt<double>(acc[l] * sixth_inv));
+        }
+      }
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (l < nl && nan_layer < 0) {
+          m[l] = mn[l];
+          if (isnan(mn[l].x)) nan_layer = l;
+        }
+      }
+      if (nan_layer >= 0) {
+        code = CMTJ_ERR_NAN;
+        break;
+      }
+    } else if (SOLVER == CMTJ_RK4) {
       // rk4_step (core/junction.hpp:1017-1041) for every layer, neighbours
       // frozen at the start of the step (runMultiLayerSolver, :1682-1707).
       // Stages run outermost so the driver values of one time serve all
@@ -501,7 +884,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
         nbB[l] = (l > 0) ? ms[l > 0 ? l - 1 : 0] : zero;
         nbT[l] = (l + 1 < nl) ? ms[l + 1 < LM ? l + 1 : l] : zero;
       }
-      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t, i, 0, sps);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t, i, 0, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -510,7 +893,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
           ms[l] = cvt<R>(m[l]) + k1 * R(0.5);
         }
       }
-      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t + 0.5 * dt, i, 1, sps);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t + 0.5 * dt, i, 1, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -527,7 +910,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
           ms[l] = cvt<R>(m[l]) + k3;
         }
       }
-      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t + dt, i, 2, sps);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t + dt, i, 2, sps);
       V3<double> mn[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
@@ -559,7 +942,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
       V3<R> mr_[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) mr_[l] = cvt<R>(m[l]);
-      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t, i, 0, sps);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t, i, 0, sps);
       V3<double> mn[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
@@ -587,7 +970,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
       V3<R> mr_[LM], fn[LM], gn[LM], dW[LM], mN[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) mr_[l] = cvt<R>(m[l]);
-      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t, i, 0, sps);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t, i, 0, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -604,7 +987,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
           mN[l] = mr_[l] + fn[l] * dtR + gn[l] * sdt;
         }
       }
-      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, g, lane, fl, t + dt, i, 1, sps);
+      eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t + dt, i, 1, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -621,7 +1004,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
       }
     }
     if ((uint64_t)i == next_log) {
-      if (logging) write_entry<R, LM, FM, LOGE>(g, J, m, er, k, lbase, NE, Rp, Rap);
+      if (logging) write_entry<R, LM, FM, LOGE>(g, J, m, er, lp + (uint64_t)k * esize, cs, Rp, Rap);
       ++k;
       next_log += W;
     }

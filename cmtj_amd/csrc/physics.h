@@ -33,9 +33,10 @@ enum Feature : uint32_t {
   F_SOT = 1u << 8,        // spin-orbit torque
   F_SOT2 = 1u << 9,       // secondary SOT reference
   F_THERMAL = 1u << 10,   // stochastic thermal field
-  F_DYN = 1u << 11,       // time-dependent drivers present
+  F_DYN = 1u << 11,       // time-dependent drivers on non-field slots
   F_TABLE = 1u << 12,     // tabulated (callback) drivers present
-  F_ALL = 0x1FFFu
+  F_DYNH = 1u << 13,      // time-dependent drivers that feed field slots only
+  F_ALL = 0x3FFFu
 };
 
 template <class R>

@@ -20,9 +20,9 @@ struct KTable {
   size_t n;
 };
 
-constexpr uint32_t FM_LEAN = F_ANIS | F_IEC | F_DYN;
-constexpr uint32_t FM_STT = F_ANIS | F_IEC | F_DYN | F_STT;
-constexpr uint32_t FM_SOTT = F_ANIS | F_IEC | F_DYN | F_SOT | F_THERMAL;
+constexpr uint32_t FM_LEAN = F_ANIS | F_IEC | F_DYNH;
+constexpr uint32_t FM_STT = F_ANIS | F_IEC | F_DYNH | F_STT;
+constexpr uint32_t FM_SOTT = F_ANIS | F_IEC | F_DYN | F_DYNH | F_SOT | F_THERMAL;
 
 // one table per instantiation unit
 KTable kernels_f64();
