@@ -32,6 +32,35 @@ def _pimm(precision, n, log_dtype=None):
     return workloads.pimm(n_points=n, seed=0, precision=precision, log="mx", log_dtype=log_dtype)
 
 
+COPY_BYTES = 1 << 30
+
+
+def _pmc_pass(b) -> int:
+    """Two sweep launches (the first loads the code object) and a plain
+    device copy of COPY_BYTES, whose FETCH_SIZE / WRITE_SIZE calibrate the
+    counters against a known byte count."""
+    import torch
+
+    b.execute()
+    b.execute()
+    x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    return 0
+
+
+def _traffic():
+    """HBM bytes per sweep launch from the committed counter summary (two
+    rocprofv3 --pmc runs of `bench.py --pmc-pass`, FETCH_SIZE then
+    WRITE_SIZE, each scaled by the copy kernel's reading), or None."""
+    f = ROOT / "profiles" / "pimm_fp32_traffic.json"
+    if not f.exists():
+        return None
+    return json.loads(f.read_text())
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,7 +69,10 @@ def main() -> int:
     ap.add_argument("--points", type=int, default=4096)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--lanes", default="auto", choices=["auto", "one", "per_layer"])
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="run the sweep kernel twice and a 1 GiB device copy, print nothing (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -57,8 +89,10 @@ def main() -> int:
     torch.cuda.set_device(dev)
 
     # ---- FP32 batched, inputs resident in HBM -------------------------------
-    b = _pimm("fp32", args.points)
+    b = _pimm("fp32", args.points).set_lanes(args.lanes)
     plan = b.plan()
+    if args.pmc_pass:
+        return _pmc_pass(b)
     info = plan.info
     stream = torch.cuda.current_stream()
     print(f"[bench] pimm: {len(b.js)} instances, {b.total_steps:.3e} junction-steps/pass, "
@@ -106,7 +140,7 @@ def main() -> int:
         nch = 3
         ent = torch.as_tensor(b.js["n_entries"].astype(np.int64), device=dev)
         reps = ent.repeat_interleave(nch)
-        chan = torch.arange(nch, device=dev).repeat(len(b.js)).repeat_interleave(reps)
+        chan = torch.arange(nch, device=dev, dtype=torch.int8).repeat(len(b.js)).repeat_interleave(reps)
         m_sel = chan < 2  # m_x of the two layers
         d = (r32.log.double() - r64.log)[m_sel].abs().max().item()
         dfin = (r32.final_m - r64.final_m).abs().max().item()
@@ -139,6 +173,7 @@ def main() -> int:
         print(f"[bench] cpu oracle: {cpu['value']:.4e} junction-steps/s on {ncores} threads", file=sys.stderr,
               flush=True)
 
+    tr = _traffic()
     out = {
         "metric": "junction-steps/s FP32 batched vs FP64 plain at max|dm|<=1e-4; p50/p99 latency",
         "value": value,
@@ -150,7 +185,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32 (anchored RK4: stages 2-4 in fp32; k1, state, clock in fp64)",
         "data": "synthetic",
         "config": {
             "workload": "pimm_2layer_mtj_sweep",
@@ -161,7 +196,8 @@ def main() -> int:
             "durations_ns": [2, 20],
             "junction_steps_per_pass": b.total_steps,
             "log": "m_x (2 layers) + R every step, float32",
-            "parallelism": "1 junction per lane",
+            "lanes_per_instance": int(info.lanes_per_instance),
+            "kernel": info.kernel_name.decode(),
         },
         "fp64_plain_junction_steps_per_s": plain_rate,
         "fp32_batched_over_fp64_plain": value / plain_rate,
@@ -169,8 +205,9 @@ def main() -> int:
         "latency_ms": {"p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99)),
                        "what": "one full sweep pass (kernel time on the launch stream)"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
-                     "flop_per_junction_step": FLOP_PER_STEP, "kernel_ms": kern_ms},
+                     "frac": achieved / PEAK_FP32, "traffic": None if tr is None else tr["bytes_per_launch"],
+                     "flop_per_junction_step": FLOP_PER_STEP, "kernel_ms": kern_ms,
+                     "traffic_algorithmic": 12.0 * b.total_steps},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -175,6 +175,8 @@ EXPORTS = (
     "cmtj_driver_eval",
     "cmtj_run",
     "cmtj_plan_create",
+    "cmtj_plan_arena_size",
+    "cmtj_plan_create_in",
     "cmtj_plan_execute",
     "cmtj_plan_query",
     "cmtj_plan_destroy",
@@ -205,6 +207,15 @@ def _load():
         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
     ]
     lib.cmtj_plan_create.restype = _P
+    lib.cmtj_plan_arena_size.argtypes = [
+        _P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+        ctypes.POINTER(ctypes.c_size_t),
+    ]
+    lib.cmtj_plan_create_in.argtypes = [
+        _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t,
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_size_t, _P,
+    ]
+    lib.cmtj_plan_create_in.restype = _P
     lib.cmtj_plan_execute.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P]
     lib.cmtj_plan_query.argtypes = [_P, ctypes.POINTER(PlanInfo)]
     lib.cmtj_plan_destroy.argtypes = [_P]
@@ -302,17 +313,37 @@ def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: 
     return log, mag, st
 
 
-class Plan:
-    """Owner of a cmtj_plan (device-resident descriptors of a batch)."""
+def plan_arena_size(js, layers, n_table: int, precision: int, log_dtype: int, lanes_per_instance: int = 0) -> int:
+    js = np.ascontiguousarray(js, dtype=JUNCTION_DTYPE)
+    layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
+    out = ctypes.c_size_t(0)
+    check(lib().cmtj_plan_arena_size(ptr(js), js.size, ptr(layers), layers.size, n_table, precision, log_dtype,
+                                     lanes_per_instance, ctypes.byref(out)), "cmtj_plan_arena_size")
+    return int(out.value)
 
-    def __init__(self, js, layers, consts, tables, precision: int, log_dtype: int, lanes_per_instance: int = 0):
+
+class Plan:
+    """Owner of a cmtj_plan (device-resident descriptors of a batch).  With
+    `arena` = (device pointer, bytes, stream) the descriptors go into that
+    caller-owned memory, which must outlive the plan's executions."""
+
+    def __init__(self, js, layers, consts, tables, precision: int, log_dtype: int, lanes_per_instance: int = 0,
+                 arena=None):
         self._js = np.ascontiguousarray(js, dtype=JUNCTION_DTYPE)
         self._layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
         tabs = None if tables is None or len(tables) == 0 else np.ascontiguousarray(tables, dtype=np.float64)
-        self._h = lib().cmtj_plan_create(
-            ptr(self._js), self._js.size, ptr(self._layers), self._layers.size, ptr(consts),
-            ptr(tabs), 0 if tabs is None else tabs.size, precision, log_dtype, lanes_per_instance,
-        )
+        if arena is None:
+            self._h = lib().cmtj_plan_create(
+                ptr(self._js), self._js.size, ptr(self._layers), self._layers.size, ptr(consts),
+                ptr(tabs), 0 if tabs is None else tabs.size, precision, log_dtype, lanes_per_instance,
+            )
+        else:
+            d_arena, nbytes, stream = arena
+            self._h = lib().cmtj_plan_create_in(
+                ptr(self._js), self._js.size, ptr(self._layers), self._layers.size, ptr(consts),
+                ptr(tabs), 0 if tabs is None else tabs.size, precision, log_dtype, lanes_per_instance,
+                d_arena, nbytes, stream,
+            )
         if not self._h:
             raise RuntimeError(f"cmtj_plan_create: {last_error()}")
         self.info = PlanInfo()

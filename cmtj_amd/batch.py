@@ -99,6 +99,9 @@ class BatchResult:
         return self.log[: n * nch * int(ne[0])].view(n, nch, int(ne[0]))[:, c, :]
 
 
+_LANES = {"auto": 0, "one": 1, "per_layer": 2}
+
+
 class JunctionBatch:
     """A packed batch of junction instances (see module docstring)."""
 
@@ -113,6 +116,7 @@ class JunctionBatch:
             log_dtype = "f32" if self.precision == _lib.FP32 else "f64"
         self.log_dtype = _lib.LOG_F32 if log_dtype in ("f32", _lib.LOG_F32) else _lib.LOG_F64
         self.set_log(log)
+        self.lanes = "auto"
         self._plan = None
         self._bufs = None
 
@@ -277,11 +281,23 @@ class JunctionBatch:
         self.log_elements = int(sizes.sum())
         self.total_steps = int(self.js["n_steps"].astype(np.uint64).sum())
 
+    def set_lanes(self, lanes: str) -> "JunctionBatch":
+        """Lane mapping of multilayer instances: "one" (all layers of an
+        instance in one GPU lane), "per_layer" (one lane per layer: lowest
+        latency per instance) or "auto" (per layer while the batch is too
+        small to fill the GPU)."""
+        if lanes not in _LANES:
+            raise ValueError(f"lanes must be one of {sorted(_LANES)}")
+        self.lanes = lanes
+        self._plan = None
+        return self
+
     # --- execution --------------------------------------------------------------
     def plan(self) -> _lib.Plan:
         if self._plan is None:
             consts = _lib.constants_get()
-            self._plan = _lib.Plan(self.js, self.layers, consts, self.tables, self.precision, self.log_dtype)
+            self._plan = _lib.Plan(self.js, self.layers, consts, self.tables, self.precision, self.log_dtype,
+                                   _LANES[self.lanes])
             self._bufs = None
         return self._plan
 

@@ -45,19 +45,30 @@ static std::mutex g_consts_mu;
 
 #include "registry.h"
 
-static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, bool loge) {
-  static const KTable kTables[] = {kernels_f64(), kernels_f64e(), kernels_f32(), kernels_spec()};
-  const KEntry* best = nullptr;
+// Waves up to which the one-layer-per-lane kernels are preferred: below
+// this a batch does not fill the chip, and the sweep time is the step count
+// of its longest instance times the latency of one wave step, which the
+// per-layer split divides; above it, the all-layers-in-one-lane kernels do
+// less redundant work per instance.
+constexpr uint64_t kAutoLaneWaves = 2048;
+
+static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, bool loge, int lanes,
+                                 uint64_t n_inst) {
+  static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),    kernels_spec(),
+                                   kernels_ll_f64(), kernels_ll_f32(), kernels_ll_spec()};
+  const KEntry* best[2] = {nullptr, nullptr};  // [0] one lane per instance, [1] one lane per layer
   for (const KTable& t : kTables)
-  for (size_t q = 0; q < t.n; ++q) {
-    const KEntry& k = t.e[q];
-    if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
-    if ((k.fm & flags) != flags) continue;
-    if (!best || k.lm < best->lm ||
-        (k.lm == best->lm && __builtin_popcount(k.fm) < __builtin_popcount(best->fm)))
-      best = &k;
-  }
-  return best;
+    for (size_t q = 0; q < t.n; ++q) {
+      const KEntry& k = t.e[q];
+      if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
+      if ((k.fm & flags) != flags) continue;
+      const KEntry*& b = best[k.lpi > 1];
+      if (!b || k.lm < b->lm || (k.lm == b->lm && __builtin_popcount(k.fm) < __builtin_popcount(b->fm))) b = &k;
+    }
+  if (nl < 2 || lanes == CMTJ_LANES_ONE || !best[1]) return best[0];
+  if (lanes == CMTJ_LANES_PER_LAYER) return best[1];
+  const uint64_t waves = (n_inst * (uint64_t)best[1]->lpi + 63) / 64;
+  return waves <= kAutoLaneWaves ? best[1] : best[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -214,7 +225,8 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 // branch on structure inside the kernel is wave-uniform.
 static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmtj_layer* layers,
                       size_t rows, const cmtj_constants* c, const double* tables, size_t n_table,
-                      int prec, int log_dtype, ArenaFn get, void* gctx, hipStream_t stream) {
+                      int prec, int log_dtype, int lanes, ArenaFn get, void* gctx, hipStream_t stream,
+                      size_t* size_only = nullptr) {
   if (n == 0) return fail(CMTJ_ERR_INVALID, "empty batch");
   if (n > 0xFFFFFFFFull) return fail(CMTJ_ERR_INVALID, "too many instances");
   if (prec != CMTJ_FP64 && prec != CMTJ_FP32) return fail(CMTJ_ERR_INVALID, "unknown precision");
@@ -291,6 +303,10 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     o_dyn[g] = off;
     off = align_up(off + nums[g].size() * sizeof(DynRec));
   }
+  if (size_only) {
+    *size_only = off;
+    return 0;
+  }
   char* arena = (char*)get(off, gctx);
   if (!arena) return CMTJ_ERR_HIP;
   P->arena = arena;
@@ -300,6 +316,7 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     HIP_TRY(hipMemcpyAsync(arena + o_tables, tables, n_table * sizeof(double), hipMemcpyHostToDevice, stream));
   P->groups.clear();
   std::string names;
+  int max_lpi = 1;
   for (size_t g = 0; g < G; ++g) {
     HIP_TRY(hipMemcpyAsync(arena + o_order[g], orders[g].data(), orders[g].size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice, stream));
@@ -308,7 +325,7 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     const cmtj_junction& j0 = js[orders[g][0]];
     const cmtj_layer* L0 = layers + j0.layer_offset;
     const bool loge = (j0.log_mask & CMTJ_LOG_ENERGY) != 0;
-    const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], loge);
+    const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], loge, lanes, orders[g].size());
     if (!k) return fail(CMTJ_ERR_INVALID, "no kernel for this junction configuration");
     Group Gr;
     Gr.k = k;
@@ -329,7 +346,8 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     for (int l = 0; l < j0.n_layers; ++l) Gr.a.lbits[l] = layer_bits(L0[l]);
     for (size_t q = 0; q < r0.size(); ++q) Gr.a.dmeta[q] = r0[q].meta;
     Gr.a.c = *c;
-    Gr.blocks = (Gr.a.n + 63) / 64;
+    Gr.blocks = (uint32_t)(((uint64_t)Gr.a.n * (uint64_t)k->lpi + 63) / 64);
+    max_lpi = std::max(max_lpi, k->lpi);
     P->groups.push_back(Gr);
     if (names.find(k->name) == std::string::npos) {
       if (!names.empty()) names += ";";
@@ -346,7 +364,7 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
   P->info.log_dtype = log_dtype;
   P->info.max_layers = max_layers;
   P->info.n_kernels = (int32_t)P->groups.size();
-  P->info.lanes_per_instance = 1;
+  P->info.lanes_per_instance = max_lpi;
   std::strncpy(P->info.kernel_name, names.c_str(), sizeof(P->info.kernel_name) - 1);
   P->log_extent = extent;
   return 0;
@@ -397,6 +415,21 @@ void* grow(void*& p, size_t& have, size_t need) {
 void* plain_arena(size_t bytes, void* ctx) {
   PlainCtx* c = (PlainCtx*)ctx;
   return grow(c->arena, c->arena_bytes, bytes);
+}
+
+struct FixedArena {
+  void* p;
+  size_t bytes;
+};
+
+void* fixed_arena(size_t bytes, void* ctx) {
+  const FixedArena* a = (const FixedArena*)ctx;
+  if (bytes > a->bytes) {
+    g_err = "plan arena too small: " + std::to_string(bytes) + " bytes needed, " + std::to_string(a->bytes) +
+            " given";
+    return nullptr;
+  }
+  return a->p;
 }
 
 void* malloc_arena(size_t bytes, void*) {
@@ -486,7 +519,7 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_consta
   cmtj_plan P;
   P.owns = false;
   int r = build_plan(&P, &jj, 1, layers, (size_t)jj.n_layers, c, tables, n_table, CMTJ_FP64, CMTJ_LOG_F64,
-                     plain_arena, &g_plain, g_plain.stream);
+                     CMTJ_LANES_AUTO, plain_arena, &g_plain, g_plain.stream);
   if (r) return r;
   const size_t o_mag = align_up(nlog * sizeof(double));
   const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS);
@@ -512,8 +545,9 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n, const cmtj_layer*
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
   }
-  if (lanes_per_instance != 0 && lanes_per_instance != 1) {
-    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 or 1");
+  if (lanes_per_instance != CMTJ_LANES_AUTO && lanes_per_instance != CMTJ_LANES_ONE &&
+      lanes_per_instance != CMTJ_LANES_PER_LAYER) {
+    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance) or 2 (one lane per layer)");
     return nullptr;
   }
   cmtj_plan* P = new cmtj_plan();
@@ -524,10 +558,45 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n, const cmtj_layer*
     return nullptr;
   }
   const int r = build_plan(P, js, n, layers, n_layer_rows, c, tables, n_table, precision, log_dtype,
-                           malloc_arena, nullptr, s);
+                           lanes_per_instance, malloc_arena, nullptr, s);
   (void)hipStreamDestroy(s);
   if (r) {
     if (P->arena) (void)hipFree(P->arena);
+    delete P;
+    return nullptr;
+  }
+  return P;
+}
+
+int cmtj_plan_arena_size(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,
+                         size_t n_table, int32_t precision, int32_t log_dtype, int32_t lanes_per_instance,
+                         size_t* bytes) {
+  if (!js || !layers || !bytes) return fail(CMTJ_ERR_INVALID, "null pointer");
+  cmtj_plan P;
+  cmtj_constants c = kDefaults;
+  return build_plan(&P, js, n, layers, n_layer_rows, &c, nullptr, n_table, precision, log_dtype,
+                    lanes_per_instance, nullptr, nullptr, nullptr, bytes);
+}
+
+cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,
+                               const cmtj_constants* c, const double* tables, size_t n_table, int32_t precision,
+                               int32_t log_dtype, int32_t lanes_per_instance, void* d_arena, size_t arena_bytes,
+                               void* stream) {
+  if (!js || !layers || !c || !d_arena) {
+    fail(CMTJ_ERR_INVALID, "null pointer");
+    return nullptr;
+  }
+  if (lanes_per_instance != CMTJ_LANES_AUTO && lanes_per_instance != CMTJ_LANES_ONE &&
+      lanes_per_instance != CMTJ_LANES_PER_LAYER) {
+    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance) or 2 (one lane per layer)");
+    return nullptr;
+  }
+  cmtj_plan* P = new cmtj_plan();
+  P->owns = false;
+  FixedArena fa{d_arena, arena_bytes};
+  const int r = build_plan(P, js, n, layers, n_layer_rows, c, tables, n_table, precision, log_dtype,
+                           lanes_per_instance, fixed_arena, &fa, (hipStream_t)stream);
+  if (r) {
     delete P;
     return nullptr;
   }

@@ -478,7 +478,7 @@ struct FieldWeights {
 };
 
 template <class R, int LM>
-__device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const GroupArgs& g) {
+__device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const GroupArgs& g, int lsh = 0) {
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
     const int meta = (k < g.ndyn) ? g.dmeta[k] : 0;
@@ -489,7 +489,7 @@ __device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const Gro
     const int axis = slot % 3;
 #pragma unroll
     for (int l = 0; l < LM; ++l) {
-      const bool on = isH && ((lmask >> l) & 1);
+      const bool on = isH && ((lmask >> (l + lsh)) & 1);
       fw.w[k][l] = mk<R>((on && axis == 0) ? R(1) : R(0), (on && axis == 1) ? R(1) : R(0),
                          (on && axis == 2) ? R(1) : R(0));
     }
@@ -524,7 +524,7 @@ template <class R, int LM, uint32_t FM, bool LOGE>
 __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base)[LM],
                                              const DynRec (&dr)[DREG], const FieldWeights<R, LM>& fw,
                                              const GroupArgs& g, uint32_t lane, uint32_t fl, double t,
-                                             uint32_t step, int stage, int sps) {
+                                             uint32_t step, int stage, int sps, int lsh = 0) {
 #pragma unroll
   for (int l = 0; l < LM; ++l) dv[l] = base[l];
   if (CMTJ_HAS(F_DYN) || (LOGE && CMTJ_HAS(F_DYNH))) {
@@ -537,7 +537,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
       const R v = wave_value<R>(type, r, g, t, step, stage, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
-        if ((lmask >> l) & 1) add_slot<R, LOGE>(dv[l], slot, v);
+        if ((lmask >> (l + lsh)) & 1) add_slot<R, LOGE>(dv[l], slot, v);
     }
   } else if (CMTJ_HAS(F_DYNH)) {
 #pragma unroll
@@ -555,7 +555,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
       const R v = wave_value<R>(meta & 0xff, r, g, t, step, stage, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
-        if (((meta >> 16) >> l) & 1) add_slot<R, false>(dv[l], (meta >> 8) & 0xff, v);
+        if (((meta >> 16) >> (l + lsh)) & 1) add_slot<R, false>(dv[l], (meta >> 8) & 0xff, v);
     }
   }
 }
@@ -566,7 +566,7 @@ template <int LM, uint32_t FM>
 __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const DynRec (&dr)[DREG],
                                                    const FieldWeights<float, LM>& fw, const GroupArgs& g,
                                                    uint32_t lane, uint32_t fl, double t0, double ts,
-                                                   uint32_t step, int stage, int sps) {
+                                                   uint32_t step, int stage, int sps, int lsh = 0) {
 #pragma unroll
   for (int l = 0; l < LM; ++l) dv_zero(dd[l]);
   if (CMTJ_HAS(F_DYN)) {
@@ -578,7 +578,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
                        wave_value<double>(type, r, g, t0, step, 0, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
-        if (((meta >> 16) >> l) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
+        if (((meta >> 16) >> (l + lsh)) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
     }
   } else if (CMTJ_HAS(F_DYNH)) {
 #pragma unroll
@@ -598,7 +598,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
                        wave_value<double>(meta & 0xff, r, g, t0, step, 0, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
-        if (((meta >> 16) >> l) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
+        if (((meta >> 16) >> (l + lsh)) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
     }
   }
 }

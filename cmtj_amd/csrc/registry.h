@@ -1,13 +1,14 @@
 // registry.h -- kernel registry shared by the instantiation units.
 #pragma once
 
-#include "kernels.h"
+#include "kernels_ll.h"
 
 namespace cmtj {
 
 struct KEntry {
   int prec;
-  int lm;
+  int lm;   // layer capacity
+  int lpi;  // lanes per instance: 1 (k_junction) or lm (k_junction_ll)
   int solver;
   uint32_t fm;
   bool loge;
@@ -29,12 +30,22 @@ KTable kernels_f64();
 KTable kernels_f64e();
 KTable kernels_f32();
 KTable kernels_spec();
+KTable kernels_ll_f64();
+KTable kernels_ll_f32();
+KTable kernels_ll_spec();
 
 }  // namespace cmtj
 
-#define CMTJ_KE(R, PREC, LM, S, FM, E)                                  \
-  {PREC, LM, S, FM, E, (const void*)&cmtj::k_junction<R, LM, S, FM, E>, \
+#define CMTJ_KE(R, PREC, LM, S, FM, E)                                     \
+  {PREC, LM, 1, S, FM, E, (const void*)&cmtj::k_junction<R, LM, S, FM, E>, \
    "k_junction<" #R "," #LM "," #S "," #FM "," #E ">"}
+
+#define CMTJ_KL(R, PREC, L, S, FM)                                               \
+  {PREC, L, L, S, FM, false, (const void*)&cmtj::k_junction_ll<R, L, S, FM, false>, \
+   "k_junction_ll<" #R "," #L "," #S "," #FM ">"}
+
+#define CMTJ_KL_SOLVERS(R, PREC, L, FM) \
+  CMTJ_KL(R, PREC, L, CMTJ_RK4, FM), CMTJ_KL(R, PREC, L, CMTJ_HEUN, FM), CMTJ_KL(R, PREC, L, CMTJ_EULER_HEUN, FM)
 
 #define CMTJ_KE_SOLVERS(R, PREC, LM, FM, E)                                     \
   CMTJ_KE(R, PREC, LM, CMTJ_RK4, FM, E), CMTJ_KE(R, PREC, LM, CMTJ_HEUN, FM, E), \

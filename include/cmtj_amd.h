@@ -246,6 +246,8 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
 
 typedef struct cmtj_plan cmtj_plan;
 
+enum cmtj_lanes { CMTJ_LANES_AUTO = 0, CMTJ_LANES_ONE = 1, CMTJ_LANES_PER_LAYER = 2 };
+
 typedef struct cmtj_plan_info {
   uint64_t n_instances;
   uint64_t log_elements;    /* sum over instances of channels*entries */
@@ -267,7 +269,10 @@ typedef struct cmtj_plan_info {
  * CMTJ_TABLE samples (may be NULL).  Batches the reference's per-point
  * runSimulation loop (cmtj/utils/procedures.py:141, cmtj/utils/parallel.py:47)
  * which has no single-call counterpart in the reference.
- * lanes_per_instance: 1 (all layers in one lane) or 0 (library picks).
+ * lanes_per_instance: CMTJ_LANES_ONE (all layers of an instance in one
+ * lane), CMTJ_LANES_PER_LAYER (one lane per layer, for multilayer
+ * instances: lowest latency per instance) or CMTJ_LANES_AUTO (one lane per
+ * layer while the batch is too small to fill the chip).
  * Returns NULL on error (see cmtj_last_error). */
 cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n,
                             const cmtj_layer* layers, size_t n_layer_rows,
@@ -283,6 +288,23 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n,
  * hipStream_t (NULL = default stream).  Asynchronous; allocates nothing. */
 int cmtj_plan_execute(cmtj_plan* p, void* d_log, size_t log_capacity,
                       double* d_mag_out, cmtj_status* d_status, void* stream);
+
+/* Device bytes a plan of these instances needs (the same validation as
+ * cmtj_plan_create; tables are not read, only counted).  No reference
+ * counterpart: sizes the caller-owned arena of cmtj_plan_create_in. */
+int cmtj_plan_arena_size(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,
+                         size_t n_table, int32_t precision, int32_t log_dtype, int32_t lanes_per_instance,
+                         size_t* bytes);
+
+/* As cmtj_plan_create, but the descriptors go into caller-owned device
+ * memory d_arena (arena_bytes >= cmtj_plan_arena_size), uploaded on `stream`
+ * (a hipStream_t, synchronised before return).  The plan does not own the
+ * arena: it must outlive every execute of the plan.  For callers that keep
+ * their own device pools (the serving loop): no device allocation. */
+cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,
+                               const cmtj_constants* c, const double* tables, size_t n_table, int32_t precision,
+                               int32_t log_dtype, int32_t lanes_per_instance, void* d_arena, size_t arena_bytes,
+                               void* stream);
 
 int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out);
 int cmtj_plan_destroy(cmtj_plan* p);

@@ -178,6 +178,25 @@ def custom_driver(J):
     return j, dict(totalTime=5e-10, timeStep=1e-13, writeFrequency=1e-12, calculateEnergies=True)
 
 
+def five_layer_chain(J, solver=SolverMode.RK4):
+    """5 layers, IEC between every adjacent pair, STRIP R (8 lanes per
+    instance in the one-layer-per-lane kernels)."""
+    layers = []
+    for q in range(5):
+        layers.append(Layer(f"L{q}", mag=CVector(1, 0.1 * (q + 1), 0.05 * q), anis=CVector(1, 0, 0),
+                            Ms=1.2 + 0.1 * q, thickness=(1 + 0.5 * q) * 1e-9, cellSurface=SURF,
+                            demagTensor=DEMAG, damping=0.05))
+    j = J(layers, [100.0 + q for q in range(5)], [50.0 + q for q in range(5)], [1.0] * 5, [1.0] * 5,
+          [0.5] * 5, [0.5] * 5, [2.0] * 5)
+    for q in range(5):
+        j.setLayerAnisotropyDriver(f"L{q}", constantDriver(20e3 + 10e3 * q))
+    for q in range(4):
+        j.setIECDriver(f"L{q}", f"L{q + 1}", constantDriver(-2e-4 * (1 + 0.3 * q)))
+    j.setLayerExternalFieldDriver("all", AxialDriver(200e3, 20e3, 5e3))
+    j.setLayerOerstedFieldDriver("all", AxialDriver(NullDriver(), stepDriver(0, 2e3, 0, 5e-12), NullDriver()))
+    return j, dict(totalTime=1e-9, timeStep=1e-13, writeFrequency=1e-12, solverMode=solver)
+
+
 CASES = {
     "rh_strip": rh_strip,
     "bilayer_iec_rk4": bilayer_iec,
@@ -197,4 +216,5 @@ CASES = {
     "exotic_eulerheun": lambda J: exotic_terms(J, SolverMode.EulerHeun),
     "energies": energies,
     "custom_driver": custom_driver,
+    "five_layer_chain": five_layer_chain,
 }

@@ -6,6 +6,8 @@ import ctypes
 import re
 from pathlib import Path
 
+import pytest
+
 from cmtj_amd import _lib
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -52,3 +54,19 @@ def test_host_entry_points_need_no_gpu():
         raise AssertionError("tabulated drivers must be rejected")
     except RuntimeError as e:
         assert "tabulated" in str(e)
+
+
+def test_plan_arena_size_runs_without_a_gpu():
+    """Sizing a plan validates and groups the batch on the host only."""
+    from cmtj_amd import workloads
+
+    b = workloads.pimm(n_points=64, t_min=1e-10, t_max=2e-10)
+    n1 = _lib.plan_arena_size(b.js, b.layers, 0, b.precision, b.log_dtype)
+    b2 = workloads.pimm(n_points=128, t_min=1e-10, t_max=2e-10)
+    n2 = _lib.plan_arena_size(b2.js, b2.layers, 0, b2.precision, b2.log_dtype)
+    assert n1 >= 64 * _lib.JUNCTION_DTYPE.itemsize + 128 * _lib.LAYER_DTYPE.itemsize
+    assert n2 > n1
+    bad = b.js.copy()
+    bad["n_layers"][3] = 0
+    with pytest.raises(RuntimeError):
+        _lib.plan_arena_size(bad, b.layers, 0, b.precision, b.log_dtype)

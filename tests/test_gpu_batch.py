@@ -32,7 +32,8 @@ def _m_channels(b, log, i):
     return {k: np.asarray(log[off + c * ne: off + (c + 1) * ne], dtype=np.float64) for c, k in enumerate(names)}
 
 
-def test_heterogeneous_fp64_batch_matches_oracle():
+@pytest.mark.parametrize("lanes", ["one", "per_layer"])
+def test_heterogeneous_fp64_batch_matches_oracle(lanes):
     names = list(parity_cases.CASES)
     names.remove("energies")  # energy channels are a drop-in feature
     js, kws = [], []
@@ -44,9 +45,9 @@ def test_heterogeneous_fp64_batch_matches_oracle():
     b = JunctionBatch.from_junctions(js, [k["totalTime"] for k in kws], [k["timeStep"] for k in kws],
                                      [k["writeFrequency"] for k in kws],
                                      [k.get("solverMode", SolverMode.RK4) for k in kws],
-                                     precision="fp64", log="full", log_dtype="f64")
-    print(f"[batch] {len(js)} instances, {b.plan().info.n_kernels} groups: {b.plan().info.kernel_name.decode()}",
-          flush=True)
+                                     precision="fp64", log="full", log_dtype="f64").set_lanes(lanes)
+    print(f"[batch] {lanes}: {len(js)} instances, {b.plan().info.n_kernels} groups: "
+          f"{b.plan().info.kernel_name.decode()}", flush=True)
     r = b.run()
     glog = r.log.cpu().numpy()
     olog, omag, ost = _oracle_blocks(b)
@@ -61,8 +62,9 @@ def test_heterogeneous_fp64_batch_matches_oracle():
     np.testing.assert_allclose(r.final_m.cpu().numpy(), omag, atol=1e-8)
 
 
+@pytest.mark.parametrize("lanes", ["one", "per_layer"])
 @pytest.mark.parametrize("builder", ["r_h_loop", "pimm", "sto"])
-def test_fp32_bulk_within_1e4_of_fp64(builder):
+def test_fp32_bulk_within_1e4_of_fp64(builder, lanes):
     if builder == "r_h_loop":
         mk = lambda p: workloads.r_h_loop(n_fields=101, total_time=2e-9, precision=p, log="full")
     elif builder == "pimm":
@@ -70,7 +72,7 @@ def test_fp32_bulk_within_1e4_of_fp64(builder):
                                       log_dtype="f64")
     else:
         mk = lambda p: workloads.sto(n_currents=128, total_time=5e-9, precision=p, log="full")
-    b64, b32 = mk("fp64"), mk("fp32")
+    b64, b32 = mk("fp64").set_lanes(lanes), mk("fp32").set_lanes(lanes)
     b32.log_dtype = _lib.LOG_F64
     b32._plan = None
     r64, r32 = b64.run(), b32.run()
@@ -81,7 +83,7 @@ def test_fp32_bulk_within_1e4_of_fp64(builder):
             if "_m" in k:
                 worst = max(worst, float(np.max(np.abs(a[k] - c[k]))))
     fm = float((r64.final_m - r32.final_m).abs().max())
-    print(f"[fp32] {builder}: max|dm| over logs = {worst:.3e}, final = {fm:.3e}", flush=True)
+    print(f"[fp32] {builder} {lanes}: max|dm| over logs = {worst:.3e}, final = {fm:.3e}", flush=True)
     assert worst <= FP32_TOL and fm <= FP32_TOL
 
 
@@ -115,6 +117,63 @@ def test_uneven_ragged_batch_and_failure_isolation():
         for k in o:
             if "_m" in k:
                 assert float(np.max(np.abs(g[k] - o[k]))) <= 1e-8
+
+
+@pytest.mark.parametrize("lanes", ["one", "per_layer"])
+@pytest.mark.parametrize("bad_layer", [0, 1])
+def test_multilayer_nan_commits_like_the_reference(bad_layer, lanes):
+    """A NaN in layer b of a bilayer stops the run; the layers up to and
+    including the first NaN layer take the step's value, later ones keep the
+    previous one (runMultiLayerSolver updates layers in turn)."""
+    js = []
+    for q in range(3):
+        j, _ = parity_cases.bilayer_iec(Junction, SolverMode.RK4)
+        if q == 1:
+            name = j.getLayerIds()[bad_layer]
+            j.setLayerExternalFieldDriver(name, AxialDriver(stepDriver(0, float("inf"), 5e-12, 1e-9),
+                                                            constantDriver(0), constantDriver(1e5)))
+        js.append(j)
+    b = JunctionBatch.from_junctions(js, 1e-10, 1e-13, 1e-13, SolverMode.RK4, precision="fp64", log="full",
+                                     log_dtype="f64").set_lanes(lanes)
+    r = b.run()
+    st = r.status()
+    olog, omag, ost = _oracle_blocks(b)
+    for i in range(3):
+        for f in ("code", "steps_done", "entries_done", "nan_layer"):
+            assert int(st[i][f]) == int(ost[i][f]), (i, f)
+    assert int(st[1]["code"]) == _lib.ERR_NAN and int(st[1]["nan_layer"]) == bad_layer
+    gm = r.final_m.cpu().numpy()
+    np.testing.assert_allclose(gm[:, :2], omag[:, :2], atol=1e-8, equal_nan=True)
+    assert np.isnan(gm[1, bad_layer]).any()
+    if bad_layer == 0:
+        assert np.isfinite(gm[1, 1]).all()  # the later layer keeps its previous value
+    for i in (0, 2):
+        g, o = _m_channels(b, r.log.cpu().numpy(), i), _m_channels(b, olog, i)
+        for k in o:
+            if "_m" in k:
+                assert float(np.max(np.abs(g[k] - o[k]))) <= 1e-8
+
+
+def test_lane_mappings_agree():
+    """One lane per instance and one lane per layer run the same arithmetic
+    per layer (FMA contraction may still differ between the two kernels)."""
+    for n in ("bilayer_iec_rk4", "trilayer_rk4", "bilayer_iec_heun", "trilayer_eulerheun", "stt_ref_top",
+              "sot_dynamic_heun", "five_layer_chain"):
+        outs = []
+        for lanes in ("one", "per_layer"):
+            j, kw = parity_cases.CASES[n](Junction)
+            j.setNoiseSeed(5)
+            b = JunctionBatch.from_junctions([j] * 3, kw["totalTime"], kw["timeStep"], kw["writeFrequency"],
+                                             kw.get("solverMode", SolverMode.RK4), precision="fp64", log="full",
+                                             log_dtype="f64").set_lanes(lanes)
+            r = b.run()
+            outs.append((r.log.cpu().numpy(), r.final_m.cpu().numpy(), b.plan().info.lanes_per_instance))
+        print(f"[lanes] {n}: lanes/instance {outs[0][2]} vs {outs[1][2]}", flush=True)
+        assert outs[0][2] == 1 and outs[1][2] > 1
+        d = max(float(np.max(np.abs(outs[0][1] - outs[1][1]))), float(np.max(np.abs(outs[0][0] - outs[1][0]))))
+        print(f"[lanes] {n}: max difference {d:.3e}", flush=True)
+        np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(outs[0][0], outs[1][0], rtol=1e-12, atol=1e-12)
 
 
 def test_drop_in_nan_raises_like_the_reference():
