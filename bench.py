@@ -61,6 +61,47 @@ def _traffic():
     return json.loads(f.read_text())
 
 
+def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
+    """Mixed request stream (PIMM / VSD / CIMS / 3-4 layer chains, 1-50 ns,
+    dt 1e-14..1e-13 s) with Poisson arrivals at `qps` for `seconds`, served
+    by the continuous-batching loop in FP32; latency = submit to result."""
+    import numpy as np
+
+    from cmtj_amd import workloads
+    from cmtj_amd.serving import Server
+
+    rng = np.random.default_rng(seed + 1)
+    mix = workloads.serving_mix(int(qps * seconds * 1.5) + 16, seed=seed)
+    futs, kinds = [], []
+    with Server(precision="fp32", slice_steps=20000, n_streams=4, log_elements=1 << 28) as srv:
+        t0 = time.perf_counter()
+        nxt = t0
+        q = 0
+        while q < len(mix):
+            now = time.perf_counter()
+            if now - t0 >= seconds:
+                break
+            if now >= nxt:
+                kind, j, T, dt, wf, sm, log = mix[q]
+                futs.append(srv.submit(j, T, dt, wf, sm, log=log))
+                kinds.append(kind)
+                q += 1
+                nxt += rng.exponential(1.0 / qps)
+            else:
+                time.sleep(min(nxt - now, 0.005))
+        res = [f.result() for f in futs]
+        ticks = srv.ticks
+    lat = np.array([r.latency for r in res])
+    steps = sum(r.steps for r in res)
+    wall = max(r.t_done for r in res) - t0
+    return {"qps": qps, "arrival_window_s": seconds, "requests": len(res),
+            "mix": {k: kinds.count(k) for k in sorted(set(kinds))},
+            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(lat.max())},
+            "junction_steps": int(steps), "junction_steps_per_s": steps / wall, "wall_s": wall, "ticks": ticks,
+            "slice_steps": 20000, "precision": "fp32", "lanes": "per_layer"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +112,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--lanes", default="auto", choices=["auto", "one", "per_layer"])
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-qps", type=float, default=20.0)
+    ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-pass", action="store_true",
                     help="run the sweep kernel twice and a 1 GiB device copy, print nothing (for counter runs)")
     args = ap.parse_args()
@@ -173,6 +217,13 @@ def main() -> int:
         print(f"[bench] cpu oracle: {cpu['value']:.4e} junction-steps/s on {ncores} threads", file=sys.stderr,
               flush=True)
 
+    serving = None
+    if not args.no_serving:
+        serving = _serving(args.serving_qps, args.serving_seconds)
+        print(f"[bench] serving: {serving['requests']} requests at {args.serving_qps} QPS, "
+              f"p50 {serving['latency_s']['p50']:.3f} s, p99 {serving['latency_s']['p99']:.3f} s, "
+              f"{serving['junction_steps_per_s']:.3e} junction-steps/s", file=sys.stderr, flush=True)
+
     tr = _traffic()
     out = {
         "metric": "junction-steps/s FP32 batched vs FP64 plain at max|dm|<=1e-4; p50/p99 latency",
@@ -209,6 +260,7 @@ def main() -> int:
                      "flop_per_junction_step": FLOP_PER_STEP, "kernel_ms": kern_ms,
                      "traffic_algorithmic": 12.0 * b.total_steps},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
     return 0

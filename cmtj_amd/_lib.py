@@ -14,6 +14,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libcmtj_amd.so"
 
+ABI_VERSION = 2
 MAX_LAYERS = 8
 NSLOT = 30
 ENERGY_CHANNELS = 23
@@ -128,6 +129,11 @@ STATUS_DTYPE = np.dtype(
     [("code", "<i4"), ("steps_done", "<u4"), ("entries_done", "<u4"), ("nan_layer", "<i4")], align=True
 )
 
+STATE_DTYPE = np.dtype(
+    [("m", "<f8", (MAX_LAYERS, 3)), ("step", "<u4"), ("entries", "<u4"), ("code", "<i4"), ("nan_layer", "<i4")],
+    align=True,
+)
+
 
 class _AbiInfo(ctypes.Structure):
     _fields_ = [
@@ -141,6 +147,7 @@ class _AbiInfo(ctypes.Structure):
             "sizeof_junction",
             "sizeof_constants",
             "sizeof_status",
+            "sizeof_state",
             "offsetof_layer_drv",
             "offsetof_layer_ref_type",
             "offsetof_junction_dt",
@@ -178,6 +185,7 @@ EXPORTS = (
     "cmtj_plan_arena_size",
     "cmtj_plan_create_in",
     "cmtj_plan_execute",
+    "cmtj_plan_execute_slice",
     "cmtj_plan_query",
     "cmtj_plan_destroy",
 )
@@ -217,12 +225,14 @@ def _load():
     ]
     lib.cmtj_plan_create_in.restype = _P
     lib.cmtj_plan_execute.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P]
+    lib.cmtj_plan_execute_slice.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.c_uint32,
+                                            ctypes.POINTER(_P), ctypes.c_int32]
     lib.cmtj_plan_query.argtypes = [_P, ctypes.POINTER(PlanInfo)]
     lib.cmtj_plan_destroy.argtypes = [_P]
     info = _AbiInfo()
     lib.cmtj_abi(ctypes.byref(info))
     checks = {
-        "version": (info.version, 1),
+        "version": (info.version, ABI_VERSION),
         "max_layers": (info.max_layers, MAX_LAYERS),
         "nslot": (info.nslot, NSLOT),
         "driver": (info.sizeof_driver, DRIVER_DTYPE.itemsize),
@@ -230,6 +240,7 @@ def _load():
         "junction": (info.sizeof_junction, JUNCTION_DTYPE.itemsize),
         "constants": (info.sizeof_constants, CONSTANTS_DTYPE.itemsize),
         "status": (info.sizeof_status, STATUS_DTYPE.itemsize),
+        "state": (info.sizeof_state, STATE_DTYPE.itemsize),
         "layer.drv": (info.offsetof_layer_drv, LAYER_DTYPE.fields["drv"][1]),
         "layer.ref_type": (info.offsetof_layer_ref_type, LAYER_DTYPE.fields["ref_type"][1]),
         "junction.dt": (info.offsetof_junction_dt, JUNCTION_DTYPE.fields["dt"][1]),
@@ -353,6 +364,16 @@ class Plan:
         check(
             lib().cmtj_plan_execute(self._h, d_log, log_capacity, d_mag, d_status, stream),
             "cmtj_plan_execute",
+    "cmtj_plan_execute_slice",
+        )
+
+    def execute_slice(self, d_log: int | None, log_capacity: int, d_mag: int | None, d_status: int | None,
+                      d_state: int, max_steps: int, streams) -> None:
+        arr = (_P * max(len(streams), 1))(*streams)
+        check(
+            lib().cmtj_plan_execute_slice(self._h, d_log, log_capacity, d_mag, d_status, d_state, max_steps,
+                                          arr, len(streams)),
+            "cmtj_plan_execute_slice",
         )
 
     def close(self) -> None:

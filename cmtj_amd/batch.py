@@ -326,6 +326,31 @@ class JunctionBatch:
         p.execute(log.data_ptr() if self.log_elements else None, log.numel(), mag.data_ptr(), st.data_ptr(), sp)
         return BatchResult(self, log, mag, st)
 
+    def run_sliced(self, slice_steps: int, n_streams: int = 1, device="cuda") -> BatchResult:
+        """Runs every instance to the end in slices of at most slice_steps
+        steps (cmtj_plan_execute_slice), state kept on the device between
+        slices; the result equals run()'s."""
+        import torch
+
+        p = self.plan()
+        log, mag, st = self._buffers(device)
+        init = np.zeros(len(self.js), dtype=_lib.STATE_DTYPE)
+        for i in range(len(self.js)):
+            off, nl = int(self.js[i]["layer_offset"]), int(self.js[i]["n_layers"])
+            init["m"][i, :nl] = self.layers["mag"][off: off + nl]
+        state = torch.as_tensor(init.view(np.uint8).reshape(len(self.js), -1)).to(device)
+        streams = [torch.cuda.Stream(device=device) for _ in range(max(1, n_streams))]
+        need = self.js["n_steps"].astype(np.int64)
+        while True:
+            torch.cuda.synchronize()
+            p.execute_slice(log.data_ptr() if self.log_elements else None, log.numel(), mag.data_ptr(),
+                            st.data_ptr(), state.data_ptr(), slice_steps, [s.cuda_stream for s in streams])
+            torch.cuda.synchronize()
+            sh = st.cpu().numpy().view(_lib.STATUS_DTYPE).reshape(-1)
+            if np.all((sh["code"] != _lib.OK) | (sh["steps_done"].astype(np.int64) >= need)):
+                break
+        return BatchResult(self, log, mag, st)
+
     def run(self, device="cuda") -> BatchResult:
         """execute + synchronize + raise on a NaN instance."""
         import torch

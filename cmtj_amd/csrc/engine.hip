@@ -255,7 +255,7 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     junction_records(j, L, recs_of[i]);
     if (recs_of[i].size() > (size_t)MAX_DYN)
       return fail(CMTJ_ERR_INVALID, "more than 16 time-dependent drivers in one junction");
-    sig.assign({j.n_layers, j.solver, j.log_mask, j.mr_mode});
+    sig.assign({j.n_layers, j.solver, j.log_mask, j.mr_mode, (int32_t)junction_flags(j, L)});
     for (int l = 0; l < j.n_layers; ++l) sig.push_back(layer_bits(L[l]));
     for (const Rec& r : recs_of[i]) sig.push_back(r.meta);
     size_t g = 0;
@@ -370,14 +370,20 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
   return 0;
 }
 
-static int launch_plan(cmtj_plan* P, void* d_log, double* d_mag, cmtj_status* d_status, hipStream_t s) {
-  for (Group& G : P->groups) {
+static int launch_plan(cmtj_plan* P, void* d_log, double* d_mag, cmtj_status* d_status, hipStream_t s,
+                       cmtj_state* d_state = nullptr, uint32_t slice = 0, void* const* streams = nullptr,
+                       int n_streams = 0) {
+  for (size_t q = 0; q < P->groups.size(); ++q) {
+    Group& G = P->groups[q];
     GroupArgs a = G.a;
     a.log = d_log;
     a.mag_out = d_mag;
     a.status = d_status;
+    a.state = d_state;
+    a.slice = slice;
     void* args[] = {&a};
-    HIP_TRY(hipLaunchKernel(G.k->fn, dim3(G.blocks), dim3(64), args, 0, s));
+    const hipStream_t st = n_streams > 0 ? (hipStream_t)streams[q % (size_t)n_streams] : s;
+    HIP_TRY(hipLaunchKernel(G.k->fn, dim3(G.blocks), dim3(64), args, 0, st));
   }
   return 0;
 }
@@ -457,6 +463,7 @@ int cmtj_abi(cmtj_abi_info* out) {
   out->sizeof_junction = (int32_t)sizeof(cmtj_junction);
   out->sizeof_constants = (int32_t)sizeof(cmtj_constants);
   out->sizeof_status = (int32_t)sizeof(cmtj_status);
+  out->sizeof_state = (int32_t)sizeof(cmtj_state);
   out->offsetof_layer_drv = (int32_t)offsetof(cmtj_layer, drv);
   out->offsetof_layer_ref_type = (int32_t)offsetof(cmtj_layer, ref_type);
   out->offsetof_junction_dt = (int32_t)offsetof(cmtj_junction, dt);
@@ -610,6 +617,19 @@ int cmtj_plan_execute(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_
     return fail(CMTJ_ERR_INVALID, "log buffer smaller than the plan's log extent (" +
                                       std::to_string(p->log_extent) + " elements)");
   return launch_plan(p, d_log, d_mag_out, d_status, (hipStream_t)stream);
+}
+
+int cmtj_plan_execute_slice(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_mag_out,
+                            cmtj_status* d_status, cmtj_state* d_state, uint32_t max_steps,
+                            void* const* streams, int32_t n_streams) {
+  if (!p) return fail(CMTJ_ERR_INVALID, "null plan");
+  if (!d_state) return fail(CMTJ_ERR_INVALID, "a slice needs the instance state");
+  if (max_steps == 0) return fail(CMTJ_ERR_INVALID, "max_steps must be positive");
+  if (n_streams < 0 || (n_streams > 0 && !streams)) return fail(CMTJ_ERR_INVALID, "bad stream list");
+  if (d_log && log_capacity < p->log_extent)
+    return fail(CMTJ_ERR_INVALID, "log buffer smaller than the plan's log extent (" +
+                                      std::to_string(p->log_extent) + " elements)");
+  return launch_plan(p, d_log, d_mag_out, d_status, nullptr, d_state, max_steps, streams, n_streams);
 }
 
 int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out) {

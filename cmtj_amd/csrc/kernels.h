@@ -40,6 +40,8 @@ struct GroupArgs {
   void* log;
   double* mag_out;  // [n][CMTJ_MAX_LAYERS][3] in junction order
   cmtj_status* status;
+  cmtj_state* state;  // resumable state [n] in junction order, or null (run from the start)
+  uint32_t slice;     // steps per launch when state is set
   uint32_t n;       // lanes of this group
   uint32_t flags;   // feature bits of the group
   int32_t log_f32;
@@ -800,11 +802,22 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
   const R sdt = (R)sqrt(dt);
   const R sixth_inv = R(1.0 / 6.0);
   const V3<R> zero = mk<R>(0, 0, 0);
-  uint64_t next_log = 0;
   uint32_t k = 0;
   int code = CMTJ_OK, nan_layer = -1;
-  uint32_t i = 0;
-  for (; i < nsteps; ++i) {
+  uint32_t i = 0, iend = nsteps;
+  cmtj_state* const S = g.state ? g.state + jid : nullptr;
+  if (S) {  // resume: state of the previous slice
+#pragma unroll
+    for (int l = 0; l < LM; ++l)
+      if (l < nl) m[l] = mk<double>(S->m[l][0], S->m[l][1], S->m[l][2]);
+    i = S->step;
+    k = S->entries;
+    code = S->code;
+    nan_layer = S->nan_layer;
+    iend = (code != CMTJ_OK) ? i : (uint32_t)min((uint64_t)nsteps, (uint64_t)i + g.slice);
+  }
+  uint64_t next_log = (uint64_t)k * W;
+  for (; i < iend; ++i) {
     const double t = (double)i * dt;
     if constexpr (ANCHORED) {
       // RK4 of the reference (core/junction.hpp:1017-1041, neighbours frozen
@@ -1027,6 +1040,20 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
     s.entries_done = k;
     s.nan_layer = nan_layer;
     g.status[jid] = s;
+  }
+  if (S) {
+#pragma unroll
+    for (int l = 0; l < LM; ++l) {
+      if (l < nl) {
+        S->m[l][0] = m[l].x;
+        S->m[l][1] = m[l].y;
+        S->m[l][2] = m[l].z;
+      }
+    }
+    S->step = i;
+    S->entries = k;
+    S->code = code;
+    S->nan_layer = nan_layer;
   }
 }
 

@@ -189,13 +189,22 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
   const R sdt = (R)sqrt(dt);
   const R sixth_inv = R(1.0 / 6.0);
   const V3<R> zero = mk<R>(0, 0, 0);
-  uint64_t next_log = 0;
   uint32_t k = 0;
   int code = CMTJ_OK, nan_layer = -1;
+  uint32_t i = 0, iend = nsteps;
+  cmtj_state* const S = g.state ? g.state + jid : nullptr;
+  if (S) {  // resume: state of the previous slice
+    m = mk<double>(S->m[lyr][0], S->m[lyr][1], S->m[lyr][2]);
+    i = S->step;
+    k = S->entries;
+    code = S->code;
+    nan_layer = S->nan_layer;
+    iend = (code != CMTJ_OK) ? i : (uint32_t)min((uint64_t)nsteps, (uint64_t)i + g.slice);
+  }
+  uint64_t next_log = (uint64_t)k * W;
   // neighbours (m of layers ly - 1 and ly + 1), refreshed after every step
   V3<double> bot = from_prev(m, hasB), top = from_next(m, hasT);
-  uint32_t i = 0;
-  for (; i < nsteps; ++i) {
+  for (; i < iend; ++i) {
     const double t = (double)i * dt;
     V3<double> mn;
     if constexpr (ANCHORED) {
@@ -325,6 +334,19 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
     s.entries_done = k;
     s.nan_layer = nan_layer;
     g.status[jid] = s;
+  }
+  if (S) {
+    if (act) {
+      S->m[ly][0] = m.x;
+      S->m[ly][1] = m.y;
+      S->m[ly][2] = m.z;
+    }
+    if (ly == 0) {
+      S->step = i;
+      S->entries = k;
+      S->code = code;
+      S->nan_layer = nan_layer;
+    }
   }
 }
 

@@ -142,4 +142,81 @@ def sto(n_currents: int = 512, total_time: float = 100e-9, dt: float = 1e-13, wf
     return b
 
 
-__all__ = ["r_h_loop", "pimm", "pimm_junction", "cims_pbit", "cims_junction", "sto", "sto_junction"]
+def vsd_junction(H: float = 100e3, f: float = 5e9) -> Junction:
+    """One free layer (Ms 1.0 T, t 1.5 nm, Ku 1e5 J/m^3 along z, alpha 0.02,
+    S 1e-16 m^2) against a fixed reference along x (CLASSIC R, Rp 100 /
+    Rap 200 Ohm); bias field H at 45 deg in the x-z plane; sine Oersted drive
+    of 500 A/m at frequency f along y (the VSD_procedure excitation,
+    cmtj/utils/procedures.py:280)."""
+    free = Layer("free", mag=CVector(0.5, 0.0, 0.866), anis=CVector(0, 0, 1), Ms=1.0, thickness=1.5e-9,
+                 cellSurface=1e-16, demagTensor=THIN, damping=0.02)
+    free.setReferenceLayer(CVector(1, 0, 0))
+    j = Junction([free], 100, 200)
+    j.setLayerAnisotropyDriver("free", constantDriver(1e5))
+    j.setLayerExternalFieldDriver("all", AxialDriver(H * math.sqrt(0.5), 0, H * math.sqrt(0.5)))
+    j.setLayerOerstedFieldDriver("all", AxialDriver(NullDriver(), ScalarDriver.getSineDriver(0, 500, f, 0),
+                                                    NullDriver()))
+    return j
+
+
+def chain_junction(n_layers: int, H: float = 200e3) -> Junction:
+    """n_layers in-plane layers (Ms 1.2..1.5 T, t 1..2.5 nm, Ku 2e4..5e4 J/m^3
+    along x, alpha 0.03) coupled by IEC J = -2e-4 J/m^2 between neighbours;
+    in-plane bias H at 30 deg; Hoe_y step 1 kA/m over the first 10 ps; STRIP
+    Rx0 100, Ry0 50, AMR 1, AHE 2 Ohm per layer."""
+    layers = []
+    for q in range(n_layers):
+        layers.append(Layer(f"L{q}", mag=CVector(1, 0.1 * (q + 1), 0.02), anis=CVector(1, 0, 0), Ms=1.2 + 0.1 * q,
+                            thickness=(1 + 0.5 * q) * 1e-9, cellSurface=1e-16, demagTensor=THIN, damping=0.03))
+    z = [0.0] * n_layers
+    j = Junction(layers, [100.0] * n_layers, [50.0] * n_layers, [1.0] * n_layers, [1.0] * n_layers, z, z,
+                 [2.0] * n_layers)
+    for q in range(n_layers):
+        j.setLayerAnisotropyDriver(f"L{q}", constantDriver(2e4 + 1e4 * q))
+    for q in range(n_layers - 1):
+        j.setIECDriver(f"L{q}", f"L{q + 1}", constantDriver(-2e-4))
+    j.setLayerExternalFieldDriver("all", AxialDriver(H * math.cos(math.pi / 6), H * math.sin(math.pi / 6), 0))
+    j.setLayerOerstedFieldDriver("all", AxialDriver(NullDriver(), ScalarDriver.getStepDriver(0, 1e3, 0, 1e-11),
+                                                    NullDriver()))
+    return j
+
+
+def serving_mix(n: int, seed: int = 0, t_min: float = 1e-9, t_max: float = 50e-9, dt_min: float = 1e-14,
+                dt_max: float = 1e-13) -> list:
+    """n request specs (kind, junction, totalTime, timeStep, writeFrequency,
+    solverMode, log) of the mixed serving stream: PIMM (2 layers, RK4),
+    VSD (1 layer, sine drive 1-20 GHz, RK4), CIMS (2 layers, static SOT at
+    300 K, Euler-Heun, dt 1e-14) and 3-4 layer IEC chains (RK4); durations
+    U(t_min, t_max), dt log-uniform in [dt_min, dt_max], logs decimated to
+    1 ps.  Parameters are drawn in the ranges of the sweep workloads."""
+    rng = np.random.default_rng(seed)
+    kinds = rng.choice(["pimm", "vsd", "cims", "chain3", "chain4"], size=n, p=[0.35, 0.25, 0.2, 0.1, 0.1])
+    out = []
+    for q in range(n):
+        k = str(kinds[q])
+        T = float(rng.uniform(t_min, t_max))
+        dt = float(np.exp(rng.uniform(np.log(dt_min), np.log(dt_max))))
+        wf = max(1e-12, dt)
+        if k == "pimm":
+            j = pimm_junction()
+            Hm = float(rng.uniform(0, 600e3))
+            j.setLayerExternalFieldDriver("all", AxialDriver(Hm * math.cos(math.pi / 6), Hm * math.sin(math.pi / 6),
+                                                             0))
+            out.append((k, j, T, dt, wf, SolverMode.RK4, "mx"))
+        elif k == "vsd":
+            j = vsd_junction(float(rng.uniform(50e3, 400e3)), float(rng.uniform(1e9, 20e9)))
+            out.append((k, j, T, dt, wf, SolverMode.RK4, "full"))
+        elif k == "cims":
+            j = cims_junction()
+            jc = float(10 ** rng.uniform(10, 12))
+            j.setLayerCurrentDriver("free", ScalarDriver.getStepDriver(0, jc, 0, T / 2))
+            j.setNoiseSeed(int(rng.integers(0, 2**62)))
+            out.append((k, j, T, 1e-14, max(1e-12, 1e-14), SolverMode.EulerHeun, "m"))
+        else:
+            j = chain_junction(3 if k == "chain3" else 4, float(rng.uniform(50e3, 400e3)))
+            out.append((k, j, T, dt, wf, SolverMode.RK4, "full"))
+    return out
+
+
+__all__ = ["r_h_loop", "pimm", "pimm_junction", "cims_pbit", "cims_junction", "sto", "sto_junction",
+           "vsd_junction", "chain_junction", "serving_mix"]

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define CMTJ_ABI_VERSION 1
+#define CMTJ_ABI_VERSION 2
 #define CMTJ_MAX_LAYERS 8
 
 /* ScalarDriver waveform kinds; numbering of reference UpdateType
@@ -192,6 +192,17 @@ typedef struct cmtj_status {
   int32_t nan_layer;      /* first layer with a NaN, else -1 */
 } cmtj_status;
 
+/* Resumable state of one instance (the serving loop's slots): the
+ * magnetisation of every layer, steps done, entries written and the status
+ * code.  A fresh instance has m = its layers' initial m and zeros elsewhere. */
+typedef struct cmtj_state {
+  double m[CMTJ_MAX_LAYERS][3];
+  uint32_t step;
+  uint32_t entries;
+  int32_t code;
+  int32_t nan_layer;
+} cmtj_state;
+
 /* Sizes and field offsets, for host bindings that mirror these structs. */
 typedef struct cmtj_abi_info {
   int32_t version;
@@ -202,6 +213,7 @@ typedef struct cmtj_abi_info {
   int32_t sizeof_junction;
   int32_t sizeof_constants;
   int32_t sizeof_status;
+  int32_t sizeof_state;
   int32_t offsetof_layer_drv;
   int32_t offsetof_layer_ref_type;
   int32_t offsetof_junction_dt;
@@ -305,6 +317,18 @@ cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_lay
                                const cmtj_constants* c, const double* tables, size_t n_table, int32_t precision,
                                int32_t log_dtype, int32_t lanes_per_instance, void* d_arena, size_t arena_bytes,
                                void* stream);
+
+/* Advances every instance of the plan by at most max_steps steps from its
+ * state d_state[n] (read, then written back), logging into d_log as
+ * cmtj_plan_execute does.  Arithmetic, step times and noise counters are
+ * those of an uninterrupted run, so a run cut into slices gives the same
+ * result.  Instances whose state holds an error code are skipped.  Groups
+ * are launched round-robin on streams[0..n_streams-1] (hipStream_t; NULL
+ * entries = default stream) so that their slices overlap on the device.
+ * No reference counterpart: the serving loop's continuous batching. */
+int cmtj_plan_execute_slice(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_mag_out,
+                            cmtj_status* d_status, cmtj_state* d_state, uint32_t max_steps,
+                            void* const* streams, int32_t n_streams);
 
 int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out);
 int cmtj_plan_destroy(cmtj_plan* p);

@@ -1,0 +1,60 @@
+"""Host logic of the serving loop: the log-arena allocator and request
+validation (no GPU needed)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cmtj_amd import workloads
+from cmtj_amd.serving import LogArena, Server
+
+
+def test_arena_first_fit_and_merge():
+    a = LogArena(100)
+    x = a.alloc(30)
+    y = a.alloc(30)
+    z = a.alloc(30)
+    assert (x, y, z) == (0, 30, 60)
+    assert a.alloc(20) is None  # 10 left
+    a.free(y, 30)
+    assert a.alloc(25) == 30    # first fit into the hole
+    a.free(x, 30)
+    a.free(30, 25)
+    a.free(z, 30)
+    assert a.used == 0 and a._free == [(0, 100)]  # everything merged back
+
+
+def test_arena_random_alloc_free_keeps_blocks_disjoint():
+    rng = np.random.default_rng(3)
+    a = LogArena(10_000)
+    live = {}
+    for _ in range(2000):
+        if live and rng.random() < 0.45:
+            off = list(live)[int(rng.integers(len(live)))]
+            a.free(off, live.pop(off))
+        else:
+            sz = int(rng.integers(1, 400))
+            off = a.alloc(sz)
+            if off is not None:
+                live[off] = sz
+        spans = sorted((o, o + s) for o, s in live.items())
+        assert all(spans[q][1] <= spans[q + 1][0] for q in range(len(spans) - 1))
+        assert all(0 <= o and e <= a.capacity for o, e in spans)
+        assert a.used == sum(live.values())
+
+
+def test_submit_rejects_a_log_larger_than_the_arena():
+    s = Server(log_elements=1000)  # not started: no device work
+    j = workloads.pimm_junction()
+    with pytest.raises(ValueError, match="exceeds"):
+        s.submit(j, 1e-9, 1e-13, 1e-13, log="mx")  # 3 x 10000 elements
+
+
+def test_serving_mix_covers_the_request_kinds():
+    mix = workloads.serving_mix(200, seed=2)
+    kinds = {m[0] for m in mix}
+    assert kinds == {"pimm", "vsd", "cims", "chain3", "chain4"}
+    layers = {m[1].layerNo for m in mix}
+    assert layers == {1, 2, 3, 4}
+    for k, j, T, dt, wf, sm, log in mix:
+        assert 1e-9 <= T <= 50e-9 and 1e-14 <= dt <= 1e-13 + 1e-27 and wf >= dt
