@@ -364,7 +364,6 @@ class Plan:
         check(
             lib().cmtj_plan_execute(self._h, d_log, log_capacity, d_mag, d_status, stream),
             "cmtj_plan_execute",
-    "cmtj_plan_execute_slice",
         )
 
     def execute_slice(self, d_log: int | None, log_capacity: int, d_mag: int | None, d_status: int | None,

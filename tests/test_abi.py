@@ -70,3 +70,26 @@ def test_plan_arena_size_runs_without_a_gpu():
     bad["n_layers"][3] = 0
     with pytest.raises(RuntimeError):
         _lib.plan_arena_size(bad, b.layers, 0, b.precision, b.log_dtype)
+
+
+def test_plan_glue_passes_arguments(monkeypatch):
+    """Python side of the plan calls, against a stand-in library object."""
+    calls = []
+
+    class Fake:
+        def __getattr__(self, name):
+            def f(*a):
+                calls.append((name, a))
+                return 0
+            return f
+
+    p = _lib.Plan.__new__(_lib.Plan)
+    p._h = 1234
+    monkeypatch.setattr(_lib, "lib", lambda: Fake())
+    p.execute(1, 2, 3, 4, 5)
+    p.execute_slice(1, 2, 3, 4, 6, 100, [7, 8])
+    assert calls[0][0] == "cmtj_plan_execute" and calls[0][1] == (1234, 1, 2, 3, 4, 5)
+    name, a = calls[1]
+    assert name == "cmtj_plan_execute_slice" and a[:7] == (1234, 1, 2, 3, 4, 6, 100) and a[8] == 2
+    assert list(a[7]) == [7, 8]
+    p._h = None
