@@ -471,12 +471,16 @@ __device__ __forceinline__ DV<float> dv_diff(const DV<double>& a, const DV<doubl
 // Numbers of the first DREG records stay in registers for the whole run.
 constexpr int DREG = 2;
 
-// Field weights of the first DREG records: w[k][l] is the unit vector of
-// the axis that record k feeds in layer l (zero if it feeds none).  Built
-// once per lane from the wave-uniform record metadata.
+// Field routing of the first DREG records: axis[k] is the field axis record
+// k feeds (-1 if it feeds no field slot; wave-uniform), on[k][l] whether it
+// feeds layer l (uniform for the all-layers-in-a-lane kernels, per lane for
+// the one-layer-per-lane ones).  Values are routed by selects, never by
+// multiplying with a 0/1 weight: 0 * inf would turn a non-finite drive of
+// one layer into a NaN of its neighbours, which the reference never sees.
 template <class R, int LM>
 struct FieldWeights {
-  V3<R> w[DREG][LM];
+  int axis[DREG];
+  bool on[DREG][LM];
 };
 
 template <class R, int LM>
@@ -488,14 +492,22 @@ __device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const Gro
     const int slot = (meta >> 8) & 0xff;
     const int lmask = (meta >> 16) & 0xff;
     const bool isH = type != CMTJ_CONSTANT && slot < CMTJ_SLOT_K;
-    const int axis = slot % 3;
+    fw.axis[k] = isH ? slot % 3 : -1;
 #pragma unroll
-    for (int l = 0; l < LM; ++l) {
-      const bool on = isH && ((lmask >> (l + lsh)) & 1);
-      fw.w[k][l] = mk<R>((on && axis == 0) ? R(1) : R(0), (on && axis == 1) ? R(1) : R(0),
-                         (on && axis == 2) ? R(1) : R(0));
-    }
+    for (int l = 0; l < LM; ++l) fw.on[k][l] = isH && ((lmask >> (l + lsh)) & 1);
   }
+}
+
+// H[axis] += v where the record feeds this layer (axis is wave-uniform)
+template <class R>
+__device__ __forceinline__ void add_field(V3<R>& H, int axis, bool on, R v) {
+  const R vv = on ? v : R(0);
+  if (axis == 0)
+    H.x += vv;
+  else if (axis == 1)
+    H.y += vv;
+  else if (axis == 2)
+    H.z += vv;
 }
 
 __device__ __forceinline__ DynRec pick_rec(const DynRec (&dr)[DREG], const GroupArgs& g, int k, uint32_t lane) {
@@ -548,7 +560,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
         const int type = g.dmeta[k] & 0xff;
         const R v = wave_value<R>(type, dr[k], g, t, step, stage, sps);
 #pragma unroll
-        for (int l = 0; l < LM; ++l) dv[l].H = dv[l].H + fw.w[k][l] * v;
+        for (int l = 0; l < LM; ++l) add_field(dv[l].H, fw.axis[k], fw.on[k][l], v);
       }
     }
     for (int k = DREG; k < g.ndyn; ++k) {
@@ -590,7 +602,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
         const float v = (float)(wave_value<double>(type, dr[k], g, ts, step, stage, sps) -
                                 wave_value<double>(type, dr[k], g, t0, step, 0, sps));
 #pragma unroll
-        for (int l = 0; l < LM; ++l) dd[l].H = dd[l].H + fw.w[k][l] * v;
+        for (int l = 0; l < LM; ++l) add_field(dd[l].H, fw.axis[k], fw.on[k][l], v);
       }
     }
     for (int k = DREG; k < g.ndyn; ++k) {
