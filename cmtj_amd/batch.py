@@ -335,6 +335,7 @@ class JunctionBatch:
         p = self.plan()
         log, mag, st = self._buffers(device)
         init = np.zeros(len(self.js), dtype=_lib.STATE_DTYPE)
+        init["nan_layer"] = -1
         for i in range(len(self.js)):
             off, nl = int(self.js[i]["layer_offset"]), int(self.js[i]["n_layers"])
             init["m"][i, :nl] = self.layers["mag"][off: off + nl]

@@ -825,7 +825,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
     i = S->step;
     k = S->entries;
     code = S->code;
-    nan_layer = S->nan_layer;
+    nan_layer = (code == CMTJ_OK) ? -1 : S->nan_layer;
     iend = (code != CMTJ_OK) ? i : (uint32_t)min((uint64_t)nsteps, (uint64_t)i + g.slice);
   }
   uint64_t next_log = (uint64_t)k * W;

@@ -150,6 +150,7 @@ class Server:
             raise ValueError(f"request log ({n_elems} elements) exceeds the server's log arena "
                              f"({self.arena.capacity} elements)")
         st = np.zeros(1, dtype=_lib.STATE_DTYPE)
+        st["nan_layer"] = -1
         st["m"][0, : L.size] = L["mag"]
         fut: Future = Future()
         r = _Req(jd, L.copy(), tabs, meta, n_elems, fut, time.perf_counter(), state=st)

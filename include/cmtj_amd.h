@@ -194,7 +194,8 @@ typedef struct cmtj_status {
 
 /* Resumable state of one instance (the serving loop's slots): the
  * magnetisation of every layer, steps done, entries written and the status
- * code.  A fresh instance has m = its layers' initial m and zeros elsewhere. */
+ * code.  A fresh instance has m = its layers' initial m, nan_layer = -1 and
+ * zeros elsewhere. */
 typedef struct cmtj_state {
   double m[CMTJ_MAX_LAYERS][3];
   uint32_t step;
