@@ -65,7 +65,8 @@ static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, b
       const KEntry*& b = best[k.lpi > 1];
       if (!b || k.lm < b->lm || (k.lm == b->lm && __builtin_popcount(k.fm) < __builtin_popcount(b->fm))) b = &k;
     }
-  if (nl < 2 || lanes == CMTJ_LANES_ONE || !best[1]) return best[0];
+  if (!best[0] || !best[1]) return best[0] ? best[0] : best[1];  // the only kernel that covers the group
+  if (nl < 2 || lanes == CMTJ_LANES_ONE) return best[0];
   if (lanes == CMTJ_LANES_PER_LAYER) return best[1];
   const uint64_t waves = (n_inst * (uint64_t)best[1]->lpi + 63) / 64;
   return waves <= kAutoLaneWaves ? best[1] : best[0];

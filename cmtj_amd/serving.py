@@ -163,6 +163,10 @@ class Server:
 
     def start(self) -> "Server":
         if self._thread is None:
+            import torch
+
+            d = torch.device(self.device)
+            self.device = torch.device(d.type, d.index if d.index is not None else torch.cuda.current_device())
             self._thread = threading.Thread(target=self._loop, name="cmtj-server", daemon=True)
             self._thread.start()
         return self
@@ -278,7 +282,7 @@ class Server:
         import torch
 
         try:
-            torch.cuda.set_device(torch.device(self.device))
+            torch.cuda.set_device(self.device)
             dtype = torch.float32 if self.log_dtype == _lib.LOG_F32 else torch.float64
             log = torch.empty(self.arena.capacity, dtype=dtype, device=self.device)
             streams = [torch.cuda.Stream(device=self.device) for _ in range(self.n_streams)]
