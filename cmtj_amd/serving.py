@@ -216,8 +216,8 @@ class Server:
                     d["table_index"][sel] += tab
                 tabs.append(r.tables)
                 tab += r.tables.size
-            js[i]["layer_offset"] = row
-            js[i]["log_offset"] = r.log_off
+            js["layer_offset"][i] = row
+            js["log_offset"][i] = r.log_off
             row += L.size
             layers.append(L)
         layers = np.concatenate(layers)
