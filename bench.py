@@ -73,7 +73,7 @@ def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
     rng = np.random.default_rng(seed + 1)
     mix = workloads.serving_mix(int(qps * seconds * 1.5) + 16, seed=seed)
     futs, kinds = [], []
-    with Server(precision="fp32", slice_steps=20000, n_streams=4, log_elements=1 << 28) as srv:
+    with Server(precision="fp32", slice_ms=20.0, n_streams=8, log_elements=1 << 28) as srv:
         t0 = time.perf_counter()
         nxt = t0
         q = 0
@@ -91,6 +91,7 @@ def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
                 time.sleep(min(nxt - now, 0.005))
         res = [f.result() for f in futs]
         ticks = srv.ticks
+        host_s = dict(srv.host_s)
     lat = np.array([r.latency for r in res])
     steps = sum(r.steps for r in res)
     wall = max(r.t_done for r in res) - t0
@@ -98,8 +99,8 @@ def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
             "mix": {k: kinds.count(k) for k in sorted(set(kinds))},
             "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "max": float(lat.max())},
-            "junction_steps": int(steps), "junction_steps_per_s": steps / wall, "wall_s": wall, "ticks": ticks,
-            "slice_steps": 20000, "precision": "fp32", "lanes": "per_layer"}
+            "junction_steps": int(steps), "junction_steps_per_s": steps / wall, "wall_s": wall,
+            "slices": ticks, "slice_ms_target": 20.0, "host_s": host_s, "precision": "fp32", "lanes": "per_layer"}
 
 
 def main() -> int:

@@ -183,6 +183,7 @@ EXPORTS = (
     "cmtj_run",
     "cmtj_plan_create",
     "cmtj_plan_arena_size",
+    "cmtj_group_key",
     "cmtj_plan_create_in",
     "cmtj_plan_execute",
     "cmtj_plan_execute_slice",
@@ -224,6 +225,7 @@ def _load():
         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_size_t, _P,
     ]
     lib.cmtj_plan_create_in.restype = _P
+    lib.cmtj_group_key.argtypes = [_P, _P, ctypes.POINTER(ctypes.c_uint64)]
     lib.cmtj_plan_execute.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P]
     lib.cmtj_plan_execute_slice.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.c_uint32,
                                             ctypes.POINTER(_P), ctypes.c_int32]
@@ -330,6 +332,16 @@ def plan_arena_size(js, layers, n_table: int, precision: int, log_dtype: int, la
     out = ctypes.c_size_t(0)
     check(lib().cmtj_plan_arena_size(ptr(js), js.size, ptr(layers), layers.size, n_table, precision, log_dtype,
                                      lanes_per_instance, ctypes.byref(out)), "cmtj_plan_arena_size")
+    return int(out.value)
+
+
+def group_key(jd, layers) -> int:
+    """Structure-signature key of one junction descriptor (jd: one record,
+    layers: its layer rows)."""
+    jd = np.ascontiguousarray(np.asarray(jd, dtype=JUNCTION_DTYPE).reshape(1))
+    layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
+    out = ctypes.c_uint64(0)
+    check(lib().cmtj_group_key(ptr(jd), ptr(layers), ctypes.byref(out)), "cmtj_group_key")
     return int(out.value)
 
 

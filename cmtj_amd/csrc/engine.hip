@@ -220,6 +220,15 @@ typedef void* (*ArenaFn)(size_t bytes, void* ctx);
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Structure signature of a junction: instances with equal signatures share a
+// kernel launch and every structural branch inside it.
+static void signature(const cmtj_junction& j, const cmtj_layer* L, const std::vector<Rec>& recs,
+                      std::vector<int32_t>& sig) {
+  sig.assign({j.n_layers, j.solver, j.log_mask, j.mr_mode, (int32_t)junction_flags(j, L)});
+  for (int l = 0; l < j.n_layers; ++l) sig.push_back(layer_bits(L[l]));
+  for (const Rec& r : recs) sig.push_back(r.meta);
+}
+
 // Builds every group of a plan into one device arena obtained from `get`.
 // A group holds the junctions of one structure signature (layer count,
 // solver, log layout, per-layer modes, driver kinds and slots), so every
@@ -256,9 +265,7 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     junction_records(j, L, recs_of[i]);
     if (recs_of[i].size() > (size_t)MAX_DYN)
       return fail(CMTJ_ERR_INVALID, "more than 16 time-dependent drivers in one junction");
-    sig.assign({j.n_layers, j.solver, j.log_mask, j.mr_mode, (int32_t)junction_flags(j, L)});
-    for (int l = 0; l < j.n_layers; ++l) sig.push_back(layer_bits(L[l]));
-    for (const Rec& r : recs_of[i]) sig.push_back(r.meta);
+    signature(j, L, recs_of[i], sig);
     size_t g = 0;
     for (; g < sigs.size(); ++g)
       if (sigs[g] == sig) break;
@@ -574,6 +581,26 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n, const cmtj_layer*
     return nullptr;
   }
   return P;
+}
+
+int cmtj_group_key(const cmtj_junction* j, const cmtj_layer* layers, uint64_t* key) {
+  if (!j || !layers || !key) return fail(CMTJ_ERR_INVALID, "null pointer");
+  cmtj_junction jj = *j;
+  jj.layer_offset = 0;
+  const int r = validate(jj, 0, layers, (size_t)std::max(jj.n_layers, 0), CMTJ_FP64, SIZE_MAX);
+  if (r) return r;
+  std::vector<Rec> recs;
+  junction_records(jj, layers, recs);
+  std::vector<int32_t> sig;
+  signature(jj, layers, recs, sig);
+  uint64_t h = 1469598103934665603ull;  // FNV-1a
+  for (int32_t v : sig)
+    for (int b = 0; b < 4; ++b) {
+      h ^= (uint64_t)(((uint32_t)v >> (8 * b)) & 0xff);
+      h *= 1099511628211ull;
+    }
+  *key = h;
+  return 0;
 }
 
 int cmtj_plan_arena_size(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,

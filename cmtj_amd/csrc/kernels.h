@@ -522,23 +522,42 @@ __device__ __forceinline__ DynRec pick_rec(const DynRec (&dr)[DREG], const Group
   return r;
 }
 
-template <class R>
+__device__ __forceinline__ bool transcendental(int type) {
+  return type == CMTJ_SINE || type == CMTJ_POSINE || type == CMTJ_HALFSINE || type == CMTJ_GAUSSIMPULSE ||
+         type == CMTJ_GAUSSSTEP;
+}
+
+// Value of one time-dependent record at t.  FT: binary32 transcendentals
+// (phase and exponent argument still formed in binary64) for the anchored
+// FP32 mode, whose stage increments need the drive only to ~1e-7 relative.
+template <class R, bool FT = false>
 __device__ __forceinline__ R wave_value(int type, const DynRec& r, const GroupArgs& g, double t, uint32_t step,
                                         int stage, int sps) {
   if (type == CMTJ_TABLE) return (R)g.tables[(size_t)r.tidx + (size_t)step * sps + stage];
   if (sizeof(R) == 4) return (R)wave_part_f(type, r.A, r.p1, r.p2, r.p3, t);
+  if (FT && transcendental(type)) return (R)wave_part_f_trans(type, r.A, r.p1, r.p2, t);
   return (R)wave_part(type, r.A, r.p1, r.p2, r.p3, t);
+}
+
+// v0[k] = v for the first DREG records (k wave-uniform, possibly a runtime index)
+__device__ __forceinline__ void keep_v0(double* v0, int k, double v) {
+  if (!v0) return;
+#pragma unroll
+  for (int q = 0; q < DREG; ++q)
+    if (q == k) v0[q] = v;
 }
 
 // Driver values at time t: the per-layer base (constant parts) plus every
 // time-dependent record.  Kinds, slots and layer masks are wave-uniform.
-// Records that feed only field slots take the fast path (one FMA per axis
-// with a uniform weight); other slots go through add_slot.
-template <class R, int LM, uint32_t FM, bool LOGE>
+// Records that feed only field slots take the fast path (a select and an
+// add on the uniform axis); other slots go through add_slot.  v0 (optional)
+// receives the values of the first DREG records.
+template <class R, int LM, uint32_t FM, bool LOGE, bool FT = false>
 __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base)[LM],
                                              const DynRec (&dr)[DREG], const FieldWeights<R, LM>& fw,
                                              const GroupArgs& g, uint32_t lane, uint32_t fl, double t,
-                                             uint32_t step, int stage, int sps, int lsh = 0) {
+                                             uint32_t step, int stage, int sps, int lsh = 0,
+                                             double* v0 = nullptr) {
 #pragma unroll
   for (int l = 0; l < LM; ++l) dv[l] = base[l];
   if (CMTJ_HAS(F_DYN) || (LOGE && CMTJ_HAS(F_DYNH))) {
@@ -548,7 +567,8 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
       const int slot = (meta >> 8) & 0xff;
       const int lmask = (meta >> 16) & 0xff;
       const DynRec r = pick_rec(dr, g, k, lane);
-      const R v = wave_value<R>(type, r, g, t, step, stage, sps);
+      const R v = wave_value<R, FT>(type, r, g, t, step, stage, sps);
+      keep_v0(v0, k, (double)v);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
         if ((lmask >> (l + lsh)) & 1) add_slot<R, LOGE>(dv[l], slot, v);
@@ -558,7 +578,8 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
     for (int k = 0; k < DREG; ++k) {
       if (k < g.ndyn) {
         const int type = g.dmeta[k] & 0xff;
-        const R v = wave_value<R>(type, dr[k], g, t, step, stage, sps);
+        const R v = wave_value<R, FT>(type, dr[k], g, t, step, stage, sps);
+        if (v0) v0[k] = (double)v;
 #pragma unroll
         for (int l = 0; l < LM; ++l) add_field(dv[l].H, fw.axis[k], fw.on[k][l], v);
       }
@@ -566,7 +587,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
     for (int k = DREG; k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
       const DynRec r = pick_rec(dr, g, k, lane);
-      const R v = wave_value<R>(meta & 0xff, r, g, t, step, stage, sps);
+      const R v = wave_value<R, FT>(meta & 0xff, r, g, t, step, stage, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
         if (((meta >> 16) >> (l + lsh)) & 1) add_slot<R, false>(dv[l], (meta >> 8) & 0xff, v);
@@ -574,13 +595,28 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
   }
 }
 
-// Driver increments dd = dv(ts) - dv(t0) of the time-dependent records
-// (binary64 difference, one rounding to binary32); constant parts cancel.
+// Record k's value at t0: from v0 for the first DREG records, else recomputed.
+__device__ __forceinline__ double value_t0(const double (&v0)[DREG], int k, int type, const DynRec& r,
+                                           const GroupArgs& g, double t0, uint32_t step, int sps) {
+  if (k < DREG) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < DREG; ++q)
+      if (q == k) v = v0[q];
+    return v;
+  }
+  return wave_value<double, true>(type, r, g, t0, step, 0, sps);
+}
+
+// Driver increments dd = dv(ts) - dv(t0) of the time-dependent records of
+// the anchored FP32 mode (binary64 difference, one rounding to binary32;
+// constant parts cancel).  v0: the record values at t0 from eval_drivers.
 template <int LM, uint32_t FM>
 __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const DynRec (&dr)[DREG],
                                                    const FieldWeights<float, LM>& fw, const GroupArgs& g,
                                                    uint32_t lane, uint32_t fl, double t0, double ts,
-                                                   uint32_t step, int stage, int sps, int lsh = 0) {
+                                                   uint32_t step, int stage, int sps, const double (&v0)[DREG],
+                                                   int lsh = 0) {
 #pragma unroll
   for (int l = 0; l < LM; ++l) dv_zero(dd[l]);
   if (CMTJ_HAS(F_DYN)) {
@@ -588,8 +624,8 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
       const int meta = g.dmeta[k];
       const int type = meta & 0xff;
       const DynRec r = pick_rec(dr, g, k, lane);
-      const double v = wave_value<double>(type, r, g, ts, step, stage, sps) -
-                       wave_value<double>(type, r, g, t0, step, 0, sps);
+      const double v = wave_value<double, true>(type, r, g, ts, step, stage, sps) -
+                       value_t0(v0, k, type, r, g, t0, step, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
         if (((meta >> 16) >> (l + lsh)) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
@@ -599,8 +635,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
     for (int k = 0; k < DREG; ++k) {
       if (k < g.ndyn) {
         const int type = g.dmeta[k] & 0xff;
-        const float v = (float)(wave_value<double>(type, dr[k], g, ts, step, stage, sps) -
-                                wave_value<double>(type, dr[k], g, t0, step, 0, sps));
+        const float v = (float)(wave_value<double, true>(type, dr[k], g, ts, step, stage, sps) - v0[k]);
 #pragma unroll
         for (int l = 0; l < LM; ++l) add_field(dd[l].H, fw.axis[k], fw.on[k][l], v);
       }
@@ -608,8 +643,8 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
     for (int k = DREG; k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
       const DynRec r = pick_rec(dr, g, k, lane);
-      const double v = wave_value<double>(meta & 0xff, r, g, ts, step, stage, sps) -
-                       wave_value<double>(meta & 0xff, r, g, t0, step, 0, sps);
+      const double v = wave_value<double, true>(meta & 0xff, r, g, ts, step, stage, sps) -
+                       wave_value<double, true>(meta & 0xff, r, g, t0, step, 0, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
         if (((meta >> 16) >> (l + lsh)) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
@@ -846,7 +881,8 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
         nbT[l] = cvt<float>(nbTd[l]);
       }
       DV<double> dv0[LM];
-      eval_drivers<double, LM, FM, false>(dv0, based, dr, fwd, g, lane, fl, t, i, 0, sps);
+      double v0[DREG];
+      eval_drivers<double, LM, FM, false, true>(dv0, based, dr, fwd, g, lane, fl, t, i, 0, sps, 0, v0);
       V3<double> k1[LM];
       Anchor A[LM];
       ERec<double> erd;
@@ -859,7 +895,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
         }
       }
       DV<float> dd[LM];
-      eval_driver_deltas<LM, FM>(dd, dr, fwf, g, lane, fl, t, t + 0.5 * dt, i, 1, sps);
+      eval_driver_deltas<LM, FM>(dd, dr, fwf, g, lane, fl, t, t + 0.5 * dt, i, 1, sps, v0);
       V3<float> acc[LM], dk[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
@@ -873,7 +909,7 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
           acc[l] = acc[l] + dk[l] * 2.f;
         }
       }
-      eval_driver_deltas<LM, FM>(dd, dr, fwf, g, lane, fl, t, t + dt, i, 2, sps);
+      eval_driver_deltas<LM, FM>(dd, dr, fwf, g, lane, fl, t, t + dt, i, 2, sps, v0);
       V3<double> mn[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {

@@ -211,13 +211,14 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
       // anchored binary32 RK4, as in k_junction
       const V3<float> nbB = cvt<float>(bot), nbT = cvt<float>(top);
       DV<double> dv0[1];
-      eval_drivers<double, 1, FM, false>(dv0, based, dr, fwd, g, inst, fl, t, i, 0, sps, lyr);
+      double v0[DREG];
+      eval_drivers<double, 1, FM, false, true>(dv0, based, dr, fwd, g, inst, fl, t, i, 0, sps, lyr, v0);
       Anchor A;
       ERec<double> erd;
       const V3<double> k1 = rhs<double, FM, false, true>(Pd, bits, fl, m, dv0[0], bot, top, erd, &A) * dt;
       const DV<float> ds0 = dv_cvt<float>(dv0[0]);
       DV<float> dd[1];
-      eval_driver_deltas<1, FM>(dd, dr, fwf, g, inst, fl, t, t + 0.5 * dt, i, 1, sps, lyr);
+      eval_driver_deltas<1, FM>(dd, dr, fwf, g, inst, fl, t, t + 0.5 * dt, i, 1, sps, v0, lyr);
       V3<float> acc, dk;
       {
         const DV<float> ds = dv_add(ds0, dd[0]);
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
         dk = rhs_delta<FM>(P, bits, fl, A, d3, dd[0], ds, nbB, nbT) * dtR;
         acc = acc + dk * 2.f;
       }
-      eval_driver_deltas<1, FM>(dd, dr, fwf, g, inst, fl, t, t + dt, i, 2, sps, lyr);
+      eval_driver_deltas<1, FM>(dd, dr, fwf, g, inst, fl, t, t + dt, i, 2, sps, v0, lyr);
       {
         const DV<float> ds = dv_add(ds0, dd[0]);
         const V3<float> d4 = cvt<float>(k1) + dk;

@@ -92,26 +92,44 @@ class _Req:
     tables: np.ndarray | None
     meta: _Meta
     n_elems: int
+    key: int
     future: Future
     t_submit: float
     t_admit: float = 0.0
     log_off: int = -1
-    state: np.ndarray | None = None  # initial state row
-    slot_rows: int = 0
-    extra: dict = field(default_factory=dict)
+    state: np.ndarray | None = None  # initial state row until the request joins its group's plan
+
+
+class _Group:
+    """Requests of one structure signature: one plan, one state array and
+    one stream; advances independently of the other groups."""
+
+    def __init__(self, key: int, stream, slice_steps: int) -> None:
+        self.key = key
+        self.stream = stream
+        self.slice_steps = slice_steps
+        self.active: list = []
+        self.incoming: list = []
+        self.plan = self.arena = self.state = self.status = None
+        self.index: dict = {}
+        self.changed = False
+        self.busy = False
+        self.ev0 = self.ev1 = None
+        self.slices = 0
 
 
 class Server:
     """Continuous-batching request loop (see module docstring).
 
     precision: "fp32" or "fp64"; log_elements: capacity of the device log
-    arena (elements of the log dtype); max_active: slots; slice_steps: steps
-    per tick; lanes: fixed lane mapping ("per_layer" or "one"); n_streams:
-    streams the groups of a tick are spread over."""
+    arena (elements of the log dtype); max_active: slots; slice_ms: target
+    device time of one slice (each group adapts its step count to it);
+    lanes: fixed lane mapping ("per_layer" or "one"); n_streams: streams
+    the groups are spread over."""
 
     def __init__(self, precision: str = "fp32", log_elements: int = 1 << 28, max_active: int = 65536,
-                 slice_steps: int = 20000, lanes: str = "per_layer", n_streams: int = 4, log_dtype=None,
-                 device: str = "cuda") -> None:
+                 slice_ms: float = 20.0, slice_steps: int = 5000, lanes: str = "per_layer", n_streams: int = 8,
+                 log_dtype=None, device: str = "cuda") -> None:
         if lanes not in ("per_layer", "one"):
             raise ValueError("a server fixes the lane mapping: 'per_layer' or 'one'")
         self.precision = _lib.FP32 if precision in ("fp32", _lib.FP32) else _lib.FP64
@@ -121,16 +139,19 @@ class Server:
         self.lanes = _LANES[lanes]
         self.arena = LogArena(log_elements)
         self.max_active = int(max_active)
-        self.slice_steps = int(slice_steps)
+        self.slice_ms = float(slice_ms)
+        self.slice_steps = int(slice_steps)  # first slice of a new group
         self.n_streams = max(1, int(n_streams))
         self.device = device
         self._pending: deque = deque()
-        self._active: list = []
+        self._groups: dict = {}
+        self._n_active = 0
         self._cv = threading.Condition()
         self._stop = False
         self._thread = None
         self.ticks = 0
         self.steps_done = 0
+        self.host_s = {"admit": 0.0, "rebuild": 0.0, "retire": 0.0, "launch": 0.0}
         self.error = None
 
     # --- client side ------------------------------------------------------------
@@ -149,11 +170,12 @@ class Server:
         if n_elems > self.arena.capacity:
             raise ValueError(f"request log ({n_elems} elements) exceeds the server's log arena "
                              f"({self.arena.capacity} elements)")
+        key = _lib.group_key(jd, L)
         st = np.zeros(1, dtype=_lib.STATE_DTYPE)
         st["nan_layer"] = -1
         st["m"][0, : L.size] = L["mag"]
         fut: Future = Future()
-        r = _Req(jd, L.copy(), tabs, meta, n_elems, fut, time.perf_counter(), state=st)
+        r = _Req(jd, L.copy(), tabs, meta, n_elems, key, fut, time.perf_counter(), state=st)
         with self._cv:
             if self._stop:
                 raise RuntimeError("server is stopped")
@@ -186,28 +208,38 @@ class Server:
         self.stop()
 
     # --- the loop ------------------------------------------------------------------
-    def _admit(self) -> bool:
-        changed = False
-        while self._pending and len(self._active) < self.max_active:
+    def _admit(self, torch, streams) -> None:
+        """FIFO admission while the log arena and the slots allow (no
+        overtaking: a request that does not fit holds the ones behind it)."""
+        while self._pending and self._n_active < self.max_active:
             r = self._pending[0]
             off = self.arena.alloc(r.n_elems)
             if off is None:
-                break  # wait for memory to free up (FIFO: no overtaking)
+                break
             self._pending.popleft()
             r.log_off = off
             r.t_admit = time.perf_counter()
-            self._active.append(r)
-            changed = True
-        return changed
+            g = self._groups.get(r.key)
+            if g is None:
+                g = _Group(r.key, streams[len(self._groups) % len(streams)], self.slice_steps)
+                self._groups[r.key] = g
+            g.incoming.append(r)
+            self._n_active += 1
 
-    def _rebuild(self, torch, old_state, old_index):
-        """Plan + state of the current active set; carried-over requests keep
-        their device state rows."""
-        n = len(self._active)
-        js = np.concatenate([r.jd for r in self._active])
+    def _rebuild(self, torch, g: _Group) -> None:
+        """Plan + state of the group's active set; carried-over requests keep
+        their device state rows.  Runs on the group's (idle) stream."""
+        g.active = g.active + g.incoming
+        g.incoming = []
+        n = len(g.active)
+        if n == 0:
+            g.plan = g.arena = g.state = g.status = None
+            g.index = {}
+            return
+        js = np.concatenate([r.jd for r in g.active])
         layers, tabs = [], []
         row = tab = 0
-        for i, r in enumerate(self._active):
+        for i, r in enumerate(g.active):
             L = r.layers.copy()
             if r.tables is not None:
                 for q in range(L.size):
@@ -224,43 +256,44 @@ class Server:
         tables = np.concatenate(tabs) if tabs else None
         nbytes = _lib.plan_arena_size(js, layers, 0 if tables is None else tables.size, self.precision,
                                       self.log_dtype, self.lanes)
-        arena = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
-        plan = _lib.Plan(js, layers, _lib.constants_get(), tables, self.precision, self.log_dtype, self.lanes,
-                         arena=(arena.data_ptr(), nbytes, torch.cuda.current_stream().cuda_stream))
         sz = _lib.STATE_DTYPE.itemsize
-        state = torch.empty((n, sz), dtype=torch.uint8, device=self.device)
-        fresh = [i for i, r in enumerate(self._active) if r.state is not None]
-        kept = [i for i, r in enumerate(self._active) if r.state is None]
-        if kept:
-            src = torch.as_tensor([old_index[id(self._active[i])] for i in kept], device=self.device)
-            state[torch.as_tensor(kept, device=self.device)] = old_state.index_select(0, src)
-        if fresh:
-            rows = np.concatenate([self._active[i].state for i in fresh]).view(np.uint8).reshape(len(fresh), sz)
-            state[torch.as_tensor(fresh, device=self.device)] = torch.as_tensor(rows).to(self.device)
-            for i in fresh:
-                self._active[i].state = None
-        index = {id(r): i for i, r in enumerate(self._active)}
-        status = torch.zeros((n, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, device=self.device)
-        return plan, arena, state, index, status
+        with torch.cuda.stream(g.stream):
+            arena = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            plan = _lib.Plan(js, layers, _lib.constants_get(), tables, self.precision, self.log_dtype, self.lanes,
+                             arena=(arena.data_ptr(), nbytes, g.stream.cuda_stream))
+            state = torch.empty((n, sz), dtype=torch.uint8, device=self.device)
+            fresh = [i for i, r in enumerate(g.active) if r.state is not None]
+            kept = [i for i, r in enumerate(g.active) if r.state is None]
+            if kept:
+                src = torch.as_tensor([g.index[id(g.active[i])] for i in kept], device=self.device)
+                state[torch.as_tensor(kept, device=self.device)] = g.state.index_select(0, src)
+            if fresh:
+                rows = np.concatenate([g.active[i].state for i in fresh]).view(np.uint8).reshape(len(fresh), sz)
+                state[torch.as_tensor(fresh, device=self.device)] = torch.as_tensor(rows).to(self.device)
+                for i in fresh:
+                    g.active[i].state = None
+            g.status = torch.zeros((n, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, device=self.device)
+        g.plan, g.arena, g.state = plan, arena, state
+        g.index = {id(r): i for i, r in enumerate(g.active)}
 
-    def _retire(self, torch, log, state, status_host):
-        keep, done = [], []
-        for i, r in enumerate(self._active):
-            st = status_host[i]
-            if int(st["code"]) != _lib.OK or int(st["steps_done"]) >= int(r.jd["n_steps"][0]):
-                done.append((i, r, st))
-            else:
-                keep.append(r)
-        if not done:
-            return False
-        states = state[torch.as_tensor([i for i, _, _ in done], device=self.device)].cpu().numpy()
-        states = states.view(_lib.STATE_DTYPE).reshape(-1)
-        for q, (i, r, st) in enumerate(done):
+    def _retire(self, torch, g: _Group, log) -> None:
+        with torch.cuda.stream(g.stream):
+            sh = g.status.cpu().numpy().view(_lib.STATUS_DTYPE).reshape(-1)
+            done = [i for i, r in enumerate(g.active)
+                    if int(sh[i]["code"]) != _lib.OK or int(sh[i]["steps_done"]) >= int(r.jd["n_steps"][0])]
+            if not done:
+                return
+            states = g.state[torch.as_tensor(done, device=self.device)].cpu().numpy()
+            states = states.view(_lib.STATE_DTYPE).reshape(-1)
+            blocks = [log[g.active[i].log_off: g.active[i].log_off + g.active[i].n_elems].double().cpu().numpy()
+                      for i in done]
+        t = time.perf_counter()
+        for q, i in enumerate(done):
+            r, st = g.active[i], sh[i]
             j = r.jd[0]
             names = _channel_names(r.meta, int(j["log_mask"]))
             ne = int(j["n_entries"])
-            blk = log[r.log_off: r.log_off + len(names) * ne].double().cpu().numpy().reshape(len(names), ne)
-            self.arena.free(r.log_off, r.n_elems)
+            blk = blocks[q][: len(names) * ne].reshape(len(names), ne)
             out = {k: blk[c] for c, k in enumerate(names)}
             if r.meta.mr_mode == _lib.MR_STRIP and int(j["log_mask"]) & _lib.LOG_MR:
                 out["Rz"] = np.zeros(ne)
@@ -274,9 +307,26 @@ class Server:
             else:
                 self.steps_done += int(st["steps_done"])
                 r.future.set_result(Result(out, states[q]["m"][:nl].copy(), int(st["steps_done"]), r.t_submit,
-                                           r.t_admit, time.perf_counter()))
-        self._active = keep
-        return True
+                                           r.t_admit, t))
+        with self._cv:
+            for i in done:
+                self.arena.free(g.active[i].log_off, g.active[i].n_elems)
+            self._n_active -= len(done)
+        gone = set(done)
+        g.active = [r for i, r in enumerate(g.active) if i not in gone]
+        g.changed = True
+
+    def _launch(self, torch, g: _Group, log) -> None:
+        if g.ev1 is None:
+            g.ev0 = torch.cuda.Event(enable_timing=True)
+            g.ev1 = torch.cuda.Event(enable_timing=True)
+        g.ev0.record(g.stream)
+        g.plan.execute_slice(log.data_ptr(), log.numel(), None, g.status.data_ptr(), g.state.data_ptr(),
+                             g.slice_steps, [g.stream.cuda_stream])
+        g.ev1.record(g.stream)
+        g.busy = True
+        g.slices += 1
+        self.ticks += 1
 
     def _loop(self) -> None:
         import torch
@@ -285,37 +335,60 @@ class Server:
             torch.cuda.set_device(self.device)
             dtype = torch.float32 if self.log_dtype == _lib.LOG_F32 else torch.float64
             log = torch.empty(self.arena.capacity, dtype=dtype, device=self.device)
+            torch.cuda.synchronize()
             streams = [torch.cuda.Stream(device=self.device) for _ in range(self.n_streams)]
-            sptr = [s.cuda_stream for s in streams]
-            plan = arena = state = status = None
-            index: dict = {}
-            changed = False
+            clk = time.perf_counter
             while True:
+                t0 = clk()
                 with self._cv:
-                    changed |= self._admit()
-                    if not self._active:
-                        if self._stop and not self._pending:
+                    self._admit(torch, streams)
+                    if not self._groups and not self._pending:
+                        if self._stop:
                             return
                         self._cv.wait(timeout=0.05)
                         continue
-                if changed:
-                    torch.cuda.synchronize()
-                    plan, arena, state, index, status = self._rebuild(torch, state, index)
-                    changed = False
-                plan.execute_slice(log.data_ptr(), log.numel(), None, status.data_ptr(), state.data_ptr(),
-                                   self.slice_steps, sptr)
-                torch.cuda.synchronize()
-                self.ticks += 1
-                sh = status.cpu().numpy().view(_lib.STATUS_DTYPE).reshape(-1)
-                with self._cv:
-                    changed |= self._retire(torch, log, state, sh)
+                self.host_s["admit"] += clk() - t0
+                progressed = False
+                for key in list(self._groups):
+                    g = self._groups[key]
+                    if g.busy:
+                        if not g.ev1.query():
+                            continue
+                        g.busy = False
+                        ms = g.ev0.elapsed_time(g.ev1)
+                        if ms > 0:  # steer the next slice towards slice_ms of device time
+                            want = g.slice_steps * self.slice_ms / ms
+                            g.slice_steps = int(min(max(want, 0.5 * g.slice_steps, 500), 2.0 * g.slice_steps,
+                                                    2_000_000))
+                        t0 = clk()
+                        self._retire(torch, g, log)
+                        self.host_s["retire"] += clk() - t0
+                        progressed = True
+                    if g.incoming or g.changed:
+                        t0 = clk()
+                        self._rebuild(torch, g)
+                        g.changed = False
+                        self.host_s["rebuild"] += clk() - t0
+                    if g.active:
+                        t0 = clk()
+                        self._launch(torch, g, log)
+                        self.host_s["launch"] += clk() - t0
+                        progressed = True
+                    elif not g.incoming:
+                        del self._groups[key]
+                if not progressed:
+                    time.sleep(0.0005)
         except BaseException as e:  # fail every open request rather than hang its caller
             self.error = e
             with self._cv:
-                for r in list(self._active) + list(self._pending):
+                for g in self._groups.values():
+                    for r in g.active + g.incoming:
+                        if not r.future.done():
+                            r.future.set_exception(e)
+                for r in self._pending:
                     if not r.future.done():
                         r.future.set_exception(e)
-                self._active, self._pending = [], deque()
+                self._groups, self._pending = {}, deque()
 
 
 __all__ = ["Server", "Result", "LogArena"]

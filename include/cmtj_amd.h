@@ -302,6 +302,13 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n,
 int cmtj_plan_execute(cmtj_plan* p, void* d_log, size_t log_capacity,
                       double* d_mag_out, cmtj_status* d_status, void* stream);
 
+/* 64-bit key of the junction's structure signature (layer count, solver,
+ * log layout, resistance mode, per-layer modes, feature set, time-dependent
+ * driver kinds and slots): instances with equal keys run in one kernel
+ * launch.  No reference counterpart; lets a serving loop keep the requests of
+ * one group together.  Host-only. */
+int cmtj_group_key(const cmtj_junction* j, const cmtj_layer* layers, uint64_t* key);
+
 /* Device bytes a plan of these instances needs (the same validation as
  * cmtj_plan_create; tables are not read, only counted).  No reference
  * counterpart: sizes the caller-owned arena of cmtj_plan_create_in. */

@@ -63,9 +63,20 @@ def test_heterogeneous_fp64_batch_matches_oracle(lanes):
 
 
 @pytest.mark.parametrize("lanes", ["one", "per_layer"])
-@pytest.mark.parametrize("builder", ["r_h_loop", "pimm", "sto"])
+def _vsd_sweep(p, n=64):
+    b = JunctionBatch.replicate(workloads.vsd_junction(), n, 2e-9, 1e-13, 1e-12, SolverMode.RK4, precision=p,
+                                log="full")
+    f = np.zeros((n, 3))
+    f[:, 1] = np.linspace(1e9, 20e9, n)
+    b.set_driver("free", "oersted", "frequency", f)
+    return b
+
+
+@pytest.mark.parametrize("builder", ["r_h_loop", "pimm", "sto", "vsd"])
 def test_fp32_bulk_within_1e4_of_fp64(builder, lanes):
-    if builder == "r_h_loop":
+    if builder == "vsd":
+        mk = _vsd_sweep
+    elif builder == "r_h_loop":
         mk = lambda p: workloads.r_h_loop(n_fields=101, total_time=2e-9, precision=p, log="full")
     elif builder == "pimm":
         mk = lambda p: workloads.pimm(n_points=256, t_min=1e-9, t_max=3e-9, precision=p, log="mx",

@@ -58,3 +58,18 @@ def test_serving_mix_covers_the_request_kinds():
     assert layers == {1, 2, 3, 4}
     for k, j, T, dt, wf, sm, log in mix:
         assert 1e-9 <= T <= 50e-9 and 1e-14 <= dt <= 1e-13 + 1e-27 and wf >= dt
+
+
+def test_group_key_follows_the_structure():
+    from cmtj_amd import _lib
+    from cmtj_amd.batch import JunctionBatch
+
+    def key(j, T=1e-9, dt=1e-13):
+        jd, L, _, _ = JunctionBatch._pack(j, T, dt, 1e-12, workloads.SolverMode.RK4)
+        return _lib.group_key(jd, L)
+
+    a, b = workloads.pimm_junction(), workloads.pimm_junction()
+    b.setLayerExternalFieldDriver("all", workloads.AxialDriver(1e5, 0, 0))  # numbers only
+    assert key(a) == key(b) == key(a, T=2e-9, dt=2e-13)
+    assert key(a) != key(workloads.chain_junction(3)) != key(workloads.chain_junction(4))
+    assert key(workloads.vsd_junction()) != key(a)  # sine drive, one layer
