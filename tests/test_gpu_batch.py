@@ -62,7 +62,6 @@ def test_heterogeneous_fp64_batch_matches_oracle(lanes):
     np.testing.assert_allclose(r.final_m.cpu().numpy(), omag, atol=1e-8)
 
 
-@pytest.mark.parametrize("lanes", ["one", "per_layer"])
 def _vsd_sweep(p, n=64):
     b = JunctionBatch.replicate(workloads.vsd_junction(), n, 2e-9, 1e-13, 1e-12, SolverMode.RK4, precision=p,
                                 log="full")
@@ -72,6 +71,7 @@ def _vsd_sweep(p, n=64):
     return b
 
 
+@pytest.mark.parametrize("lanes", ["one", "per_layer"])
 @pytest.mark.parametrize("builder", ["r_h_loop", "pimm", "sto", "vsd"])
 def test_fp32_bulk_within_1e4_of_fp64(builder, lanes):
     if builder == "vsd":
