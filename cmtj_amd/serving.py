@@ -111,6 +111,7 @@ class _Group:
         self.active: list = []
         self.incoming: list = []
         self.plan = self.arena = self.state = self.status = None
+        self.state_host = self.status_host = None  # pinned mirrors, filled after every slice
         self.index: dict = {}
         self.changed = False
         self.busy = False
@@ -227,13 +228,14 @@ class Server:
             self._n_active += 1
 
     def _rebuild(self, torch, g: _Group) -> None:
-        """Plan + state of the group's active set; carried-over requests keep
-        their device state rows.  Runs on the group's (idle) stream."""
+        """Plan + state of the group's active set.  Carried-over requests keep
+        their state (from the host mirror of the last slice), new ones start
+        from their initial row.  Runs when the group's stream is idle."""
         g.active = g.active + g.incoming
         g.incoming = []
         n = len(g.active)
         if n == 0:
-            g.plan = g.arena = g.state = g.status = None
+            g.plan = g.arena = g.state = g.status = g.state_host = g.status_host = None
             g.index = {}
             return
         js = np.concatenate([r.jd for r in g.active])
@@ -257,43 +259,50 @@ class Server:
         nbytes = _lib.plan_arena_size(js, layers, 0 if tables is None else tables.size, self.precision,
                                       self.log_dtype, self.lanes)
         sz = _lib.STATE_DTYPE.itemsize
+        host = torch.empty((n, sz), dtype=torch.uint8, pin_memory=True)
+        hv = host.numpy()
+        old = g.state_host.numpy() if g.state_host is not None else None
+        for i, r in enumerate(g.active):
+            if r.state is not None:
+                hv[i] = r.state.view(np.uint8).reshape(sz)
+                r.state = None
+            else:
+                hv[i] = old[g.index[id(r)]]
         with torch.cuda.stream(g.stream):
             arena = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
             plan = _lib.Plan(js, layers, _lib.constants_get(), tables, self.precision, self.log_dtype, self.lanes,
                              arena=(arena.data_ptr(), nbytes, g.stream.cuda_stream))
             state = torch.empty((n, sz), dtype=torch.uint8, device=self.device)
-            fresh = [i for i, r in enumerate(g.active) if r.state is not None]
-            kept = [i for i, r in enumerate(g.active) if r.state is None]
-            if kept:
-                src = torch.as_tensor([g.index[id(g.active[i])] for i in kept], device=self.device)
-                state[torch.as_tensor(kept, device=self.device)] = g.state.index_select(0, src)
-            if fresh:
-                rows = np.concatenate([g.active[i].state for i in fresh]).view(np.uint8).reshape(len(fresh), sz)
-                state[torch.as_tensor(fresh, device=self.device)] = torch.as_tensor(rows).to(self.device)
-                for i in fresh:
-                    g.active[i].state = None
+            state.copy_(host, non_blocking=True)
             g.status = torch.zeros((n, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, device=self.device)
-        g.plan, g.arena, g.state = plan, arena, state
+        g.status_host = torch.empty((n, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, pin_memory=True)
+        g.plan, g.arena, g.state, g.state_host = plan, arena, state, host
         g.index = {id(r): i for i, r in enumerate(g.active)}
 
     def _retire(self, torch, g: _Group, log) -> None:
+        """Called once the group's slice (and its status/state copies) are
+        done: hands finished requests their results."""
+        sh = g.status_host.numpy().view(_lib.STATUS_DTYPE).reshape(-1)
+        done = [i for i, r in enumerate(g.active)
+                if int(sh[i]["code"]) != _lib.OK or int(sh[i]["steps_done"]) >= int(r.jd["n_steps"][0])]
+        if not done:
+            return
+        states = g.state_host.numpy().view(_lib.STATE_DTYPE).reshape(-1)
         with torch.cuda.stream(g.stream):
-            sh = g.status.cpu().numpy().view(_lib.STATUS_DTYPE).reshape(-1)
-            done = [i for i, r in enumerate(g.active)
-                    if int(sh[i]["code"]) != _lib.OK or int(sh[i]["steps_done"]) >= int(r.jd["n_steps"][0])]
-            if not done:
-                return
-            states = g.state[torch.as_tensor(done, device=self.device)].cpu().numpy()
-            states = states.view(_lib.STATE_DTYPE).reshape(-1)
-            blocks = [log[g.active[i].log_off: g.active[i].log_off + g.active[i].n_elems].double().cpu().numpy()
-                      for i in done]
+            blocks = []
+            for i in done:
+                r = g.active[i]
+                hb = torch.empty(r.n_elems, dtype=log.dtype, pin_memory=True)
+                hb.copy_(log[r.log_off: r.log_off + r.n_elems], non_blocking=True)
+                blocks.append(hb)
+        g.stream.synchronize()
         t = time.perf_counter()
         for q, i in enumerate(done):
             r, st = g.active[i], sh[i]
             j = r.jd[0]
             names = _channel_names(r.meta, int(j["log_mask"]))
             ne = int(j["n_entries"])
-            blk = blocks[q][: len(names) * ne].reshape(len(names), ne)
+            blk = blocks[q].numpy().astype(np.float64)[: len(names) * ne].reshape(len(names), ne)
             out = {k: blk[c] for c, k in enumerate(names)}
             if r.meta.mr_mode == _lib.MR_STRIP and int(j["log_mask"]) & _lib.LOG_MR:
                 out["Rz"] = np.zeros(ne)
@@ -306,7 +315,7 @@ class Server:
                 r.future.set_exception(e)
             else:
                 self.steps_done += int(st["steps_done"])
-                r.future.set_result(Result(out, states[q]["m"][:nl].copy(), int(st["steps_done"]), r.t_submit,
+                r.future.set_result(Result(out, states[i]["m"][:nl].copy(), int(st["steps_done"]), r.t_submit,
                                            r.t_admit, t))
         with self._cv:
             for i in done:
@@ -317,12 +326,17 @@ class Server:
         g.changed = True
 
     def _launch(self, torch, g: _Group, log) -> None:
+        """One slice on the group's stream, followed by the copies of status
+        and state into the group's pinned host mirrors."""
         if g.ev1 is None:
             g.ev0 = torch.cuda.Event(enable_timing=True)
             g.ev1 = torch.cuda.Event(enable_timing=True)
         g.ev0.record(g.stream)
         g.plan.execute_slice(log.data_ptr(), log.numel(), None, g.status.data_ptr(), g.state.data_ptr(),
                              g.slice_steps, [g.stream.cuda_stream])
+        with torch.cuda.stream(g.stream):
+            g.status_host.copy_(g.status, non_blocking=True)
+            g.state_host.copy_(g.state, non_blocking=True)
         g.ev1.record(g.stream)
         g.busy = True
         g.slices += 1
