@@ -52,18 +52,38 @@ static std::mutex g_consts_mu;
 // less redundant work per instance.
 constexpr uint64_t kAutoLaneWaves = 2048;
 
-static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, bool loge, int lanes,
+// kinds: bit 0 = a record of another kind than step/sine, bit 1 = a step
+// record, bit 2 = a sine record.
+constexpr uint32_t KIND_OTHER = 1, KIND_STEP = 2, KIND_SINE = 4;
+
+static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds) {
+  const uint32_t feat = k.fm & F_ALL;
+  if ((feat & flags) != flags) return false;
+  if ((k.fm & F_EXACT) && feat != flags) return false;
+  if ((k.fm & F_KSTEP) && (kinds & ~KIND_STEP)) return false;
+  if ((k.fm & F_KSINE) && (kinds & ~KIND_SINE)) return false;
+  return true;
+}
+
+// Lower is better: fewer layers of capacity, then exact feature sets, then
+// driver-kind specialisations, then fewer compiled-in features.
+static int rank(const KEntry& k) {
+  return k.lm * 1000 + ((k.fm & F_EXACT) ? 0 : 200) + ((k.fm & (F_KSTEP | F_KSINE)) ? 0 : 100) +
+         __builtin_popcount(k.fm & F_ALL);
+}
+
+static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, uint32_t kinds, bool loge, int lanes,
                                  uint64_t n_inst) {
   static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),    kernels_spec(),
-                                   kernels_ll_f64(), kernels_ll_f32(), kernels_ll_spec()};
+                                   kernels_ll_f64(), kernels_ll_f32(), kernels_ll_spec(), kernels_exact()};
   const KEntry* best[2] = {nullptr, nullptr};  // [0] one lane per instance, [1] one lane per layer
   for (const KTable& t : kTables)
     for (size_t q = 0; q < t.n; ++q) {
       const KEntry& k = t.e[q];
       if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
-      if ((k.fm & flags) != flags) continue;
+      if (!covers(k, flags, kinds)) continue;
       const KEntry*& b = best[k.lpi > 1];
-      if (!b || k.lm < b->lm || (k.lm == b->lm && __builtin_popcount(k.fm) < __builtin_popcount(b->fm))) b = &k;
+      if (!b || rank(k) < rank(*b)) b = &k;
     }
   if (!best[0] || !best[1]) return best[0] ? best[0] : best[1];  // the only kernel that covers the group
   if (nl < 2 || lanes == CMTJ_LANES_ONE) return best[0];
@@ -333,7 +353,12 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     const cmtj_junction& j0 = js[orders[g][0]];
     const cmtj_layer* L0 = layers + j0.layer_offset;
     const bool loge = (j0.log_mask & CMTJ_LOG_ENERGY) != 0;
-    const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], loge, lanes, orders[g].size());
+    uint32_t kinds = 0;
+    for (const Rec& r : recs_of[orders[g][0]]) {
+      const int ty = r.meta & 0xff;
+      kinds |= (ty == CMTJ_STEP) ? KIND_STEP : (ty == CMTJ_SINE) ? KIND_SINE : KIND_OTHER;
+    }
+    const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], kinds, loge, lanes, orders[g].size());
     if (!k) return fail(CMTJ_ERR_INVALID, "no kernel for this junction configuration");
     Group Gr;
     Gr.k = k;

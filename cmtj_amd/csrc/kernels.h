@@ -136,7 +136,7 @@ struct ERec {
   V3<R> Hext, Hiec, Hanis, Hdemag, Hfl, Hdl;
 };
 
-#define CMTJ_HAS(b) ((FM & (b)) && (fl & (b)))
+#define CMTJ_HAS(b) ((FM & (b)) && ((FM & F_EXACT) || (fl & (b))))
 
 // Quantities of the binary64 anchor evaluation of one layer at (m0, t),
 // kept in binary32 for the increment stages of the anchored RK4.
@@ -530,9 +530,15 @@ __device__ __forceinline__ bool transcendental(int type) {
 // Value of one time-dependent record at t.  FT: binary32 transcendentals
 // (phase and exponent argument still formed in binary64) for the anchored
 // FP32 mode, whose stage increments need the drive only to ~1e-7 relative.
-template <class R, bool FT = false>
+template <class R, bool FT = false, uint32_t FM = 0>
 __device__ __forceinline__ R wave_value(int type, const DynRec& r, const GroupArgs& g, double t, uint32_t step,
                                         int stage, int sps) {
+  if constexpr ((FM & F_KSTEP) != 0) {  // stepUpdate (drivers.hpp:73-78), the only kind of this group
+    return (t >= r.p1 && t <= r.p2) ? (R)r.A : R(0);
+  } else if constexpr ((FM & F_KSINE) != 0) {
+    if (sizeof(R) == 4 || FT) return (R)wave_part_f_trans(CMTJ_SINE, r.A, r.p1, r.p2, t);
+    return (R)wave_part_trans(CMTJ_SINE, r.A, r.p1, r.p2, t);
+  }
   if (type == CMTJ_TABLE) return (R)g.tables[(size_t)r.tidx + (size_t)step * sps + stage];
   if (sizeof(R) == 4) return (R)wave_part_f(type, r.A, r.p1, r.p2, r.p3, t);
   if (FT && transcendental(type)) return (R)wave_part_f_trans(type, r.A, r.p1, r.p2, t);
@@ -567,7 +573,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
       const int slot = (meta >> 8) & 0xff;
       const int lmask = (meta >> 16) & 0xff;
       const DynRec r = pick_rec(dr, g, k, lane);
-      const R v = wave_value<R, FT>(type, r, g, t, step, stage, sps);
+      const R v = wave_value<R, FT, FM>(type, r, g, t, step, stage, sps);
       keep_v0(v0, k, (double)v);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
@@ -578,7 +584,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
     for (int k = 0; k < DREG; ++k) {
       if (k < g.ndyn) {
         const int type = g.dmeta[k] & 0xff;
-        const R v = wave_value<R, FT>(type, dr[k], g, t, step, stage, sps);
+        const R v = wave_value<R, FT, FM>(type, dr[k], g, t, step, stage, sps);
         if (v0) v0[k] = (double)v;
 #pragma unroll
         for (int l = 0; l < LM; ++l) add_field(dv[l].H, fw.axis[k], fw.on[k][l], v);
@@ -587,7 +593,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
     for (int k = DREG; k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
       const DynRec r = pick_rec(dr, g, k, lane);
-      const R v = wave_value<R, FT>(meta & 0xff, r, g, t, step, stage, sps);
+      const R v = wave_value<R, FT, FM>(meta & 0xff, r, g, t, step, stage, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
         if (((meta >> 16) >> (l + lsh)) & 1) add_slot<R, false>(dv[l], (meta >> 8) & 0xff, v);
@@ -596,6 +602,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
 }
 
 // Record k's value at t0: from v0 for the first DREG records, else recomputed.
+template <uint32_t FM>
 __device__ __forceinline__ double value_t0(const double (&v0)[DREG], int k, int type, const DynRec& r,
                                            const GroupArgs& g, double t0, uint32_t step, int sps) {
   if (k < DREG) {
@@ -605,7 +612,7 @@ __device__ __forceinline__ double value_t0(const double (&v0)[DREG], int k, int 
       if (q == k) v = v0[q];
     return v;
   }
-  return wave_value<double, true>(type, r, g, t0, step, 0, sps);
+  return wave_value<double, true, FM>(type, r, g, t0, step, 0, sps);
 }
 
 // Driver increments dd = dv(ts) - dv(t0) of the time-dependent records of
@@ -624,8 +631,8 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
       const int meta = g.dmeta[k];
       const int type = meta & 0xff;
       const DynRec r = pick_rec(dr, g, k, lane);
-      const double v = wave_value<double, true>(type, r, g, ts, step, stage, sps) -
-                       value_t0(v0, k, type, r, g, t0, step, sps);
+      const double v = wave_value<double, true, FM>(type, r, g, ts, step, stage, sps) -
+                       value_t0<FM>(v0, k, type, r, g, t0, step, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
         if (((meta >> 16) >> (l + lsh)) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);
@@ -635,7 +642,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
     for (int k = 0; k < DREG; ++k) {
       if (k < g.ndyn) {
         const int type = g.dmeta[k] & 0xff;
-        const float v = (float)(wave_value<double, true>(type, dr[k], g, ts, step, stage, sps) - v0[k]);
+        const float v = (float)(wave_value<double, true, FM>(type, dr[k], g, ts, step, stage, sps) - v0[k]);
 #pragma unroll
         for (int l = 0; l < LM; ++l) add_field(dd[l].H, fw.axis[k], fw.on[k][l], v);
       }
@@ -643,8 +650,8 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
     for (int k = DREG; k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
       const DynRec r = pick_rec(dr, g, k, lane);
-      const double v = wave_value<double, true>(meta & 0xff, r, g, ts, step, stage, sps) -
-                       wave_value<double, true>(meta & 0xff, r, g, t0, step, 0, sps);
+      const double v = wave_value<double, true, FM>(meta & 0xff, r, g, ts, step, stage, sps) -
+                       wave_value<double, true, FM>(meta & 0xff, r, g, t0, step, 0, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
         if (((meta >> 16) >> (l + lsh)) & 1) add_slot<float, false>(dd[l], (meta >> 8) & 0xff, (float)v);

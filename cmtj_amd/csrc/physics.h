@@ -36,7 +36,11 @@ enum Feature : uint32_t {
   F_DYN = 1u << 11,       // time-dependent drivers on non-field slots
   F_TABLE = 1u << 12,     // tabulated (callback) drivers present
   F_DYNH = 1u << 13,      // time-dependent drivers that feed field slots only
-  F_ALL = 0x3FFFu
+  F_ALL = 0x3FFFu,        // every feature bit
+  // kernel-only bits (never set in a group's runtime flags):
+  F_EXACT = 1u << 16,     // the group's features are exactly the kernel's: feature tests are compile-time
+  F_KSTEP = 1u << 17,     // every time-dependent record is a step driver
+  F_KSINE = 1u << 18,     // every time-dependent record is a sine driver
 };
 
 template <class R>

@@ -25,6 +25,13 @@ constexpr uint32_t FM_LEAN = F_ANIS | F_IEC | F_DYNH;
 constexpr uint32_t FM_STT = F_ANIS | F_IEC | F_DYNH | F_STT;
 constexpr uint32_t FM_SOTT = F_ANIS | F_IEC | F_DYN | F_DYNH | F_SOT | F_THERMAL;
 
+// exact feature sets of the benchmark workloads (see workloads.py)
+constexpr uint32_t FM_PIMM = F_ANIS | F_IEC | F_DYNH | F_EXACT | F_KSTEP;          // 2-layer MTJ, Hoe step
+constexpr uint32_t FM_RH = F_ANIS | F_EXACT | F_KSTEP;                             // R(H) loop, static drive
+constexpr uint32_t FM_VSD = F_ANIS | F_DYNH | F_EXACT | F_KSINE;                   // sine Oersted drive
+constexpr uint32_t FM_CIMS = F_ANIS | F_DYN | F_SOT | F_THERMAL | F_EXACT | F_KSTEP;  // SOT pulse at T
+constexpr uint32_t FM_STO = F_ANIS | F_STT | F_EXACT | F_KSTEP;                    // DC spin-torque oscillator
+
 // one table per instantiation unit
 KTable kernels_f64();
 KTable kernels_f64e();
@@ -33,6 +40,7 @@ KTable kernels_spec();
 KTable kernels_ll_f64();
 KTable kernels_ll_f32();
 KTable kernels_ll_spec();
+KTable kernels_exact();
 
 }  // namespace cmtj
 

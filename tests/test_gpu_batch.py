@@ -98,6 +98,43 @@ def test_fp32_bulk_within_1e4_of_fp64(builder, lanes):
     assert worst <= FP32_TOL and fm <= FP32_TOL
 
 
+def _workload(name):
+    if name == "pimm":
+        return workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp64", log="mx", log_dtype="f64")
+    if name == "r_h_loop":
+        return workloads.r_h_loop(n_fields=11, total_time=0.5e-9, precision="fp64")
+    if name == "vsd":
+        return _vsd_sweep("fp64", n=8)
+    if name == "sto":
+        return workloads.sto(n_currents=8, total_time=0.5e-9, precision="fp64")
+    return workloads.cims_pbit(n_currents=4, n_seeds=4, total_time=0.1e-9, dt=1e-14, pulse=0.05e-9,
+                               precision="fp64")
+
+
+@pytest.mark.parametrize("lanes", ["one", "per_layer"])
+@pytest.mark.parametrize("name", ["pimm", "r_h_loop", "vsd", "sto", "cims"])
+def test_workload_kernels_match_oracle(name, lanes):
+    """The kernels specialised to the workloads' exact feature sets against
+    the oracle (the CIMS ensemble pathwise: same Philox stream)."""
+    b = _workload(name).set_lanes(lanes)
+    r = b.run()
+    print(f"[workload] {name} {lanes}: {b.plan().info.kernel_name.decode()}", flush=True)
+    olog, omag, ost = _oracle_blocks(b)
+    st = r.status()
+    assert (st["code"] == 0).all() and (st["steps_done"] == ost["steps_done"]).all()
+    glog = r.log.cpu().numpy()
+    worst = 0.0
+    for i in range(len(b.js)):
+        g, o = _m_channels(b, glog, i), _m_channels(b, olog, i)
+        for k in o:
+            if "_m" in k:
+                worst = max(worst, float(np.max(np.abs(g[k] - o[k]))))
+    nl = int(b.js[0]["n_layers"])
+    worst = max(worst, float(np.max(np.abs(r.final_m.cpu().numpy()[:, :nl] - omag[:, :nl]))))
+    print(f"[workload] {name} {lanes}: max|dm| vs oracle {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
 def test_uneven_ragged_batch_and_failure_isolation():
     """Different layer counts, durations, time steps and solvers in one batch;
     one instance blows up (NaN) without touching the others."""
