@@ -56,10 +56,10 @@ constexpr uint64_t kAutoLaneWaves = 2048;
 // record, bit 2 = a sine record.
 constexpr uint32_t KIND_OTHER = 1, KIND_STEP = 2, KIND_SINE = 4;
 
-static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds) {
+static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds, int ndyn) {
   const uint32_t feat = k.fm & F_ALL;
   if ((feat & flags) != flags) return false;
-  if ((k.fm & F_EXACT) && feat != flags) return false;
+  if ((k.fm & F_EXACT) && (feat != flags || ndyn > DREG)) return false;  // exact kernels keep every record in registers
   if ((k.fm & F_KSTEP) && (kinds & ~KIND_STEP)) return false;
   if ((k.fm & F_KSINE) && (kinds & ~KIND_SINE)) return false;
   return true;
@@ -72,8 +72,8 @@ static int rank(const KEntry& k) {
          __builtin_popcount(k.fm & F_ALL);
 }
 
-static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, uint32_t kinds, bool loge, int lanes,
-                                 uint64_t n_inst) {
+static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, uint32_t kinds, int ndyn, bool loge,
+                                 int lanes, uint64_t n_inst) {
   static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),    kernels_spec(),
                                    kernels_ll_f64(), kernels_ll_f32(), kernels_ll_spec(), kernels_exact()};
   const KEntry* best[2] = {nullptr, nullptr};  // [0] one lane per instance, [1] one lane per layer
@@ -81,7 +81,7 @@ static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, u
     for (size_t q = 0; q < t.n; ++q) {
       const KEntry& k = t.e[q];
       if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
-      if (!covers(k, flags, kinds)) continue;
+      if (!covers(k, flags, kinds, ndyn)) continue;
       const KEntry*& b = best[k.lpi > 1];
       if (!b || rank(k) < rank(*b)) b = &k;
     }
@@ -358,7 +358,8 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
       const int ty = r.meta & 0xff;
       kinds |= (ty == CMTJ_STEP) ? KIND_STEP : (ty == CMTJ_SINE) ? KIND_SINE : KIND_OTHER;
     }
-    const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], kinds, loge, lanes, orders[g].size());
+    const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], kinds,
+                                  (int)recs_of[orders[g][0]].size(), loge, lanes, orders[g].size());
     if (!k) return fail(CMTJ_ERR_INVALID, "no kernel for this junction configuration");
     Group Gr;
     Gr.k = k;

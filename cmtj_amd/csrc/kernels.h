@@ -498,16 +498,15 @@ __device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const Gro
   }
 }
 
-// H[axis] += v where the record feeds this layer (axis is wave-uniform)
+// H[axis] += v where the record feeds this layer.  The layer choice is a
+// select (another layer never sees v); the axis (wave-uniform) is a 0/1
+// weight, so a non-finite v reaches only the layer it drives.
 template <class R>
 __device__ __forceinline__ void add_field(V3<R>& H, int axis, bool on, R v) {
   const R vv = on ? v : R(0);
-  if (axis == 0)
-    H.x += vv;
-  else if (axis == 1)
-    H.y += vv;
-  else if (axis == 2)
-    H.z += vv;
+  H.x = fma(R(axis == 0 ? 1 : 0), vv, H.x);
+  H.y = fma(R(axis == 1 ? 1 : 0), vv, H.y);
+  H.z = fma(R(axis == 2 ? 1 : 0), vv, H.z);
 }
 
 __device__ __forceinline__ DynRec pick_rec(const DynRec (&dr)[DREG], const GroupArgs& g, int k, uint32_t lane) {
@@ -590,7 +589,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
         for (int l = 0; l < LM; ++l) add_field(dv[l].H, fw.axis[k], fw.on[k][l], v);
       }
     }
-    for (int k = DREG; k < g.ndyn; ++k) {
+    for (int k = DREG; !(FM & F_EXACT) && k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
       const DynRec r = pick_rec(dr, g, k, lane);
       const R v = wave_value<R, FT, FM>(meta & 0xff, r, g, t, step, stage, sps);
@@ -647,7 +646,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
         for (int l = 0; l < LM; ++l) add_field(dd[l].H, fw.axis[k], fw.on[k][l], v);
       }
     }
-    for (int k = DREG; k < g.ndyn; ++k) {
+    for (int k = DREG; !(FM & F_EXACT) && k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
       const DynRec r = pick_rec(dr, g, k, lane);
       const double v = wave_value<double, true, FM>(meta & 0xff, r, g, ts, step, stage, sps) -
