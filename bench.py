@@ -103,6 +103,158 @@ def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
             "slices": ticks, "slice_ms_target": 20.0, "host_s": host_s, "precision": "fp32", "lanes": "per_layer"}
 
 
+def _timed_run(b, reps: int = 1):
+    """Warm-up launch on a tiny slice of the batch, then `reps` timed
+    launches of the whole batch (HIP events on the launch stream)."""
+    import numpy as np
+    import torch
+
+    b.subset(np.arange(min(64, len(b.js)))).run()
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        r = b.execute(stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return r, float(np.mean(ms))
+
+
+def _bench_rh() -> dict:
+    """R(H) loop, 101 fields x 10 ns, 1 layer, RK4 dt 1e-13, W = 10: the FP64
+    drop-in run field by field (as the reference's scripts do), the same
+    sweep as one FP64 and one FP32 launch, and the oracle on the host."""
+    import numpy as np
+
+    import oracle
+    from cmtj_amd import AxialDriver, _lib, workloads
+
+    b64 = workloads.r_h_loop(precision="fp64")
+    r64, ms64 = _timed_run(b64)
+    b32 = workloads.r_h_loop(precision="fp32")
+    b32.log_dtype = _lib.LOG_F64
+    r32, ms32 = _timed_run(b32)
+    steps = b64.total_steps
+    # drop-in: a few fields, one runSimulation each
+    from cmtj_amd import CVector, Junction, Layer, constantDriver
+
+    base = b64.js[0]
+    t = []
+    for H in (-800e3, -400e3, 0.0, 400e3, 800e3):
+        l = Layer("free", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=1.2, thickness=2e-9, cellSurface=1e-16,
+                  demagTensor=[CVector(0.00024, 0, 0), CVector(0, 0.0022, 0), CVector(0, 0, 0.99756)], damping=0.01)
+        jj = Junction([l], [100.0], [0.0], [1.0], [0.0], [0.0], [0.0], [0.0])
+        jj.setLayerAnisotropyDriver("free", constantDriver(5e3))
+        jj.setLayerExternalFieldDriver("all", AxialDriver(H, 0, 0))
+        t0 = time.perf_counter()
+        jj.runSimulation(10e-9, 1e-13, 1e-12)
+        t.append(time.perf_counter() - t0)
+    plain_rate = int(base["n_steps"]) / float(np.median(t))
+    t0 = time.perf_counter()
+    olog, omag, ost = oracle.run_batch(b64.js, b64.layers, _lib.constants_get(), None, b64.log_elements,
+                                       n_threads=min(16, os.cpu_count() or 1))
+    cpu_s = time.perf_counter() - t0
+    glog = r64.log.cpu().numpy()
+    dm64 = float(np.max(np.abs(r64.final_m.cpu().numpy()[:, 0] - omag[:, 0])))
+    rx = r64.channel("Rx")[:, -1].cpu().numpy()
+    dm32 = float((r32.log.double() - r64.log).abs().max())
+    return {"workload": "r_h_loop_101_fields", "junction_steps": steps,
+            "fp64_batched_junction_steps_per_s": steps / (ms64 * 1e-3), "fp64_batched_ms": ms64,
+            "fp32_batched_junction_steps_per_s": steps / (ms32 * 1e-3), "fp32_batched_ms": ms32,
+            "fp64_plain_junction_steps_per_s": plain_rate,
+            "cpu_oracle_junction_steps_per_s": steps / cpu_s, "cpu_threads": min(16, os.cpu_count() or 1),
+            "max_dm_fp64_vs_oracle_final": dm64, "max_dlog_fp32_vs_fp64": dm32,
+            "log_max_abs_difference_fp64_vs_oracle": float(np.max(np.abs(glog - olog))),
+            "final_Rx_first_mid_last": [float(rx[0]), float(rx[50]), float(rx[-1])]}
+
+
+def _bench_cims(n_currents: int = 256, n_seeds: int = 4096) -> dict:
+    """CIMS / p-bit ensemble: 256 currents x 4096 seeds, 2 layers, static SOT
+    at 300 K, Euler-Heun dt 1e-14 s, 5 ns; switching probability per current
+    in FP32, an FP64 sample with the same noise, and the oracle on a smaller
+    sample (pathwise: the same Philox streams)."""
+    import numpy as np
+
+    import oracle
+    from cmtj_amd import _lib, analysis, workloads
+
+    b = workloads.cims_pbit(n_currents=n_currents, n_seeds=n_seeds, precision="fp32")
+    r, ms = _timed_run(b)
+    P, se = analysis.switching_probability(r.final_m[:, 0, 2], n_currents)
+    ns64 = 64
+    idx64 = (np.arange(n_currents)[:, None] * n_seeds + np.arange(ns64)[None, :]).ravel()
+    b64 = b.subset(idx64, precision="fp64")
+    r64, ms64 = _timed_run(b64)
+    s32 = (r.final_m[idx64, 0, 2] < 0).cpu().numpy()
+    s64 = (r64.final_m[:, 0, 2] < 0).cpu().numpy()
+    dm = float((r.final_m[idx64] - r64.final_m).abs().max())
+    idxo = (np.arange(0, n_currents, 8)[:, None] * n_seeds + np.arange(2)[None, :]).ravel()
+    bo = b.subset(idxo, precision="fp64")
+    t0 = time.perf_counter()
+    _, omag, _ = oracle.run_batch(bo.js, bo.layers, _lib.constants_get(), None, 0,
+                                  n_threads=min(16, os.cpu_count() or 1))
+    cpu_s = time.perf_counter() - t0
+    ro = b64.subset(np.searchsorted(idx64, idxo)).run()  # the GPU FP64 of the oracle's sample
+    dmo = float(np.max(np.abs(ro.final_m.cpu().numpy()[:, :2] - omag[:, :2])))
+    Pn = P.cpu().numpy()
+    return {"workload": "cims_pbit_256x4096", "instances": len(b.js), "junction_steps": b.total_steps,
+            "fp32_junction_steps_per_s": b.total_steps / (ms * 1e-3), "fp32_ms": ms,
+            "fp64_sample_junction_steps_per_s": b64.total_steps / (ms64 * 1e-3),
+            "switching_probability_first_mid_last": [float(Pn[0]), float(Pn[n_currents // 2]), float(Pn[-1])],
+            "switching_stderr_max": float(se.max()),
+            "fp32_vs_fp64_same_outcome_fraction": float(np.mean(s32 == s64)),
+            "fp32_vs_fp64_max_dm_final": dm, "fp64_sample": f"{ns64} seeds x {n_currents} currents",
+            "fp64_gpu_vs_oracle_max_dm_final": dmo, "oracle_sample": f"{len(idxo)} instances",
+            "cpu_oracle_junction_steps_per_s": bo.total_steps / cpu_s}
+
+
+def _bench_sto(n_currents: int = 512) -> dict:
+    """Spin-torque oscillator precision study: 512 DC currents x 1e6 RK4 steps
+    (100 ns at 1e-13 s), FP32 and FP64; oscillation frequency and linewidth
+    from the FFT of the free layer's m_x (logged every 99 steps), max|dm|,
+    and the oracle on a sample."""
+    import numpy as np
+
+    import oracle
+    from cmtj_amd import _lib, analysis, workloads
+
+    b32 = workloads.sto(n_currents=n_currents, precision="fp32", log="full")
+    b32.log_dtype = _lib.LOG_F64
+    b64 = workloads.sto(n_currents=n_currents, precision="fp64", log="full")
+    r32, ms32 = _timed_run(b32)
+    r64, ms64 = _timed_run(b64)
+    W = int(b64.js[0]["write_every"])
+    dtl = W * float(b64.js[0]["dt"])
+    f32, w32, _ = analysis.spectrum_peaks(r32.channel("free_mx"), dtl)
+    f64, w64, _ = analysis.spectrum_peaks(r64.channel("free_mx"), dtl)
+    dm = float((r32.log - r64.log).abs().max())
+    dm_m = max(float((r32.channel(c) - r64.channel(c)).abs().max()) for c in ("free_mx", "free_my", "free_mz"))
+    osc = (f64 > 0.2e9)
+    idx = np.arange(0, n_currents, n_currents // 8)
+    bo = b64.subset(idx)
+    t0 = time.perf_counter()
+    olog, omag, _ = oracle.run_batch(bo.js, bo.layers, _lib.constants_get(), None, bo.log_elements,
+                                     n_threads=min(16, os.cpu_count() or 1))
+    cpu_s = time.perf_counter() - t0
+    ro = bo.run()
+    dmo = float(np.max(np.abs(ro.log.cpu().numpy() - olog)))
+    df = (f32 - f64).abs()
+    dw = (w32 - w64).abs()
+    return {"workload": "sto_512_currents_1e6_steps", "junction_steps": b64.total_steps,
+            "fp32_junction_steps_per_s": b32.total_steps / (ms32 * 1e-3), "fp32_ms": ms32,
+            "fp64_junction_steps_per_s": b64.total_steps / (ms64 * 1e-3), "fp64_ms": ms64,
+            "oscillating_points": int(osc.sum()),
+            "frequency_GHz_first_mid_last": [float(f64[0]) / 1e9, float(f64[n_currents // 2]) / 1e9,
+                                             float(f64[-1]) / 1e9],
+            "fp32_vs_fp64_max_dfreq_Hz": float(df.max()), "fp32_vs_fp64_max_dlinewidth_Hz": float(dw.max()),
+            "frequency_resolution_Hz": 1.0 / (0.8 * r64.channel("free_mx").shape[1] * dtl),
+            "fp32_vs_fp64_max_dm": dm_m, "fp32_vs_fp64_max_dlog": dm,
+            "fp64_gpu_vs_oracle_max_dlog": dmo, "oracle_sample": f"{len(idx)} currents",
+            "cpu_oracle_junction_steps_per_s": bo.total_steps / cpu_s}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +268,9 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
+    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto"],
+                    help="pimm: the headline sweep (default); rh, cims, sto: the other sweep workloads, "
+                         "one JSON line each")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="run the sweep kernel twice and a 1 GiB device copy, print nothing (for counter runs)")
     args = ap.parse_args()
@@ -132,6 +287,10 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload != "pimm":
+        fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto}[args.workload]
+        print(json.dumps(fn()))
+        return 0
 
     # ---- FP32 batched, inputs resident in HBM -------------------------------
     b = _pimm("fp32", args.points).set_lanes(args.lanes)

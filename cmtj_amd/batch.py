@@ -281,6 +281,23 @@ class JunctionBatch:
         self.log_elements = int(sizes.sum())
         self.total_steps = int(self.js["n_steps"].astype(np.uint64).sum())
 
+    def subset(self, idx, precision=None, log=None, log_dtype=None) -> "JunctionBatch":
+        """A batch of the instances idx (same descriptors, seeds and noise
+        streams), e.g. to re-run a sample in another precision."""
+        idx = np.asarray(idx, dtype=np.int64)
+        js = self.js[idx].copy()
+        rows, off = [], 0
+        for q, i in enumerate(idx):
+            a, nl = int(self.js[i]["layer_offset"]), int(self.js[i]["n_layers"])
+            rows.append(self.layers[a: a + nl])
+            js["layer_offset"][q] = off
+            off += nl
+        b = JunctionBatch(js, np.concatenate(rows), self.tables, self.metas, self.meta_index[idx],
+                          precision=self.precision if precision is None else precision,
+                          log=int(self.js[0]["log_mask"]) if log is None else log, log_dtype=log_dtype)
+        b.lanes = self.lanes
+        return b
+
     def set_lanes(self, lanes: str) -> "JunctionBatch":
         """Lane mapping of multilayer instances: "one" (all layers of an
         instance in one GPU lane), "per_layer" (one lane per layer: lowest
