@@ -6,6 +6,8 @@ namespace cmtj {
 static const KEntry kE[] = {
     CMTJ_KL(float, CMTJ_FP32, 2, CMTJ_RK4, FM_PIMM),
     CMTJ_KL(double, CMTJ_FP64, 2, CMTJ_RK4, FM_PIMM),
+    CMTJ_KL(float, CMTJ_FP32, 4, CMTJ_RK4, FM_PIMM),   // 3-4 layer IEC chains with a field step
+    CMTJ_KL(double, CMTJ_FP64, 4, CMTJ_RK4, FM_PIMM),
     CMTJ_KE(float, CMTJ_FP32, 2, CMTJ_RK4, FM_PIMM, false),
     CMTJ_KE(double, CMTJ_FP64, 2, CMTJ_RK4, FM_PIMM, false),
     CMTJ_KE(float, CMTJ_FP32, 1, CMTJ_RK4, FM_RH, false),
