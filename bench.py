@@ -268,7 +268,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto"],
+    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving"],
                     help="pimm: the headline sweep (default); rh, cims, sto: the other sweep workloads, "
                          "one JSON line each")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -288,7 +288,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if args.workload != "pimm":
-        fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto}[args.workload]
+        fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto,
+              "serving": lambda: _serving(args.serving_qps, args.serving_seconds)}[args.workload]
         print(json.dumps(fn()))
         return 0
 
