@@ -21,6 +21,10 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# The serving leg runs one stream per structure group; with HIP's default of 4
+# hardware queues a fifth group would queue behind another group's slices.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 FLOP_PER_STEP = 666.0  # algorithmic FLOP per junction-step of this 2-layer RK4 workload
 PEAK_FP32 = 157.3      # MI355X published FP32 vector peak, TFLOP/s
 PEAK_FP64 = 78.6

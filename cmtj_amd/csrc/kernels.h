@@ -481,10 +481,23 @@ template <class R, int LM>
 struct FieldWeights {
   int axis[DREG];
   bool on[DREG][LM];
+  double win_lo, win_hi;  // step-driver kernels: every record is zero outside [win_lo, win_hi]
 };
 
 template <class R, int LM>
-__device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const GroupArgs& g, int lsh = 0) {
+__device__ __forceinline__ void field_weights(FieldWeights<R, LM>& fw, const GroupArgs& g, int lsh = 0,
+                                              const DynRec* dr = nullptr) {
+  fw.win_lo = __builtin_inf();
+  fw.win_hi = -__builtin_inf();
+  if (dr) {
+#pragma unroll
+    for (int k = 0; k < DREG; ++k) {
+      if (k < g.ndyn) {
+        fw.win_lo = fmin(fw.win_lo, dr[k].p1);
+        fw.win_hi = fmax(fw.win_hi, dr[k].p2);
+      }
+    }
+  }
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
     const int meta = (k < g.ndyn) ? g.dmeta[k] : 0;
@@ -565,6 +578,14 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
                                              double* v0 = nullptr) {
 #pragma unroll
   for (int l = 0; l < LM; ++l) dv[l] = base[l];
+  if constexpr ((FM & F_KSTEP) != 0) {  // stepUpdate is zero outside [t_start, t_stop]
+    if (!(t >= fw.win_lo && t <= fw.win_hi)) {
+      if (v0)
+#pragma unroll
+        for (int k = 0; k < DREG; ++k) v0[k] = 0.0;
+      return;
+    }
+  }
   if (CMTJ_HAS(F_DYN) || (LOGE && CMTJ_HAS(F_DYNH))) {
     for (int k = 0; k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
@@ -625,6 +646,9 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
                                                    int lsh = 0) {
 #pragma unroll
   for (int l = 0; l < LM; ++l) dv_zero(dd[l]);
+  if constexpr ((FM & F_KSTEP) != 0) {  // both times outside every step window: no increment
+    if (!(ts >= fw.win_lo && t0 <= fw.win_hi)) return;
+  }
   if (CMTJ_HAS(F_DYN)) {
     for (int k = 0; k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
@@ -844,11 +868,11 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
   }
 
   FieldWeights<R, LM> fw;
-  field_weights<R, LM>(fw, g);
+  field_weights<R, LM>(fw, g, 0, dr);
   FieldWeights<float, LM> fwf;
-  if constexpr (ANCHORED) field_weights<float, LM>(fwf, g);
+  if constexpr (ANCHORED) field_weights<float, LM>(fwf, g, 0, dr);
   FieldWeights<double, LM> fwd;
-  if constexpr (ANCHORED) field_weights<double, LM>(fwd, g);
+  if constexpr (ANCHORED) field_weights<double, LM>(fwd, g, 0, dr);
   ERec<R> er[LM];
   DV<R> dv[LM];
   const R dtR = (R)dt;

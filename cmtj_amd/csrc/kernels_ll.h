@@ -178,11 +178,11 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
   }
   FieldWeights<R, 1> fw;
-  field_weights<R, 1>(fw, g, lyr);
+  field_weights<R, 1>(fw, g, lyr, dr);
   FieldWeights<float, 1> fwf;
-  if constexpr (ANCHORED) field_weights<float, 1>(fwf, g, lyr);
+  if constexpr (ANCHORED) field_weights<float, 1>(fwf, g, lyr, dr);
   FieldWeights<double, 1> fwd;
-  if constexpr (ANCHORED) field_weights<double, 1>(fwd, g, lyr);
+  if constexpr (ANCHORED) field_weights<double, 1>(fwd, g, lyr, dr);
   ERec<R> er;
   DV<R> dv[1];
   const R dtR = (R)dt;
