@@ -28,6 +28,7 @@ _LOG_MODES = {
     "full": _lib.LOG_FULL,                       # m (x, y, z) of every layer + resistance
     "mx": _lib.LOG_MX | _lib.LOG_MR,             # m_x of every layer + resistance (PIMM spectra)
     "m": _lib.LOG_MX | _lib.LOG_MYZ,             # m only
+    "r": _lib.LOG_MR,                            # resistance only (spin-diode voltages)
     "none": 0,                                   # final state only
 }
 

@@ -1,0 +1,71 @@
+"""PIMM / VSD procedures, reference (chained) semantics on the oracle
+backend: validation, output shapes, and the PIMM peak at the Kittel
+frequency of a thin film."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cmtj_amd import AxialDriver, Axis, CVector, Layer, constantDriver
+from cmtj_amd.procedures import PIMM_procedure, ResistanceParameters, VSD_procedure, compute_spectrum_strip
+from cmtj_amd.utils import FieldScan
+from oracle_backend import OracleJunction
+
+THIN = [CVector(0, 0, 0), CVector(0, 0, 0), CVector(0, 0, 1)]
+
+
+def _film(Ms=1.0, K=1e4):
+    l = Layer("free", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=Ms, thickness=2e-9, cellSurface=1e-16,
+              demagTensor=THIN, damping=0.01)
+    j = OracleJunction([l])
+    j.setLayerAnisotropyDriver("free", constantDriver(K))
+    return j
+
+
+def test_spectrum_strip_matches_a_direct_fft():
+    x = np.sin(2 * np.pi * 5e9 * np.arange(1000) * 1e-12)
+    y, f = compute_spectrum_strip(x, 1e-12, 80e9)
+    assert f[np.argmax(y)] == pytest.approx(5e9, abs=1.1e9)
+    assert len(y) == len(f) <= 500
+
+
+def test_pimm_peak_is_the_kittel_frequency():
+    Ms, K = 1.0, 1e4
+    j = _film(Ms, K)
+    _, Hvecs = FieldScan.amplitude_scan(100e3, 300e3, 3, theta=90, phi=0)
+    spec, f, out = PIMM_procedure(j, Hvecs, 1e-12, [ResistanceParameters(Rxx0=100, Ramr=1, l=1, w=1)],
+                                  Hoe_direction=Axis.zaxis, simulation_duration=5e-9, disturbance=0.0,
+                                  disable_tqdm=True, independent=False)
+    assert spec.shape == (3, len(f)) and set(out) >= {"H", "Rx", "Ry", "m_avg"}
+    mu0, gamma = 12.566e-7, 220880.0
+    df = 1 / 5e-9
+    for p, H in enumerate(Hvecs):
+        Hx = float(H[0])
+        Hk = 2 * K / Ms
+        fk = gamma / (2 * math.pi) * math.sqrt((Hx + Hk) * (Hx + Hk + Ms / mu0))
+        assert abs(float(f[np.argmax(spec[p]), 0]) - fk) <= 2 * df, (Hx, fk)
+
+
+def test_procedure_validation_messages():
+    j = _film()
+    with pytest.raises(ValueError, match="wait_time"):
+        PIMM_procedure(j, [[1e5, 0, 0]], 1e-12, [ResistanceParameters()], wait_time=1e-8, simulation_duration=1e-9)
+    with pytest.raises(ValueError, match="number of layers"):
+        PIMM_procedure(j, [[1e5, 0, 0]], 1e-12, [])
+    with pytest.raises(ValueError, match="number of layers"):
+        VSD_procedure(j, [[1e5, 0, 0]], [1e9], 1e-12, [], Rtype="Rx")
+
+
+def test_vsd_chained_runs():
+    j = _film()
+    j2 = OracleJunction([j.layers[0]], [100.0], [0.0], [1.0], [0.0], [0.0], [0.0], [0.0])
+    j2.setLayerAnisotropyDriver("free", constantDriver(1e4))
+    j2.setLayerExternalFieldDriver("all", AxialDriver(0, 0, 0))
+    H = [[0, 200e3, 0]]
+    fs = np.array([5e9, 16e9, 30e9])
+    v = VSD_procedure(j2, H, fs, 1e-12, [ResistanceParameters(Rxx0=100, Ramr=1, l=1, w=1)],
+                      Hoe_direction=Axis.zaxis, simulation_duration=3e-9, disturbance=0.0, Rtype="Rx",
+                      independent=False)
+    assert v.shape == (1, 3) and np.all(np.isfinite(v))
