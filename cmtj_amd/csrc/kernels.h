@@ -717,23 +717,12 @@ __device__ __forceinline__ V3<double> normalize_exact(V3<double> a) {
 
 // Same result to ~1 ulp: binary32 reciprocal square root refined by two
 // Newton steps in binary64 (used by the FP32-bulk kernels).
-// Near |a| = 1 (every step after the first: the update is nearly orthogonal
-// to m, so e = |a|^2 - 1 is of the order of the squared step) the binomial
-// series 1/sqrt(1 + e) = 1 - e/2 + 3e^2/8 - 5e^3/16 + 35e^4/128 is exact to
-// binary64 for |e| < 2^-14 (the first dropped term, 63e^5/256, is below
-// 2^-72); elsewhere the Newton path.
 __device__ __forceinline__ V3<double> normalize_fast(V3<double> a) {
   const double s = a.x * a.x + a.y * a.y + a.z * a.z;
-  const double e = s - 1.0;
-  double y;
-  if (fabs(e) < 0x1p-14) {
-    y = 1.0 + e * (-0.5 + e * (0.375 + e * (-0.3125 + e * 0.2734375)));
-  } else {
-    if (s == 0.0) return a;
-    y = (double)__frsqrt_rn((float)s);
-    y = y * (1.5 - 0.5 * s * y * y);
-    y = y * (1.5 - 0.5 * s * y * y);
-  }
+  if (s == 0.0) return a;
+  double y = (double)__frsqrt_rn((float)s);
+  y = y * (1.5 - 0.5 * s * y * y);
+  y = y * (1.5 - 0.5 * s * y * y);
   return mk<double>(a.x * y, a.y * y, a.z * y);
 }
 

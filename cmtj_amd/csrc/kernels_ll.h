@@ -33,7 +33,6 @@ __device__ __forceinline__ double dpp_d(double v) {
 }
 constexpr int DPP_ROW_SHR1 = 0x111;
 constexpr int DPP_ROW_SHL1 = 0x101;
-constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1, 0, 3, 2]: the other lane of a pair
 
 template <class R>
 __device__ __forceinline__ V3<R> from_prev(V3<R> v, bool on) {
@@ -46,23 +45,6 @@ __device__ __forceinline__ V3<R> from_next(V3<R> v, bool on) {
   const V3<double> d = cvt<double>(v);
   const V3<double> r = mk<double>(dpp_d<DPP_ROW_SHL1>(d.x), dpp_d<DPP_ROW_SHL1>(d.y), dpp_d<DPP_ROW_SHL1>(d.z));
   return on ? cvt<R>(r) : mk<R>(0, 0, 0);
-}
-
-// Neighbours (m of layers ly - 1 and ly + 1, zero where absent).  With two
-// lanes per instance both are the pair partner: one exchange instead of two.
-template <int L, class R>
-__device__ __forceinline__ void neighbours(V3<R> v, bool hasB, bool hasT, V3<R>& bot, V3<R>& top) {
-  if constexpr (L == 2) {
-    const V3<double> d = cvt<double>(v);
-    const V3<R> p = cvt<R>(mk<double>(dpp_d<DPP_QUAD_XOR1>(d.x), dpp_d<DPP_QUAD_XOR1>(d.y),
-                                      dpp_d<DPP_QUAD_XOR1>(d.z)));
-    const V3<R> z = mk<R>(0, 0, 0);
-    bot = hasB ? p : z;
-    top = hasT ? p : z;
-  } else {
-    bot = from_prev(v, hasB);
-    top = from_next(v, hasT);
-  }
 }
 
 // Sum of v over the L lanes of an instance (butterfly; for two layers the
@@ -221,8 +203,7 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
   }
   uint64_t next_log = (uint64_t)k * W;
   // neighbours (m of layers ly - 1 and ly + 1), refreshed after every step
-  V3<double> bot, top;
-  neighbours<L>(m, hasB, hasT, bot, top);
+  V3<double> bot = from_prev(m, hasB), top = from_next(m, hasT);
   for (; i < iend; ++i) {
     const double t = (double)i * dt;
     V3<double> mn;
@@ -309,8 +290,7 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
         gn = stoch<R>(P, mr_, dW);
       }
       const V3<R> mN = mr_ + fn * dtR + gn * sdt;
-      V3<R> bN, tN;
-      neighbours<L>(mN, hasB, hasT, bN, tN);
+      const V3<R> bN = from_prev(mN, hasB), tN = from_next(mN, hasT);
       eval_drivers<R, 1, FM, LOGE>(dv, base, dr, fw, g, inst, fl, t + dt, i, 1, sps, lyr);
       const V3<R> f2 = rhs<R, FM, LOGE>(P, bits, fl, mN, dv[0], bN, tN, er);
       V3<double> acc = m + cvt<double>((fn + f2) * (R(0.5) * dtR));
@@ -333,7 +313,8 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
       }
     }
     m = mn;
-    neighbours<L>(m, hasB, hasT, bot, top);
+    bot = from_prev(m, hasB);
+    top = from_next(m, hasT);
     if ((uint64_t)i == next_log) {
       if (logging)
         write_entry_ll<R, L, FM, LOGE>(g, J, ly, act, m, top, er, lp + (uint64_t)k * esize, cs, Rp, Rap);
