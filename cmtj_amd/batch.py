@@ -100,7 +100,7 @@ class BatchResult:
         return self.log[: n * nch * int(ne[0])].view(n, nch, int(ne[0]))[:, c, :]
 
 
-_LANES = {"auto": 0, "one": 1, "per_layer": 2}
+_LANES = {"auto": 0, "one": 1, "per_layer": 2, "one_occ": 3}
 
 
 class JunctionBatch:

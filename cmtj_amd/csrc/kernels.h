@@ -817,7 +817,7 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
 }
 
 template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
-__global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
+__device__ __forceinline__ void junction_body(const GroupArgs& g) {
   const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= g.n) return;
   const uint32_t fl = g.flags;
@@ -1133,6 +1133,19 @@ __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
     S->code = code;
     S->nan_layer = nan_layer;
   }
+}
+
+// One instance per lane.  k_junction_o4 is the same kernel compiled for at
+// least 4 waves per SIMD (at most 128 VGPRs): batches that fill the chip trade
+// registers for latency hiding.
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
+__global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
+  junction_body<R, LM, SOLVER, FM, LOGE>(g);
+}
+
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_junction_o4(GroupArgs g) {
+  junction_body<R, LM, SOLVER, FM, LOGE>(g);
 }
 
 }  // namespace cmtj

@@ -259,7 +259,12 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
 
 typedef struct cmtj_plan cmtj_plan;
 
-enum cmtj_lanes { CMTJ_LANES_AUTO = 0, CMTJ_LANES_ONE = 1, CMTJ_LANES_PER_LAYER = 2 };
+enum cmtj_lanes {
+  CMTJ_LANES_AUTO = 0,
+  CMTJ_LANES_ONE = 1,
+  CMTJ_LANES_PER_LAYER = 2,
+  CMTJ_LANES_ONE_OCC = 3  /* one lane per instance, kernels compiled for >= 4 waves per SIMD where available */
+};
 
 typedef struct cmtj_plan_info {
   uint64_t n_instances;
