@@ -100,7 +100,7 @@ class BatchResult:
         return self.log[: n * nch * int(ne[0])].view(n, nch, int(ne[0]))[:, c, :]
 
 
-_LANES = {"auto": 0, "one": 1, "per_layer": 2, "one_occ": 3}
+_LANES = {"auto": 0, "one": 1, "per_layer": 2}
 
 
 class JunctionBatch:
@@ -301,9 +301,8 @@ class JunctionBatch:
 
     def set_lanes(self, lanes: str) -> "JunctionBatch":
         """Lane mapping of multilayer instances: "one" (all layers of an
-        instance in one GPU lane), "per_layer" (one lane per layer: lowest
-        latency per instance) or "auto" (per layer while the batch is too
-        small to fill the GPU)."""
+        instance in one GPU lane), "per_layer" (one lane per layer) or "auto"
+        (per layer wherever a kernel for the group exists)."""
         if lanes not in _LANES:
             raise ValueError(f"lanes must be one of {sorted(_LANES)}")
         self.lanes = lanes

@@ -45,13 +45,6 @@ static std::mutex g_consts_mu;
 
 #include "registry.h"
 
-// Automatic lane mapping: one lane per layer wherever such a kernel exists.
-// On a batch too small to fill the chip the per-layer split divides the
-// latency of a wave step, which sets the sweep time; on full-chip batches it
-// measured no slower either (1M-instance CIMS ensemble 6.6e10 vs 6.3e10
-// junction-steps/s, 65,536-point PIMM sweep 2.56e10 vs 2.44e10 on MI355X):
-// its register footprint is smaller, so more waves fit per SIMD.
-constexpr uint64_t kAutoLaneWaves = ~0ull;
 
 // kinds: bit 0 = a record of another kind than step/sine, bit 1 = a step
 // record, bit 2 = a sine record.
@@ -77,23 +70,25 @@ static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, u
                                  int lanes, uint64_t n_inst) {
   static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),    kernels_spec(),
                                    kernels_ll_f64(), kernels_ll_f32(), kernels_ll_spec(), kernels_exact()};
-  // [0] one lane per instance, [1] one lane per layer, [2] one lane per instance
-  // compiled for >= 4 waves per SIMD
-  const KEntry* best[3] = {nullptr, nullptr, nullptr};
+  const KEntry* best[2] = {nullptr, nullptr};  // [0] one lane per instance, [1] one lane per layer
   for (const KTable& t : kTables)
     for (size_t q = 0; q < t.n; ++q) {
       const KEntry& k = t.e[q];
       if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
       if (!covers(k, flags, kinds, ndyn)) continue;
-      const KEntry*& b = best[k.lpi > 1 ? 1 : (k.occ > 0 ? 2 : 0)];
+      const KEntry*& b = best[k.lpi > 1];
       if (!b || rank(k) < rank(*b)) b = &k;
     }
-  if (lanes == CMTJ_LANES_ONE_OCC) return best[2] ? best[2] : best[0] ? best[0] : best[1];
   if (!best[0] || !best[1]) return best[0] ? best[0] : best[1];  // the only kernel that covers the group
   if (nl < 2 || lanes == CMTJ_LANES_ONE) return best[0];
-  if (lanes == CMTJ_LANES_PER_LAYER) return best[1];
-  const uint64_t waves = (n_inst * (uint64_t)best[1]->lpi + 63) / 64;
-  return waves <= kAutoLaneWaves ? best[1] : best[0];
+  // CMTJ_LANES_PER_LAYER, and the automatic choice: on a batch too small to
+  // fill the chip the per-layer split divides the latency of a wave step,
+  // which sets the sweep time; on full-chip batches it measured no slower
+  // (1M-instance CIMS ensemble 6.6e10 vs 6.3e10 junction-steps/s, 65,536-point
+  // PIMM sweep 2.56e10 vs 2.44e10 on MI355X): its register footprint is
+  // smaller, so more waves fit per SIMD.
+  (void)n_inst;
+  return best[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -590,9 +585,8 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n, const cmtj_layer*
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
   }
-  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_ONE_OCC) {
-    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per "
-                           "layer) or 3 (one lane per instance, occupancy-bound kernels)");
+  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_LAYER) {
+    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance) or 2 (one lane per layer)");
     return nullptr;
   }
   cmtj_plan* P = new cmtj_plan();
@@ -651,9 +645,8 @@ cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_lay
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
   }
-  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_ONE_OCC) {
-    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per "
-                           "layer) or 3 (one lane per instance, occupancy-bound kernels)");
+  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_LAYER) {
+    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance) or 2 (one lane per layer)");
     return nullptr;
   }
   cmtj_plan* P = new cmtj_plan();

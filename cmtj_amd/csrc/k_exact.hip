@@ -17,8 +17,6 @@ static const KEntry kE[] = {
     CMTJ_KL(float, CMTJ_FP32, 2, CMTJ_EULER_HEUN, FM_CIMS),
     CMTJ_KL(double, CMTJ_FP64, 2, CMTJ_EULER_HEUN, FM_CIMS),
     CMTJ_KE(float, CMTJ_FP32, 2, CMTJ_EULER_HEUN, FM_CIMS, false),
-    CMTJ_KE_O4(float, CMTJ_FP32, 2, CMTJ_EULER_HEUN, FM_CIMS),
-    CMTJ_KE_O4(float, CMTJ_FP32, 2, CMTJ_RK4, FM_PIMM),
     CMTJ_KL(float, CMTJ_FP32, 2, CMTJ_RK4, FM_STO),
     CMTJ_KL(double, CMTJ_FP64, 2, CMTJ_RK4, FM_STO),
 };

@@ -1135,16 +1135,10 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   }
 }
 
-// One instance per lane.  k_junction_o4 is the same kernel compiled for at
-// least 4 waves per SIMD (at most 128 VGPRs): batches that fill the chip trade
-// registers for latency hiding.
+// One instance per lane.  (A variant compiled for >= 4 waves per SIMD spilled
+// 88-300 VGPRs and ran 1.6-5x slower on full-chip batches; not kept.)
 template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
 __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
-  junction_body<R, LM, SOLVER, FM, LOGE>(g);
-}
-
-template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_junction_o4(GroupArgs g) {
   junction_body<R, LM, SOLVER, FM, LOGE>(g);
 }
 

@@ -9,7 +9,6 @@ struct KEntry {
   int prec;
   int lm;   // layer capacity
   int lpi;  // lanes per instance: 1 (k_junction) or lm (k_junction_ll)
-  int occ;  // minimum waves per SIMD the kernel was compiled for (0: no bound)
   int solver;
   uint32_t fm;
   bool loge;
@@ -46,15 +45,11 @@ KTable kernels_exact();
 }  // namespace cmtj
 
 #define CMTJ_KE(R, PREC, LM, S, FM, E)                                     \
-  {PREC, LM, 1, 0, S, FM, E, (const void*)&cmtj::k_junction<R, LM, S, FM, E>, \
+  {PREC, LM, 1, S, FM, E, (const void*)&cmtj::k_junction<R, LM, S, FM, E>, \
    "k_junction<" #R "," #LM "," #S "," #FM "," #E ">"}
 
-#define CMTJ_KE_O4(R, PREC, LM, S, FM)                                                \
-  {PREC, LM, 1, 4, S, FM, false, (const void*)&cmtj::k_junction_o4<R, LM, S, FM, false>, \
-   "k_junction_o4<" #R "," #LM "," #S "," #FM ">"}
-
 #define CMTJ_KL(R, PREC, L, S, FM)                                               \
-  {PREC, L, L, 0, S, FM, false, (const void*)&cmtj::k_junction_ll<R, L, S, FM, false>, \
+  {PREC, L, L, S, FM, false, (const void*)&cmtj::k_junction_ll<R, L, S, FM, false>, \
    "k_junction_ll<" #R "," #L "," #S "," #FM ">"}
 
 #define CMTJ_KL_SOLVERS(R, PREC, L, FM) \

@@ -259,12 +259,7 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
 
 typedef struct cmtj_plan cmtj_plan;
 
-enum cmtj_lanes {
-  CMTJ_LANES_AUTO = 0,
-  CMTJ_LANES_ONE = 1,
-  CMTJ_LANES_PER_LAYER = 2,
-  CMTJ_LANES_ONE_OCC = 3  /* one lane per instance, kernels compiled for >= 4 waves per SIMD where available */
-};
+enum cmtj_lanes { CMTJ_LANES_AUTO = 0, CMTJ_LANES_ONE = 1, CMTJ_LANES_PER_LAYER = 2 };
 
 typedef struct cmtj_plan_info {
   uint64_t n_instances;
@@ -289,8 +284,8 @@ typedef struct cmtj_plan_info {
  * which has no single-call counterpart in the reference.
  * lanes_per_instance: CMTJ_LANES_ONE (all layers of an instance in one
  * lane), CMTJ_LANES_PER_LAYER (one lane per layer, for multilayer
- * instances: lowest latency per instance) or CMTJ_LANES_AUTO (one lane per
- * layer while the batch is too small to fill the chip).
+ * instances) or CMTJ_LANES_AUTO (one lane per layer wherever a kernel for the
+ * group exists).
  * Returns NULL on error (see cmtj_last_error). */
 cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n,
                             const cmtj_layer* layers, size_t n_layer_rows,
