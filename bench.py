@@ -280,6 +280,8 @@ def main() -> int:
                          "one JSON line each")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="run the sweep kernel twice and a 1 GiB device copy, print nothing (for counter runs)")
+    ap.add_argument("--pmc-fixed-ns", type=float, default=0.0,
+                    help="with --pmc-pass: give every sweep point this duration (ns)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -301,7 +303,15 @@ def main() -> int:
         return 0
 
     # ---- FP32 batched, inputs resident in HBM -------------------------------
-    b = _pimm("fp32", args.points).set_lanes(args.lanes)
+    if args.pmc_pass and args.pmc_fixed_ns:
+        # every point the same duration: counters divide evenly by waves x steps
+        from cmtj_amd import workloads
+
+        T = args.pmc_fixed_ns * 1e-9
+        b = workloads.pimm(n_points=args.points, seed=0, t_min=T, t_max=T, precision="fp32", log="mx")
+        b.set_lanes(args.lanes)
+    else:
+        b = _pimm("fp32", args.points).set_lanes(args.lanes)
     plan = b.plan()
     if args.pmc_pass:
         return _pmc_pass(b)
