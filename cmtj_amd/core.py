@@ -1099,7 +1099,11 @@ class Junction:
         return out
 
     def _append(self, key: str, arr: np.ndarray) -> None:
-        self._log.setdefault(key, []).append(np.asarray(arr, dtype=np.float64))
+        # the reference creates a key on its first logged entry: a run that
+        # logs nothing leaves the log as it was (logLayerParams, junction.hpp:1580)
+        arr = np.asarray(arr, dtype=np.float64)
+        if arr.size:
+            self._log.setdefault(key, []).append(arr)
 
     def saveLog(self, filename: str) -> None:
         # saveLogs, core/junction.hpp:1638-1656

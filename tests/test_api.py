@@ -223,6 +223,12 @@ def test_step_and_log_counts_follow_fp64_truncation():
     assert int(jd["n_entries"]) == (999999 + 98) // 99
 
 
+def test_a_run_shorter_than_one_step_logs_nothing():
+    j = OracleJunction([_layer(mag=CVector(1, 0, 0))])
+    j.runSimulation(0.5e-13, 1e-13, 1e-13)  # (unsigned)(0.5) = 0 steps
+    assert j.getLog() == {}
+
+
 def test_log_times_and_continuation():
     j = OracleJunction([_layer(mag=CVector(1, 0, 0))])
     j.setLayerExternalFieldDriver("all", AxialDriver(0, 0, 1e5))

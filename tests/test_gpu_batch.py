@@ -285,3 +285,23 @@ def test_thermal_equilibrium_on_gpu(precision):
     want = 1 / math.tanh(x) - 1 / x
     print(f"[thermal] {precision}: <mz> = {mz:.4f}, Langevin = {want:.4f}", flush=True)
     assert abs(mz - want) < 0.01
+
+
+def test_zero_and_short_runs():
+    """totalTime below one step: no step, no entry, m unchanged (the
+    reference's truncation); a write frequency longer than the run logs only
+    the first step."""
+    j, kw = parity_cases.CASES["bilayer_iec_rk4"](Junction)
+    m0 = [j.getLayerMagnetisation(l).tolist() for l in j.getLayerIds()]
+    j.runSimulation(0.5e-13, 1e-13, 1e-13)
+    assert j.getLog() == {}
+    assert [j.getLayerMagnetisation(l).tolist() for l in j.getLayerIds()] == m0
+    js = [parity_cases.CASES["bilayer_iec_rk4"](Junction)[0] for _ in range(3)]
+    b = JunctionBatch.from_junctions(js, [0.0, 5e-13, 3e-12], 1e-13, [1e-13, 1e-11, 1e-12], SolverMode.RK4,
+                                     precision="fp64", log="full", log_dtype="f64")
+    r = b.run()
+    st = r.status()
+    assert list(st["steps_done"]) == [0, 5, 30] and list(st["entries_done"]) == [0, 1, 3]
+    olog, omag, ost = _oracle_blocks(b)
+    assert list(ost["steps_done"]) == [0, 5, 30] and list(ost["entries_done"]) == [0, 1, 3]
+    np.testing.assert_allclose(r.final_m.cpu().numpy()[:, :2], omag[:, :2], atol=1e-12)
