@@ -112,6 +112,7 @@ class _Group:
         self.incoming: list = []
         self.plan = self.arena = self.state = self.status = None
         self.state_host = self.status_host = None  # pinned mirrors, filled after every slice
+        self.nsteps = np.zeros(0, dtype=np.int64)
         self.index: dict = {}
         self.changed = False
         self.busy = False
@@ -239,35 +240,33 @@ class Server:
             g.index = {}
             return
         js = np.concatenate([r.jd for r in g.active])
-        layers, tabs = [], []
-        row = tab = 0
-        for i, r in enumerate(g.active):
-            L = r.layers.copy()
+        nls = js["n_layers"].astype(np.int64)
+        js["layer_offset"] = np.concatenate([[0], np.cumsum(nls)[:-1]]).astype(np.uint32)
+        js["log_offset"] = np.fromiter((r.log_off for r in g.active), dtype=np.uint64, count=n)
+        g.nsteps = js["n_steps"].astype(np.int64)
+        layers = np.concatenate([r.layers for r in g.active])
+        tabs, tab = [], 0
+        for i, r in enumerate(g.active):  # only requests with callback drivers carry tables
             if r.tables is not None:
-                for q in range(L.size):
-                    d = L[q]["drv"]
-                    sel = d["type"] == _lib.TABLE
-                    d["table_index"][sel] += tab
+                a0 = int(js["layer_offset"][i])
+                d = layers["drv"][a0: a0 + int(nls[i])]
+                d["table_index"][d["type"] == _lib.TABLE] += tab
                 tabs.append(r.tables)
                 tab += r.tables.size
-            js["layer_offset"][i] = row
-            js["log_offset"][i] = r.log_off
-            row += L.size
-            layers.append(L)
-        layers = np.concatenate(layers)
         tables = np.concatenate(tabs) if tabs else None
         nbytes = _lib.plan_arena_size(js, layers, 0 if tables is None else tables.size, self.precision,
                                       self.log_dtype, self.lanes)
         sz = _lib.STATE_DTYPE.itemsize
         host = torch.empty((n, sz), dtype=torch.uint8, pin_memory=True)
         hv = host.numpy()
-        old = g.state_host.numpy() if g.state_host is not None else None
+        kept = [(i, g.index[id(r)]) for i, r in enumerate(g.active) if r.state is None]
+        if kept:
+            dst, src = np.asarray(kept, dtype=np.int64).T
+            hv[dst] = g.state_host.numpy()[src]
         for i, r in enumerate(g.active):
             if r.state is not None:
                 hv[i] = r.state.view(np.uint8).reshape(sz)
                 r.state = None
-            else:
-                hv[i] = old[g.index[id(r)]]
         with torch.cuda.stream(g.stream):
             arena = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
             plan = _lib.Plan(js, layers, _lib.constants_get(), tables, self.precision, self.log_dtype, self.lanes,
@@ -283,26 +282,27 @@ class Server:
         """Called once the group's slice (and its status/state copies) are
         done: hands finished requests their results."""
         sh = g.status_host.numpy().view(_lib.STATUS_DTYPE).reshape(-1)
-        done = [i for i, r in enumerate(g.active)
-                if int(sh[i]["code"]) != _lib.OK or int(sh[i]["steps_done"]) >= int(r.jd["n_steps"][0])]
-        if not done:
+        done = np.flatnonzero((sh["code"] != _lib.OK) | (sh["steps_done"].astype(np.int64) >= g.nsteps))
+        if done.size == 0:
             return
+        done = done.tolist()
         states = g.state_host.numpy().view(_lib.STATE_DTYPE).reshape(-1)
+        sizes = [g.active[i].n_elems for i in done]
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        buf = torch.empty(int(offs[-1]), dtype=log.dtype, pin_memory=True)
         with torch.cuda.stream(g.stream):
-            blocks = []
-            for i in done:
+            for q, i in enumerate(done):  # one pinned buffer, one synchronisation
                 r = g.active[i]
-                hb = torch.empty(r.n_elems, dtype=log.dtype, pin_memory=True)
-                hb.copy_(log[r.log_off: r.log_off + r.n_elems], non_blocking=True)
-                blocks.append(hb)
+                buf[int(offs[q]): int(offs[q + 1])].copy_(log[r.log_off: r.log_off + r.n_elems], non_blocking=True)
         g.stream.synchronize()
+        allv = buf.numpy().astype(np.float64)
         t = time.perf_counter()
         for q, i in enumerate(done):
             r, st = g.active[i], sh[i]
             j = r.jd[0]
             names = _channel_names(r.meta, int(j["log_mask"]))
             ne = int(j["n_entries"])
-            blk = blocks[q].numpy().astype(np.float64)[: len(names) * ne].reshape(len(names), ne)
+            blk = allv[int(offs[q]): int(offs[q]) + len(names) * ne].reshape(len(names), ne)
             out = {k: blk[c] for c, k in enumerate(names)}
             if r.meta.mr_mode == _lib.MR_STRIP and int(j["log_mask"]) & _lib.LOG_MR:
                 out["Rz"] = np.zeros(ne)
