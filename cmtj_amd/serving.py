@@ -112,6 +112,7 @@ class _Group:
         self.incoming: list = []
         self.plan = self.arena = self.state = self.status = None
         self.state_host = self.status_host = None  # pinned mirrors, filled after every slice
+        self.state_pin = self.status_pin = None    # their grow-only backing buffers
         self.nsteps = np.zeros(0, dtype=np.int64)
         self.index: dict = {}
         self.changed = False
@@ -154,6 +155,7 @@ class Server:
         self.ticks = 0
         self.steps_done = 0
         self.host_s = {"admit": 0.0, "rebuild": 0.0, "retire": 0.0, "launch": 0.0}
+        self._stage = None  # pinned staging buffer for finished logs, grow-only
         self.error = None
 
     # --- client side ------------------------------------------------------------
@@ -257,12 +259,22 @@ class Server:
         nbytes = _lib.plan_arena_size(js, layers, 0 if tables is None else tables.size, self.precision,
                                       self.log_dtype, self.lanes)
         sz = _lib.STATE_DTYPE.itemsize
-        host = torch.empty((n, sz), dtype=torch.uint8, pin_memory=True)
+        # pinned host mirrors of state and status, grow-only per group (a
+        # pinned allocation costs far more than a slice's bookkeeping)
+        if g.state_pin is None or g.state_pin.shape[0] < n:
+            cap = max(n, 2 * (0 if g.state_pin is None else g.state_pin.shape[0]), 64)
+            new_pin = torch.empty((cap, sz), dtype=torch.uint8, pin_memory=True)
+            if g.state_host is not None:
+                new_pin[: g.state_host.shape[0]] = g.state_host
+            g.state_pin = new_pin
+            g.status_pin = torch.empty((cap, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, pin_memory=True)
+        host = g.state_pin[:n]
         hv = host.numpy()
         kept = [(i, g.index[id(r)]) for i, r in enumerate(g.active) if r.state is None]
         if kept:
             dst, src = np.asarray(kept, dtype=np.int64).T
-            hv[dst] = g.state_host.numpy()[src]
+            rows = g.state_pin.numpy()[src]  # fancy indexing copies: safe to write back in place
+            hv[dst] = rows
         for i, r in enumerate(g.active):
             if r.state is not None:
                 hv[i] = r.state.view(np.uint8).reshape(sz)
@@ -274,7 +286,7 @@ class Server:
             state = torch.empty((n, sz), dtype=torch.uint8, device=self.device)
             state.copy_(host, non_blocking=True)
             g.status = torch.zeros((n, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, device=self.device)
-        g.status_host = torch.empty((n, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, pin_memory=True)
+        g.status_host = g.status_pin[:n]
         g.plan, g.arena, g.state, g.state_host = plan, arena, state, host
         g.index = {id(r): i for i, r in enumerate(g.active)}
 
@@ -289,7 +301,10 @@ class Server:
         states = g.state_host.numpy().view(_lib.STATE_DTYPE).reshape(-1)
         sizes = [g.active[i].n_elems for i in done]
         offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-        buf = torch.empty(int(offs[-1]), dtype=log.dtype, pin_memory=True)
+        if self._stage is None or self._stage.numel() < int(offs[-1]):
+            self._stage = torch.empty(max(int(offs[-1]), 2 * (0 if self._stage is None else self._stage.numel()),
+                                          1 << 20), dtype=log.dtype, pin_memory=True)
+        buf = self._stage[: int(offs[-1])]
         with torch.cuda.stream(g.stream):
             for q, i in enumerate(done):  # one pinned buffer, one synchronisation
                 r = g.active[i]
