@@ -128,7 +128,7 @@ class Server:
     arena (elements of the log dtype); max_active: slots; slice_ms: target
     device time of one slice (each group adapts its step count to it);
     lanes: fixed lane mapping ("per_layer" or "one"); n_streams: streams
-    the groups are spread over."""
+    created up front (more are created when more groups are active)."""
 
     def __init__(self, precision: str = "fp32", log_elements: int = 1 << 28, max_active: int = 65536,
                  slice_ms: float = 20.0, slice_steps: int = 5000, lanes: str = "per_layer", n_streams: int = 8,
@@ -225,7 +225,10 @@ class Server:
             r.t_admit = time.perf_counter()
             g = self._groups.get(r.key)
             if g is None:
-                g = _Group(r.key, streams[len(self._groups) % len(streams)], self.slice_steps)
+                # a stream of its own: a group syncs its stream when it
+                # rebuilds or retires and must not wait for another group's slice
+                st = streams.pop() if streams else torch.cuda.Stream(device=self.device)
+                g = _Group(r.key, st, self.slice_steps)
                 self._groups[r.key] = g
             g.incoming.append(r)
             self._n_active += 1
@@ -404,6 +407,7 @@ class Server:
                         self.host_s["launch"] += clk() - t0
                         progressed = True
                     elif not g.incoming:
+                        streams.append(g.stream)  # back to the pool
                         del self._groups[key]
                 if not progressed:
                     time.sleep(0.0005)
