@@ -35,6 +35,12 @@ from .batch import _LANES, _LOG_MODES, JunctionBatch, _channel_names, _Meta
 from .core import SolverMode
 
 
+def _cat(arrs, dtype) -> np.ndarray:
+    """Concatenation of structured records through their bytes (numpy's
+    structured-dtype promotion is slow per array)."""
+    return np.concatenate([np.ascontiguousarray(a).view(np.uint8).reshape(-1) for a in arrs]).view(dtype)
+
+
 class LogArena:
     """First-fit allocator of element ranges in a fixed device log buffer."""
 
@@ -154,8 +160,8 @@ class Server:
         self._thread = None
         self.ticks = 0
         self.steps_done = 0
-        self.host_s = {"admit": 0.0, "rebuild": 0.0, "retire": 0.0, "launch": 0.0}
-        self._stage = None  # pinned staging buffer for finished logs, grow-only
+        self.host_s = {"admit": 0.0, "rebuild": 0.0, "retire": 0.0, "launch": 0.0, "complete": 0.0}
+        self._finishing: list = []  # (event, pinned log copies, offsets, requests) of retired requests
         self.error = None
 
     # --- client side ------------------------------------------------------------
@@ -244,12 +250,12 @@ class Server:
             g.plan = g.arena = g.state = g.status = g.state_host = g.status_host = None
             g.index = {}
             return
-        js = np.concatenate([r.jd for r in g.active])
+        js = _cat([r.jd for r in g.active], _lib.JUNCTION_DTYPE)
         nls = js["n_layers"].astype(np.int64)
         js["layer_offset"] = np.concatenate([[0], np.cumsum(nls)[:-1]]).astype(np.uint32)
         js["log_offset"] = np.fromiter((r.log_off for r in g.active), dtype=np.uint64, count=n)
         g.nsteps = js["n_steps"].astype(np.int64)
-        layers = np.concatenate([r.layers for r in g.active])
+        layers = _cat([r.layers for r in g.active], _lib.LAYER_DTYPE)
         tabs, tab = [], 0
         for i, r in enumerate(g.active):  # only requests with callback drivers carry tables
             if r.tables is not None:
@@ -295,53 +301,64 @@ class Server:
 
     def _retire(self, torch, g: _Group, log) -> None:
         """Called once the group's slice (and its status/state copies) are
-        done: hands finished requests their results."""
+        done: takes finished requests out of the group and starts the copies
+        of their logs; _complete hands out the results when those land."""
         sh = g.status_host.numpy().view(_lib.STATUS_DTYPE).reshape(-1)
         done = np.flatnonzero((sh["code"] != _lib.OK) | (sh["steps_done"].astype(np.int64) >= g.nsteps))
         if done.size == 0:
             return
         done = done.tolist()
         states = g.state_host.numpy().view(_lib.STATE_DTYPE).reshape(-1)
-        sizes = [g.active[i].n_elems for i in done]
-        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-        if self._stage is None or self._stage.numel() < int(offs[-1]):
-            self._stage = torch.empty(max(int(offs[-1]), 2 * (0 if self._stage is None else self._stage.numel()),
-                                          1 << 20), dtype=log.dtype, pin_memory=True)
-        buf = self._stage[: int(offs[-1])]
+        reqs = [(g.active[i], sh[i].copy(), states[i]["m"].copy()) for i in done]
+        offs = np.concatenate([[0], np.cumsum([r.n_elems for r, _, _ in reqs])]).astype(np.int64)
+        buf = torch.empty(int(offs[-1]), dtype=log.dtype, pin_memory=True)
         with torch.cuda.stream(g.stream):
-            for q, i in enumerate(done):  # one pinned buffer, one synchronisation
-                r = g.active[i]
+            for q, (r, _, _) in enumerate(reqs):
                 buf[int(offs[q]): int(offs[q + 1])].copy_(log[r.log_off: r.log_off + r.n_elems], non_blocking=True)
-        g.stream.synchronize()
-        allv = buf.numpy().astype(np.float64)
-        t = time.perf_counter()
-        for q, i in enumerate(done):
-            r, st = g.active[i], sh[i]
-            j = r.jd[0]
-            names = _channel_names(r.meta, int(j["log_mask"]))
-            ne = int(j["n_entries"])
-            blk = allv[int(offs[q]): int(offs[q]) + len(names) * ne].reshape(len(names), ne)
-            out = {k: blk[c] for c, k in enumerate(names)}
-            if r.meta.mr_mode == _lib.MR_STRIP and int(j["log_mask"]) & _lib.LOG_MR:
-                out["Rz"] = np.zeros(ne)
-            out["time"] = (np.arange(ne, dtype=np.float64) * int(j["write_every"])) * float(j["dt"])
-            nl = int(j["n_layers"])
-            if int(st["code"]) == _lib.ERR_NAN:
-                k = int(st["entries_done"])
-                e = RuntimeError("NAN magnetisation")
-                e.partial_log = {key: v[:k] for key, v in out.items()}
-                r.future.set_exception(e)
-            else:
-                self.steps_done += int(st["steps_done"])
-                r.future.set_result(Result(out, states[i]["m"][:nl].copy(), int(st["steps_done"]), r.t_submit,
-                                           r.t_admit, t))
-        with self._cv:
-            for i in done:
-                self.arena.free(g.active[i].log_off, g.active[i].n_elems)
-            self._n_active -= len(done)
+            ev = torch.cuda.Event()
+            ev.record(g.stream)
+        self._finishing.append((ev, buf, offs, reqs))
         gone = set(done)
         g.active = [r for i, r in enumerate(g.active) if i not in gone]
         g.changed = True
+
+    def _complete(self) -> bool:
+        """Results of retired requests whose log copies have landed; their
+        log blocks return to the arena only now."""
+        ready, waiting = [], []
+        for f in self._finishing:
+            (ready if f[0].query() else waiting).append(f)
+        if not ready:
+            return False
+        self._finishing = waiting
+        t = time.perf_counter()
+        freed = []
+        for ev, buf, offs, reqs in ready:
+            allv = buf.numpy().astype(np.float64)
+            for q, (r, st, m) in enumerate(reqs):
+                j = r.jd[0]
+                names = _channel_names(r.meta, int(j["log_mask"]))
+                ne = int(j["n_entries"])
+                blk = allv[int(offs[q]): int(offs[q]) + len(names) * ne].reshape(len(names), ne)
+                out = {k: blk[c] for c, k in enumerate(names)}
+                if r.meta.mr_mode == _lib.MR_STRIP and int(j["log_mask"]) & _lib.LOG_MR:
+                    out["Rz"] = np.zeros(ne)
+                out["time"] = (np.arange(ne, dtype=np.float64) * int(j["write_every"])) * float(j["dt"])
+                nl = int(j["n_layers"])
+                if int(st["code"]) == _lib.ERR_NAN:
+                    k = int(st["entries_done"])
+                    e = RuntimeError("NAN magnetisation")
+                    e.partial_log = {key: v[:k] for key, v in out.items()}
+                    r.future.set_exception(e)
+                else:
+                    self.steps_done += int(st["steps_done"])
+                    r.future.set_result(Result(out, m[:nl].copy(), int(st["steps_done"]), r.t_submit, r.t_admit, t))
+                freed.append(r)
+        with self._cv:
+            for r in freed:
+                self.arena.free(r.log_off, r.n_elems)
+            self._n_active -= len(freed)
+        return True
 
     def _launch(self, torch, g: _Group, log) -> None:
         """One slice on the group's stream, followed by the copies of status
@@ -374,13 +391,16 @@ class Server:
                 t0 = clk()
                 with self._cv:
                     self._admit(torch, streams)
-                    if not self._groups and not self._pending:
+                    idle = not self._groups and not self._pending
+                    if idle and not self._finishing:
                         if self._stop:
                             return
                         self._cv.wait(timeout=0.05)
                         continue
                 self.host_s["admit"] += clk() - t0
-                progressed = False
+                t0 = clk()
+                progressed = self._complete()
+                self.host_s["complete"] += clk() - t0
                 for key in list(self._groups):
                     g = self._groups[key]
                     if g.busy:
@@ -414,6 +434,11 @@ class Server:
         except BaseException as e:  # fail every open request rather than hang its caller
             self.error = e
             with self._cv:
+                for _, _, _, reqs in self._finishing:
+                    for r, _, _ in reqs:
+                        if not r.future.done():
+                            r.future.set_exception(e)
+                self._finishing = []
                 for g in self._groups.values():
                     for r in g.active + g.incoming:
                         if not r.future.done():
