@@ -184,6 +184,8 @@ EXPORTS = (
     "cmtj_plan_create",
     "cmtj_plan_arena_size",
     "cmtj_group_key",
+    "cmtj_host_alloc",
+    "cmtj_host_free",
     "cmtj_plan_create_in",
     "cmtj_plan_execute",
     "cmtj_plan_execute_slice",
@@ -222,8 +224,10 @@ def _load():
     ]
     lib.cmtj_plan_create_in.argtypes = [
         _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t,
-        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_size_t, _P,
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_size_t, _P, ctypes.c_int32,
     ]
+    lib.cmtj_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(_P), ctypes.POINTER(_P)]
+    lib.cmtj_host_free.argtypes = [_P]
     lib.cmtj_plan_create_in.restype = _P
     lib.cmtj_group_key.argtypes = [_P, _P, ctypes.POINTER(ctypes.c_uint64)]
     lib.cmtj_plan_execute.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P]
@@ -345,6 +349,29 @@ def group_key(jd, layers) -> int:
     return int(out.value)
 
 
+class HostBuffer:
+    """Pinned host memory mapped for the device (cmtj_host_alloc): `array`
+    is a uint8 numpy view of it, `device` the address kernels use."""
+
+    def __init__(self, nbytes: int) -> None:
+        h, d = _P(), _P()
+        check(lib().cmtj_host_alloc(int(nbytes), ctypes.byref(h), ctypes.byref(d)), "cmtj_host_alloc")
+        self.host, self.device, self.nbytes = int(h.value), int(d.value), int(nbytes)
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * self.nbytes).from_address(self.host))
+
+    def close(self) -> None:
+        if self.host:
+            self.array = None
+            lib().cmtj_host_free(self.host)
+            self.host = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Plan:
     """Owner of a cmtj_plan (device-resident descriptors of a batch).  With
     `arena` = (device pointer, bytes, stream) the descriptors go into that
@@ -361,11 +388,12 @@ class Plan:
                 ptr(tabs), 0 if tabs is None else tabs.size, precision, log_dtype, lanes_per_instance,
             )
         else:
-            d_arena, nbytes, stream = arena
+            d_arena, nbytes, stream = arena[:3]
+            on_host = len(arena) > 3 and bool(arena[3])
             self._h = lib().cmtj_plan_create_in(
                 ptr(self._js), self._js.size, ptr(self._layers), self._layers.size, ptr(consts),
                 ptr(tabs), 0 if tabs is None else tabs.size, precision, log_dtype, lanes_per_instance,
-                d_arena, nbytes, stream,
+                d_arena, nbytes, stream, int(on_host),
             )
         if not self._h:
             raise RuntimeError(f"cmtj_plan_create: {last_error()}")

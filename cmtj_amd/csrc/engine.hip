@@ -255,7 +255,7 @@ static void signature(const cmtj_junction& j, const cmtj_layer* L, const std::ve
 static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmtj_layer* layers,
                       size_t rows, const cmtj_constants* c, const double* tables, size_t n_table,
                       int prec, int log_dtype, int lanes, ArenaFn get, void* gctx, hipStream_t stream,
-                      size_t* size_only = nullptr) {
+                      size_t* size_only = nullptr, char* host_view = nullptr) {
   if (n == 0) return fail(CMTJ_ERR_INVALID, "empty batch");
   if (n > 0xFFFFFFFFull) return fail(CMTJ_ERR_INVALID, "too many instances");
   if (prec != CMTJ_FP64 && prec != CMTJ_FP32) return fail(CMTJ_ERR_INVALID, "unknown precision");
@@ -337,18 +337,26 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
   char* arena = (char*)get(off, gctx);
   if (!arena) return CMTJ_ERR_HIP;
   P->arena = arena;
-  HIP_TRY(hipMemcpyAsync(arena + o_js, js, n * sizeof(cmtj_junction), hipMemcpyHostToDevice, stream));
-  HIP_TRY(hipMemcpyAsync(arena + o_layers, layers, rows * sizeof(cmtj_layer), hipMemcpyHostToDevice, stream));
-  if (n_table)
-    HIP_TRY(hipMemcpyAsync(arena + o_tables, tables, n_table * sizeof(double), hipMemcpyHostToDevice, stream));
+  // host-mapped arena: the kernels read the descriptors through the mapping,
+  // the host writes them in place (no copy engine involved)
+  auto upload = [&](size_t off, const void* src, size_t bytes) -> int {
+    if (!bytes) return 0;
+    if (host_view) {
+      std::memcpy(host_view + off, src, bytes);
+      return 0;
+    }
+    HIP_TRY(hipMemcpyAsync(arena + off, src, bytes, hipMemcpyHostToDevice, stream));
+    return 0;
+  };
+  if (upload(o_js, js, n * sizeof(cmtj_junction))) return CMTJ_ERR_HIP;
+  if (upload(o_layers, layers, rows * sizeof(cmtj_layer))) return CMTJ_ERR_HIP;
+  if (n_table && upload(o_tables, tables, n_table * sizeof(double))) return CMTJ_ERR_HIP;
   P->groups.clear();
   std::string names;
   int max_lpi = 1;
   for (size_t g = 0; g < G; ++g) {
-    HIP_TRY(hipMemcpyAsync(arena + o_order[g], orders[g].data(), orders[g].size() * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemcpyAsync(arena + o_dyn[g], nums[g].data(), nums[g].size() * sizeof(DynRec),
-                           hipMemcpyHostToDevice, stream));
+    if (upload(o_order[g], orders[g].data(), orders[g].size() * sizeof(uint32_t))) return CMTJ_ERR_HIP;
+    if (upload(o_dyn[g], nums[g].data(), nums[g].size() * sizeof(DynRec))) return CMTJ_ERR_HIP;
     const cmtj_junction& j0 = js[orders[g][0]];
     const cmtj_layer* L0 = layers + j0.layer_offset;
     const bool loge = (j0.log_mask & CMTJ_LOG_ENERGY) != 0;
@@ -388,7 +396,7 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     }
   }
   // host staging vectors die with this function: finish the copies first
-  HIP_TRY(hipStreamSynchronize(stream));
+  if (!host_view) HIP_TRY(hipStreamSynchronize(stream));
   P->info.n_instances = n;
   P->info.log_elements = extent;
   P->info.total_steps = total_steps;
@@ -640,7 +648,7 @@ int cmtj_plan_arena_size(const cmtj_junction* js, size_t n, const cmtj_layer* la
 cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,
                                const cmtj_constants* c, const double* tables, size_t n_table, int32_t precision,
                                int32_t log_dtype, int32_t lanes_per_instance, void* d_arena, size_t arena_bytes,
-                               void* stream) {
+                               void* stream, int32_t arena_is_host) {
   if (!js || !layers || !c || !d_arena) {
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
@@ -649,16 +657,41 @@ cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_lay
     fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance) or 2 (one lane per layer)");
     return nullptr;
   }
+  void* dev = d_arena;
+  if (arena_is_host && hipHostGetDevicePointer(&dev, d_arena, 0) != hipSuccess) {
+    fail(CMTJ_ERR_INVALID, "arena is not host memory mapped for the device (use cmtj_host_alloc)");
+    return nullptr;
+  }
   cmtj_plan* P = new cmtj_plan();
   P->owns = false;
-  FixedArena fa{d_arena, arena_bytes};
+  FixedArena fa{dev, arena_bytes};
   const int r = build_plan(P, js, n, layers, n_layer_rows, c, tables, n_table, precision, log_dtype,
-                           lanes_per_instance, fixed_arena, &fa, (hipStream_t)stream);
+                           lanes_per_instance, fixed_arena, &fa, (hipStream_t)stream, nullptr,
+                           arena_is_host ? (char*)d_arena : nullptr);
   if (r) {
     delete P;
     return nullptr;
   }
   return P;
+}
+
+int cmtj_host_alloc(size_t bytes, void** host, void** device) {
+  if (!host || !device || bytes == 0) return fail(CMTJ_ERR_INVALID, "bad host allocation request");
+  *host = *device = nullptr;
+  if (hipHostMalloc(host, bytes, hipHostMallocMapped | hipHostMallocPortable | hipHostMallocCoherent) !=
+      hipSuccess)
+    return fail(CMTJ_ERR_HIP, "hipHostMalloc failed for " + std::to_string(bytes) + " bytes");
+  if (hipHostGetDevicePointer(device, *host, 0) != hipSuccess) {
+    (void)hipHostFree(*host);
+    *host = nullptr;
+    return fail(CMTJ_ERR_HIP, "hipHostGetDevicePointer failed");
+  }
+  return 0;
+}
+
+int cmtj_host_free(void* host) {
+  if (host && hipHostFree(host) != hipSuccess) return fail(CMTJ_ERR_HIP, "hipHostFree failed");
+  return 0;
 }
 
 int cmtj_plan_execute(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_mag_out,

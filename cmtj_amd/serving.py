@@ -116,9 +116,11 @@ class _Group:
         self.slice_steps = slice_steps
         self.active: list = []
         self.incoming: list = []
-        self.plan = self.arena = self.state = self.status = None
-        self.state_host = self.status_host = None  # pinned mirrors, filled after every slice
-        self.state_pin = self.status_pin = None    # their grow-only backing buffers
+        self.plan = None
+        # host-mapped buffers (cmtj_host_alloc): descriptors, slice state and
+        # status live in host memory that the kernels read and write directly
+        self.harena = self.hstate = self.hstatus = None
+        self.n = 0
         self.nsteps = np.zeros(0, dtype=np.int64)
         self.index: dict = {}
         self.changed = False
@@ -162,6 +164,7 @@ class Server:
         self.steps_done = 0
         self.host_s = {"admit": 0.0, "rebuild": 0.0, "retire": 0.0, "launch": 0.0, "complete": 0.0}
         self._finishing: list = []  # (event, pinned log copies, offsets, requests) of retired requests
+        self._hpool: list = []      # free host-mapped buffers
         self.error = None
 
     # --- client side ------------------------------------------------------------
@@ -239,16 +242,34 @@ class Server:
             g.incoming.append(r)
             self._n_active += 1
 
+    def _hget(self, nbytes: int, old=None):
+        """A host-mapped buffer of at least nbytes (old is kept if it is big
+        enough, else returned to the pool)."""
+        if old is not None and old.nbytes >= nbytes:
+            return old
+        if old is not None:
+            self._hpool.append(old)
+        fit = [b for b in self._hpool if b.nbytes >= nbytes]
+        if fit:
+            b = min(fit, key=lambda x: x.nbytes)
+            self._hpool.remove(b)
+            return b
+        return _lib.HostBuffer(max(1 << max(int(nbytes) - 1, 1).bit_length(), 1 << 16))
+
     def _rebuild(self, torch, g: _Group) -> None:
-        """Plan + state of the group's active set.  Carried-over requests keep
-        their state (from the host mirror of the last slice), new ones start
-        from their initial row.  Runs when the group's stream is idle."""
+        """Plan + state of the group's active set, written in place into the
+        group's host-mapped buffers.  Carried-over requests keep their state
+        row, new ones start from their initial row.  Runs while the group's
+        stream is idle (no launch of the group reads the buffers)."""
         g.active = g.active + g.incoming
         g.incoming = []
         n = len(g.active)
         if n == 0:
-            g.plan = g.arena = g.state = g.status = g.state_host = g.status_host = None
-            g.index = {}
+            for b in (g.harena, g.hstate, g.hstatus):
+                if b is not None:
+                    self._hpool.append(b)
+            g.plan = g.harena = g.hstate = g.hstatus = None
+            g.index, g.n = {}, 0
             return
         js = _cat([r.jd for r in g.active], _lib.JUNCTION_DTYPE)
         nls = js["n_layers"].astype(np.int64)
@@ -268,47 +289,38 @@ class Server:
         nbytes = _lib.plan_arena_size(js, layers, 0 if tables is None else tables.size, self.precision,
                                       self.log_dtype, self.lanes)
         sz = _lib.STATE_DTYPE.itemsize
-        # pinned host mirrors of state and status, grow-only per group (a
-        # pinned allocation costs far more than a slice's bookkeeping)
-        if g.state_pin is None or g.state_pin.shape[0] < n:
-            cap = max(n, 2 * (0 if g.state_pin is None else g.state_pin.shape[0]), 64)
-            new_pin = torch.empty((cap, sz), dtype=torch.uint8, pin_memory=True)
-            if g.state_host is not None:
-                new_pin[: g.state_host.shape[0]] = g.state_host
-            g.state_pin = new_pin
-            g.status_pin = torch.empty((cap, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, pin_memory=True)
-        host = g.state_pin[:n]
-        hv = host.numpy()
+        # state rows: gather the carried-over rows (a copy) before the buffer is rewritten
         kept = [(i, g.index[id(r)]) for i, r in enumerate(g.active) if r.state is None]
+        old_rows = None
         if kept:
             dst, src = np.asarray(kept, dtype=np.int64).T
-            rows = g.state_pin.numpy()[src]  # fancy indexing copies: safe to write back in place
-            hv[dst] = rows
+            old_rows = g.hstate.array[: g.n * sz].reshape(g.n, sz)[src]
+        g.hstate = self._hget(n * sz, g.hstate)
+        hv = g.hstate.array[: n * sz].reshape(n, sz)
+        if kept:
+            hv[dst] = old_rows
         for i, r in enumerate(g.active):
             if r.state is not None:
                 hv[i] = r.state.view(np.uint8).reshape(sz)
                 r.state = None
-        with torch.cuda.stream(g.stream):
-            arena = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
-            plan = _lib.Plan(js, layers, _lib.constants_get(), tables, self.precision, self.log_dtype, self.lanes,
-                             arena=(arena.data_ptr(), nbytes, g.stream.cuda_stream))
-            state = torch.empty((n, sz), dtype=torch.uint8, device=self.device)
-            state.copy_(host, non_blocking=True)
-            g.status = torch.zeros((n, _lib.STATUS_DTYPE.itemsize), dtype=torch.uint8, device=self.device)
-        g.status_host = g.status_pin[:n]
-        g.plan, g.arena, g.state, g.state_host = plan, arena, state, host
+        g.hstatus = self._hget(n * _lib.STATUS_DTYPE.itemsize, g.hstatus)
+        g.hstatus.array[: n * _lib.STATUS_DTYPE.itemsize] = 0
+        g.harena = self._hget(nbytes, g.harena)
+        g.plan = _lib.Plan(js, layers, _lib.constants_get(), tables, self.precision, self.log_dtype, self.lanes,
+                           arena=(g.harena.host, g.harena.nbytes, g.stream.cuda_stream, True))
+        g.n = n
         g.index = {id(r): i for i, r in enumerate(g.active)}
 
     def _retire(self, torch, g: _Group, log) -> None:
         """Called once the group's slice (and its status/state copies) are
         done: takes finished requests out of the group and starts the copies
         of their logs; _complete hands out the results when those land."""
-        sh = g.status_host.numpy().view(_lib.STATUS_DTYPE).reshape(-1)
+        sh = g.hstatus.array[: g.n * _lib.STATUS_DTYPE.itemsize].view(_lib.STATUS_DTYPE)
         done = np.flatnonzero((sh["code"] != _lib.OK) | (sh["steps_done"].astype(np.int64) >= g.nsteps))
         if done.size == 0:
             return
         done = done.tolist()
-        states = g.state_host.numpy().view(_lib.STATE_DTYPE).reshape(-1)
+        states = g.hstate.array[: g.n * _lib.STATE_DTYPE.itemsize].view(_lib.STATE_DTYPE)
         reqs = [(g.active[i], sh[i].copy(), states[i]["m"].copy()) for i in done]
         offs = np.concatenate([[0], np.cumsum([r.n_elems for r, _, _ in reqs])]).astype(np.int64)
         buf = torch.empty(int(offs[-1]), dtype=log.dtype, pin_memory=True)
@@ -361,17 +373,14 @@ class Server:
         return True
 
     def _launch(self, torch, g: _Group, log) -> None:
-        """One slice on the group's stream, followed by the copies of status
-        and state into the group's pinned host mirrors."""
+        """One slice on the group's stream; status and state land in the
+        group's host-mapped buffers."""
         if g.ev1 is None:
             g.ev0 = torch.cuda.Event(enable_timing=True)
             g.ev1 = torch.cuda.Event(enable_timing=True)
         g.ev0.record(g.stream)
-        g.plan.execute_slice(log.data_ptr(), log.numel(), None, g.status.data_ptr(), g.state.data_ptr(),
-                             g.slice_steps, [g.stream.cuda_stream])
-        with torch.cuda.stream(g.stream):
-            g.status_host.copy_(g.status, non_blocking=True)
-            g.state_host.copy_(g.state, non_blocking=True)
+        g.plan.execute_slice(log.data_ptr(), log.numel(), None, g.hstatus.device, g.hstate.device, g.slice_steps,
+                             [g.stream.cuda_stream])
         g.ev1.record(g.stream)
         g.busy = True
         g.slices += 1
@@ -447,6 +456,19 @@ class Server:
                     if not r.future.done():
                         r.future.set_exception(e)
                 self._groups, self._pending = {}, deque()
+        finally:
+            # host-mapped buffers go back to the driver once no launch can read them
+            try:
+                torch.cuda.synchronize()
+            except Exception:
+                pass
+            for g in self._groups.values():
+                for b in (g.harena, g.hstate, g.hstatus):
+                    if b is not None:
+                        self._hpool.append(b)
+            for b in self._hpool:
+                b.close()
+            self._hpool = []
 
 
 __all__ = ["Server", "Result", "LogArena"]

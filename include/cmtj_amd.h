@@ -316,15 +316,26 @@ int cmtj_plan_arena_size(const cmtj_junction* js, size_t n, const cmtj_layer* la
                          size_t n_table, int32_t precision, int32_t log_dtype, int32_t lanes_per_instance,
                          size_t* bytes);
 
-/* As cmtj_plan_create, but the descriptors go into caller-owned device
- * memory d_arena (arena_bytes >= cmtj_plan_arena_size), uploaded on `stream`
- * (a hipStream_t, synchronised before return).  The plan does not own the
- * arena: it must outlive every execute of the plan.  For callers that keep
- * their own device pools (the serving loop): no device allocation. */
+/* As cmtj_plan_create, but the descriptors go into caller-owned memory
+ * d_arena (arena_bytes >= cmtj_plan_arena_size): device memory, uploaded on
+ * `stream` (a hipStream_t, synchronised before return), or, with
+ * arena_is_host, host memory from cmtj_host_alloc (its host address), written
+ * in place and read by the kernels through the mapping (no copy engine, no
+ * synchronisation; the caller must not rewrite it while a launch of the
+ * plan runs).  The plan does not own the arena: it must outlive every
+ * execute of the plan.  No reference counterpart: for callers that keep
+ * their own pools (the serving loop); no allocation. */
 cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_layer* layers, size_t n_layer_rows,
                                const cmtj_constants* c, const double* tables, size_t n_table, int32_t precision,
                                int32_t log_dtype, int32_t lanes_per_instance, void* d_arena, size_t arena_bytes,
-                               void* stream);
+                               void* stream, int32_t arena_is_host);
+
+/* Pinned host memory mapped into the device's address space (hipHostMalloc,
+ * coherent): *host is the host address, *device the address kernels use.
+ * For plan arenas, slice state and status that the host reads and writes
+ * between launches without copies.  No reference counterpart. */
+int cmtj_host_alloc(size_t bytes, void** host, void** device);
+int cmtj_host_free(void* host);
 
 /* Advances every instance of the plan by at most max_steps steps from its
  * state d_state[n] (read, then written back), logging into d_log as
