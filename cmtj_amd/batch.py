@@ -167,8 +167,9 @@ class JunctionBatch:
         if tb is not None:
             raise ValueError("callback drivers cannot be replicated; use from_junctions")
         nl = L.size
-        js = np.repeat(jd.reshape(1), n)
-        layers = np.tile(L, n)
+        # byte-level copies: numpy's structured-dtype repeat is element-wise and slow
+        js = np.tile(np.ascontiguousarray(jd.reshape(1)).view(np.uint8), n).view(_lib.JUNCTION_DTYPE)
+        layers = np.tile(np.ascontiguousarray(L).view(np.uint8), n).view(_lib.LAYER_DTYPE)
         js["layer_offset"] = np.arange(n, dtype=np.uint32) * nl
         js["stream"] = np.arange(n, dtype=np.uint64)
         b = cls(js, layers, None, [meta], np.zeros(n, dtype=np.int32), precision, log, log_dtype)
