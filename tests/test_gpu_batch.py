@@ -101,6 +101,12 @@ def test_fp32_bulk_within_1e4_of_fp64(builder, lanes):
 def _workload(name):
     if name == "pimm":
         return workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp64", log="mx", log_dtype="f64")
+    if name == "pimm_delayed":  # the Oersted step switches on and off mid-run
+        b = workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp64", log="mx", log_dtype="f64")
+        n = len(b.js)
+        b.set_driver("all", "oersted", "t_start", np.full((n, 3), 5.05e-12))
+        b.set_driver("all", "oersted", "t_stop", np.full((n, 3), 2.0e-11))
+        return b
     if name == "r_h_loop":
         return workloads.r_h_loop(n_fields=11, total_time=0.5e-9, precision="fp64")
     if name == "vsd":
@@ -112,7 +118,7 @@ def _workload(name):
 
 
 @pytest.mark.parametrize("lanes", ["one", "per_layer"])
-@pytest.mark.parametrize("name", ["pimm", "r_h_loop", "vsd", "sto", "cims"])
+@pytest.mark.parametrize("name", ["pimm", "pimm_delayed", "r_h_loop", "vsd", "sto", "cims"])
 def test_workload_kernels_match_oracle(name, lanes):
     """The kernels specialised to the workloads' exact feature sets against
     the oracle (the CIMS ensemble pathwise: same Philox stream)."""
