@@ -271,7 +271,7 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2048)
-    ap.add_argument("--lanes", default="auto", choices=["auto", "one", "per_layer"])
+    ap.add_argument("--lanes", default="auto", choices=["auto", "one", "per_layer", "per_component"])
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)

@@ -100,7 +100,7 @@ class BatchResult:
         return self.log[: n * nch * int(ne[0])].view(n, nch, int(ne[0]))[:, c, :]
 
 
-_LANES = {"auto": 0, "one": 1, "per_layer": 2}
+_LANES = {"auto": 0, "one": 1, "per_layer": 2, "per_component": 3}
 
 
 class JunctionBatch:
@@ -301,9 +301,12 @@ class JunctionBatch:
         return b
 
     def set_lanes(self, lanes: str) -> "JunctionBatch":
-        """Lane mapping of multilayer instances: "one" (all layers of an
-        instance in one GPU lane), "per_layer" (one lane per layer) or "auto"
-        (per layer wherever a kernel for the group exists)."""
+        """Lane mapping of the instances: "one" (all layers of an instance in
+        one GPU lane), "per_layer" (one lane per layer), "per_component" (one
+        lane per vector component of each layer; binary32 RK4 groups of the
+        lean feature set) or "auto" (per component where such a kernel covers
+        the group and the batch leaves SIMDs idle, else per layer for
+        multilayer instances)."""
         if lanes not in _LANES:
             raise ValueError(f"lanes must be one of {sorted(_LANES)}")
         self.lanes = lanes

@@ -66,29 +66,43 @@ static int rank(const KEntry& k) {
          __builtin_popcount(k.fm & F_ALL);
 }
 
+// SIMDs of the target (MI355X: 256 CUs x 4): a batch whose waves number at
+// most this many runs one wave per SIMD.
+constexpr uint64_t kSimds = 1024;
+
 static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, uint32_t kinds, int ndyn, bool loge,
                                  int lanes, uint64_t n_inst) {
-  static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),    kernels_spec(),
-                                   kernels_ll_f64(), kernels_ll_f32(), kernels_ll_spec(), kernels_exact()};
-  const KEntry* best[2] = {nullptr, nullptr};  // [0] one lane per instance, [1] one lane per layer
+  static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),
+                                   kernels_spec(),   kernels_ll_f64(),  kernels_ll_f32(),
+                                   kernels_ll_spec(), kernels_exact(),  kernels_cl()};
+  // [0] one lane per instance, [1] one lane per layer, [2] one lane per component
+  const KEntry* best[3] = {nullptr, nullptr, nullptr};
   for (const KTable& t : kTables)
     for (size_t q = 0; q < t.n; ++q) {
       const KEntry& k = t.e[q];
       if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
       if (!covers(k, flags, kinds, ndyn)) continue;
-      const KEntry*& b = best[k.lpi > 1];
+      const int cat = (k.lpi == 1) ? 0 : (k.lpi == k.lm ? 1 : 2);
+      const KEntry*& b = best[cat];
       if (!b || rank(k) < rank(*b)) b = &k;
     }
-  if (!best[0] || !best[1]) return best[0] ? best[0] : best[1];  // the only kernel that covers the group
-  if (nl < 2 || lanes == CMTJ_LANES_ONE) return best[0];
-  // CMTJ_LANES_PER_LAYER, and the automatic choice: on a batch too small to
-  // fill the chip the per-layer split divides the latency of a wave step,
-  // which sets the sweep time; on full-chip batches it measured no slower
-  // (1M-instance CIMS ensemble 6.6e10 vs 6.3e10 junction-steps/s, 65,536-point
-  // PIMM sweep 2.56e10 vs 2.44e10 on MI355X): its register footprint is
-  // smaller, so more waves fit per SIMD.
-  (void)n_inst;
-  return best[1];
+  if (lanes == CMTJ_LANES_ONE && best[0]) return best[0];
+  if (lanes == CMTJ_LANES_PER_COMPONENT && best[2]) return best[2];
+  // automatic choice: on a batch too small to fill the chip, the time of a
+  // sweep is its longest instance's step count times the latency of one
+  // wave step.  One lane per component issues about half the instructions
+  // per wave step of one lane per layer, at four times the lanes: it wins
+  // while its waves still fit one per SIMD, and costs about twice the
+  // instructions per junction-step beyond that.
+  if (lanes == CMTJ_LANES_AUTO && best[2] && n_inst * (uint64_t)best[2]->lpi <= 64 * kSimds) return best[2];
+  // one lane per layer for multilayer instances: on full-chip batches it
+  // measured no slower than one lane per instance (1M-instance CIMS ensemble
+  // 6.6e10 vs 6.3e10 junction-steps/s, 65,536-point PIMM sweep 2.56e10 vs
+  // 2.44e10 on MI355X): its register footprint is smaller, so more waves fit
+  // per SIMD.
+  if (best[1] && (nl >= 2 || !best[0])) return best[1];
+  if (best[0]) return best[0];
+  return best[1] ? best[1] : best[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -593,8 +607,10 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n, const cmtj_layer*
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
   }
-  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_LAYER) {
-    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance) or 2 (one lane per layer)");
+  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_COMPONENT) {
+    fail(CMTJ_ERR_INVALID,
+         "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per layer) or 3 (one lane "
+         "per component)");
     return nullptr;
   }
   cmtj_plan* P = new cmtj_plan();
@@ -653,8 +669,10 @@ cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_lay
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
   }
-  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_LAYER) {
-    fail(CMTJ_ERR_INVALID, "lanes_per_instance must be 0 (auto), 1 (one lane per instance) or 2 (one lane per layer)");
+  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_COMPONENT) {
+    fail(CMTJ_ERR_INVALID,
+         "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per layer) or 3 (one lane "
+         "per component)");
     return nullptr;
   }
   void* dev = d_arena;

@@ -1,0 +1,352 @@
+// kernels_cl.h -- the junction kernel with one vector component per lane.
+//
+// Layer l of an instance occupies the lane quad 4l..4l+3: lanes 0, 1, 2 of
+// the quad hold the x, y and z components of every per-layer vector (m, the
+// effective field, the torques), lane 3 holds zeros.  An instance of at most
+// L layers spans 4L lanes (L = 1, 2 or 4: 4L divides a 16-lane DPP row).
+//
+// The vector algebra of a step maps onto DPP quad permutations, which the
+// compiler folds into the arithmetic instruction that consumes them:
+//   (a x b)_c = a_{c+1} b_{c+2} - a_{c+2} b_{c+1}  -- rotations [1,2,0,3] and
+//                                                    [2,0,1,3] of the quad
+//   a . b      = butterfly sum of a_c b_c over the quad (swaps [1,0,3,2],
+//                then [2,3,0,1]); every lane of the quad gets the same sum
+// and the neighbour layers (start-of-step values, core/junction.hpp:
+// 1682-1707) arrive by a row shift of 4 lanes.  A step of one wave then
+// issues about half the instructions of the one-layer-per-lane kernel, and a
+// batch occupies four times as many waves: on a batch too small to fill the
+// chip both shorten the sweep, whose time is the step count of the longest
+// instance times the latency of one wave step.
+//
+// Integrator: the anchored binary32 RK4 of the other FP32 kernels (rk4_step,
+// core/junction.hpp:1017-1041): k1 and the state in binary64, stages 2-4 as
+// binary32 increments f(m0 + d, t_s) - f(m0, t) (rhs_delta in kernels.h,
+// same terms, component form).  Features: demagnetisation with a diagonal
+// tensor, first-order anisotropy, bilinear IEC with constant coupling,
+// time-dependent drivers that feed field slots only; log channels m and the
+// classic or strip magnetoresistance.  Sums over the quad associate
+// differently from the sequential dot products of the other kernels, so
+// results agree with them to rounding, not bitwise.
+#pragma once
+
+#include "kernels_ll.h"
+
+namespace cmtj {
+
+constexpr int QP_ROT1 = 0xC9;   // quad_perm [1,2,0,3]: component c reads c + 1 (mod 3)
+constexpr int QP_ROT2 = 0xD2;   // quad_perm [2,0,1,3]: component c reads c + 2 (mod 3)
+constexpr int QP_SWAP1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int QP_SWAP2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int QP_BC0 = 0x00;    // quad_perm [0,0,0,0]: x to every lane of the quad
+constexpr int QP_BC1 = 0x55;    // [1,1,1,1]: y
+constexpr int QP_BC2 = 0xAA;    // [2,2,2,2]: z
+constexpr int DPP_ROW_SHR4 = 0x114;  // lane i reads lane i - 4 (0 outside the row)
+constexpr int DPP_ROW_SHL4 = 0x104;  // lane i reads lane i + 4
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_x(double v) {
+  return dpp_d<CTRL>(v);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_x(float v) {
+  return dpp_f<CTRL>(v);
+}
+
+// Sum over the quad, identical in its four lanes: (v0 + v1) + (v2 + v3).
+template <class T>
+__device__ __forceinline__ T qsum(T v) {
+  v = v + dpp_x<QP_SWAP1>(v);
+  return v + dpp_x<QP_SWAP2>(v);
+}
+
+// (a x b) at this lane's component, given a's rotated components a1 = a_{c+1}
+// and a2 = a_{c+2}.
+template <class T>
+__device__ __forceinline__ T cross_c(T a1, T a2, T b) {
+  return a1 * dpp_x<QP_ROT2>(b) - a2 * dpp_x<QP_ROT1>(b);
+}
+
+template <class T>
+__device__ __forceinline__ T comp(V3<T> v, int c) {
+  return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : T(0)));
+}
+
+// Whether record k (meta: kind | slot << 8 | layer mask << 16) feeds
+// component c of layer ly's field.
+__device__ __forceinline__ bool feeds(int meta, int ly, int c) {
+  const int type = meta & 0xff;
+  const int slot = (meta >> 8) & 0xff;
+  const int lmask = (meta >> 16) & 0xff;
+  return type != CMTJ_CONSTANT && slot < CMTJ_SLOT_K && slot % 3 == c && ((lmask >> ly) & 1);
+}
+
+// Per-lane constants of one layer component.
+struct CLConst {
+  double hb, an, nms, alpha, gconv, kan, jb, jt, ref;  // binary64 anchor
+  float anf, nmsf, alphaf, gconvf, kanf;               // binary32 increments
+};
+
+template <int L, uint32_t FM>
+__global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
+  static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
+  constexpr int LPI = 4 * L;
+  const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t inst = gl / LPI;
+  if (inst >= g.n) return;
+  const int q = (int)(gl % LPI);
+  const int ly = q >> 2, c = q & 3;
+  const uint32_t fl = g.flags;
+  const int nl = g.nl;
+  const bool lact = ly < nl;          // the lane's layer exists
+  const bool live = lact && c < 3;    // the lane holds a component
+  const int lyr = lact ? ly : 0;
+  const uint32_t jid = g.order[inst];
+  const cmtj_junction* J = g.js + jid;
+  const uint32_t nsteps = J->n_steps;
+  const uint32_t W = J->write_every;
+  const uint32_t NE = J->n_entries;
+  const double dt = J->dt;
+  const uint64_t lbase = J->log_offset;
+  const bool logging = (g.log != nullptr) && g.log_mask != 0;
+  const bool f32log = g.log_f32;
+  const uint32_t esize = f32log ? 4u : 8u;
+  char* const lp = logging ? (char*)g.log + lbase * esize : nullptr;
+  const uint64_t cs = (uint64_t)NE * esize;
+  const double Rp = J->Rp, Rap = J->Rap;
+  constexpr int sps = 3;
+  const bool hasB = lact && ly > 0, hasT = lact && ly + 1 < nl;
+  const uint32_t gshift = threadIdx.x & ~(uint32_t)(LPI - 1);
+
+  // per-lane constants (loadLayer parameters of this layer, this component)
+  CLConst K;
+  double m;
+  {
+    const cmtj_layer& Ly = g.layers[J->layer_offset + lyr];
+    LP<double> P;
+    DV<double> base;
+    V3<double> m3;
+    load_layer<double>(Ly, g.c, P, base, m3);
+    const double z = 0.0;
+    m = live ? comp(m3, c) : z;
+    K.hb = live ? comp(base.H, c) : z;
+    K.an = live ? comp(P.anis, c) : z;
+    K.nms = live ? comp(mk<double>(P.N[0], P.N[4], P.N[8]), c) * P.ms_mu0 : z;
+    K.alpha = P.alpha;
+    K.gconv = P.gconv;
+    K.kan = live ? (2.0 * base.K) * P.inv_ms : z;
+    K.jb = live ? base.Jb * P.inv_mst : z;
+    K.jt = live ? base.Jt * P.inv_mst : z;
+    K.ref = live ? comp(mk<double>(Ly.ref[0], Ly.ref[1], Ly.ref[2]), c) : z;
+    K.anf = (float)K.an;
+    K.nmsf = (float)K.nms;
+    K.alphaf = (float)K.alpha;
+    K.gconvf = (float)K.gconv;
+    K.kanf = (float)K.kan;
+  }
+  DynRec dr[DREG];
+  bool sel[DREG];
+  double win_lo = __builtin_inf(), win_hi = -__builtin_inf();
+#pragma unroll
+  for (int k = 0; k < DREG; ++k) {
+    if (k < g.ndyn) {
+      dr[k] = g.dyn[(size_t)k * g.n + inst];
+      win_lo = fmin(win_lo, dr[k].p1);
+      win_hi = fmax(win_hi, dr[k].p2);
+      sel[k] = live && feeds(g.dmeta[k], ly, c);
+    } else {
+      dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
+      sel[k] = false;
+    }
+  }
+  const float dtR = (float)dt;
+  const float sixth_inv = 1.0f / 6.0f;
+  uint32_t k = 0;
+  int code = CMTJ_OK, nan_layer = -1;
+  uint32_t i = 0, iend = nsteps;
+  cmtj_state* const S = g.state ? g.state + jid : nullptr;
+  if (S) {  // resume: state of the previous slice
+    m = live ? S->m[lyr][c] : 0.0;
+    i = S->step;
+    k = S->entries;
+    code = S->code;
+    nan_layer = (code == CMTJ_OK) ? -1 : S->nan_layer;
+    iend = (code != CMTJ_OK) ? i : (uint32_t)min((uint64_t)nsteps, (uint64_t)i + g.slice);
+  }
+  uint64_t next_log = (uint64_t)k * W;
+  double bot = dpp_d<DPP_ROW_SHR4>(m), top = dpp_d<DPP_ROW_SHL4>(m);
+  bot = hasB ? bot : 0.0;
+  top = hasT ? top : 0.0;
+
+  for (; i < iend; ++i) {
+    const double t = (double)i * dt;
+    // ---- anchor: dm/dt at (m, t) in binary64 (calculateHeffDipoleInjection,
+    // core/junction.hpp:590-622; solveLLG, :716-815)
+    double v0[DREG] = {0.0, 0.0};
+    double H = K.hb;
+    bool inwin = true;
+    if constexpr ((FM & F_KSTEP) != 0) inwin = t >= win_lo && t <= win_hi;
+    if (CMTJ_HAS(F_DYNH) && inwin) {
+#pragma unroll
+      for (int r = 0; r < DREG; ++r) {
+        if (r < g.ndyn) {
+          v0[r] = wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, t, i, 0, sps);
+          H = H + (sel[r] ? v0[r] : 0.0);
+        }
+      }
+      for (int r = DREG; !(FM & F_EXACT) && r < g.ndyn; ++r) {
+        const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+        const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, rr, g, t, i, 0, sps);
+        H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
+      }
+    }
+    const double m1 = dpp_d<QP_ROT1>(m), m2 = dpp_d<QP_ROT2>(m);
+    double u = 0.0;
+    if (CMTJ_HAS(F_ANIS)) {
+      u = qsum(K.an * m);
+      H = H + K.an * (K.kan * u);
+    }
+    if (CMTJ_HAS(F_IEC)) H = H + (bot * K.jb + top * K.jt);
+    H = H - K.nms * m;
+    const double P0 = cross_c(m1, m2, H);
+    const double Q0 = cross_c(m1, m2, P0);
+    const double k1 = ((P0 + Q0 * K.alpha) * K.gconv) * dt;
+
+    // ---- stages 2-4: binary32 increments about the anchor
+    const float m0a = (float)m1, m0b = (float)m2;
+    const float H0 = (float)H, Pf = (float)P0;
+    auto delta = [&](float d, float ddH) -> float {
+      // f(m0 + d, ts) - f(m0, t), component c (rhs_delta, kernels.h)
+      float dH = ddH - K.nmsf * d;
+      if (CMTJ_HAS(F_ANIS)) dH = dH + K.anf * (K.kanf * qsum(K.anf * d));
+      const float d1 = dpp_f<QP_ROT1>(d), d2 = dpp_f<QP_ROT2>(d);
+      const float Hs = H0 + dH;
+      const float dP = cross_c(m0a, m0b, dH) + cross_c(d1, d2, Hs);
+      const float dQ = cross_c(m0a, m0b, dP) + cross_c(d1, d2, Pf + dP);
+      return ((dP + dQ * K.alphaf) * K.gconvf) * dtR;
+    };
+    auto drive_delta = [&](double ts, int stage) -> float {
+      float dd = 0.f;
+      if (!CMTJ_HAS(F_DYNH)) return dd;
+      if constexpr ((FM & F_KSTEP) != 0) {
+        if (!(ts >= win_lo && t <= win_hi)) return dd;
+      }
+#pragma unroll
+      for (int r = 0; r < DREG; ++r) {
+        if (r < g.ndyn) {
+          const float v = (float)(wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, ts, i, stage, sps) - v0[r]);
+          dd = dd + (sel[r] ? v : 0.f);
+        }
+      }
+      for (int r = DREG; !(FM & F_EXACT) && r < g.ndyn; ++r) {
+        const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+        const int ty = g.dmeta[r] & 0xff;
+        const float v = (float)(wave_value<double, true, FM>(ty, rr, g, ts, i, stage, sps) -
+                                wave_value<double, true, FM>(ty, rr, g, t, i, 0, sps));
+        dd = dd + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.f);
+      }
+      return dd;
+    };
+    const float dmid = drive_delta(t + 0.5 * dt, 1);
+    const float d2 = (float)(k1 * 0.5);
+    float dk = delta(d2, dmid);
+    float acc = dk * 2.f;
+    const float d3 = d2 + dk * 0.5f;
+    dk = delta(d3, dmid);
+    acc = acc + dk * 2.f;
+    const float dend = drive_delta(t + dt, 2);
+    const float d4 = (float)k1 + dk;
+    acc = acc + delta(d4, dend);
+    double mn = m + k1 + (double)(acc * sixth_inv);
+    // normalize_fast (kernels.h) over the quad
+    {
+      const double s = qsum(mn * mn);
+      if (s != 0.0) {
+        double y = (double)__frsqrt_rn((float)s);
+        y = y * (1.5 - 0.5 * s * y * y);
+        y = y * (1.5 - 0.5 * s * y * y);
+        mn = mn * y;
+      }
+    }
+    // the reference updates layers in turn and stops at the first NaN
+    // layer; layers up to and including it take the new value
+    const unsigned long long bad = __ballot(live && c == 0 && isnan(mn));
+    const uint32_t gb = (uint32_t)((bad >> gshift) & ((LPI == 64) ? ~0ull : ((1ull << LPI) - 1)));
+    if (gb) {
+      nan_layer = __builtin_ctz(gb) >> 2;
+      if (ly <= nan_layer) m = mn;
+      code = CMTJ_ERR_NAN;
+      break;
+    }
+    m = mn;
+    bot = dpp_d<DPP_ROW_SHR4>(m);
+    top = dpp_d<DPP_ROW_SHL4>(m);
+    bot = hasB ? bot : 0.0;
+    top = hasT ? top : 0.0;
+    if ((uint64_t)i == next_log) {
+      if (logging) {
+        // logLayerParams (core/junction.hpp:1580-1636)
+        char* p = lp + (uint64_t)k * esize;
+        const int lmask = g.log_mask, mr = g.mr_mode;
+        const int nm = ((lmask & CMTJ_LOG_MX) ? 1 : 0) + ((lmask & CMTJ_LOG_MYZ) ? 2 : 0);
+        if (live && nm) {
+          const int ch = (lmask & CMTJ_LOG_MX) ? c : c - 1;
+          if (ch >= 0 && ((lmask & CMTJ_LOG_MYZ) || c == 0)) log_put(f32log, p + (uint64_t)(ly * nm + ch) * cs, m);
+        }
+        p += (uint64_t)(nl * nm) * cs;
+        if (lmask & CMTJ_LOG_MR) {
+          if (mr == CMTJ_MR_CLASSIC) {
+            // calculateMagnetoresistance (:1865-1867): layer 0 against layer
+            // 1, one layer against the fixed reference vector (:1623-1629)
+            const double cosT = qsum(m * (nl == 1 ? K.ref : top));
+            if (q == 0) log_put(f32log, p, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
+          } else if (mr == CMTJ_MR_STRIP) {
+            // stripMagnetoResistance (:1835-1857), parallel sum of the layers
+            const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
+            double rx = 0.0, ry = 0.0;
+            if (lact && c == 0) {
+              const double Rx = J->Rx0[ly] + J->AMR_X[ly] * (mx * mx) + J->SMR_X[ly] * (my * my);
+              const double Ry = J->Ry0[ly] + 0.5 * J->AHE[ly] * mz + (J->AMR_Y[ly] + J->SMR_Y[ly]) * mx * my;
+              rx = 1. / Rx;
+              ry = 1. / Ry;
+            }
+#pragma unroll
+            for (int o = 4; o < LPI; o <<= 1) {
+              rx += __shfl_xor(rx, o, 64);
+              ry += __shfl_xor(ry, o, 64);
+            }
+            if (q == 0) {
+              log_put(f32log, p, 1 / rx);
+              log_put(f32log, p + cs, 1 / ry);
+            }
+          }
+        }
+      }
+      ++k;
+      next_log += W;
+    }
+  }
+  if (g.mag_out && live) g.mag_out[((size_t)jid * CMTJ_MAX_LAYERS + ly) * 3 + c] = m;
+  if (g.status && q == 0) {
+    cmtj_status s;
+    s.code = code;
+    s.steps_done = i;
+    s.entries_done = k;
+    s.nan_layer = nan_layer;
+    g.status[jid] = s;
+  }
+  if (S) {
+    if (live) S->m[ly][c] = m;
+    if (q == 0) {
+      S->step = i;
+      S->entries = k;
+      S->code = code;
+      S->nan_layer = nan_layer;
+    }
+  }
+}
+
+}  // namespace cmtj

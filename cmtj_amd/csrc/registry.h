@@ -1,14 +1,14 @@
 // registry.h -- kernel registry shared by the instantiation units.
 #pragma once
 
-#include "kernels_ll.h"
+#include "kernels_cl.h"
 
 namespace cmtj {
 
 struct KEntry {
   int prec;
   int lm;   // layer capacity
-  int lpi;  // lanes per instance: 1 (k_junction) or lm (k_junction_ll)
+  int lpi;  // lanes per instance: 1 (k_junction), lm (k_junction_ll) or 4 lm (k_junction_cl)
   int solver;
   uint32_t fm;
   bool loge;
@@ -41,6 +41,7 @@ KTable kernels_ll_f64();
 KTable kernels_ll_f32();
 KTable kernels_ll_spec();
 KTable kernels_exact();
+KTable kernels_cl();
 
 }  // namespace cmtj
 
@@ -51,6 +52,10 @@ KTable kernels_exact();
 #define CMTJ_KL(R, PREC, L, S, FM)                                               \
   {PREC, L, L, S, FM, false, (const void*)&cmtj::k_junction_ll<R, L, S, FM, false>, \
    "k_junction_ll<" #R "," #L "," #S "," #FM ">"}
+
+#define CMTJ_KC(L, FM)                                                                   \
+  {CMTJ_FP32, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM>, \
+   "k_junction_cl<" #L "," #FM ">"}
 
 #define CMTJ_KL_SOLVERS(R, PREC, L, FM) \
   CMTJ_KL(R, PREC, L, CMTJ_RK4, FM), CMTJ_KL(R, PREC, L, CMTJ_HEUN, FM), CMTJ_KL(R, PREC, L, CMTJ_EULER_HEUN, FM)

@@ -259,7 +259,12 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
 
 typedef struct cmtj_plan cmtj_plan;
 
-enum cmtj_lanes { CMTJ_LANES_AUTO = 0, CMTJ_LANES_ONE = 1, CMTJ_LANES_PER_LAYER = 2 };
+enum cmtj_lanes {
+  CMTJ_LANES_AUTO = 0,
+  CMTJ_LANES_ONE = 1,
+  CMTJ_LANES_PER_LAYER = 2,
+  CMTJ_LANES_PER_COMPONENT = 3
+};
 
 typedef struct cmtj_plan_info {
   uint64_t n_instances;
@@ -284,8 +289,10 @@ typedef struct cmtj_plan_info {
  * which has no single-call counterpart in the reference.
  * lanes_per_instance: CMTJ_LANES_ONE (all layers of an instance in one
  * lane), CMTJ_LANES_PER_LAYER (one lane per layer, for multilayer
- * instances) or CMTJ_LANES_AUTO (one lane per layer wherever a kernel for the
- * group exists).
+ * instances), CMTJ_LANES_PER_COMPONENT (four lanes per layer, one per vector
+ * component; binary32 RK4 groups of the lean feature set) or CMTJ_LANES_AUTO
+ * (one lane per component where such a kernel covers the group and its waves
+ * fit one per SIMD, else one lane per layer for multilayer instances).
  * Returns NULL on error (see cmtj_last_error). */
 cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n,
                             const cmtj_layer* layers, size_t n_layer_rows,

@@ -1,0 +1,179 @@
+"""The one-component-per-lane kernels (anchored binary32 RK4) against the
+oracle (binary64): workloads and multilayer cases of the lean feature set,
+the magnetoresistance channels, the NaN semantics of a multilayer step and
+the automatic lane choice."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import parity_cases
+from cmtj_amd import (AxialDriver, CVector, Junction, Layer, NullDriver, SolverMode, _lib, constantDriver,
+                      sineDriver, stepDriver, workloads)
+from cmtj_amd.batch import JunctionBatch, _channel_names
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-4   # max |dm| of the FP32-bulk mode against FP64
+
+
+def _chain(n_layers, mr="strip"):
+    """n layers coupled by bilinear IEC, constant and step Oersted drives;
+    mr: "strip", "classic" (bilayers) or "none"."""
+    S, D = parity_cases.SURF, parity_cases.DEMAG
+    layers = [Layer(f"L{q}", mag=CVector(1, 0.1 * (q + 1), 0.05 * q), anis=CVector(1, 0, 0), Ms=1.2 + 0.1 * q,
+                    thickness=(1 + 0.5 * q) * 1e-9, cellSurface=S, demagTensor=D, damping=0.05)
+              for q in range(n_layers)]
+    if mr == "strip":
+        j = Junction(layers, [100.0 + q for q in range(n_layers)], [50.0 + q for q in range(n_layers)],
+                     [1.0] * n_layers, [1.0] * n_layers, [0.5] * n_layers, [0.5] * n_layers, [2.0] * n_layers)
+    elif mr == "classic":
+        j = Junction(layers, 100.0, 200.0)
+    else:
+        j = Junction(layers)
+    for q in range(n_layers):
+        j.setLayerAnisotropyDriver(f"L{q}", constantDriver(20e3 + 10e3 * q))
+    for q in range(n_layers - 1):
+        j.setIECDriver(f"L{q}", f"L{q + 1}", constantDriver(-2e-4 * (1 + 0.3 * q)))
+    j.setLayerExternalFieldDriver("all", AxialDriver(200e3, 20e3, 5e3))
+    j.setLayerOerstedFieldDriver("all", AxialDriver(NullDriver(), stepDriver(0, 2e3, 0, 5e-12), NullDriver()))
+    return j
+
+
+def _trilayer_sine():
+    """Three layers, bilinear IEC, a sine and a step drive on different axes
+    (the generic lean kernel: drive kinds resolved at run time)."""
+    j = _chain(3, mr="none")
+    j.setLayerExternalFieldDriver("all", AxialDriver(sineDriver(50e3, 20e3, 3e9, 0.3), constantDriver(0),
+                                                     constantDriver(100e3)))
+    return j
+
+
+def _batch(name):
+    if name == "pimm":
+        return workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp32", log="full",
+                              log_dtype="f64")
+    if name == "pimm_delayed":
+        b = workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp32", log="mx", log_dtype="f64")
+        n = len(b.js)
+        b.set_driver("all", "oersted", "t_start", np.full((n, 3), 5.05e-12))
+        b.set_driver("all", "oersted", "t_stop", np.full((n, 3), 2.0e-11))
+        return b
+    if name == "r_h_loop":
+        b = workloads.r_h_loop(n_fields=11, total_time=0.5e-9, precision="fp32", log="full")
+        b.log_dtype = _lib.LOG_F64
+        return b
+    if name == "vsd":
+        b = JunctionBatch.replicate(workloads.vsd_junction(), 8, 1e-9, 1e-13, 1e-12, SolverMode.RK4,
+                                    precision="fp32", log="full", log_dtype="f64")
+        f = np.zeros((8, 3))
+        f[:, 1] = np.linspace(1e9, 20e9, 8)
+        b.set_driver("free", "oersted", "frequency", f)
+        return b
+    if name == "rh_strip":
+        j, kw = parity_cases.rh_strip(Junction)
+        return JunctionBatch.from_junctions([j] * 5, 0.5e-9, 1e-13, 1e-12, SolverMode.RK4, precision="fp32",
+                                            log="full", log_dtype="f64")
+    if name == "chain2_strip":
+        return JunctionBatch.from_junctions([_chain(2)] * 3, 0.5e-9, 1e-13, 1e-12, SolverMode.RK4,
+                                            precision="fp32", log="full", log_dtype="f64")
+    if name == "chain4_strip":
+        return JunctionBatch.from_junctions([_chain(4)] * 3, 0.5e-9, 1e-13, 1e-12, SolverMode.RK4,
+                                            precision="fp32", log="full", log_dtype="f64")
+    if name == "trilayer_sine":
+        return JunctionBatch.from_junctions([_trilayer_sine()] * 3, 1e-9, 1e-13, 3e-13, SolverMode.RK4,
+                                            precision="fp32", log="full", log_dtype="f64")
+    raise KeyError(name)
+
+
+def _channels(b, log, i):
+    j = b.js[i]
+    names = _channel_names(b.metas[b.meta_index[i]], int(j["log_mask"]))
+    ne, off = int(j["n_entries"]), int(j["log_offset"])
+    return {k: np.asarray(log[off + c * ne: off + (c + 1) * ne], dtype=np.float64) for c, k in enumerate(names)}
+
+
+@pytest.mark.parametrize("name", ["pimm", "pimm_delayed", "r_h_loop", "vsd", "rh_strip", "chain2_strip",
+                                  "chain4_strip", "trilayer_sine"])
+def test_component_lanes_match_oracle(name):
+    b = _batch(name).set_lanes("per_component")
+    kname = b.plan().info.kernel_name.decode()
+    print(f"[cl] {name}: {kname}", flush=True)
+    assert "k_junction_cl" in kname
+    r = b.run()
+    olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
+    st = r.status()
+    assert (st["code"] == 0).all()
+    assert (st["steps_done"] == ost["steps_done"]).all() and (st["entries_done"] == ost["entries_done"]).all()
+    glog = r.log.cpu().numpy()
+    wm, wr = 0.0, 0.0
+    for i in range(len(b.js)):
+        g, o = _channels(b, glog, i), _channels(b, olog, i)
+        assert g.keys() == o.keys()
+        for k in o:
+            d = float(np.max(np.abs(g[k] - o[k]))) if len(o[k]) else 0.0
+            if "_m" in k:
+                wm = max(wm, d)
+            else:  # resistance channels, relative
+                wr = max(wr, d / max(1.0, float(np.max(np.abs(o[k])))))
+    nl = int(b.js[0]["n_layers"])
+    fm = float(np.max(np.abs(r.final_m.cpu().numpy()[:, :nl] - omag[:, :nl])))
+    print(f"[cl] {name}: max|dm| logs {wm:.3e}, final {fm:.3e}, R rel {wr:.3e}", flush=True)
+    assert wm <= FP32_TOL and fm <= FP32_TOL and wr <= FP32_TOL
+
+
+@pytest.mark.parametrize("bad_layer", [0, 1])
+def test_component_lanes_nan_commits_like_the_reference(bad_layer):
+    """A NaN in layer b stops the run: layers up to and including it take
+    the step's value, later layers keep the previous one."""
+    js = []
+    for q in range(3):
+        j = _chain(2, mr="classic")
+        if q == 1:
+            j.setLayerExternalFieldDriver(f"L{bad_layer}", AxialDriver(stepDriver(0, float("inf"), 5e-12, 1e-9),
+                                                                       constantDriver(0), constantDriver(1e5)))
+        js.append(j)
+    b = JunctionBatch.from_junctions(js, 1e-10, 1e-13, 1e-13, SolverMode.RK4, precision="fp32", log="full",
+                                     log_dtype="f64").set_lanes("per_component")
+    assert "k_junction_cl" in b.plan().info.kernel_name.decode()
+    r = b.run()
+    st = r.status()
+    olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
+    for i in range(3):
+        for f in ("code", "steps_done", "entries_done", "nan_layer"):
+            assert int(st[i][f]) == int(ost[i][f]), (i, f)
+    assert int(st[1]["code"]) == _lib.ERR_NAN and int(st[1]["nan_layer"]) == bad_layer
+    gm = r.final_m.cpu().numpy()
+    assert np.array_equal(np.isnan(gm[:, :2]), np.isnan(omag[:, :2]))
+    ok = np.isfinite(omag[:, :2])
+    assert float(np.max(np.abs(gm[:, :2][ok] - omag[:, :2][ok]))) <= FP32_TOL
+
+
+def test_component_lanes_chosen_automatically_on_small_batches():
+    small = workloads.pimm(n_points=64, t_min=1e-11, t_max=2e-11, precision="fp32", log="mx")
+    assert "k_junction_cl" in small.plan().info.kernel_name.decode()
+    assert int(small.plan().info.lanes_per_instance) == 8
+    big = workloads.pimm(n_points=16384, t_min=1e-11, t_max=2e-11, precision="fp32", log="mx")
+    assert "k_junction_ll" in big.plan().info.kernel_name.decode()
+    fp64 = workloads.pimm(n_points=64, t_min=1e-11, t_max=2e-11, precision="fp64", log="mx")
+    assert "k_junction_cl" not in fp64.plan().info.kernel_name.decode()
+
+
+def test_component_lanes_sliced_run_equals_uninterrupted():
+    """Resumable slices of the component-lane kernels are bit-identical to an
+    uninterrupted run."""
+    js = [_chain(2), _chain(4), _trilayer_sine(), parity_cases.rh_strip(Junction)[0]]
+    mk = lambda: JunctionBatch.from_junctions(js, [0.5e-9, 0.4e-9, 0.3e-9, 0.6e-9], 1e-13, 1e-12, SolverMode.RK4,
+                                              precision="fp32", log="full").set_lanes("per_component")
+    a = mk()
+    kname = a.plan().info.kernel_name.decode()
+    assert kname.startswith("k_junction_cl") and "k_junction_ll" not in kname and "k_junction<" not in kname
+    ra = a.run()
+    rb = mk().run_sliced(slice_steps=777, n_streams=3)
+    la, lb = ra.log.cpu().numpy(), rb.log.cpu().numpy()
+    print(f"[cl slices] max log difference {np.max(np.abs(la - lb)):.3e}", flush=True)
+    np.testing.assert_array_equal(la, lb)
+    np.testing.assert_array_equal(ra.final_m.cpu().numpy(), rb.final_m.cpu().numpy())
+    sa, sb = ra.status(), rb.status()
+    assert (sa["steps_done"] == sb["steps_done"]).all() and (sa["entries_done"] == sb["entries_done"]).all()
