@@ -56,14 +56,16 @@ static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds, int ndyn) {
   if ((k.fm & F_EXACT) && (feat != flags || ndyn > DREG)) return false;  // exact kernels keep every record in registers
   if ((k.fm & F_KSTEP) && (kinds & ~KIND_STEP)) return false;
   if ((k.fm & F_KSINE) && (kinds & ~KIND_SINE)) return false;
+  if (k.nd1 && k.nd1 != ndyn + 1) return false;
   return true;
 }
 
 // Lower is better: fewer layers of capacity, then exact feature sets, then
-// driver-kind specialisations, then fewer compiled-in features.
+// driver-kind specialisations, then a compiled-in record count, then fewer
+// compiled-in features.
 static int rank(const KEntry& k) {
   return k.lm * 1000 + ((k.fm & F_EXACT) ? 0 : 200) + ((k.fm & (F_KSTEP | F_KSINE)) ? 0 : 100) +
-         __builtin_popcount(k.fm & F_ALL);
+         (k.nd1 ? 0 : 50) + __builtin_popcount(k.fm & F_ALL);
 }
 
 // SIMDs of the target (MI355X: 256 CUs x 4): a batch whose waves number at

@@ -4,8 +4,9 @@
 #include "registry.h"
 namespace cmtj {
 static const KEntry kC[] = {
-    CMTJ_KC(2, FM_PIMM), CMTJ_KC(4, FM_PIMM), CMTJ_KC(1, FM_RH), CMTJ_KC(1, FM_VSD),
-    CMTJ_KC(1, FM_LEAN), CMTJ_KC(2, FM_LEAN), CMTJ_KC(4, FM_LEAN),
+    CMTJ_KC(2, FM_PIMM, 1),  // the PIMM sweep: one Oersted step record
+    CMTJ_KC(2, FM_PIMM, -1), CMTJ_KC(4, FM_PIMM, -1), CMTJ_KC(1, FM_RH, 0), CMTJ_KC(1, FM_VSD, 1),
+    CMTJ_KC(1, FM_LEAN, -1), CMTJ_KC(2, FM_LEAN, -1), CMTJ_KC(4, FM_LEAN, -1),
 };
 KTable kernels_cl() { return {kC, sizeof(kC) / sizeof(kC[0])}; }
 }  // namespace cmtj

@@ -86,11 +86,13 @@ __device__ __forceinline__ bool feeds(int meta, int ly, int c) {
 
 // Per-lane constants of one layer component.
 struct CLConst {
-  double hb, an, nms, alpha, gconv, kan, jb, jt, ref;  // binary64 anchor
-  float anf, nmsf, alphaf, gconvf, kanf;               // binary32 increments
+  double hb, an, nms, alpha, gdt, akn, jb, jt, ref;  // binary64 anchor
+  float anf, nmsf, alphaf, gdtf, aknf;               // binary32 increments
 };
 
-template <int L, uint32_t FM>
+// ND: number of time-dependent records when fixed at compile time (exact
+// kernels), -1 to read it from the group.
+template <int L, uint32_t FM, int ND>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
   constexpr int LPI = 4 * L;
@@ -136,23 +138,25 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.an = live ? comp(P.anis, c) : z;
     K.nms = live ? comp(mk<double>(P.N[0], P.N[4], P.N[8]), c) * P.ms_mu0 : z;
     K.alpha = P.alpha;
-    K.gconv = P.gconv;
-    K.kan = live ? (2.0 * base.K) * P.inv_ms : z;
+    K.gdt = P.gconv * dt;
+    K.akn = live ? K.an * ((2.0 * base.K) * P.inv_ms) : z;  // Ha_c = akn (anis . m)
     K.jb = live ? base.Jb * P.inv_mst : z;
     K.jt = live ? base.Jt * P.inv_mst : z;
     K.ref = live ? comp(mk<double>(Ly.ref[0], Ly.ref[1], Ly.ref[2]), c) : z;
     K.anf = (float)K.an;
     K.nmsf = (float)K.nms;
     K.alphaf = (float)K.alpha;
-    K.gconvf = (float)K.gconv;
-    K.kanf = (float)K.kan;
+    K.gdtf = (float)P.gconv * (float)dt;
+    K.aknf = (float)K.akn;
   }
+  constexpr int NR = (ND >= 0) ? ND : DREG;  // records kept in registers
+  static_assert(NR <= DREG, "at most DREG records in registers");
   DynRec dr[DREG];
   bool sel[DREG];
   double win_lo = __builtin_inf(), win_hi = -__builtin_inf();
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
-    if (k < g.ndyn) {
+    if (k < NR && (ND >= 0 || k < g.ndyn)) {
       dr[k] = g.dyn[(size_t)k * g.n + inst];
       win_lo = fmin(win_lo, dr[k].p1);
       win_hi = fmax(win_hi, dr[k].p2);
@@ -162,8 +166,18 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       sel[k] = false;
     }
   }
-  const float dtR = (float)dt;
   const float sixth_inv = 1.0f / 6.0f;
+  // log layout of this lane (logLayerParams, core/junction.hpp:1580-1636):
+  // its m channel, if it writes one, and the resistance channels after the
+  // m channels of all layers
+  const int lmask = logging ? g.log_mask : 0, mrm = g.mr_mode;
+  const int nm = ((lmask & CMTJ_LOG_MX) ? 1 : 0) + ((lmask & CMTJ_LOG_MYZ) ? 2 : 0);
+  const int mch = (lmask & CMTJ_LOG_MX) ? c : c - 1;
+  const bool wm = live && nm && mch >= 0 && ((lmask & CMTJ_LOG_MYZ) || c == 0);
+  const uint64_t om = wm ? (uint64_t)(ly * nm + mch) * cs : 0;
+  const uint64_t orr = (uint64_t)(nl * nm) * cs;
+  const bool classic = (lmask & CMTJ_LOG_MR) && mrm == CMTJ_MR_CLASSIC;
+  const bool strip = (lmask & CMTJ_LOG_MR) && mrm == CMTJ_MR_STRIP;
   uint32_t k = 0;
   int code = CMTJ_OK, nan_layer = -1;
   uint32_t i = 0, iend = nsteps;
@@ -191,42 +205,38 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     if constexpr ((FM & F_KSTEP) != 0) inwin = t >= win_lo && t <= win_hi;
     if (CMTJ_HAS(F_DYNH) && inwin) {
 #pragma unroll
-      for (int r = 0; r < DREG; ++r) {
-        if (r < g.ndyn) {
+      for (int r = 0; r < NR; ++r) {
+        if (ND >= 0 || r < g.ndyn) {
           v0[r] = wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, t, i, 0, sps);
           H = H + (sel[r] ? v0[r] : 0.0);
         }
       }
-      for (int r = DREG; !(FM & F_EXACT) && r < g.ndyn; ++r) {
+      for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
         const DynRec rr = g.dyn[(size_t)r * g.n + inst];
         const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, rr, g, t, i, 0, sps);
         H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
       }
     }
     const double m1 = dpp_d<QP_ROT1>(m), m2 = dpp_d<QP_ROT2>(m);
-    double u = 0.0;
-    if (CMTJ_HAS(F_ANIS)) {
-      u = qsum(K.an * m);
-      H = H + K.an * (K.kan * u);
-    }
-    if (CMTJ_HAS(F_IEC)) H = H + (bot * K.jb + top * K.jt);
-    H = H - K.nms * m;
+    if (CMTJ_HAS(F_ANIS)) H = fma(K.akn, qsum(K.an * m), H);
+    if (CMTJ_HAS(F_IEC)) H = fma(top, K.jt, fma(bot, K.jb, H));
+    H = fma(-K.nms, m, H);
     const double P0 = cross_c(m1, m2, H);
     const double Q0 = cross_c(m1, m2, P0);
-    const double k1 = ((P0 + Q0 * K.alpha) * K.gconv) * dt;
+    const double k1 = fma(Q0, K.alpha, P0) * K.gdt;
 
     // ---- stages 2-4: binary32 increments about the anchor
     const float m0a = (float)m1, m0b = (float)m2;
     const float H0 = (float)H, Pf = (float)P0;
     auto delta = [&](float d, float ddH) -> float {
       // f(m0 + d, ts) - f(m0, t), component c (rhs_delta, kernels.h)
-      float dH = ddH - K.nmsf * d;
-      if (CMTJ_HAS(F_ANIS)) dH = dH + K.anf * (K.kanf * qsum(K.anf * d));
+      float dH = fmaf(-K.nmsf, d, ddH);
+      if (CMTJ_HAS(F_ANIS)) dH = fmaf(K.aknf, qsum(K.anf * d), dH);
       const float d1 = dpp_f<QP_ROT1>(d), d2 = dpp_f<QP_ROT2>(d);
       const float Hs = H0 + dH;
       const float dP = cross_c(m0a, m0b, dH) + cross_c(d1, d2, Hs);
       const float dQ = cross_c(m0a, m0b, dP) + cross_c(d1, d2, Pf + dP);
-      return ((dP + dQ * K.alphaf) * K.gconvf) * dtR;
+      return fmaf(dQ, K.alphaf, dP) * K.gdtf;
     };
     auto drive_delta = [&](double ts, int stage) -> float {
       float dd = 0.f;
@@ -235,13 +245,13 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         if (!(ts >= win_lo && t <= win_hi)) return dd;
       }
 #pragma unroll
-      for (int r = 0; r < DREG; ++r) {
-        if (r < g.ndyn) {
+      for (int r = 0; r < NR; ++r) {
+        if (ND >= 0 || r < g.ndyn) {
           const float v = (float)(wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, ts, i, stage, sps) - v0[r]);
           dd = dd + (sel[r] ? v : 0.f);
         }
       }
-      for (int r = DREG; !(FM & F_EXACT) && r < g.ndyn; ++r) {
+      for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
         const DynRec rr = g.dyn[(size_t)r * g.n + inst];
         const int ty = g.dmeta[r] & 0xff;
         const float v = (float)(wave_value<double, true, FM>(ty, rr, g, ts, i, stage, sps) -
@@ -261,25 +271,27 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     const float d4 = (float)k1 + dk;
     acc = acc + delta(d4, dend);
     double mn = m + k1 + (double)(acc * sixth_inv);
-    // normalize_fast (kernels.h) over the quad
+    // CVector::normalize (core/cvector.hpp:185-192) over the quad: binary32
+    // reciprocal square root and one Newton step in binary64 (|m| = 1 to
+    // ~2e-14, far inside the increments' binary32 rounding); a zero vector
+    // stays as it is
     {
       const double s = qsum(mn * mn);
-      if (s != 0.0) {
-        double y = (double)__frsqrt_rn((float)s);
-        y = y * (1.5 - 0.5 * s * y * y);
-        y = y * (1.5 - 0.5 * s * y * y);
-        mn = mn * y;
-      }
+      double y = (double)__frsqrt_rn((float)s);
+      y = y * (1.5 - (0.5 * s) * (y * y));
+      mn = (s != 0.0) ? mn * y : mn;
     }
     // the reference updates layers in turn and stops at the first NaN
     // layer; layers up to and including it take the new value
     const unsigned long long bad = __ballot(live && c == 0 && isnan(mn));
-    const uint32_t gb = (uint32_t)((bad >> gshift) & ((LPI == 64) ? ~0ull : ((1ull << LPI) - 1)));
-    if (gb) {
-      nan_layer = __builtin_ctz(gb) >> 2;
-      if (ly <= nan_layer) m = mn;
-      code = CMTJ_ERR_NAN;
-      break;
+    if (bad) {  // wave-uniform test first: the per-instance work only on the rare path
+      const uint32_t gb = (uint32_t)((bad >> gshift) & ((1ull << LPI) - 1));
+      if (gb) {
+        nan_layer = __builtin_ctz(gb) >> 2;
+        if (ly <= nan_layer) m = mn;
+        code = CMTJ_ERR_NAN;
+        break;
+      }
     }
     m = mn;
     bot = dpp_d<DPP_ROW_SHR4>(m);
@@ -287,42 +299,31 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     bot = hasB ? bot : 0.0;
     top = hasT ? top : 0.0;
     if ((uint64_t)i == next_log) {
-      if (logging) {
-        // logLayerParams (core/junction.hpp:1580-1636)
-        char* p = lp + (uint64_t)k * esize;
-        const int lmask = g.log_mask, mr = g.mr_mode;
-        const int nm = ((lmask & CMTJ_LOG_MX) ? 1 : 0) + ((lmask & CMTJ_LOG_MYZ) ? 2 : 0);
-        if (live && nm) {
-          const int ch = (lmask & CMTJ_LOG_MX) ? c : c - 1;
-          if (ch >= 0 && ((lmask & CMTJ_LOG_MYZ) || c == 0)) log_put(f32log, p + (uint64_t)(ly * nm + ch) * cs, m);
+      char* const p = lp + (uint64_t)k * esize;
+      if (wm) log_put(f32log, p + om, m);
+      if (classic) {
+        // calculateMagnetoresistance (:1865-1867): layer 0 against layer 1,
+        // one layer against the fixed reference vector (:1623-1629)
+        const double cosT = qsum(m * (nl == 1 ? K.ref : top));
+        if (q == 0) log_put(f32log, p + orr, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
+      } else if (strip) {
+        // stripMagnetoResistance (:1835-1857), parallel sum of the layers
+        const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
+        double rx = 0.0, ry = 0.0;
+        if (lact && c == 0) {
+          const double Rx = J->Rx0[ly] + J->AMR_X[ly] * (mx * mx) + J->SMR_X[ly] * (my * my);
+          const double Ry = J->Ry0[ly] + 0.5 * J->AHE[ly] * mz + (J->AMR_Y[ly] + J->SMR_Y[ly]) * mx * my;
+          rx = 1. / Rx;
+          ry = 1. / Ry;
         }
-        p += (uint64_t)(nl * nm) * cs;
-        if (lmask & CMTJ_LOG_MR) {
-          if (mr == CMTJ_MR_CLASSIC) {
-            // calculateMagnetoresistance (:1865-1867): layer 0 against layer
-            // 1, one layer against the fixed reference vector (:1623-1629)
-            const double cosT = qsum(m * (nl == 1 ? K.ref : top));
-            if (q == 0) log_put(f32log, p, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
-          } else if (mr == CMTJ_MR_STRIP) {
-            // stripMagnetoResistance (:1835-1857), parallel sum of the layers
-            const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
-            double rx = 0.0, ry = 0.0;
-            if (lact && c == 0) {
-              const double Rx = J->Rx0[ly] + J->AMR_X[ly] * (mx * mx) + J->SMR_X[ly] * (my * my);
-              const double Ry = J->Ry0[ly] + 0.5 * J->AHE[ly] * mz + (J->AMR_Y[ly] + J->SMR_Y[ly]) * mx * my;
-              rx = 1. / Rx;
-              ry = 1. / Ry;
-            }
 #pragma unroll
-            for (int o = 4; o < LPI; o <<= 1) {
-              rx += __shfl_xor(rx, o, 64);
-              ry += __shfl_xor(ry, o, 64);
-            }
-            if (q == 0) {
-              log_put(f32log, p, 1 / rx);
-              log_put(f32log, p + cs, 1 / ry);
-            }
-          }
+        for (int o = 4; o < LPI; o <<= 1) {
+          rx += __shfl_xor(rx, o, 64);
+          ry += __shfl_xor(ry, o, 64);
+        }
+        if (q == 0) {
+          log_put(f32log, p + orr, 1 / rx);
+          log_put(f32log, p + orr + cs, 1 / ry);
         }
       }
       ++k;

@@ -14,6 +14,7 @@ struct KEntry {
   bool loge;
   const void* fn;
   const char* name;
+  int nd1;  // 1 + the number of time-dependent records the kernel is compiled for; 0: any
 };
 
 struct KTable {
@@ -53,9 +54,9 @@ KTable kernels_cl();
   {PREC, L, L, S, FM, false, (const void*)&cmtj::k_junction_ll<R, L, S, FM, false>, \
    "k_junction_ll<" #R "," #L "," #S "," #FM ">"}
 
-#define CMTJ_KC(L, FM)                                                                   \
-  {CMTJ_FP32, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM>, \
-   "k_junction_cl<" #L "," #FM ">"}
+#define CMTJ_KC(L, FM, ND)                                                                     \
+  {CMTJ_FP32, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND>,  \
+   "k_junction_cl<" #L "," #FM "," #ND ">", (ND) + 1}
 
 #define CMTJ_KL_SOLVERS(R, PREC, L, FM) \
   CMTJ_KL(R, PREC, L, CMTJ_RK4, FM), CMTJ_KL(R, PREC, L, CMTJ_HEUN, FM), CMTJ_KL(R, PREC, L, CMTJ_EULER_HEUN, FM)
