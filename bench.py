@@ -65,7 +65,7 @@ def _traffic():
     return json.loads(f.read_text())
 
 
-def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
+def _serving(qps: float, seconds: float, seed: int = 0, lanes: str = "per_component") -> dict:
     """Mixed request stream (PIMM / VSD / CIMS / 3-4 layer chains, 1-50 ns,
     dt 1e-14..1e-13 s) with Poisson arrivals at `qps` for `seconds`, served
     by the continuous-batching loop in FP32; latency = submit to result."""
@@ -77,7 +77,7 @@ def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
     rng = np.random.default_rng(seed + 1)
     mix = workloads.serving_mix(int(qps * seconds * 1.5) + 16, seed=seed)
     futs, kinds = [], []
-    with Server(precision="fp32", slice_ms=20.0, n_streams=8, log_elements=1 << 28) as srv:
+    with Server(precision="fp32", slice_ms=20.0, n_streams=8, log_elements=1 << 28, lanes=lanes) as srv:
         t0 = time.perf_counter()
         nxt = t0
         q = 0
@@ -104,7 +104,7 @@ def _serving(qps: float, seconds: float, seed: int = 0) -> dict:
             "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "max": float(lat.max())},
             "junction_steps": int(steps), "junction_steps_per_s": steps / wall, "wall_s": wall,
-            "slices": ticks, "slice_ms_target": 20.0, "host_s": host_s, "precision": "fp32", "lanes": "per_layer"}
+            "slices": ticks, "slice_ms_target": 20.0, "host_s": host_s, "precision": "fp32", "lanes": lanes}
 
 
 def _timed_run(b, reps: int = 1):
@@ -272,6 +272,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--lanes", default="auto", choices=["auto", "one", "per_layer", "per_component"])
+    ap.add_argument("--serving-lanes", default="per_component", choices=["one", "per_layer", "per_component"])
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
@@ -298,7 +299,7 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.workload != "pimm":
         fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto,
-              "serving": lambda: _serving(args.serving_qps, args.serving_seconds)}[args.workload]
+              "serving": lambda: _serving(args.serving_qps, args.serving_seconds, lanes=args.serving_lanes)}[args.workload]
         print(json.dumps(fn()))
         return 0
 
@@ -397,7 +398,7 @@ def main() -> int:
 
     serving = None
     if not args.no_serving:
-        serving = _serving(args.serving_qps, args.serving_seconds)
+        serving = _serving(args.serving_qps, args.serving_seconds, lanes=args.serving_lanes)
         print(f"[bench] serving: {serving['requests']} requests at {args.serving_qps} QPS, "
               f"p50 {serving['latency_s']['p50']:.3f} s, p99 {serving['latency_s']['p99']:.3f} s, "
               f"{serving['junction_steps_per_s']:.3e} junction-steps/s", file=sys.stderr, flush=True)

@@ -135,14 +135,17 @@ class Server:
     precision: "fp32" or "fp64"; log_elements: capacity of the device log
     arena (elements of the log dtype); max_active: slots; slice_ms: target
     device time of one slice (each group adapts its step count to it);
-    lanes: fixed lane mapping ("per_layer" or "one"); n_streams: streams
-    created up front (more are created when more groups are active)."""
+    lanes: fixed lane mapping ("per_component", "per_layer" or "one"; a
+    group's kernel then depends on its structure only, never on how many
+    requests share it, so a result is the same alone and in a mix);
+    n_streams: streams created up front (more are created when more groups
+    are active)."""
 
     def __init__(self, precision: str = "fp32", log_elements: int = 1 << 28, max_active: int = 65536,
-                 slice_ms: float = 20.0, slice_steps: int = 5000, lanes: str = "per_layer", n_streams: int = 8,
+                 slice_ms: float = 20.0, slice_steps: int = 5000, lanes: str = "per_component", n_streams: int = 8,
                  log_dtype=None, device: str = "cuda") -> None:
-        if lanes not in ("per_layer", "one"):
-            raise ValueError("a server fixes the lane mapping: 'per_layer' or 'one'")
+        if lanes not in ("per_component", "per_layer", "one"):
+            raise ValueError("a server fixes the lane mapping: 'per_component', 'per_layer' or 'one'")
         self.precision = _lib.FP32 if precision in ("fp32", _lib.FP32) else _lib.FP64
         if log_dtype is None:
             log_dtype = "f32" if self.precision == _lib.FP32 else "f64"

@@ -40,7 +40,7 @@ def test_sliced_run_equals_uninterrupted(precision):
 
 def _alone(spec, precision):
     kind, j, T, dt, wf, sm, log = spec
-    b = JunctionBatch.from_junctions([j], T, dt, wf, sm, precision=precision, log=log).set_lanes("per_layer")
+    b = JunctionBatch.from_junctions([j], T, dt, wf, sm, precision=precision, log=log).set_lanes("per_component")
     r = b.run()
     return r.instance_log(0), r.final_m.cpu().numpy()[0, : j.layerNo]
 
