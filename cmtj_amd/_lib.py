@@ -191,6 +191,7 @@ EXPORTS = (
     "cmtj_plan_execute_slice",
     "cmtj_plan_query",
     "cmtj_plan_destroy",
+    "cmtj_vsd_mix",
 )
 
 _P = ctypes.c_void_p
@@ -235,6 +236,8 @@ def _load():
                                             ctypes.POINTER(_P), ctypes.c_int32]
     lib.cmtj_plan_query.argtypes = [_P, ctypes.POINTER(PlanInfo)]
     lib.cmtj_plan_destroy.argtypes = [_P]
+    lib.cmtj_vsd_mix.argtypes = [_P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _P, ctypes.c_double, _P, _P,
+                                 ctypes.c_int32, _P, _P]
     info = _AbiInfo()
     lib.cmtj_abi(ctypes.byref(info))
     checks = {
@@ -347,6 +350,29 @@ def group_key(jd, layers) -> int:
     out = ctypes.c_uint64(0)
     check(lib().cmtj_group_key(ptr(jd), ptr(layers), ctypes.byref(out)), "cmtj_group_key")
     return int(out.value)
+
+
+def vsd_mix(R, freq, dt: float, b, a, stream: int = 0):
+    """Mixing voltage of each row of R (cmtj_vsd_mix): R a float64 device
+    tensor [n, len] with unit column stride, freq [n] frequencies; b, a the
+    filter coefficients.  Returns a float64 device tensor [n]."""
+    import torch
+
+    if R.dtype != torch.float64 or R.dim() != 2 or (R.shape[1] > 1 and R.stride(1) != 1):
+        raise ValueError("R must be a 2-D float64 tensor with contiguous rows")
+    n, length = R.shape
+    f = torch.as_tensor(np.asarray(freq, dtype=np.float64).ravel(), device=R.device)
+    if f.numel() != n:
+        raise ValueError("one frequency per series")
+    bb = np.ascontiguousarray(b, dtype=np.float64).ravel()
+    aa = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    taps = max(len(bb), len(aa))
+    bb = np.pad(bb, (0, taps - len(bb)))
+    aa = np.pad(aa, (0, taps - len(aa)))
+    out = torch.empty(n, dtype=torch.float64, device=R.device)
+    check(lib().cmtj_vsd_mix(R.data_ptr(), R.stride(0) if n > 1 else length, n, length, f.data_ptr(), float(dt),
+                             ptr(bb), ptr(aa), taps, out.data_ptr(), stream or None), "cmtj_vsd_mix")
+    return out
 
 
 class HostBuffer:

@@ -508,6 +508,54 @@ void* malloc_arena(size_t bytes, void*) {
 // ---------------------------------------------------------------------------
 // C ABI
 
+// ---------------------------------------------------------------------------
+// spin-diode mixing voltage: compute_sd (cmtj/utils/resistance.py:12-22) over
+// many series, one lane per series.  x_k = -sin(2 pi f t_k) R_k with
+// t_k = k dt, y = the IIR filter (b, a) of x in scipy.signal.lfilter's
+// transposed direct form II (zero initial state), V = mean of y.  Operation
+// order as in lfilter (no FMA contraction); the mean is a compensated sum.
+
+constexpr int kMaxTaps = 9;
+struct IirTaps {
+  double b[kMaxTaps], a[kMaxTaps];
+  int n;
+};
+
+__global__ __launch_bounds__(64) void k_vsd_mix(const double* __restrict__ R, uint64_t ld, uint32_t n, uint32_t len,
+                                                const double* __restrict__ freq, double dt, IirTaps f,
+                                                double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double* r = R + (uint64_t)q * ld;
+  const double w = (2.0 * M_PI) * freq[q];
+  double z[kMaxTaps - 1];
+#pragma unroll
+  for (int i = 0; i < kMaxTaps - 1; ++i) z[i] = 0.0;
+  double sum = 0.0, comp = 0.0;
+  const int nt = f.n;
+  for (uint32_t k = 0; k < len; ++k) {
+    const double x = -sin(w * ((double)k * dt)) * r[k];
+    double y;
+    if (nt > 1) {
+      y = z[0] + f.b[0] * x;
+#pragma unroll
+      for (int i = 0; i < kMaxTaps - 2; ++i)
+        if (i < nt - 2) z[i] = (z[i + 1] + x * f.b[i + 1]) - y * f.a[i + 1];
+#pragma unroll
+      for (int i = 0; i < kMaxTaps - 1; ++i)
+        if (i == nt - 2) z[i] = x * f.b[i + 1] - y * f.a[i + 1];
+    } else {
+      y = x * f.b[0];
+    }
+    // Neumaier summation
+    const double tsum = sum + y;
+    comp += (fabs(sum) >= fabs(y)) ? (sum - tsum) + y : (y - tsum) + sum;
+    sum = tsum;
+  }
+  out[q] = len ? (sum + comp) / (double)len : 0.0;
+}
+
 extern "C" {
 
 int cmtj_abi(cmtj_abi_info* out) {
@@ -734,6 +782,30 @@ int cmtj_plan_execute_slice(cmtj_plan* p, void* d_log, size_t log_capacity, doub
     return fail(CMTJ_ERR_INVALID, "log buffer smaller than the plan's log extent (" +
                                       std::to_string(p->log_extent) + " elements)");
   return launch_plan(p, d_log, d_mag_out, d_status, nullptr, d_state, max_steps, streams, n_streams);
+}
+
+int cmtj_vsd_mix(const double* d_R, size_t ld, size_t n, size_t len, const double* d_freq, double dt,
+                 const double* b, const double* a, int32_t n_taps, double* d_out, void* stream) {
+  if (n == 0) return 0;
+  if (!d_R || !d_freq || !d_out || !b || !a) return fail(CMTJ_ERR_INVALID, "null pointer");
+  if (n_taps < 1 || n_taps > kMaxTaps)
+    return fail(CMTJ_ERR_INVALID, "n_taps must be 1.." + std::to_string(kMaxTaps));
+  if (a[0] == 0.0) return fail(CMTJ_ERR_INVALID, "a[0] must be nonzero");
+  if (len > 0xffffffffull || n > 0xffffffffull) return fail(CMTJ_ERR_INVALID, "series too long");
+  if (ld < len) return fail(CMTJ_ERR_INVALID, "row stride shorter than a series");
+  if (!(dt > 0.0)) return fail(CMTJ_ERR_INVALID, "dt must be positive");
+  IirTaps f;
+  std::memset(&f, 0, sizeof(f));
+  f.n = n_taps;
+  for (int i = 0; i < n_taps; ++i) {  // lfilter normalises by a[0]
+    f.b[i] = b[i] / a[0];
+    f.a[i] = a[i] / a[0];
+  }
+  const uint32_t blocks = (uint32_t)((n + 63) / 64);
+  hipLaunchKernelGGL(k_vsd_mix, dim3(blocks), dim3(64), 0, (hipStream_t)stream, d_R, (uint64_t)ld, (uint32_t)n,
+                     (uint32_t)len, d_freq, dt, f, d_out);
+  HIP_TRY(hipGetLastError());
+  return 0;
 }
 
 int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out) {

@@ -86,8 +86,8 @@ __device__ __forceinline__ bool feeds(int meta, int ly, int c) {
 
 // Per-lane constants of one layer component.
 struct CLConst {
-  double hb, an, nms, alpha, gdt, akn, jb, jt, ref;  // binary64 anchor
-  float anf, nmsf, alphaf, gdtf, aknf;               // binary32 increments
+  double hb, an, an1, an2, nms, alpha, gdt, akn, jb, jt, ref;  // binary64 anchor
+  float anf, an1f, an2f, nmsf, alphaf, gdtf, aknf;             // binary32 increments
 };
 
 // ND: number of time-dependent records when fixed at compile time (exact
@@ -136,6 +136,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     m = live ? comp(m3, c) : z;
     K.hb = live ? comp(base.H, c) : z;
     K.an = live ? comp(P.anis, c) : z;
+    K.an1 = live ? comp(P.anis, c == 2 ? 0 : c + 1) : z;  // components c + 1, c + 2 (mod 3)
+    K.an2 = live ? comp(P.anis, c == 0 ? 2 : c - 1) : z;
     K.nms = live ? comp(mk<double>(P.N[0], P.N[4], P.N[8]), c) * P.ms_mu0 : z;
     K.alpha = P.alpha;
     K.gdt = P.gconv * dt;
@@ -144,6 +146,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.jt = live ? base.Jt * P.inv_mst : z;
     K.ref = live ? comp(mk<double>(Ly.ref[0], Ly.ref[1], Ly.ref[2]), c) : z;
     K.anf = (float)K.an;
+    K.an1f = (float)K.an1;
+    K.an2f = (float)K.an2;
     K.nmsf = (float)K.nms;
     K.alphaf = (float)K.alpha;
     K.gdtf = (float)P.gconv * (float)dt;
@@ -217,10 +221,16 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
       }
     }
+    // m of the other two components (dot products below sum the three
+    // products of each lane's own view: the quad's lanes may differ in the
+    // last bit of a dot product, never more)
     const double m1 = dpp_d<QP_ROT1>(m), m2 = dpp_d<QP_ROT2>(m);
-    if (CMTJ_HAS(F_ANIS)) H = fma(K.akn, qsum(K.an * m), H);
+    if (CMTJ_HAS(F_ANIS)) H = fma(K.akn, fma(K.an2, m2, fma(K.an1, m1, K.an * m)), H);
     if (CMTJ_HAS(F_IEC)) H = fma(top, K.jt, fma(bot, K.jb, H));
     H = fma(-K.nms, m, H);
+    // m x H and m x (m x H) as cross products: the initial m of a run need
+    // not have unit length (the reference normalises its constructor
+    // argument, not the stored vector), so m (m . H) - H is no substitute
     const double P0 = cross_c(m1, m2, H);
     const double Q0 = cross_c(m1, m2, P0);
     const double k1 = fma(Q0, K.alpha, P0) * K.gdt;
@@ -230,9 +240,9 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     const float H0 = (float)H, Pf = (float)P0;
     auto delta = [&](float d, float ddH) -> float {
       // f(m0 + d, ts) - f(m0, t), component c (rhs_delta, kernels.h)
-      float dH = fmaf(-K.nmsf, d, ddH);
-      if (CMTJ_HAS(F_ANIS)) dH = fmaf(K.aknf, qsum(K.anf * d), dH);
       const float d1 = dpp_f<QP_ROT1>(d), d2 = dpp_f<QP_ROT2>(d);
+      float dH = fmaf(-K.nmsf, d, ddH);
+      if (CMTJ_HAS(F_ANIS)) dH = fmaf(K.aknf, fmaf(K.an2f, d2, fmaf(K.an1f, d1, K.anf * d)), dH);
       const float Hs = H0 + dH;
       const float dP = cross_c(m0a, m0b, dH) + cross_c(d1, d2, Hs);
       const float dQ = cross_c(m0a, m0b, dP) + cross_c(d1, d2, Pf + dP);

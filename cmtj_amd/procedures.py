@@ -12,7 +12,6 @@ Signatures, defaults, validation messages and outputs follow the reference.
 """
 from __future__ import annotations
 
-import math
 from collections import defaultdict
 from dataclasses import dataclass
 from typing import Any, Callable
@@ -21,7 +20,7 @@ import numpy as np
 
 from .batch import JunctionBatch
 from .core import AxialDriver, Axis, CVector, NullDriver, ScalarDriver, SolverMode
-from .utils import Filters, calculate_resistance_series
+from .utils import calculate_resistance_series
 
 
 @dataclass
@@ -222,7 +221,7 @@ def VSD_procedure(junction, Hvecs: np.ndarray, frequencies: np.ndarray, int_step
         r = b.run()
         ne = int(b.js[0]["n_entries"])
         if Rtype == "Rz":
-            R = r.channel(rkey).cpu().numpy()
+            R = r.channel(rkey)  # stays on the device
         else:
             R = resistance(r.log[: n * 3 * nl * ne].view(n, nl, 3, ne).cpu().numpy())
         fcol = np.tile(frequencies, nH)
@@ -247,10 +246,28 @@ def VSD_procedure(junction, Hvecs: np.ndarray, frequencies: np.ndarray, int_step
         R = np.asarray(Rs)
         fcol = np.asarray(fcol)
         ne = R.shape[1]
-    t = np.arange(ne, dtype=np.float64) * int_step
-    I = np.sin(2 * math.pi * fcol[:, None] * t[None, :])
-    sd = Filters.butter_lowpass_filter(-I * R, cutoff=10e6, fs=1.0 / int_step, order=3)
-    return sd.mean(axis=-1).reshape(nH, nF)
+    return _mixing_voltage(R, fcol, int_step).reshape(nH, nF)
+
+
+def _mixing_voltage(R, freqs, int_step: float) -> np.ndarray:
+    """compute_sd (resistance.py:12-22) of every series at once on the
+    device: -sin(2 pi f t) R through the order-3 10 MHz Butterworth low-pass
+    (filters.py:33-43; coefficients from scipy.signal.butter as there), then
+    the mean."""
+    import torch
+    from scipy.signal import butter
+
+    from . import _lib
+
+    fs = 1.0 / int_step
+    nyq = 0.5 * fs
+    b, a = butter(3, 10e6 / nyq, btype="low", analog=False)
+    if not isinstance(R, torch.Tensor):
+        R = torch.as_tensor(np.ascontiguousarray(R, dtype=np.float64), device="cuda")
+    R = R.to(torch.float64)
+    if R.dim() == 2 and R.shape[1] > 1 and R.stride(1) != 1:
+        R = R.contiguous()
+    return _lib.vsd_mix(R, freqs, int_step, b, a).cpu().numpy()
 
 
 __all__ = ["ResistanceParameters", "PIMM_procedure", "VSD_procedure", "compute_spectrum_strip"]

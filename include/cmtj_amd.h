@@ -359,6 +359,21 @@ int cmtj_plan_execute_slice(cmtj_plan* p, void* d_log, size_t log_capacity, doub
 int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out);
 int cmtj_plan_destroy(cmtj_plan* p);
 
+/* ---- post-processing ---------------------------------------------------- */
+
+/* Spin-diode mixing voltage of n resistance series on the device, one value
+ * per series: x_k = -sin(2 pi freq[q] k dt) R[q][k], filtered by the IIR
+ * filter (b, a) of n_taps coefficients each (scipy.signal.lfilter's
+ * transposed direct form II, zero initial state, normalised by a[0]), then
+ * averaged.  Replaces compute_sd (cmtj/utils/resistance.py:12-22) with its
+ * Filters.butter_lowpass_filter (cmtj/utils/filters.py:33-43), which the
+ * reference's VSD_procedure (cmtj/utils/procedures.py:280-331) applies to
+ * each sweep point's log on the host.  d_R: device, series q at
+ * d_R + q * ld (ld >= len elements); d_freq, d_out: device [n]; b, a: host,
+ * n_taps <= 9.  Asynchronous on `stream` (hipStream_t; NULL = default). */
+int cmtj_vsd_mix(const double* d_R, size_t ld, size_t n, size_t len, const double* d_freq, double dt,
+                 const double* b, const double* a, int32_t n_taps, double* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,7 @@ def test_procedure_validation_messages():
         VSD_procedure(j, [[1e5, 0, 0]], [1e9], 1e-12, [], Rtype="Rx")
 
 
+@pytest.mark.gpu  # the mixing voltage is taken on the device
 def test_vsd_chained_runs():
     j = _film()
     j2 = OracleJunction([j.layers[0]], [100.0], [0.0], [1.0], [0.0], [0.0], [0.0], [0.0])
