@@ -50,9 +50,10 @@ def install_as_cmtj() -> None:
     mod = _sys.modules[__name__]
     _sys.modules.setdefault("cmtj", mod)
     _sys.modules.setdefault("cmtj.constants", constants)
-    from . import utils
+    from . import stack, utils
 
     _sys.modules.setdefault("cmtj.utils", utils)
+    _sys.modules.setdefault("cmtj.stack", stack)
 
 
 __all__ = [

@@ -14,8 +14,9 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libcmtj_amd.so"
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_LAYERS = 8
+MAX_STACK = 64
 NSLOT = 30
 ENERGY_CHANNELS = 23
 
@@ -129,6 +130,22 @@ STATUS_DTYPE = np.dtype(
     [("code", "<i4"), ("steps_done", "<u4"), ("entries_done", "<u4"), ("nan_layer", "<i4")], align=True
 )
 
+STACK_DTYPE = np.dtype(
+    [
+        ("series", "<i4"),
+        ("kcl", "<i4"),
+        ("n_junctions", "<i4"),
+        ("two_layer", "<i4"),
+        ("phase_offset", "<f8"),
+        ("pol", "<f8", (3,)),
+        ("current", DRIVER_DTYPE),
+        ("coupling", "<f8", (MAX_STACK,)),
+        ("top", "<i4", (MAX_STACK,)),
+        ("bottom", "<i4", (MAX_STACK,)),
+    ],
+    align=True,
+)
+
 STATE_DTYPE = np.dtype(
     [("m", "<f8", (MAX_LAYERS, 3)), ("step", "<u4"), ("entries", "<u4"), ("code", "<i4"), ("nan_layer", "<i4")],
     align=True,
@@ -148,6 +165,7 @@ class _AbiInfo(ctypes.Structure):
             "sizeof_constants",
             "sizeof_status",
             "sizeof_state",
+            "sizeof_stack",
             "offsetof_layer_drv",
             "offsetof_layer_ref_type",
             "offsetof_junction_dt",
@@ -192,6 +210,7 @@ EXPORTS = (
     "cmtj_plan_query",
     "cmtj_plan_destroy",
     "cmtj_vsd_mix",
+    "cmtj_run_stack",
 )
 
 _P = ctypes.c_void_p
@@ -238,6 +257,7 @@ def _load():
     lib.cmtj_plan_destroy.argtypes = [_P]
     lib.cmtj_vsd_mix.argtypes = [_P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _P, ctypes.c_double, _P, _P,
                                  ctypes.c_int32, _P, _P]
+    lib.cmtj_run_stack.argtypes = [_P, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P]
     info = _AbiInfo()
     lib.cmtj_abi(ctypes.byref(info))
     checks = {
@@ -250,6 +270,7 @@ def _load():
         "constants": (info.sizeof_constants, CONSTANTS_DTYPE.itemsize),
         "status": (info.sizeof_status, STATUS_DTYPE.itemsize),
         "state": (info.sizeof_state, STATE_DTYPE.itemsize),
+        "stack": (info.sizeof_stack, STACK_DTYPE.itemsize),
         "layer.drv": (info.offsetof_layer_drv, LAYER_DTYPE.fields["drv"][1]),
         "layer.ref_type": (info.offsetof_layer_ref_type, LAYER_DTYPE.fields["ref_type"][1]),
         "junction.dt": (info.offsetof_junction_dt, JUNCTION_DTYPE.fields["dt"][1]),
@@ -330,6 +351,26 @@ def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: 
         ptr(log) if log.size else None, log.size, ptr(mag), ptr(st),
     )
     check(rc, "cmtj_run")
+    return log, mag, st
+
+
+def run_stack(stack: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray):
+    """A coupled stack through cmtj_run_stack; js[q]["log_offset"] must place
+    each member's block of n_channels + 2 channels.  Returns (log, mag [n, 8, 3],
+    status [n])."""
+    stack = np.ascontiguousarray(stack, dtype=STACK_DTYPE).reshape(())
+    js = np.ascontiguousarray(js, dtype=JUNCTION_DTYPE)
+    layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
+    n = js.size
+    need = 0
+    for q in range(n):
+        nch = n_channels(int(js[q]["n_layers"]), int(js[q]["mr_mode"]), int(js[q]["log_mask"])) + 2
+        need = max(need, int(js[q]["log_offset"]) + nch * int(js[q]["n_entries"]))
+    log = np.zeros(need, dtype=np.float64)
+    mag = np.zeros((n, MAX_LAYERS, 3), dtype=np.float64)
+    st = np.zeros(n, dtype=STATUS_DTYPE)
+    check(lib().cmtj_run_stack(ptr(stack), ptr(js), ptr(layers), layers.size, ptr(consts),
+                               ptr(log) if log.size else None, log.size, ptr(mag), ptr(st)), "cmtj_run_stack")
     return log, mag, st
 
 

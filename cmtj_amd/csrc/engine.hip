@@ -569,6 +569,7 @@ int cmtj_abi(cmtj_abi_info* out) {
   out->sizeof_constants = (int32_t)sizeof(cmtj_constants);
   out->sizeof_status = (int32_t)sizeof(cmtj_status);
   out->sizeof_state = (int32_t)sizeof(cmtj_state);
+  out->sizeof_stack = (int32_t)sizeof(cmtj_stack);
   out->offsetof_layer_drv = (int32_t)offsetof(cmtj_layer, drv);
   out->offsetof_layer_ref_type = (int32_t)offsetof(cmtj_layer, ref_type);
   out->offsetof_junction_dt = (int32_t)offsetof(cmtj_junction, dt);
@@ -647,6 +648,80 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_consta
   HIP_TRY(hipMemcpyAsync(st, out + o_st, sizeof(cmtj_status), hipMemcpyDeviceToHost, g_plain.stream));
   HIP_TRY(hipStreamSynchronize(g_plain.stream));
   std::memcpy(mag_out_host, mag, sizeof(double) * 3 * (size_t)jj.n_layers);
+  return 0;
+}
+
+int cmtj_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_layer* layers, size_t n_layer_rows,
+                   const cmtj_constants* c, double* log_host, size_t log_cap, double* mag_out_host,
+                   cmtj_status* st_host) {
+  if (!s || !js || !layers || !c || !mag_out_host || !st_host) return fail(CMTJ_ERR_INVALID, "null pointer");
+  const int n = s->n_junctions;
+  if (n < 2 || n > CMTJ_MAX_STACK)
+    return fail(CMTJ_ERR_INVALID, "a stack has 2.." + std::to_string(CMTJ_MAX_STACK) + " junctions");
+  if (s->current.type == CMTJ_TABLE || s->current.type < 0 || s->current.type > CMTJ_GAUSSSTEP)
+    return fail(CMTJ_ERR_INVALID, "the stack current needs a closed-form driver");
+  const cmtj_junction& j0 = js[0];
+  if (j0.solver != CMTJ_RK4 && j0.solver != CMTJ_EULER_HEUN)
+    return fail(CMTJ_ERR_INVALID, "stacks step with RK4 or Euler-Heun");
+  size_t need = 0;
+  for (int q = 0; q < n; ++q) {
+    const cmtj_junction& j = js[q];
+    if (j.solver != j0.solver || j.n_steps != j0.n_steps || j.write_every != j0.write_every ||
+        j.n_entries != j0.n_entries || j.dt != j0.dt)
+      return fail(CMTJ_ERR_INVALID, "stack members must share solver, step count, time step and log cadence");
+    if (j.n_layers < 1 || j.n_layers > 2) return fail(CMTJ_ERR_INVALID, "stack members have 1 or 2 layers");
+    if (s->top[q] < 0 || s->top[q] >= j.n_layers || (s->two_layer && (s->bottom[q] < 0 || s->bottom[q] >= j.n_layers)))
+      return fail(CMTJ_ERR_INVALID, "top/bottom layer index out of range");
+    const size_t nch = (size_t)n_channels(j.n_layers, j.mr_mode, j.log_mask) + 2;
+    need = std::max<size_t>(need, j.log_offset + nch * j.n_entries);
+  }
+  if (need > 0 && (!log_host || log_cap < need)) return fail(CMTJ_ERR_INVALID, "log buffer too small");
+  const KTable kt = kernels_stack();
+  const KEntry* ke = nullptr;
+  for (size_t q = 0; q < kt.n; ++q)
+    if (kt.e[q].solver == j0.solver) ke = &kt.e[q];
+  if (!ke) return fail(CMTJ_ERR_INVALID, "no stack kernel for this solver");
+
+  std::lock_guard<std::mutex> lk(g_plain.mu);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (g_plain.stream == nullptr || g_plain.device != dev) {
+    if (g_plain.stream) (void)hipStreamDestroy(g_plain.stream);
+    if (g_plain.arena) (void)hipFree(g_plain.arena);
+    if (g_plain.out) (void)hipFree(g_plain.out);
+    g_plain.arena = g_plain.out = nullptr;
+    g_plain.arena_bytes = g_plain.out_bytes = 0;
+    HIP_TRY(hipStreamCreateWithFlags(&g_plain.stream, hipStreamNonBlocking));
+    g_plain.device = dev;
+  }
+  cmtj_plan P;
+  P.owns = false;
+  int r = build_plan(&P, js, (size_t)n, layers, n_layer_rows, c, nullptr, 0, CMTJ_FP64, CMTJ_LOG_F64,
+                     CMTJ_LANES_ONE, plain_arena, &g_plain, g_plain.stream);
+  if (r) return r;
+  if (P.groups.size() != 1)
+    return fail(CMTJ_ERR_INVALID,
+                "the junctions of a stack must share one structure (layers, modes, driver kinds)");
+  // equal step counts keep the plan's order the caller's: member j on lane j
+  const size_t o_mag = align_up(need * sizeof(double));
+  const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * (size_t)n);
+  const size_t o_stk = align_up(o_st + sizeof(cmtj_status) * (size_t)n);
+  const size_t out_bytes = o_stk + sizeof(cmtj_stack);
+  char* out = (char*)grow(g_plain.out, g_plain.out_bytes, out_bytes);
+  if (!out) return CMTJ_ERR_HIP;
+  HIP_TRY(hipMemcpyAsync(out + o_stk, s, sizeof(cmtj_stack), hipMemcpyHostToDevice, g_plain.stream));
+  Group& G = P.groups[0];
+  G.k = ke;
+  G.blocks = 1;
+  G.a.stack = (const cmtj_stack*)(out + o_stk);
+  r = launch_plan(&P, need ? out : nullptr, (double*)(out + o_mag), (cmtj_status*)(out + o_st), g_plain.stream);
+  if (r) return r;
+  if (need) HIP_TRY(hipMemcpyAsync(log_host, out, need * sizeof(double), hipMemcpyDeviceToHost, g_plain.stream));
+  HIP_TRY(hipMemcpyAsync(mag_out_host, out + o_mag, sizeof(double) * 3 * CMTJ_MAX_LAYERS * (size_t)n,
+                         hipMemcpyDeviceToHost, g_plain.stream));
+  HIP_TRY(hipMemcpyAsync(st_host, out + o_st, sizeof(cmtj_status) * (size_t)n, hipMemcpyDeviceToHost,
+                         g_plain.stream));
+  HIP_TRY(hipStreamSynchronize(g_plain.stream));
   return 0;
 }
 

@@ -1,0 +1,17 @@
+// Coupled-stack kernels: one member junction per lane (every feature, FP64),
+// the stack's current exchanged between the lanes each step.  Members have
+// at most two layers (Stack<T>::isTwoLayerMemberStack, core/stack.hpp:
+// 272-283); the reference's stacks step with RK4, or Euler-Heun when a
+// member is stochastic (initializeSimulation, :36-62).
+#include "registry.h"
+namespace cmtj {
+constexpr uint32_t FM_STACK = F_ALL;
+static const KEntry kS[] = {
+    {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_STACK, false,
+     (const void*)&k_junction<double, 2, CMTJ_RK4, FM_STACK, false, true>, "k_junction<double,2,RK4,stack>"},
+    {CMTJ_FP64, 2, 1, CMTJ_EULER_HEUN, FM_STACK, false,
+     (const void*)&k_junction<double, 2, CMTJ_EULER_HEUN, FM_STACK, false, true>,
+     "k_junction<double,2,EulerHeun,stack>"},
+};
+KTable kernels_stack() { return {kS, sizeof(kS) / sizeof(kS[0])}; }
+}  // namespace cmtj

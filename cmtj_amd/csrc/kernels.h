@@ -41,6 +41,7 @@ struct GroupArgs {
   double* mag_out;  // [n][CMTJ_MAX_LAYERS][3] in junction order
   cmtj_status* status;
   cmtj_state* state;  // resumable state [n] in junction order, or null (run from the start)
+  const cmtj_stack* stack;  // coupled stack (k_junction<..., STK>): junction j on lane j
   uint32_t slice;     // steps per launch when state is set
   uint32_t n;       // lanes of this group
   uint32_t flags;   // feature bits of the group
@@ -732,6 +733,17 @@ __device__ __forceinline__ V3<double> normalize_m(V3<double> a) {
   return normalize_fast(a);
 }
 
+// Channels of one junction's log entry (logLayerParams, core/junction.hpp:
+// 1580-1636): m, resistance, energies.
+__device__ __forceinline__ int log_channels(int nl, int mr, int mask) {
+  int n = 0;
+  if (mask & CMTJ_LOG_MX) n += nl;
+  if (mask & CMTJ_LOG_MYZ) n += 2 * nl;
+  if (mask & CMTJ_LOG_MR) n += (mr == CMTJ_MR_CLASSIC) ? 1 : (mr == CMTJ_MR_STRIP ? 2 : 0);
+  if (mask & CMTJ_LOG_ENERGY) n += CMTJ_ENERGY_CHANNELS * nl;
+  return n;
+}
+
 // One log value at byte address p (binary32 or binary64 log).
 __device__ __forceinline__ void log_put(bool f32, char* p, double v) {
   if (f32)
@@ -816,7 +828,61 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
   }
 }
 
-template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
+// Current of stack member j for the stage times of the step at t
+// (Stack<T>::runSimulationKCL / runSimulationNonKCL, core/stack.hpp:305-410):
+// the pair couplings come from the members' magnetisations at the start of
+// the step (storeMagnetisations, :65-76) and travel between the lanes of the
+// wave.  I[s]: the current at stage time s (RK4: t, t + dt/2, t + dt);
+// Ilog, Clog: the stack log's I_j and C_j_{j+1} of this step.
+template <int LM>
+__device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n, int ti, int bi,
+                                              const V3<double> (&m)[LM], double t, double dt, double (&I)[3],
+                                              double& Ilog, double& Clog) {
+  V3<double> mt = m[0], mb = m[0];
+#pragma unroll
+  for (int l = 0; l < LM; ++l) {
+    mt = (l == ti) ? m[l] : mt;
+    mb = (l == bi) ? m[l] : mb;
+  }
+  const V3<double> p = sk->two_layer ? mb : mk<double>(sk->pol[0], sk->pol[1], sk->pol[2]);
+  const double d = mt.x * p.x + mt.y * p.y + mt.z * p.z;  // c_dot, core/junction.hpp:82-84
+  const double dn = __shfl(d, (j + 1 < n) ? j + 1 : j, 64);
+  const double Cj = (j + 1 < n) ? sk->coupling[j] * (sk->series ? d + dn : d - dn) : 0.0;
+  if (sk->kcl) {
+    // totalCurrent = I0 + sum_j C_j I0, the same constant for every member
+    const double I0 = driver_value(sk->current, t);
+    double total = I0;
+    for (int q = 0; q + 1 < n; ++q) total += __shfl(Cj, q, 64) * I0;
+    I[0] = I[1] = I[2] = total;
+    Ilog = total;
+    Clog = Cj;
+  } else {
+    // effectiveCoupling *= (1 + C_{j-1}) along the chain; the member's driver
+    // is current * effectiveCoupling with its phase shifted
+    double eff = 1.0;
+    for (int q = 0; q + 1 < n; ++q) {
+      const double cq = __shfl(Cj, q, 64);
+      if (q < j) eff *= (1 + cq);
+    }
+    Clog = eff * (1 + Cj);
+    cmtj_driver dl = sk->current;
+    dl.constant *= eff;
+    dl.amplitude *= eff;
+    dl.phase += sk->series ? sk->phase_offset * j : sk->phase_offset;
+    I[0] = driver_value(dl, t);
+    I[1] = driver_value(dl, t + 0.5 * dt);
+    I[2] = driver_value(dl, t + dt);
+    Ilog = I[0];
+  }
+}
+
+template <class R, int LM>
+__device__ __forceinline__ void set_current(DV<R> (&dv)[LM], double I) {
+#pragma unroll
+  for (int l = 0; l < LM; ++l) dv[l].I = (R)I;
+}
+
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false>
 __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= g.n) return;
@@ -894,8 +960,17 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     iend = (code != CMTJ_OK) ? i : (uint32_t)min((uint64_t)nsteps, (uint64_t)i + g.slice);
   }
   uint64_t next_log = (uint64_t)k * W;
+  [[maybe_unused]] int s_n = 0, s_ti = 0, s_bi = 0;
+  if constexpr (STK) {
+    s_n = g.stack->n_junctions;
+    s_ti = g.stack->top[lane];
+    s_bi = g.stack->bottom[lane];
+  }
   for (; i < iend; ++i) {
     const double t = (double)i * dt;
+    [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
+    [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
+    if constexpr (STK) stack_current<LM>(g.stack, (int)lane, s_n, s_ti, s_bi, m, t, dt, sI, sIlog, sClog);
     if constexpr (ANCHORED) {
       // RK4 of the reference (core/junction.hpp:1017-1041, neighbours frozen
       // at the start of the step) in anchored binary32: k1 and the anchor
@@ -976,6 +1051,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
         nbT[l] = (l + 1 < nl) ? ms[l + 1 < LM ? l + 1 : l] : zero;
       }
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t, i, 0, sps);
+      if constexpr (STK) set_current<R, LM>(dv, sI[0]);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -985,6 +1061,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
         }
       }
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t + 0.5 * dt, i, 1, sps);
+      if constexpr (STK) set_current<R, LM>(dv, sI[1]);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -1002,6 +1079,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
         }
       }
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t + dt, i, 2, sps);
+      if constexpr (STK) set_current<R, LM>(dv, sI[2]);
       V3<double> mn[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
@@ -1024,7 +1102,14 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
           if (isnan(mn[l].x)) nan_layer = l;
         }
       }
-      if (nan_layer >= 0) {
+      if constexpr (STK) {
+        // a member's NaN ends the stack's run (the reference throws): every
+        // member stops at this step and the lanes leave the loop together
+        if (__any(nan_layer >= 0)) {
+          if (nan_layer >= 0) code = CMTJ_ERR_NAN;
+          break;
+        }
+      } else if (nan_layer >= 0) {
         code = CMTJ_ERR_NAN;
         break;
       }
@@ -1034,6 +1119,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
 #pragma unroll
       for (int l = 0; l < LM; ++l) mr_[l] = cvt<R>(m[l]);
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t, i, 0, sps);
+      if constexpr (STK) set_current<R, LM>(dv, sI[0]);
       V3<double> mn[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
@@ -1096,6 +1182,14 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     }
     if ((uint64_t)i == next_log) {
       if (logging) write_entry<R, LM, FM, LOGE>(g, J, m, er, lp + (uint64_t)k * esize, cs, Rp, Rap);
+      if constexpr (STK) {
+        if (logging) {  // the member's stack channels follow its junction channels
+          const uint64_t nch = (uint64_t)log_channels(nl, g.mr_mode, g.log_mask);
+          char* q = lp + (uint64_t)k * esize + nch * cs;
+          log_put(g.log_f32, q, sIlog);
+          log_put(g.log_f32, q + cs, sClog);
+        }
+      }
       ++k;
       next_log += W;
     }
@@ -1137,9 +1231,9 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
 
 // One instance per lane.  (A variant compiled for >= 4 waves per SIMD spilled
 // 88-300 VGPRs and ran 1.6-5x slower on full-chip batches; not kept.)
-template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE>
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false>
 __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
-  junction_body<R, LM, SOLVER, FM, LOGE>(g);
+  junction_body<R, LM, SOLVER, FM, LOGE, STK>(g);
 }
 
 }  // namespace cmtj

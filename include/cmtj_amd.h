@@ -20,8 +20,9 @@
 extern "C" {
 #endif
 
-#define CMTJ_ABI_VERSION 2
+#define CMTJ_ABI_VERSION 3
 #define CMTJ_MAX_LAYERS 8
+#define CMTJ_MAX_STACK 64 /* junctions of one coupled stack (one wave) */
 
 /* ScalarDriver waveform kinds; numbering of reference UpdateType
  * (core/drivers.hpp:18-29).  CMTJ_TABLE is this library's tabulated form of
@@ -215,6 +216,7 @@ typedef struct cmtj_abi_info {
   int32_t sizeof_constants;
   int32_t sizeof_status;
   int32_t sizeof_state;
+  int32_t sizeof_stack;
   int32_t offsetof_layer_drv;
   int32_t offsetof_layer_ref_type;
   int32_t offsetof_junction_dt;
@@ -254,6 +256,46 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
              const cmtj_constants* c, const double* tables, size_t n_table,
              double* log_host, size_t log_cap, double* mag_out_host,
              cmtj_status* st);
+
+/* ---- electrically coupled stacks (FP64, host buffers) ------------------ */
+
+/* A SeriesStack / ParallelStack (core/stack.hpp:414-480): junctions that
+ * share one current.  Per step, with the magnetisations of the start of the
+ * step, pair (j, j+1) has the coupling
+ *   C_j = coupling[j] * (m_j . p_j  +  m_{j+1} . p_{j+1})   (series)
+ *   C_j = coupling[j] * (m_j . p_j  -  m_{j+1} . p_{j+1})   (parallel)
+ * with m = each junction's top layer and p = its bottom layer (two-layer
+ * members) or `pol` (getEffectiveCouplingStrength, :428-436, :459-466).
+ * KCL (runSimulationKCL, :372-410): every junction takes the constant
+ * current I0(t) + sum_j C_j I0(t), I0 = current(t).  Non-KCL with delayed
+ * coupling (runSimulationNonKCL, :305-370): junction j takes the driver
+ * current * prod_{k<j} (1 + C_k), phase-shifted by phase_offset * j
+ * (series) or phase_offset (parallel), evaluated at its stage times. */
+typedef struct cmtj_stack {
+  int32_t series;      /* 1: SeriesStack, 0: ParallelStack */
+  int32_t kcl;         /* 1: KCL, 0: non-KCL (delayed coupling) */
+  int32_t n_junctions; /* 2..CMTJ_MAX_STACK */
+  int32_t two_layer;   /* 1: p_j = bottom layer of junction j, 0: p_j = pol */
+  double phase_offset;
+  double pol[3];
+  cmtj_driver current; /* setCoupledCurrentDriver (:196-202); not CMTJ_TABLE */
+  double coupling[CMTJ_MAX_STACK]; /* pair (j, j+1), j < n_junctions - 1 */
+  int32_t top[CMTJ_MAX_STACK];     /* layer index of topId in junction j */
+  int32_t bottom[CMTJ_MAX_STACK];  /* layer index of bottomId (two-layer members) */
+} cmtj_stack;
+
+/* Runs a stack on the current GPU, one wave: junction j on lane j.  js[j]
+ * are the members (same structure, solver RK4 or Euler-Heun, n_steps,
+ * write_every, dt); js[j].log_offset places member j's block in log_host:
+ * the channels of cmtj_run's log followed by two stack channels, the current
+ * of junction j at the logged step and, for j < n - 1, the logged coupling
+ * of pair (j, j+1) (C_j under KCL, prod_{k<=j} (1 + C_k) otherwise: the
+ * stack log's "I_j" and "C_j_{j+1}", :206-217).  mag_out_host: [n][8][3],
+ * st_host: [n].  A NaN in any member stops every member at that step.
+ * Replaces Stack<T>::runSimulation (:296-300). */
+int cmtj_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_layer* layers,
+                   size_t n_layer_rows, const cmtj_constants* c, double* log_host, size_t log_cap,
+                   double* mag_out_host, cmtj_status* st_host);
 
 /* ---- batched plans: many uneven instances, device buffers -------------- */
 

@@ -52,6 +52,7 @@ def _load():
         lib.oracle_thermal_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                                ctypes.c_uint32, P]
         lib.oracle_n_channels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.oracle_run_stack.argtypes = [P, P, P, P, P, P, P]
         _lib = (lib, abi)
     return _lib
 
@@ -88,6 +89,27 @@ def run_batch(js, layers, consts, tables=None, log_elements=0, n_threads=None):
     lib.oracle_run_batch(_p(js), js.size, _p(layers), _p(np.ascontiguousarray(consts)), _p(tabs),
                          0 if tabs is None else tabs.size, _p(log), _p(mag), _p(st), int(n_threads))
     return log, mag, st
+
+
+def run_stack(stack, js, layers, consts):
+    """A coupled stack, same layout as cmtj_run_stack: returns (log, mag
+    [n, 8, 3], status [n])."""
+    lib, abi = _load()
+    stack = np.ascontiguousarray(stack, dtype=abi.STACK_DTYPE).reshape(())
+    js = np.ascontiguousarray(js, dtype=abi.JUNCTION_DTYPE)
+    layers = np.ascontiguousarray(layers, dtype=abi.LAYER_DTYPE)
+    need = 0
+    for q in range(js.size):
+        nch = lib.oracle_n_channels(int(js[q]["n_layers"]), int(js[q]["mr_mode"]), int(js[q]["log_mask"])) + 2
+        need = max(need, int(js[q]["log_offset"]) + nch * int(js[q]["n_entries"]))
+    log = np.zeros(max(need, 1), dtype=np.float64)
+    mag = np.zeros((js.size, abi.MAX_LAYERS, 3), dtype=np.float64)
+    st = np.zeros(js.size, dtype=abi.STATUS_DTYPE)
+    rc = lib.oracle_run_stack(_p(stack), _p(js), _p(layers), _p(np.ascontiguousarray(consts)), _p(log), _p(mag),
+                              _p(st))
+    if rc:
+        raise ValueError("oracle_run_stack: bad stack descriptor")
+    return log[:need], mag, st
 
 
 def driver_value(rec, t: float) -> float:

@@ -406,12 +406,11 @@ static void log_entry(const cmtj_junction* j, const olayer* Ls, double* log, uin
   }
 }
 
-int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
-               const double* tables, size_t n_table, double* log, double* mag_out, cmtj_status* st) {
-  (void)n_table;
+/* Layer states of one junction at the start of a run. */
+static void load_layers(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
+                        const double* tables, olayer* Ls) {
   const int nl = j->n_layers;
-  olayer Ls[CMTJ_MAX_LAYERS];
-  memset(Ls, 0, sizeof(Ls));
+  memset(Ls, 0, sizeof(olayer) * CMTJ_MAX_LAYERS);
   const int sps = (j->solver == CMTJ_RK4) ? 3 : (j->solver == CMTJ_HEUN ? 2 : 1);
   for (int l = 0; l < nl; ++l) {
     Ls[l].p = &layers[l];
@@ -421,70 +420,93 @@ int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_cons
     Ls[l].mag = vload(layers[l].mag);
     Ls[l].cellVolume = layers[l].cell_surface * layers[l].thickness;
   }
+}
+
+/* One step i of every layer of a junction with its solver.  Returns 0, or
+ * nonzero with st->code / st->nan_layer set. */
+static int junction_step(const cmtj_junction* j, olayer* Ls, uint32_t i, cmtj_status* st) {
+  const int nl = j->n_layers;
+  const double dt = j->dt;
+  const vec3 zero = v3(0, 0, 0);
+  const double t = i * dt; /* core/junction.hpp:1974 */
+  if (j->solver == CMTJ_RK4) {
+    /* runSingleLayerSolver / runMultiLayerSolver, :1668-1707: neighbours
+     * frozen at the start of the step for every stage of every layer */
+    vec3 copies[CMTJ_MAX_LAYERS + 2];
+    copies[0] = zero;
+    copies[nl + 1] = zero;
+    for (int l = 0; l < nl; ++l) copies[l + 1] = Ls[l].mag;
+    for (int l = 0; l < nl; ++l) {
+      if (rk4_step(&Ls[l], t, dt, i, copies[l], copies[l + 2])) {
+        st->code = CMTJ_ERR_NAN;
+        st->nan_layer = l;
+        return 1;
+      }
+    }
+  } else if (j->solver == CMTJ_EULER_HEUN) {
+    /* eulerHeunSolverStep, :1709-1754 */
+    vec3 mPrime[CMTJ_MAX_LAYERS];
+    for (int l = 0; l < nl; ++l) {
+      olayer* L = &Ls[l];
+      const vec3 dW = langevin_vector(L, j, t, i, (uint32_t)l);
+      const vec3 bottom = (l == 0) ? zero : Ls[l - 1].mag;
+      const vec3 top = (l == nl - 1) ? zero : Ls[l + 1].mag;
+      const vec3 fn = llg(L, t, i, 0, L->mag, bottom, top);
+      const vec3 gn = stochastic_torque(L, L->mag, dW);
+      const vec3 mNext = vadd(L->mag, vmul(gn, sqrt(dt)));
+      const vec3 gnp = stochastic_torque(L, mNext, dW);
+      mPrime[l] = vadd(vadd(L->mag, vmul(fn, dt)), vmul(smul(0.5, vadd(gn, gnp)), sqrt(dt)));
+    }
+    for (int l = 0; l < nl; ++l) Ls[l].mag = vnormalize(mPrime[l]);
+  } else if (j->solver == CMTJ_HEUN) {
+    /* heunSolverStep, :1756-1815 */
+    vec3 fn[CMTJ_MAX_LAYERS], gn[CMTJ_MAX_LAYERS], dW[CMTJ_MAX_LAYERS], mNext[CMTJ_MAX_LAYERS];
+    for (int l = 0; l < nl; ++l) {
+      olayer* L = &Ls[l];
+      const vec3 bottom = (l == 0) ? zero : Ls[l - 1].mag;
+      const vec3 top = (l == nl - 1) ? zero : Ls[l + 1].mag;
+      fn[l] = llg(L, t, i, 0, L->mag, bottom, top);
+      dW[l] = langevin_vector(L, j, t, i, (uint32_t)l);
+      gn[l] = stochastic_torque(L, L->mag, dW[l]);
+      mNext[l] = vadd(vadd(L->mag, vmul(fn[l], dt)), vmul(gn[l], sqrt(dt)));
+    }
+    for (int l = 0; l < nl; ++l) {
+      olayer* L = &Ls[l];
+      const vec3 bottom = (l == 0) ? zero : mNext[l - 1];
+      const vec3 top = (l == nl - 1) ? zero : mNext[l + 1];
+      const vec3 f2 = llg(L, t + dt, i, 1, mNext[l], bottom, top);
+      const vec3 g2 = stochastic_torque(L, mNext[l], dW[l]);
+      vec3 m = vadd(L->mag, smul(0.5 * dt, vadd(fn[l], f2)));
+      m = vadd(m, vmul(smul(0.5, vadd(gn[l], g2)), sqrt(dt)));
+      L->mag = vnormalize(m);
+    }
+  } else {
+    st->code = CMTJ_ERR_INVALID;
+    return 2;
+  }
+  return 0;
+}
+
+static void store_mag(const cmtj_junction* j, const olayer* Ls, double* mag_out) {
+  for (int l = 0; l < j->n_layers; ++l) {
+    mag_out[3 * l + 0] = Ls[l].mag.x;
+    mag_out[3 * l + 1] = Ls[l].mag.y;
+    mag_out[3 * l + 2] = Ls[l].mag.z;
+  }
+}
+
+int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
+               const double* tables, size_t n_table, double* log, double* mag_out, cmtj_status* st) {
+  (void)n_table;
+  olayer Ls[CMTJ_MAX_LAYERS];
+  load_layers(j, layers, c, tables, Ls);
   st->code = CMTJ_OK;
   st->steps_done = 0;
   st->entries_done = 0;
   st->nan_layer = -1;
-  const double dt = j->dt;
-  const vec3 zero = v3(0, 0, 0);
   uint32_t k = 0;
   for (uint32_t i = 0; i < j->n_steps; ++i) {
-    const double t = i * dt; /* core/junction.hpp:1974 */
-    if (j->solver == CMTJ_RK4) {
-      /* runSingleLayerSolver / runMultiLayerSolver, :1668-1707: neighbours
-       * frozen at the start of the step for every stage of every layer */
-      vec3 copies[CMTJ_MAX_LAYERS + 2];
-      copies[0] = zero;
-      copies[nl + 1] = zero;
-      for (int l = 0; l < nl; ++l) copies[l + 1] = Ls[l].mag;
-      for (int l = 0; l < nl; ++l) {
-        if (rk4_step(&Ls[l], t, dt, i, copies[l], copies[l + 2])) {
-          st->code = CMTJ_ERR_NAN;
-          st->nan_layer = l;
-          goto done;
-        }
-      }
-    } else if (j->solver == CMTJ_EULER_HEUN) {
-      /* eulerHeunSolverStep, :1709-1754 */
-      vec3 mPrime[CMTJ_MAX_LAYERS];
-      for (int l = 0; l < nl; ++l) {
-        olayer* L = &Ls[l];
-        const vec3 dW = langevin_vector(L, j, t, i, (uint32_t)l);
-        const vec3 bottom = (l == 0) ? zero : Ls[l - 1].mag;
-        const vec3 top = (l == nl - 1) ? zero : Ls[l + 1].mag;
-        const vec3 fn = llg(L, t, i, 0, L->mag, bottom, top);
-        const vec3 gn = stochastic_torque(L, L->mag, dW);
-        const vec3 mNext = vadd(L->mag, vmul(gn, sqrt(dt)));
-        const vec3 gnp = stochastic_torque(L, mNext, dW);
-        mPrime[l] = vadd(vadd(L->mag, vmul(fn, dt)), vmul(smul(0.5, vadd(gn, gnp)), sqrt(dt)));
-      }
-      for (int l = 0; l < nl; ++l) Ls[l].mag = vnormalize(mPrime[l]);
-    } else if (j->solver == CMTJ_HEUN) {
-      /* heunSolverStep, :1756-1815 */
-      vec3 fn[CMTJ_MAX_LAYERS], gn[CMTJ_MAX_LAYERS], dW[CMTJ_MAX_LAYERS], mNext[CMTJ_MAX_LAYERS];
-      for (int l = 0; l < nl; ++l) {
-        olayer* L = &Ls[l];
-        const vec3 bottom = (l == 0) ? zero : Ls[l - 1].mag;
-        const vec3 top = (l == nl - 1) ? zero : Ls[l + 1].mag;
-        fn[l] = llg(L, t, i, 0, L->mag, bottom, top);
-        dW[l] = langevin_vector(L, j, t, i, (uint32_t)l);
-        gn[l] = stochastic_torque(L, L->mag, dW[l]);
-        mNext[l] = vadd(vadd(L->mag, vmul(fn[l], dt)), vmul(gn[l], sqrt(dt)));
-      }
-      for (int l = 0; l < nl; ++l) {
-        olayer* L = &Ls[l];
-        const vec3 bottom = (l == 0) ? zero : mNext[l - 1];
-        const vec3 top = (l == nl - 1) ? zero : mNext[l + 1];
-        const vec3 f2 = llg(L, t + dt, i, 1, mNext[l], bottom, top);
-        const vec3 g2 = stochastic_torque(L, mNext[l], dW[l]);
-        vec3 m = vadd(L->mag, smul(0.5 * dt, vadd(fn[l], f2)));
-        m = vadd(m, vmul(smul(0.5, vadd(gn[l], g2)), sqrt(dt)));
-        L->mag = vnormalize(m);
-      }
-    } else {
-      st->code = CMTJ_ERR_INVALID;
-      goto done;
-    }
+    if (junction_step(j, Ls, i, st)) break;
     st->steps_done = i + 1;
     if (i % j->write_every == 0) { /* :1978-1980 */
       log_entry(j, Ls, log, k);
@@ -492,12 +514,107 @@ int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_cons
       st->entries_done = k;
     }
   }
-done:
-  for (int l = 0; l < nl; ++l) {
-    mag_out[3 * l + 0] = Ls[l].mag.x;
-    mag_out[3 * l + 1] = Ls[l].mag.y;
-    mag_out[3 * l + 2] = Ls[l].mag.z;
+  store_mag(j, Ls, mag_out);
+  return 0;
+}
+
+/* ---- electrically coupled stacks (core/stack.hpp) ----------------------- */
+
+/* Stack<T>::runSimulationKCL (:372-410) and runSimulationNonKCL with delayed
+ * coupling (:305-370) of SeriesStack / ParallelStack (:414-480).  Member q
+ * logs into log + js[q].log_offset: its junction channels, then the stack's
+ * I_q and C_q_{q+1} (logStackData, :206-217).  Per step the coupling uses
+ * the members' magnetisations at the start of the step (storeMagnetisations,
+ * :65-76); each member's current driver is replaced for the step by the
+ * local driver (executeJunctionStep, :78-91: setLayerCurrentDriver("all")),
+ * then the member takes one step of its solver. */
+int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_layer* layers,
+                     const cmtj_constants* c, double* log, double* mag_out, cmtj_status* st) {
+  const int n = s->n_junctions;
+  if (n < 2 || n > CMTJ_MAX_STACK) return 1;
+  cmtj_layer* own = (cmtj_layer*)malloc(sizeof(cmtj_layer) * CMTJ_MAX_LAYERS * (size_t)n);
+  olayer* Ls = (olayer*)malloc(sizeof(olayer) * CMTJ_MAX_LAYERS * (size_t)n);
+  if (!own || !Ls) {
+    free(own);
+    free(Ls);
+    return 1;
   }
+  for (int q = 0; q < n; ++q) {
+    const cmtj_junction* j = &js[q];
+    memcpy(&own[q * CMTJ_MAX_LAYERS], layers + j->layer_offset, sizeof(cmtj_layer) * (size_t)j->n_layers);
+    load_layers(j, &own[q * CMTJ_MAX_LAYERS], c, NULL, &Ls[q * CMTJ_MAX_LAYERS]);
+    st[q].code = CMTJ_OK;
+    st[q].steps_done = 0;
+    st[q].entries_done = 0;
+    st[q].nan_layer = -1;
+  }
+  const cmtj_junction* j0 = &js[0];
+  double C[CMTJ_MAX_STACK], Ilog[CMTJ_MAX_STACK], Clog[CMTJ_MAX_STACK];
+  uint32_t k = 0;
+  int stop = 0;
+  for (uint32_t i = 0; i < j0->n_steps && !stop; ++i) {
+    const double t = i * j0->dt;
+    /* storeMagnetisations and getEffectiveCouplingStrength (:428-436, :459-466) */
+    double d[CMTJ_MAX_STACK];
+    for (int q = 0; q < n; ++q) {
+      const olayer* L = &Ls[q * CMTJ_MAX_LAYERS];
+      const vec3 m = L[s->top[q]].mag;
+      const vec3 p = s->two_layer ? L[s->bottom[q]].mag : v3(s->pol[0], s->pol[1], s->pol[2]);
+      d[q] = vdot(m, p);
+    }
+    for (int q = 0; q + 1 < n; ++q) C[q] = s->coupling[q] * (s->series ? d[q] + d[q + 1] : d[q] - d[q + 1]);
+    cmtj_driver local[CMTJ_MAX_STACK];
+    if (s->kcl) {
+      const double I0 = oracle_driver_value(&s->current, t, i, 0, 1, NULL);
+      double total = I0;
+      for (int q = 0; q + 1 < n; ++q) {
+        total += C[q] * I0;
+        Clog[q] = C[q];
+      }
+      for (int q = 0; q < n; ++q) {
+        memset(&local[q], 0, sizeof(cmtj_driver));
+        local[q].type = CMTJ_CONSTANT; /* getConstantDriver(totalCurrent); phaseShift leaves it */
+        local[q].constant = total;
+      }
+    } else {
+      double eff = 1.0;
+      for (int q = 0; q < n; ++q) {
+        if (q > 0) {
+          eff *= (1 + C[q - 1]);
+          Clog[q - 1] = eff;
+        }
+        local[q] = s->current; /* currentDriver * effectiveCoupling (drivers.hpp:313-318) */
+        local[q].constant = s->current.constant * eff;
+        local[q].amplitude = s->current.amplitude * eff;
+        local[q].phase += s->series ? s->phase_offset * q : s->phase_offset; /* phaseShift */
+      }
+    }
+    for (int q = 0; q < n && !stop; ++q) {
+      const cmtj_junction* j = &js[q];
+      for (int l = 0; l < j->n_layers; ++l) own[q * CMTJ_MAX_LAYERS + l].drv[CMTJ_SLOT_CURRENT] = local[q];
+      if (junction_step(j, &Ls[q * CMTJ_MAX_LAYERS], i, &st[q])) stop = 1;
+      Ilog[q] = oracle_driver_value(&local[q], t, i, 0, 1, NULL);
+    }
+    if (stop) break;
+    for (int q = 0; q < n; ++q) st[q].steps_done = i + 1;
+    if (i % j0->write_every == 0) { /* logSimulationStep, :93-104 */
+      for (int q = 0; q < n; ++q) {
+        const cmtj_junction* j = &js[q];
+        double* blk = log + j->log_offset;
+        log_entry(j, &Ls[q * CMTJ_MAX_LAYERS], blk, k);
+        const size_t E = j->n_entries;
+        const size_t nch = (size_t)oracle_n_channels(j->n_layers, j->mr_mode, j->log_mask);
+        blk[nch * E + k] = Ilog[q];
+        blk[(nch + 1) * E + k] = (q + 1 < n) ? Clog[q] : 0.0;
+        st[q].entries_done = k + 1;
+      }
+      ++k;
+    }
+  }
+  for (int q = 0; q < n; ++q)
+    store_mag(&js[q], &Ls[q * CMTJ_MAX_LAYERS], mag_out + (size_t)q * 3 * CMTJ_MAX_LAYERS);
+  free(own);
+  free(Ls);
   return 0;
 }
 

@@ -10,3 +10,16 @@ from cmtj_amd import Junction
 class OracleJunction(Junction):
     def _execute(self, j, L, consts, tables):
         return oracle.run(j, L, consts, tables)
+
+
+from cmtj_amd.stack import ParallelStack, SeriesStack  # noqa: E402
+
+
+class OracleSeriesStack(SeriesStack):
+    def _execute(self, stack, js, layers, consts):
+        return oracle.run_stack(stack, js, layers, consts)
+
+
+class OracleParallelStack(ParallelStack):
+    def _execute(self, stack, js, layers, consts):
+        return oracle.run_stack(stack, js, layers, consts)

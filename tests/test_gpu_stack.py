@@ -1,0 +1,90 @@
+"""Coupled stacks on the GPU (cmtj_run_stack, one wave) against the oracle's
+stack runner: series / parallel, KCL / non-KCL, one- and two-layer members,
+RK4 and the thermal Euler-Heun path, and a NaN that stops the stack."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cmtj_amd import AxialDriver, ScalarDriver, constantDriver, stepDriver
+from cmtj_amd.stack import ParallelStack, SeriesStack
+from oracle_backend import OracleParallelStack, OracleSeriesStack
+from test_stack_cpu import _single, _two
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(member, n=3):
+    mags = [(0, 0, 1), (0.4, 0.1, 0.9), (0.1, 0.5, 0.8), (0.7, 0.1, 0.3)]
+    return [member(stt=True, mag=mags[q % len(mags)]) for q in range(n)]
+
+
+def _compare(a, b, tol=1e-8):
+    la, lb = a.getLog(), b.getLog()
+    assert la.keys() == lb.keys()
+    worst = 0.0
+    for k in la:
+        x, y = np.asarray(la[k]), np.asarray(lb[k])
+        assert x.shape == y.shape, k
+        scale = max(1.0, float(np.max(np.abs(y)))) if y.size else 1.0
+        worst = max(worst, float(np.max(np.abs(x - y))) / scale if y.size else 0.0)
+    for q in range(a.stackSize):
+        ja, jb = a.getLog(q), b.getLog(q)
+        assert ja.keys() == jb.keys()
+        for k in ja:
+            x, y = np.asarray(ja[k]), np.asarray(jb[k])
+            scale = max(1.0, float(np.max(np.abs(y)))) if y.size else 1.0
+            worst = max(worst, float(np.max(np.abs(x - y))) / scale if y.size else 0.0)
+    return worst
+
+
+@pytest.mark.parametrize("member", [_single, _two])
+@pytest.mark.parametrize("kcl", [True, False])
+@pytest.mark.parametrize("series", [True, False])
+def test_stack_matches_oracle(member, kcl, series):
+    js = _members(member)
+    drv = ScalarDriver.getSineDriver(8e11, 5e11, 3e9, 0.4)
+    gcls, ocls = (SeriesStack, OracleSeriesStack) if series else (ParallelStack, OracleParallelStack)
+    stacks = [cls(js, useKCL=kcl, phaseOffset=0.3) for cls in (gcls, ocls)]
+    for s in stacks:
+        s.setCouplingStrength([0.25, -0.4])
+        s.setCoupledCurrentDriver(drv)
+        s.setExternalFieldDriver(AxialDriver(constantDriver(1e4), constantDriver(0), constantDriver(2e4)))
+        s.runSimulation(0.5e-9, 1e-13, 1e-12)
+    worst = _compare(*stacks)
+    print(f"[stack] {member.__name__} kcl={kcl} series={series}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_stochastic_stack_matches_oracle_pathwise():
+    """Members at 300 K step with Euler-Heun; the engine and the oracle share
+    the Philox stream, so the paths agree."""
+    js = _members(_single, n=4)
+    for j in js:
+        j.setLayerTemperatureDriver("all", constantDriver(300.0))
+        j.setNoiseSeed(7)
+    stacks = [cls(js) for cls in (SeriesStack, OracleSeriesStack)]
+    for s in stacks:
+        s.setCouplingStrength(0.3)
+        s.setCoupledCurrentDriver(constantDriver(1e12))
+        s.runSimulation(0.2e-9, 1e-13, 1e-12)
+    worst = _compare(*stacks)
+    print(f"[stack] Euler-Heun 300 K: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_nan_member_stops_the_stack():
+    js = _members(_single, n=3)
+    for q, j in enumerate(js):  # one structure for every member; member 1's step is infinite
+        amp = float("inf") if q == 1 else 1e3
+        j.setLayerExternalFieldDriver("all", AxialDriver(stepDriver(0, amp, 5e-12, 1e-9), constantDriver(0),
+                                                         constantDriver(1e5)))
+    g, o = SeriesStack(js), OracleSeriesStack(js)
+    for s in (g, o):
+        s.setCoupledCurrentDriver(constantDriver(1e12))
+        with pytest.raises(RuntimeError, match="NAN magnetisation"):
+            s.runSimulation(1e-10, 1e-13, 1e-13)
+    assert 0 < len(g.getLog()["time"]) < 1000
+    assert len(g.getLog()["time"]) == len(o.getLog()["time"])
+    for q in range(3):
+        assert len(g.getLog(q)["time"]) == len(g.getLog()["time"])
