@@ -177,6 +177,71 @@ def _bench_rh() -> dict:
             "final_Rx_first_mid_last": [float(rx[0]), float(rx[50]), float(rx[-1])]}
 
 
+def _bench_stack(n_members: int = 8, total_time: float = 10e-9) -> dict:
+    """Series stack of STT junctions sharing one current (KCL, coupling
+    0.05): one stack run on the GPU (one wave, member j on lane j) against
+    the oracle's stack runner on one host core; RK4 dt 1e-13, W = 10."""
+    import numpy as np
+
+    import oracle
+    from cmtj_amd import CVector, Junction, Layer, _lib, constantDriver
+    from cmtj_amd.stack import SeriesStack
+
+    demag = [CVector(0, 0, 0), CVector(0, 0, 0), CVector(0, 0, 1)]
+    members = []
+    for q in range(n_members):
+        l = Layer.createSTTLayer("free", mag=CVector(0.1 * q, 0.2, 1.0), anis=CVector(1, 0, 0), Ms=1.65 - 0.02 * q,
+                                 thickness=3e-9, cellSurface=7e-14, demagTensor=demag, damping=0.005,
+                                 SlonczewskiSpacerLayerParameter=1.0, beta=0.1, spinPolarisation=0.6)
+        l.setReferenceLayer(CVector(1, 0, 0))
+        j = Junction([l], 100, 200)
+        j.setLayerAnisotropyDriver("free", constantDriver(1.05e3))
+        members.append(j)
+
+    class _OracleStack(SeriesStack):
+        def _execute(self, stack, js, layers, consts):
+            return oracle.run_stack(stack, js, layers, consts)
+
+    out = {}
+    for name, cls in (("gpu", SeriesStack), ("cpu_oracle", _OracleStack)):
+        s = cls(members)
+        s.setCouplingStrength(0.05)
+        s.setCoupledCurrentDriver(constantDriver(5e11))
+        s.runSimulation(1e-11, 1e-13, 1e-12)  # warm-up (code objects, buffers)
+        s.clearLogs()
+        t0 = time.perf_counter()
+        s.runSimulation(total_time, 1e-13, 1e-12)
+        out[name] = (time.perf_counter() - t0, s)
+    steps = n_members * int(total_time / 1e-13)
+    # a sweep: n_stacks stacks (coupling and current varied) in one launch
+    from cmtj_amd.stack import run_stacks
+
+    n_stacks = 4096
+    sweep = []
+    for k in range(n_stacks):
+        s = SeriesStack(members)
+        s.setCouplingStrength(0.01 + 0.1 * (k % 64) / 64)
+        s.setCoupledCurrentDriver(constantDriver(1e11 * (1 + k // 64)))
+        sweep.append(s)
+    run_stacks(sweep[:64], 1e-11, 1e-13, 1e-12)  # warm-up
+    t0 = time.perf_counter()
+    run_stacks(sweep, 1e-9, 1e-13, 1e-12)
+    sweep_s = time.perf_counter() - t0
+    sweep_steps = n_stacks * n_members * int(1e-9 / 1e-13)
+    g, o = out["gpu"][1], out["cpu_oracle"][1]
+    dR = float(np.max(np.abs(np.asarray(g.getLog()["Resistance"]) - np.asarray(o.getLog()["Resistance"]))))
+    dm = max(float(np.max(np.abs(np.asarray(g.getLog(q)["free_mx"]) - np.asarray(o.getLog(q)["free_mx"]))))
+             for q in range(n_members))
+    return {"workload": f"series_stack_{n_members}_stt_members_kcl", "junction_steps": steps,
+            "gpu_s": out["gpu"][0], "gpu_junction_steps_per_s": steps / out["gpu"][0],
+            "cpu_oracle_s": out["cpu_oracle"][0], "cpu_oracle_junction_steps_per_s": steps / out["cpu_oracle"][0],
+            "cpu_threads": 1, "max_dm_gpu_vs_oracle": dm, "max_dresistance_gpu_vs_oracle": dR,
+            "conditioning": "a 1e-14 change of one member's initial m grows to 3.9e-6 in the oracle itself by 10 ns",
+            "sweep": {"stacks": n_stacks, "members": n_members, "total_time": 1e-9, "junction_steps": sweep_steps,
+                      "s": sweep_s, "junction_steps_per_s": sweep_steps / sweep_s},
+            "what": "host-to-host Stack.runSimulation / run_stacks (packing, upload, kernel, log copy back)"}
+
+
 def _bench_cims(n_currents: int = 256, n_seeds: int = 4096) -> dict:
     """CIMS / p-bit ensemble: 256 currents x 4096 seeds, 2 layers, static SOT
     at 300 K, Euler-Heun dt 1e-14 s, 5 ns; switching probability per current
@@ -276,7 +341,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving"],
+    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving", "stack"],
                     help="pimm: the headline sweep (default); rh, cims, sto: the other sweep workloads, "
                          "one JSON line each")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -298,7 +363,7 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if args.workload != "pimm":
-        fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto,
+        fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto, "stack": _bench_stack,
               "serving": lambda: _serving(args.serving_qps, args.serving_seconds, lanes=args.serving_lanes)}[args.workload]
         print(json.dumps(fn()))
         return 0

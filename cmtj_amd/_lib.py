@@ -210,7 +210,7 @@ EXPORTS = (
     "cmtj_plan_query",
     "cmtj_plan_destroy",
     "cmtj_vsd_mix",
-    "cmtj_run_stack",
+    "cmtj_run_stacks",
 )
 
 _P = ctypes.c_void_p
@@ -257,7 +257,7 @@ def _load():
     lib.cmtj_plan_destroy.argtypes = [_P]
     lib.cmtj_vsd_mix.argtypes = [_P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _P, ctypes.c_double, _P, _P,
                                  ctypes.c_int32, _P, _P]
-    lib.cmtj_run_stack.argtypes = [_P, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P]
+    lib.cmtj_run_stacks.argtypes = [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P]
     info = _AbiInfo()
     lib.cmtj_abi(ctypes.byref(info))
     checks = {
@@ -354,11 +354,11 @@ def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: 
     return log, mag, st
 
 
-def run_stack(stack: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray):
-    """A coupled stack through cmtj_run_stack; js[q]["log_offset"] must place
-    each member's block of n_channels + 2 channels.  Returns (log, mag [n, 8, 3],
-    status [n])."""
-    stack = np.ascontiguousarray(stack, dtype=STACK_DTYPE).reshape(())
+def run_stacks(stacks: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray):
+    """Stacks through cmtj_run_stacks (js[s * members + j]: member j of stack
+    s); js[q]["log_offset"] must place each member's block of n_channels + 2
+    channels.  Returns (log, mag [n, 8, 3], status [n])."""
+    stacks = np.ascontiguousarray(stacks, dtype=STACK_DTYPE).reshape(-1)
     js = np.ascontiguousarray(js, dtype=JUNCTION_DTYPE)
     layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
     n = js.size
@@ -369,8 +369,8 @@ def run_stack(stack: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.
     log = np.zeros(need, dtype=np.float64)
     mag = np.zeros((n, MAX_LAYERS, 3), dtype=np.float64)
     st = np.zeros(n, dtype=STATUS_DTYPE)
-    check(lib().cmtj_run_stack(ptr(stack), ptr(js), ptr(layers), layers.size, ptr(consts),
-                               ptr(log) if log.size else None, log.size, ptr(mag), ptr(st)), "cmtj_run_stack")
+    check(lib().cmtj_run_stacks(ptr(stacks), stacks.size, ptr(js), ptr(layers), layers.size, ptr(consts),
+                                ptr(log) if log.size else None, log.size, ptr(mag), ptr(st)), "cmtj_run_stacks")
     return log, mag, st
 
 

@@ -651,26 +651,36 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_consta
   return 0;
 }
 
-int cmtj_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_layer* layers, size_t n_layer_rows,
-                   const cmtj_constants* c, double* log_host, size_t log_cap, double* mag_out_host,
-                   cmtj_status* st_host) {
-  if (!s || !js || !layers || !c || !mag_out_host || !st_host) return fail(CMTJ_ERR_INVALID, "null pointer");
-  const int n = s->n_junctions;
+int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_junction* js, const cmtj_layer* layers,
+                    size_t n_layer_rows, const cmtj_constants* c, double* log_host, size_t log_cap,
+                    double* mag_out_host, cmtj_status* st_host) {
+  if (!stacks || !js || !layers || !c || !mag_out_host || !st_host) return fail(CMTJ_ERR_INVALID, "null pointer");
+  if (n_stacks == 0) return 0;
+  if (n_stacks > (1u << 24)) return fail(CMTJ_ERR_INVALID, "too many stacks");
+  const int n = stacks[0].n_junctions;
   if (n < 2 || n > CMTJ_MAX_STACK)
     return fail(CMTJ_ERR_INVALID, "a stack has 2.." + std::to_string(CMTJ_MAX_STACK) + " junctions");
-  if (s->current.type == CMTJ_TABLE || s->current.type < 0 || s->current.type > CMTJ_GAUSSSTEP)
-    return fail(CMTJ_ERR_INVALID, "the stack current needs a closed-form driver");
+  for (size_t q = 0; q < n_stacks; ++q) {
+    const cmtj_stack& s = stacks[q];
+    if (s.n_junctions != n) return fail(CMTJ_ERR_INVALID, "every stack of a batch has the same member count");
+    if (s.current.type == CMTJ_TABLE || s.current.type < 0 || s.current.type > CMTJ_GAUSSSTEP)
+      return fail(CMTJ_ERR_INVALID, "the stack current needs a closed-form driver");
+  }
+  const size_t nj = n_stacks * (size_t)n;
   const cmtj_junction& j0 = js[0];
   if (j0.solver != CMTJ_RK4 && j0.solver != CMTJ_EULER_HEUN)
     return fail(CMTJ_ERR_INVALID, "stacks step with RK4 or Euler-Heun");
   size_t need = 0;
-  for (int q = 0; q < n; ++q) {
+  for (size_t q = 0; q < nj; ++q) {
     const cmtj_junction& j = js[q];
+    const cmtj_stack& s = stacks[q / (size_t)n];
+    const int m = (int)(q % (size_t)n);
+    // equal step counts keep the plan's instance order the caller's
     if (j.solver != j0.solver || j.n_steps != j0.n_steps || j.write_every != j0.write_every ||
         j.n_entries != j0.n_entries || j.dt != j0.dt)
       return fail(CMTJ_ERR_INVALID, "stack members must share solver, step count, time step and log cadence");
     if (j.n_layers < 1 || j.n_layers > 2) return fail(CMTJ_ERR_INVALID, "stack members have 1 or 2 layers");
-    if (s->top[q] < 0 || s->top[q] >= j.n_layers || (s->two_layer && (s->bottom[q] < 0 || s->bottom[q] >= j.n_layers)))
+    if (s.top[m] < 0 || s.top[m] >= j.n_layers || (s.two_layer && (s.bottom[m] < 0 || s.bottom[m] >= j.n_layers)))
       return fail(CMTJ_ERR_INVALID, "top/bottom layer index out of range");
     const size_t nch = (size_t)n_channels(j.n_layers, j.mr_mode, j.log_mask) + 2;
     need = std::max<size_t>(need, j.log_offset + nch * j.n_entries);
@@ -681,6 +691,8 @@ int cmtj_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_laye
   for (size_t q = 0; q < kt.n; ++q)
     if (kt.e[q].solver == j0.solver) ke = &kt.e[q];
   if (!ke) return fail(CMTJ_ERR_INVALID, "no stack kernel for this solver");
+  uint32_t pad = 1;
+  while (pad < (uint32_t)n) pad <<= 1;
 
   std::lock_guard<std::mutex> lk(g_plain.mu);
   int dev = 0;
@@ -696,31 +708,32 @@ int cmtj_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_laye
   }
   cmtj_plan P;
   P.owns = false;
-  int r = build_plan(&P, js, (size_t)n, layers, n_layer_rows, c, nullptr, 0, CMTJ_FP64, CMTJ_LOG_F64,
-                     CMTJ_LANES_ONE, plain_arena, &g_plain, g_plain.stream);
+  int r = build_plan(&P, js, nj, layers, n_layer_rows, c, nullptr, 0, CMTJ_FP64, CMTJ_LOG_F64, CMTJ_LANES_ONE,
+                     plain_arena, &g_plain, g_plain.stream);
   if (r) return r;
   if (P.groups.size() != 1)
     return fail(CMTJ_ERR_INVALID,
                 "the junctions of a stack must share one structure (layers, modes, driver kinds)");
-  // equal step counts keep the plan's order the caller's: member j on lane j
   const size_t o_mag = align_up(need * sizeof(double));
-  const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * (size_t)n);
-  const size_t o_stk = align_up(o_st + sizeof(cmtj_status) * (size_t)n);
-  const size_t out_bytes = o_stk + sizeof(cmtj_stack);
+  const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj);
+  const size_t o_stk = align_up(o_st + sizeof(cmtj_status) * nj);
+  const size_t out_bytes = o_stk + sizeof(cmtj_stack) * n_stacks;
   char* out = (char*)grow(g_plain.out, g_plain.out_bytes, out_bytes);
   if (!out) return CMTJ_ERR_HIP;
-  HIP_TRY(hipMemcpyAsync(out + o_stk, s, sizeof(cmtj_stack), hipMemcpyHostToDevice, g_plain.stream));
+  HIP_TRY(hipMemcpyAsync(out + o_stk, stacks, sizeof(cmtj_stack) * n_stacks, hipMemcpyHostToDevice,
+                         g_plain.stream));
   Group& G = P.groups[0];
   G.k = ke;
-  G.blocks = 1;
+  G.blocks = (uint32_t)((n_stacks * pad + 63) / 64);
   G.a.stack = (const cmtj_stack*)(out + o_stk);
+  G.a.stack_n = (uint32_t)n_stacks;
+  G.a.stack_pad = pad;
   r = launch_plan(&P, need ? out : nullptr, (double*)(out + o_mag), (cmtj_status*)(out + o_st), g_plain.stream);
   if (r) return r;
   if (need) HIP_TRY(hipMemcpyAsync(log_host, out, need * sizeof(double), hipMemcpyDeviceToHost, g_plain.stream));
-  HIP_TRY(hipMemcpyAsync(mag_out_host, out + o_mag, sizeof(double) * 3 * CMTJ_MAX_LAYERS * (size_t)n,
+  HIP_TRY(hipMemcpyAsync(mag_out_host, out + o_mag, sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj,
                          hipMemcpyDeviceToHost, g_plain.stream));
-  HIP_TRY(hipMemcpyAsync(st_host, out + o_st, sizeof(cmtj_status) * (size_t)n, hipMemcpyDeviceToHost,
-                         g_plain.stream));
+  HIP_TRY(hipMemcpyAsync(st_host, out + o_st, sizeof(cmtj_status) * nj, hipMemcpyDeviceToHost, g_plain.stream));
   HIP_TRY(hipStreamSynchronize(g_plain.stream));
   return 0;
 }

@@ -41,7 +41,9 @@ struct GroupArgs {
   double* mag_out;  // [n][CMTJ_MAX_LAYERS][3] in junction order
   cmtj_status* status;
   cmtj_state* state;  // resumable state [n] in junction order, or null (run from the start)
-  const cmtj_stack* stack;  // coupled stack (k_junction<..., STK>): junction j on lane j
+  const cmtj_stack* stack;  // coupled stacks (k_junction<..., STK>): [stack_n]
+  uint32_t stack_n;         // stacks of the launch; member j of stack s is instance s * members + j
+  uint32_t stack_pad;       // lanes per stack: a power of two >= members, <= 64 (a stack never straddles a wave)
   uint32_t slice;     // steps per launch when state is set
   uint32_t n;       // lanes of this group
   uint32_t flags;   // feature bits of the group
@@ -835,7 +837,7 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
 // wave.  I[s]: the current at stage time s (RK4: t, t + dt/2, t + dt);
 // Ilog, Clog: the stack log's I_j and C_j_{j+1} of this step.
 template <int LM>
-__device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n, int ti, int bi,
+__device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n, int base, int ti, int bi,
                                               const V3<double> (&m)[LM], double t, double dt, double (&I)[3],
                                               double& Ilog, double& Clog) {
   V3<double> mt = m[0], mb = m[0];
@@ -846,13 +848,14 @@ __device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n
   }
   const V3<double> p = sk->two_layer ? mb : mk<double>(sk->pol[0], sk->pol[1], sk->pol[2]);
   const double d = mt.x * p.x + mt.y * p.y + mt.z * p.z;  // c_dot, core/junction.hpp:82-84
-  const double dn = __shfl(d, (j + 1 < n) ? j + 1 : j, 64);
+  // base: the wave lane of member 0 of this stack
+  const double dn = __shfl(d, base + ((j + 1 < n) ? j + 1 : j), 64);
   const double Cj = (j + 1 < n) ? sk->coupling[j] * (sk->series ? d + dn : d - dn) : 0.0;
   if (sk->kcl) {
     // totalCurrent = I0 + sum_j C_j I0, the same constant for every member
     const double I0 = driver_value(sk->current, t);
     double total = I0;
-    for (int q = 0; q + 1 < n; ++q) total += __shfl(Cj, q, 64) * I0;
+    for (int q = 0; q + 1 < n; ++q) total += __shfl(Cj, base + q, 64) * I0;
     I[0] = I[1] = I[2] = total;
     Ilog = total;
     Clog = Cj;
@@ -861,7 +864,7 @@ __device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n
     // is current * effectiveCoupling with its phase shifted
     double eff = 1.0;
     for (int q = 0; q + 1 < n; ++q) {
-      const double cq = __shfl(Cj, q, 64);
+      const double cq = __shfl(Cj, base + q, 64);
       if (q < j) eff *= (1 + cq);
     }
     Clog = eff * (1 + Cj);
@@ -884,7 +887,20 @@ __device__ __forceinline__ void set_current(DV<R> (&dv)[LM], double I) {
 
 template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false>
 __device__ __forceinline__ void junction_body(const GroupArgs& g) {
-  const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  [[maybe_unused]] const cmtj_stack* sk = nullptr;
+  [[maybe_unused]] int s_j = 0, s_base = 0;
+  if constexpr (STK) {
+    // stack s on lanes [s pad, s pad + members) of one wave; the members of
+    // all stacks are the group's instances in the caller's order
+    const uint32_t pad = g.stack_pad, sid = lane / pad;
+    if (sid >= g.stack_n) return;
+    sk = g.stack + sid;
+    s_j = (int)(lane % pad);
+    if (s_j >= sk->n_junctions) return;
+    s_base = (int)(threadIdx.x & ~(pad - 1));
+    lane = sid * (uint32_t)sk->n_junctions + (uint32_t)s_j;
+  }
   if (lane >= g.n) return;
   const uint32_t fl = g.flags;
   const int nl = g.nl;
@@ -962,15 +978,15 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   uint64_t next_log = (uint64_t)k * W;
   [[maybe_unused]] int s_n = 0, s_ti = 0, s_bi = 0;
   if constexpr (STK) {
-    s_n = g.stack->n_junctions;
-    s_ti = g.stack->top[lane];
-    s_bi = g.stack->bottom[lane];
+    s_n = sk->n_junctions;
+    s_ti = sk->top[s_j];
+    s_bi = sk->bottom[s_j];
   }
   for (; i < iend; ++i) {
     const double t = (double)i * dt;
     [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
     [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
-    if constexpr (STK) stack_current<LM>(g.stack, (int)lane, s_n, s_ti, s_bi, m, t, dt, sI, sIlog, sClog);
+    if constexpr (STK) stack_current<LM>(sk, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, sI, sIlog, sClog);
     if constexpr (ANCHORED) {
       // RK4 of the reference (core/junction.hpp:1017-1041, neighbours frozen
       // at the start of the step) in anchored binary32: k1 and the anchor
@@ -1103,9 +1119,13 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
         }
       }
       if constexpr (STK) {
-        // a member's NaN ends the stack's run (the reference throws): every
-        // member stops at this step and the lanes leave the loop together
-        if (__any(nan_layer >= 0)) {
+        // a member's NaN ends its stack's run (the reference throws): every
+        // member of that stack stops at this step, the lanes of the stack
+        // leave the loop together, other stacks go on
+        const unsigned long long bad = __ballot(nan_layer >= 0);
+        const unsigned long long mine =
+            ((g.stack_pad >= 64) ? ~0ull : ((1ull << g.stack_pad) - 1)) << (unsigned)s_base;
+        if (bad & mine) {
           if (nan_layer >= 0) code = CMTJ_ERR_NAN;
           break;
         }

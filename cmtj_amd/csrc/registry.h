@@ -43,7 +43,7 @@ KTable kernels_ll_f32();
 KTable kernels_ll_spec();
 KTable kernels_exact();
 KTable kernels_cl();
-KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stack only
+KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stacks only
 
 }  // namespace cmtj
 

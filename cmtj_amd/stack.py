@@ -2,7 +2,7 @@
 ``ParallelStack`` with the reference's Python API (python/cmtj.cpp:361-426,
 core/stack.hpp).  A stack runs on the GPU as one wave, member j on lane j,
 the members' current recomputed every step from their magnetisations
-(``cmtj_run_stack``); logs, errors and defaults follow the reference.
+(``cmtj_run_stacks``); logs, errors and defaults follow the reference.
 """
 from __future__ import annotations
 
@@ -141,11 +141,11 @@ class _Stack:
 
     # --- run -------------------------------------------------------------------
     def _execute(self, stack, js, layers, consts):
-        return _lib.run_stack(stack, js, layers, consts)
+        return _lib.run_stacks(stack, js, layers, consts)
 
-    def runSimulation(self, totalTime: float, timeStep: float = 1e-13, writeFrequency: float = 1e-11) -> None:
-        """Stack<T>::runSimulation (:296-300): KCL or non-KCL coupling, every
-        member stepping with RK4 (Euler-Heun when stochastic)."""
+    def _pack(self, totalTime: float, timeStep: float, writeFrequency: float):
+        """Descriptors of one run: (stack record, member records with layer
+        and log offsets from 0, layer rows, channels per member, steps, W)."""
         if not self.useKCL and not self.delayed:
             raise NotImplementedError("non-delayed coupling (each member seeing the previous member's updated "
                                       "state within a step) is not provided by the GPU engine")
@@ -164,30 +164,28 @@ class _Stack:
         two = self.isTwoLayerMemberStack()
         n_steps, W = packed[0][3], packed[0][4]
         n = self.stackSize
+        if n > _lib.MAX_STACK:
+            raise NotImplementedError(f"stacks of more than {_lib.MAX_STACK} junctions")
         st = np.zeros((), dtype=_lib.STACK_DTYPE)
         st["series"] = 1 if self._series else 0
         st["kcl"] = 1 if self.useKCL else 0
         st["n_junctions"] = n
         st["two_layer"] = 1 if two else 0
         st["phase_offset"] = self.phaseOffset
-        if n > _lib.MAX_STACK:
-            raise NotImplementedError(f"stacks of more than {_lib.MAX_STACK} junctions")
         if not two and n_steps > 0:
             st["pol"] = self.getPolarisationVector().tolist()
         st["current"] = self.currentDriver._record()
         for q in range(n - 1):
             st["coupling"][q] = self.getCoupling(q)
-        ids = [j.getLayerIds() for j in self.junctionList]
         for q, j in enumerate(self.junctionList):
             if n_steps > 0:
-                st["top"][q] = ids[q].index(j.getLayer(self.topId).id)
+                ids = j.getLayerIds()
+                st["top"][q] = ids.index(j.getLayer(self.topId).id)
                 if two:
-                    st["bottom"][q] = ids[q].index(j.getLayer(self.bottomId).id)
+                    st["bottom"][q] = ids.index(j.getLayer(self.bottomId).id)
         js = np.zeros(n, dtype=_lib.JUNCTION_DTYPE)
-        layers = []
-        rows = 0
-        off = 0
-        nchs = []
+        layers, nchs = [], []
+        rows = off = 0
         for q, (jd, L, _t, _ns, _w) in enumerate(packed):
             js[q] = jd
             js[q]["layer_offset"] = rows
@@ -197,14 +195,15 @@ class _Stack:
             nch = _lib.n_channels(int(jd["n_layers"]), int(jd["mr_mode"]), int(jd["log_mask"]))
             nchs.append(nch)
             off += (nch + 2) * int(jd["n_entries"])
-        layers = np.concatenate(layers)
-        if n_steps == 0:
-            return
-        log, mag, status = self._execute(st, js, layers, _lib.constants_get())
+        return st, js, np.concatenate(layers), nchs, n_steps, W, off
+
+    def _finish(self, js, log, mag, status, nchs, W: int, timeStep: float) -> None:
+        """Members take over their logs and final state; the stack log."""
         ne_done = int(status["entries_done"].min())
         for q, j in enumerate(self.junctionList):
             ne = int(js[q]["n_entries"])
-            blk = log[int(js[q]["log_offset"]):int(js[q]["log_offset"]) + (nchs[q] + 2) * ne].reshape(nchs[q] + 2, ne)
+            o = int(js[q]["log_offset"])
+            blk = log[o:o + (nchs[q] + 2) * ne].reshape(nchs[q] + 2, ne)
             stq = status[q].copy()
             stq["entries_done"] = ne_done
             j._runs += 1
@@ -214,6 +213,15 @@ class _Stack:
             raise RuntimeError("NAN magnetisation")
         if (status["code"] != _lib.OK).any():
             raise RuntimeError(f"engine status {int(status['code'].max())}")
+
+    def runSimulation(self, totalTime: float, timeStep: float = 1e-13, writeFrequency: float = 1e-11) -> None:
+        """Stack<T>::runSimulation (:296-300): KCL or non-KCL coupling, every
+        member stepping with RK4 (Euler-Heun when stochastic)."""
+        st, js, layers, nchs, n_steps, W, _ = self._pack(totalTime, timeStep, writeFrequency)
+        if n_steps == 0:
+            return
+        log, mag, status = self._execute(st, js, layers, _lib.constants_get())
+        self._finish(js, log, mag, status, nchs, W, timeStep)
 
     def _stack_resistance(self, R: list) -> np.ndarray:
         raise NotImplementedError
@@ -271,4 +279,43 @@ class ParallelStack(_Stack):
         return 1.0 / inv
 
 
-__all__ = ["SeriesStack", "ParallelStack"]
+def run_stacks(stacks, totalTime: float, timeStep: float = 1e-13, writeFrequency: float = 1e-11) -> None:
+    """Runs many stacks in one launch (no reference counterpart: a parameter
+    sweep over stacks, each stack as by its own runSimulation).  Every stack
+    has the same member count and every member the same structure; logs and
+    final states go to each stack as runSimulation leaves them.  Raises
+    RuntimeError("NAN magnetisation") after the run if any stack hit a NaN."""
+    stacks = list(stacks)
+    if not stacks:
+        return
+    parts = [s._pack(totalTime, timeStep, writeFrequency) for s in stacks]
+    if len({p[0]["n_junctions"].item() for p in parts}) != 1:
+        raise ValueError("every stack of a batch has the same member count")
+    if parts[0][4] == 0:
+        return
+    recs = np.stack([p[0] for p in parts])
+    js, layers = [], []
+    rows = off = 0
+    for st, jq, L, _n, _s, _w, size in parts:
+        jq = jq.copy()
+        jq["layer_offset"] += rows
+        jq["log_offset"] += off
+        js.append(jq)
+        layers.append(L)
+        rows += len(L)
+        off += size
+    js = np.concatenate(js)
+    log, mag, status = stacks[0]._execute(recs, js, np.concatenate(layers), _lib.constants_get())
+    failed = False
+    for k, (s, p) in enumerate(zip(stacks, parts)):
+        n = s.stackSize
+        sl = slice(k * n, (k + 1) * n)
+        try:
+            s._finish(js[sl], log, mag[sl], status[sl], p[3], p[5], timeStep)
+        except RuntimeError:
+            failed = True
+    if failed:
+        raise RuntimeError("NAN magnetisation")
+
+
+__all__ = ["SeriesStack", "ParallelStack", "run_stacks"]

@@ -284,18 +284,23 @@ typedef struct cmtj_stack {
   int32_t bottom[CMTJ_MAX_STACK];  /* layer index of bottomId (two-layer members) */
 } cmtj_stack;
 
-/* Runs a stack on the current GPU, one wave: junction j on lane j.  js[j]
- * are the members (same structure, solver RK4 or Euler-Heun, n_steps,
- * write_every, dt); js[j].log_offset places member j's block in log_host:
- * the channels of cmtj_run's log followed by two stack channels, the current
- * of junction j at the logged step and, for j < n - 1, the logged coupling
- * of pair (j, j+1) (C_j under KCL, prod_{k<=j} (1 + C_k) otherwise: the
- * stack log's "I_j" and "C_j_{j+1}", :206-217).  mag_out_host: [n][8][3],
- * st_host: [n].  A NaN in any member stops every member at that step.
- * Replaces Stack<T>::runSimulation (:296-300). */
-int cmtj_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_layer* layers,
-                   size_t n_layer_rows, const cmtj_constants* c, double* log_host, size_t log_cap,
-                   double* mag_out_host, cmtj_status* st_host);
+/* Runs n_stacks stacks of the same member count on the current GPU in one
+ * launch: stack s on lanes [s P, s P + members) of one wave (P the member
+ * count rounded up to a power of two), its members exchanging the coupling
+ * terms by lane shuffles every step.  js[s * members + j] is member j of
+ * stack s; every member shares one structure, the solver (RK4 or Euler-Heun),
+ * n_steps, write_every and dt.  js[q].log_offset places member q's block in
+ * log_host: the channels of cmtj_run's log followed by two stack channels,
+ * the member's current at the logged step and, for j < members - 1, the
+ * logged coupling of pair (j, j+1) (C_j under KCL, prod_{k<=j} (1 + C_k)
+ * otherwise: the stack log's "I_j" and "C_j_{j+1}", :206-217).
+ * mag_out_host: [n_stacks * members][8][3], st_host: [n_stacks * members].
+ * A NaN in a member stops every member of its stack at that step.  One stack
+ * replaces Stack<T>::runSimulation (:296-300); a batch replaces a loop of
+ * them over a parameter sweep. */
+int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_junction* js,
+                    const cmtj_layer* layers, size_t n_layer_rows, const cmtj_constants* c,
+                    double* log_host, size_t log_cap, double* mag_out_host, cmtj_status* st_host);
 
 /* ---- batched plans: many uneven instances, device buffers -------------- */
 

@@ -91,11 +91,12 @@ def run_batch(js, layers, consts, tables=None, log_elements=0, n_threads=None):
     return log, mag, st
 
 
-def run_stack(stack, js, layers, consts):
-    """A coupled stack, same layout as cmtj_run_stack: returns (log, mag
-    [n, 8, 3], status [n])."""
+def run_stack(stacks, js, layers, consts):
+    """Coupled stacks, same layout as cmtj_run_stacks (js[s * members + j]:
+    member j of stack s), one after another: returns (log, mag [n, 8, 3],
+    status [n])."""
     lib, abi = _load()
-    stack = np.ascontiguousarray(stack, dtype=abi.STACK_DTYPE).reshape(())
+    stacks = np.ascontiguousarray(stacks, dtype=abi.STACK_DTYPE).reshape(-1)
     js = np.ascontiguousarray(js, dtype=abi.JUNCTION_DTYPE)
     layers = np.ascontiguousarray(layers, dtype=abi.LAYER_DTYPE)
     need = 0
@@ -105,10 +106,14 @@ def run_stack(stack, js, layers, consts):
     log = np.zeros(max(need, 1), dtype=np.float64)
     mag = np.zeros((js.size, abi.MAX_LAYERS, 3), dtype=np.float64)
     st = np.zeros(js.size, dtype=abi.STATUS_DTYPE)
-    rc = lib.oracle_run_stack(_p(stack), _p(js), _p(layers), _p(np.ascontiguousarray(consts)), _p(log), _p(mag),
-                              _p(st))
-    if rc:
-        raise ValueError("oracle_run_stack: bad stack descriptor")
+    consts = np.ascontiguousarray(consts)
+    n = int(stacks[0]["n_junctions"]) if stacks.size else 0
+    for s in range(stacks.size):
+        sub = js[s * n:(s + 1) * n]
+        rc = lib.oracle_run_stack(_p(stacks[s:s + 1]), _p(sub), _p(layers), _p(consts), _p(log),
+                                  _p(mag[s * n:(s + 1) * n]), _p(st[s * n:(s + 1) * n]))
+        if rc:
+            raise ValueError("oracle_run_stack: bad stack descriptor")
     return log[:need], mag, st
 
 

@@ -40,7 +40,7 @@ int oracle_run_batch(const cmtj_junction* js, size_t n,
                      double* mag_out, cmtj_status* st, int n_threads);
 
 /* A coupled stack (core/stack.hpp), same descriptors and log layout as
- * cmtj_run_stack; mag_out [n][CMTJ_MAX_LAYERS][3], st [n].  Returns 0. */
+ * cmtj_run_stacks for one stack; mag_out [n][CMTJ_MAX_LAYERS][3], st [n].  Returns 0. */
 int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js,
                      const cmtj_layer* layers, const cmtj_constants* c,
                      double* log, double* mag_out, cmtj_status* st);

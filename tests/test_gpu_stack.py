@@ -1,4 +1,4 @@
-"""Coupled stacks on the GPU (cmtj_run_stack, one wave) against the oracle's
+"""Coupled stacks on the GPU (cmtj_run_stacks) against the oracle's
 stack runner: series / parallel, KCL / non-KCL, one- and two-layer members,
 RK4 and the thermal Euler-Heun path, and a NaN that stops the stack."""
 from __future__ import annotations
@@ -88,3 +88,41 @@ def test_nan_member_stops_the_stack():
     assert len(g.getLog()["time"]) == len(o.getLog()["time"])
     for q in range(3):
         assert len(g.getLog(q)["time"]) == len(g.getLog()["time"])
+
+
+def test_batched_stacks_match_oracle():
+    """Many 3-member stacks (4 lanes each, 16 per wave) in one launch: every
+    stack as the oracle runs it alone; a NaN in one stack stops that stack
+    only."""
+    from cmtj_amd.stack import run_stacks
+
+    def build(k, cls_s, cls_p, bad=False):
+        js = _members(_single, n=3)
+        for q, j in enumerate(js):
+            amp = float("inf") if (bad and q == 1) else 1e3 * (k % 5)
+            j.setLayerExternalFieldDriver("all", AxialDriver(stepDriver(0, amp, 5e-12, 1e-9), constantDriver(0),
+                                                             constantDriver(1e5)))
+        s = (cls_s if k % 2 else cls_p)(js, useKCL=bool(k % 3), phaseOffset=0.1 * k)
+        s.setCouplingStrength([0.1 + 0.01 * k, -0.2])
+        s.setCoupledCurrentDriver(ScalarDriver.getSineDriver(5e11, 2e11 * (k % 4), 2e9, 0.0))
+        return s
+
+    K = 40
+    g = [build(k, SeriesStack, ParallelStack, bad=(k == 7)) for k in range(K)]
+    o = [build(k, OracleSeriesStack, OracleParallelStack, bad=(k == 7)) for k in range(K)]
+    for a, b in zip(g, o):
+        for x, y in zip(a.junctionList, b.junctionList):
+            y._seed = x._seed
+    with pytest.raises(RuntimeError, match="NAN magnetisation"):
+        run_stacks(g, 0.3e-9, 1e-13, 1e-12)
+    worst = 0.0
+    for k, b in enumerate(o):
+        if k == 7:
+            with pytest.raises(RuntimeError, match="NAN magnetisation"):
+                b.runSimulation(0.3e-9, 1e-13, 1e-12)
+        else:
+            b.runSimulation(0.3e-9, 1e-13, 1e-12)
+        worst = max(worst, _compare(g[k], b))
+    print(f"[stack] {K} stacks in one launch: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+    assert 0 < len(g[7].getLog()["time"]) < len(g[6].getLog()["time"])

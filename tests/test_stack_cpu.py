@@ -233,3 +233,33 @@ def test_current_moves_the_stt_members():
         s.runSimulation(2e-10, 1e-13, 1e-12)
         return np.asarray(s.getLog(0)["free_mx"])
     assert np.max(np.abs(run(5e12) - run(0.0))) > 1e-4
+
+
+def test_batched_stacks_equal_separate_runs_oracle():
+    """run_stacks over several stacks gives each stack what its own
+    runSimulation gives (the oracle backend runs them one by one)."""
+    from cmtj_amd.stack import run_stacks
+
+    def build(k):
+        s = (OracleSeriesStack if k % 2 else OracleParallelStack)(
+            [_single(stt=True, mag=(0.1 * k, 0.2, 1.0)), _single(stt=True), _single(stt=True, mag=(0.5, 0, 0.8))],
+            useKCL=bool(k % 3))
+        s.setCouplingStrength(0.1 * (k + 1))
+        s.setCoupledCurrentDriver(ScalarDriver.getConstantDriver(1e12 * (k + 1)))
+        return s
+
+    batch = [build(k) for k in range(4)]
+    alone = [build(k) for k in range(4)]
+    for k in range(4):  # same noise seeds for the comparison
+        for a, b in zip(batch[k].junctionList, alone[k].junctionList):
+            b._seed = a._seed
+    run_stacks(batch, 1e-10, 1e-13, 1e-12)
+    for s in alone:
+        s.runSimulation(1e-10, 1e-13, 1e-12)
+    for a, b in zip(batch, alone):
+        assert a.getLog().keys() == b.getLog().keys()
+        for k in a.getLog():
+            np.testing.assert_array_equal(np.asarray(a.getLog()[k]), np.asarray(b.getLog()[k]))
+        for q in range(3):
+            for k in a.getLog(q):
+                np.testing.assert_array_equal(np.asarray(a.getLog(q)[k]), np.asarray(b.getLog(q)[k]))
