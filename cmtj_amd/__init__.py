@@ -13,6 +13,7 @@ import sys as _sys
 from . import _lib  # noqa: F401  (loads libcmtj_amd.so; raises if it is missing)
 from . import constants
 from .core import (  # noqa: F401
+    AdaptiveIntegrationParams,
     AxialDriver,
     Axis,
     CVector,
@@ -57,7 +58,7 @@ def install_as_cmtj() -> None:
 
 
 __all__ = [
-    "AxialDriver", "Axis", "CVector", "Junction", "Layer", "NullDriver", "Reference", "ScalarDriver",
+    "AdaptiveIntegrationParams", "AxialDriver", "Axis", "CVector", "Junction", "Layer", "NullDriver", "Reference", "ScalarDriver",
     "SolverMode", "UpdateType", "c_dot", "constantDriver", "gaussianImpulseDriver", "gaussianStepDriver",
     "posSineDriver", "pulseDriver", "sineDriver", "stepDriver", "trapezoidDriver", "constants",
     "xaxis", "yaxis", "zaxis", "all", "none", "fixed", "top", "bottom", "RK4", "Heun", "EulerHeun",

@@ -14,7 +14,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libcmtj_amd.so"
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_LAYERS = 8
 MAX_STACK = 64
 NSLOT = 30
@@ -38,13 +38,13 @@ EULER_HEUN, RK4, DORMAND_PRINCE, HEUN = range(4)
 MR_NONE, MR_CLASSIC, MR_STRIP = range(3)
 
 # enum cmtj_log_bits
-LOG_MX, LOG_MYZ, LOG_MR, LOG_ENERGY = 1, 2, 4, 8
+LOG_MX, LOG_MYZ, LOG_MR, LOG_ENERGY, LOG_TIME = 1, 2, 4, 8, 16
 LOG_FULL = LOG_MX | LOG_MYZ | LOG_MR
 
 FP64, FP32 = 0, 1
 LOG_F64, LOG_F32 = 0, 1
 
-OK, ERR_INVALID, ERR_HIP, ERR_NAN = 0, 1, 2, 3
+OK, ERR_INVALID, ERR_HIP, ERR_NAN, ERR_LIMIT = 0, 1, 2, 3, 4
 
 DRIVER_DTYPE = np.dtype(
     [
@@ -90,6 +90,11 @@ LAYER_DTYPE = np.dtype(
         ("alternative_stt", "<i4"),
         ("temperature_set", "<i4"),
         ("drv", DRIVER_DTYPE, (NSLOT,)),
+        ("dp_abs_tol", "<f8"),
+        ("dp_rel_tol", "<f8"),
+        ("dp_max_factor", "<f8"),
+        ("dp_min_factor", "<f8"),
+        ("dp_safety", "<f8"),
     ],
     align=True,
 )
@@ -117,6 +122,8 @@ JUNCTION_DTYPE = np.dtype(
         ("SMR_X", "<f8", (MAX_LAYERS,)),
         ("SMR_Y", "<f8", (MAX_LAYERS,)),
         ("AHE", "<f8", (MAX_LAYERS,)),
+        ("total_time", "<f8"),
+        ("write_freq", "<f8"),
     ],
     align=True,
 )
@@ -312,6 +319,8 @@ def n_channels(n_layers: int, mr_mode: int, log_mask: int) -> int:
         n += 1 if mr_mode == MR_CLASSIC else (2 if mr_mode == MR_STRIP else 0)
     if log_mask & LOG_ENERGY:
         n += ENERGY_CHANNELS * n_layers
+    if log_mask & LOG_TIME:
+        n += 1
     return n
 
 

@@ -52,6 +52,8 @@ def _channel_names(meta: _Meta, mask: int) -> list:
             names.append(meta.rtag)
         elif meta.mr_mode == _lib.MR_STRIP:
             names += ["Rx", "Ry"]
+    if mask & _lib.LOG_TIME:  # Dormand-Prince: the adaptive entry times
+        names.append("time")
     return names
 
 
@@ -79,10 +81,14 @@ class BatchResult:
         ne = int(j["n_entries"])
         off = int(j["log_offset"])
         block = self.log[off: off + len(names) * ne].double().cpu().numpy().reshape(len(names), ne)
+        dp = int(j["log_mask"]) & _lib.LOG_TIME
+        if dp:  # adaptive runs fill their log capacity only in part
+            block = block[:, : int(self.status()[i]["entries_done"])]
         out = {k: block[c] for c, k in enumerate(names)}
         if meta.mr_mode == _lib.MR_STRIP and int(j["log_mask"]) & _lib.LOG_MR:
-            out["Rz"] = np.zeros(ne)
-        out["time"] = (np.arange(ne, dtype=np.float64) * int(j["write_every"])) * float(j["dt"])
+            out["Rz"] = np.zeros(block.shape[1])
+        if not dp:
+            out["time"] = (np.arange(ne, dtype=np.float64) * int(j["write_every"])) * float(j["dt"])
         return out
 
     def channel(self, name: str):
@@ -242,6 +248,8 @@ class JunctionBatch:
     def set_times(self, totalTime, timeStep=None, writeFrequency=None) -> None:
         """Per-instance durations / steps with the reference's truncations
         (core/junction.hpp:1956-1959)."""
+        if np.any(self.js["solver"] == _lib.DORMAND_PRINCE):
+            raise NotImplementedError("Dormand-Prince batches take their times from from_junctions")
         n = len(self.js)
         dt = np.broadcast_to(np.asarray(self.js["dt"] if timeStep is None else timeStep, dtype=np.float64), (n,))
         T = np.broadcast_to(np.asarray(totalTime, dtype=np.float64), (n,))
@@ -265,14 +273,17 @@ class JunctionBatch:
         mask = _LOG_MODES[log] if isinstance(log, str) else int(log)
         if mask & _lib.LOG_ENERGY:
             raise ValueError("energy channels are a drop-in (Junction.runSimulation) feature")
-        self.js["log_mask"] = mask
+        # Dormand-Prince instances log their adaptive entry times as well
+        dp = self.js["solver"] == _lib.DORMAND_PRINCE
+        self.js["log_mask"] = np.where(dp & (mask != 0), mask | _lib.LOG_TIME, mask)
         self._layout()
         self._plan = None
 
     def _layout(self) -> None:
         nch = np.array([_lib.n_channels(int(self.js[i]["n_layers"]), int(self.js[i]["mr_mode"]),
                                         int(self.js[i]["log_mask"]))
-                        for i in range(len(self.js))], dtype=np.uint64) if len(self.metas) > 1 else \
+                        for i in range(len(self.js))], dtype=np.uint64) \
+            if len(self.metas) > 1 or len(np.unique(self.js["log_mask"])) > 1 else \
             np.full(len(self.js), _lib.n_channels(int(self.js[0]["n_layers"]), int(self.js[0]["mr_mode"]),
                                                   int(self.js[0]["log_mask"])), dtype=np.uint64)
         sizes = nch * self.js["n_entries"].astype(np.uint64)

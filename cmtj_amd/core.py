@@ -500,6 +500,31 @@ def _fill_driver(rec: np.ndarray, d: ScalarDriver, table_index: int = -1) -> Non
 
 
 # ---------------------------------------------------------------------------
+# AdaptiveIntegrationParams (core/junction.hpp:125-139, python/cmtj.cpp:151-163)
+
+class AdaptiveIntegrationParams:
+    """Step control of the Dormand-Prince solver; defaults of the reference."""
+
+    def __init__(self) -> None:
+        self.abs_tol = 1e-6
+        self.rel_tol = 1e-3
+        self.max_factor = 5.0
+        self.min_factor = 0.1
+        self.safety_factor = 0.9
+        self.use_pid_control = False
+        self.ki = 0.0
+        self.kp = 0.2
+        self.kd = 0.0
+        self.prev_error_ratio = 1.0
+        self.integral_error = 0.0
+
+    def __copy__(self) -> "AdaptiveIntegrationParams":
+        p = AdaptiveIntegrationParams()
+        p.__dict__.update(self.__dict__)
+        return p
+
+
+# ---------------------------------------------------------------------------
 # Layer (core/junction.hpp:145-1137, python/cmtj.cpp:226-289)
 
 _SCALAR_SLOTS = {
@@ -591,7 +616,7 @@ class Layer:
         self.scaleNoise = 0.0
         self.noiseAxis = Axis.all
         self.oneFNoise = None
-        self.adaptiveParams = None
+        self.adaptiveParams = AdaptiveIntegrationParams()
         self._scalar = {k: ScalarDriver() for k in _SCALAR_SLOTS}
         self._axial = {k: AxialDriver() for k in _AXIAL_SLOTS}
 
@@ -774,7 +799,8 @@ class Layer:
         return CVector(0, 0, 0)
 
     def setAdaptiveParams(self, params) -> None:
-        self.adaptiveParams = params
+        # the layer keeps its own copy (core/junction.hpp:355-357)
+        self.adaptiveParams = copy.copy(params)
 
     def __copy__(self) -> "Layer":
         L = Layer.__new__(Layer)
@@ -806,6 +832,12 @@ class Layer:
         rec["polarisation"] = self.spinPolarisation
         rec["fl_torque"] = self.fieldLikeTorque
         rec["dl_torque"] = self.dampingLikeTorque
+        ap = self.adaptiveParams
+        rec["dp_abs_tol"] = ap.abs_tol
+        rec["dp_rel_tol"] = ap.rel_tol
+        rec["dp_max_factor"] = ap.max_factor
+        rec["dp_min_factor"] = ap.min_factor
+        rec["dp_safety"] = ap.safety_factor
         rec["ref_type"] = int(self.referenceType)
         rec["include_stt"] = int(self.includeSTT)
         rec["include_sot"] = int(self.includeSOT)
@@ -1127,7 +1159,13 @@ class Junction:
             if l.scaleNoise != 0 or l.oneFNoise is not None:
                 raise NotImplementedError("coloured (1/f, alpha) noise is not provided by the GPU engine")
         if local == SolverMode.DormandPrince:
-            raise NotImplementedError("DormandPrince adaptive stepping is not provided by the GPU engine")
+            if self.layerNo != 1:
+                # the reference's multi-layer Dormand-Prince sets the step to
+                # -1 after any rejected step (layers' hopt, junction.hpp:1699-1704)
+                # and then runs its clock backwards
+                raise NotImplementedError("DormandPrince runs single-layer junctions on the GPU engine")
+            if self.layers[0].adaptiveParams.use_pid_control:
+                raise NotImplementedError("the PID step controller of DormandPrince is not provided")
         return int(local)
 
     def _descriptor(self, totalTime: float, timeStep: float, writeFrequency: float, calculateEnergies: bool,
@@ -1144,8 +1182,14 @@ class Junction:
                 raise RuntimeError("Failed to divide vector by zero!")
             if l.temperatureSet and l.cellVolume == 0.0:
                 raise RuntimeError("Cell surface cannot be 0 during temp. calculations!")
+        dp = solver == int(SolverMode.DormandPrince)
         if log_mask is None:
             log_mask = _lib.LOG_FULL | (_lib.LOG_ENERGY if calculateEnergies else 0)
+        if dp:
+            if calculateEnergies:
+                raise NotImplementedError("energy channels with DormandPrince are not provided")
+            if log_mask:
+                log_mask |= _lib.LOG_TIME
         tables = _Tables()
         L = np.zeros(self.layerNo, dtype=_lib.LAYER_DTYPE)
         for i, l in enumerate(self.layers):
@@ -1158,6 +1202,18 @@ class Junction:
         j["n_steps"] = n_steps
         j["write_every"] = W
         j["n_entries"] = (n_steps + W - 1) // W
+        if dp:
+            if tables.array() is not None and len(tables.array()):
+                raise NotImplementedError("callback drivers need fixed stage times (not DormandPrince)")
+            # the adaptive loop of runSimulation (:1995-2044): at most one entry
+            # per write interval before totalTime and one after; attempts capped
+            # (at an exact equilibrium the error ratio is 0, the controller's
+            # factor saturates at min_factor and the step shrinks to zero
+            # before resetting: about 320 attempts per initial step of time)
+            j["n_steps"] = min(2**32 - 1, max(1_000_000, 400 * max(1, int(totalTime / timeStep))))
+            j["n_entries"] = int(totalTime / writeFrequency) + 4
+            j["total_time"] = totalTime
+            j["write_freq"] = writeFrequency
         j["seed"] = self._seed
         j["stream"] = self._runs
         j["dt"] = timeStep
@@ -1209,8 +1265,12 @@ class Junction:
                     for base in ("Hfl", "Hdl"):
                         for c in ("x", "y", "z"):
                             self._append(f"{l.id}_{base}{c}", vals[base + c])
-        # entry k is labelled with the pre-step time (k W) dt (core/junction.hpp:1974)
-        self._append("time", (np.arange(ne, dtype=np.float64) * W) * timeStep)
+        if mask & _lib.LOG_TIME:  # Dormand-Prince: adaptive entry times
+            self._append("time", log[ch, :ne])
+            ch += 1
+        else:
+            # entry k is labelled with the pre-step time (k W) dt (core/junction.hpp:1974)
+            self._append("time", (np.arange(ne, dtype=np.float64) * W) * timeStep)
         self.logLength += ne
         for i, l in enumerate(self.layers):
             l.mag = CVector(*mag[i])
@@ -1239,6 +1299,8 @@ class Junction:
         self._apply(j, log, mag, st, timeStep, W)
         if int(st["code"]) == _lib.ERR_NAN:
             raise RuntimeError("NAN magnetisation")
+        if int(st["code"]) == _lib.ERR_LIMIT:
+            raise RuntimeError("DormandPrince: the step-attempt or log-entry limit of the run was reached")
         if int(st["code"]) != _lib.OK:
             raise RuntimeError(f"engine status {int(st['code'])}")
         if verbose:
@@ -1249,6 +1311,7 @@ class Junction:
 
 __all__ = [
     "Axis", "Reference", "SolverMode", "UpdateType", "CVector", "c_dot", "ScalarDriver", "NullDriver",
+    "AdaptiveIntegrationParams",
     "AxialDriver", "Layer", "Junction", "constantDriver", "pulseDriver", "sineDriver", "posSineDriver",
     "stepDriver", "trapezoidDriver", "gaussianImpulseDriver", "gaussianStepDriver",
 ]

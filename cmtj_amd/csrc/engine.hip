@@ -76,7 +76,8 @@ static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, u
                                  int lanes, uint64_t n_inst) {
   static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),
                                    kernels_spec(),   kernels_ll_f64(),  kernels_ll_f32(),
-                                   kernels_ll_spec(), kernels_exact(),  kernels_cl()};
+                                   kernels_ll_spec(), kernels_exact(),  kernels_cl(),
+                                   kernels_dp()};
   // [0] one lane per instance, [1] one lane per layer, [2] one lane per component
   const KEntry* best[3] = {nullptr, nullptr, nullptr};
   for (const KTable& t : kTables)
@@ -116,6 +117,7 @@ static int n_channels(int nl, int mr, int mask) {
   if (mask & CMTJ_LOG_MYZ) n += 2 * nl;
   if (mask & CMTJ_LOG_MR) n += (mr == CMTJ_MR_CLASSIC) ? 1 : (mr == CMTJ_MR_STRIP ? 2 : 0);
   if (mask & CMTJ_LOG_ENERGY) n += CMTJ_ENERGY_CHANNELS * nl;
+  if (mask & CMTJ_LOG_TIME) n += 1;
   return n;
 }
 
@@ -160,14 +162,32 @@ static int validate(const cmtj_junction& j, size_t idx, const cmtj_layer* layers
     return fail(CMTJ_ERR_INVALID, "n_layers must be in [1, 8]" + at);
   if ((size_t)j.layer_offset + (size_t)j.n_layers > rows)
     return fail(CMTJ_ERR_INVALID, "layer rows out of range" + at);
-  if (j.solver == CMTJ_DORMAND_PRINCE)
-    return fail(CMTJ_ERR_INVALID, "DormandPrince adaptive stepping is not provided by the engine" + at);
-  if (j.solver != CMTJ_RK4 && j.solver != CMTJ_HEUN && j.solver != CMTJ_EULER_HEUN)
+  const bool dp = j.solver == CMTJ_DORMAND_PRINCE;
+  if (j.solver != CMTJ_RK4 && j.solver != CMTJ_HEUN && j.solver != CMTJ_EULER_HEUN && !dp)
     return fail(CMTJ_ERR_INVALID, "unknown solver" + at);
   if (!(j.dt > 0.0) || !std::isfinite(j.dt)) return fail(CMTJ_ERR_INVALID, "time step must be > 0" + at);
-  if (j.write_every < 1) return fail(CMTJ_ERR_INVALID, "write_every must be >= 1" + at);
-  const uint64_t ne = j.n_steps == 0 ? 0 : ((uint64_t)j.n_steps + j.write_every - 1) / j.write_every;
-  if (ne != j.n_entries) return fail(CMTJ_ERR_INVALID, "n_entries != ceil(n_steps / write_every)" + at);
+  if (dp) {
+    // adaptive steps: one layer (the reference's multi-layer variant sets
+    // the step to -1 after any rejection and does not terminate), binary64,
+    // closed-form drivers only (callback tables are sampled at fixed stage
+    // times), no energy channels, the time logged per entry
+    if (j.n_layers != 1) return fail(CMTJ_ERR_INVALID, "Dormand-Prince runs take one layer" + at);
+    if (prec != CMTJ_FP64) return fail(CMTJ_ERR_INVALID, "Dormand-Prince runs are FP64" + at);
+    if (j.log_mask & CMTJ_LOG_ENERGY)
+      return fail(CMTJ_ERR_INVALID, "no energy channels with Dormand-Prince" + at);
+    if (j.log_mask && !(j.log_mask & CMTJ_LOG_TIME))
+      return fail(CMTJ_ERR_INVALID, "a Dormand-Prince log needs the time channel" + at);
+    if (!(j.total_time >= 0.0) || !std::isfinite(j.total_time) || !(j.write_freq > 0.0))
+      return fail(CMTJ_ERR_INVALID, "total_time >= 0 and write_freq > 0 needed" + at);
+    for (int s = 0; s < CMTJ_NSLOT; ++s)
+      if (layers[j.layer_offset].drv[s].type == CMTJ_TABLE)
+        return fail(CMTJ_ERR_INVALID, "callback drivers need fixed stage times (not Dormand-Prince)" + at);
+  } else {
+    if (j.log_mask & CMTJ_LOG_TIME) return fail(CMTJ_ERR_INVALID, "the time channel is for Dormand-Prince" + at);
+    if (j.write_every < 1) return fail(CMTJ_ERR_INVALID, "write_every must be >= 1" + at);
+    const uint64_t ne = j.n_steps == 0 ? 0 : ((uint64_t)j.n_steps + j.write_every - 1) / j.write_every;
+    if (ne != j.n_entries) return fail(CMTJ_ERR_INVALID, "n_entries != ceil(n_steps / write_every)" + at);
+  }
   if ((j.log_mask & CMTJ_LOG_ENERGY) && prec != CMTJ_FP64)
     return fail(CMTJ_ERR_INVALID, "energy channels need an FP64 run" + at);
   if (j.mr_mode == CMTJ_MR_CLASSIC && j.n_layers > 2)
@@ -865,6 +885,9 @@ int cmtj_plan_execute_slice(cmtj_plan* p, void* d_log, size_t log_capacity, doub
   if (!p) return fail(CMTJ_ERR_INVALID, "null plan");
   if (!d_state) return fail(CMTJ_ERR_INVALID, "a slice needs the instance state");
   if (max_steps == 0) return fail(CMTJ_ERR_INVALID, "max_steps must be positive");
+  for (const Group& G : p->groups)
+    if (G.k->solver == CMTJ_DORMAND_PRINCE)
+      return fail(CMTJ_ERR_INVALID, "Dormand-Prince runs are not sliced (their step count is adaptive)");
   if (n_streams < 0 || (n_streams > 0 && !streams)) return fail(CMTJ_ERR_INVALID, "bad stream list");
   if (d_log && log_capacity < p->log_extent)
     return fail(CMTJ_ERR_INVALID, "log buffer smaller than the plan's log extent (" +

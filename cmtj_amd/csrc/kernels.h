@@ -743,6 +743,7 @@ __device__ __forceinline__ int log_channels(int nl, int mr, int mask) {
   if (mask & CMTJ_LOG_MYZ) n += 2 * nl;
   if (mask & CMTJ_LOG_MR) n += (mr == CMTJ_MR_CLASSIC) ? 1 : (mr == CMTJ_MR_STRIP ? 2 : 0);
   if (mask & CMTJ_LOG_ENERGY) n += CMTJ_ENERGY_CHANNELS * nl;
+  if (mask & CMTJ_LOG_TIME) n += 1;
   return n;
 }
 

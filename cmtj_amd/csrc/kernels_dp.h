@@ -1,0 +1,163 @@
+// kernels_dp.h -- Dormand-Prince 5(4) with adaptive steps, one single-layer
+// instance per lane (FP64).
+//
+// Each lane runs its own step-size history: Layer::dormand_prince_step
+// (core/junction.hpp:866-1002, the standard controller) inside the adaptive
+// branch of Junction::runSimulation (:1995-2044).  Quirks of the reference
+// kept on purpose, since they define its results:
+//   * the clock advances by the step size proposed for the *next* step
+//     (dormand_prince_step overwrites timeStep before `t += current_timestep`);
+//   * the rejection counter is reset at the top of every attempt, so a step
+//     is forced through only when the proposed step falls below 1e-15 s;
+//   * a log entry follows each accepted step that reaches the next multiple
+//     of the write frequency, labelled with the new time.
+// Lanes of a wave take different numbers of attempts; every lane leaves
+// its loop after at most n_steps attempts (a run that needs more ends with
+// CMTJ_ERR_LIMIT), so the wave always drains.
+#pragma once
+
+#include "kernels.h"
+
+namespace cmtj {
+
+__device__ __forceinline__ double vlen(V3<double> v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+template <uint32_t FM>
+__global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
+  const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= g.n) return;
+  const uint32_t fl = g.flags;
+  const uint32_t jid = g.order[lane];
+  const cmtj_junction* J = g.js + jid;
+  const cmtj_layer& Ly = g.layers[J->layer_offset];
+  LP<double> P;
+  DV<double> base[1];
+  V3<double> m[1];
+  load_layer<double>(Ly, g.c, P, base[0], m[0]);
+  DynRec dr[DREG];
+#pragma unroll
+  for (int k = 0; k < DREG; ++k) {
+    if (k < g.ndyn)
+      dr[k] = g.dyn[(size_t)k * g.n + lane];
+    else
+      dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
+  }
+  FieldWeights<double, 1> fw;
+  field_weights<double, 1>(fw, g, 0, dr);
+  ERec<double> er[1];
+  const int bits = g.lbits[0];
+  const V3<double> zero = mk<double>(0, 0, 0);
+  // dm/dt at (t, m) (calculateLLGWithFieldTorque, :851-864)
+  auto f = [&](double t, V3<double> mm) -> V3<double> {
+    DV<double> dv[1];
+    eval_drivers<double, 1, FM, false>(dv, base, dr, fw, g, lane, fl, t, 0, 0, 1);
+    return rhs<double, FM, false>(P, bits, fl, mm, dv[0], zero, zero, er[0]);
+  };
+  const double T = J->total_time, wf = J->write_freq, h0 = J->dt;
+  const double abs_tol = Ly.dp_abs_tol, rel_tol = Ly.dp_rel_tol;
+  const double max_factor = Ly.dp_max_factor, min_factor = Ly.dp_min_factor, safety = Ly.dp_safety;
+  const uint32_t cap = J->n_steps, NE = J->n_entries;
+  const bool logging = (g.log != nullptr) && g.log_mask != 0;
+  const uint32_t esize = g.log_f32 ? 4u : 8u;
+  char* const lp = logging ? (char*)g.log + J->log_offset * esize : nullptr;
+  const uint64_t cs = (uint64_t)NE * esize;
+  const uint64_t tch = (uint64_t)(log_channels(1, g.mr_mode, g.log_mask) - 1);  // the time channel is last
+
+  double t = 0.0, h = h0, next_write = 0.0;
+  uint32_t it = 0, k = 0, accepted_steps = 0;
+  int code = CMTJ_OK, nan_layer = -1;
+  while (t < T) {
+    if (it >= cap) {
+      code = CMTJ_ERR_LIMIT;
+      break;
+    }
+    ++it;
+    if (t + h > T) h = T - t;
+    int rejections = 0;
+    if (isnan(h) || isinf(h) || h <= 0) {
+      h = h0;
+      ++rejections;
+    }
+    // dormand_prince_step(t, h, 0, 0)
+    const V3<double> mt = m[0];
+    const V3<double> k1 = f(t, mt) * h;
+    const V3<double> k2 = f(t + (1.0 / 5.0) * h, mt + k1 * (1.0 / 5.0)) * h;
+    const V3<double> k3 = f(t + (3.0 / 10.0) * h, mt + k1 * (3.0 / 40.0) + k2 * (9.0 / 40.0)) * h;
+    const V3<double> k4 =
+        f(t + (4.0 / 5.0) * h, mt + k1 * (44.0 / 45.0) + k2 * (-56.0 / 15.0) + k3 * (32.0 / 9.0)) * h;
+    const V3<double> k5 = f(t + (8.0 / 9.0) * h, mt + k1 * (19372.0 / 6561.0) + k2 * (-25360.0 / 2187.0) +
+                                                      k3 * (64448.0 / 6561.0) + k4 * (-212.0 / 729.0)) *
+                          h;
+    const V3<double> k6 = f(t + 1.0 * h, mt + k1 * (9017.0 / 3168.0) + k2 * (-355.0 / 33.0) +
+                                             k3 * (46732.0 / 5247.0) + k4 * (49.0 / 176.0) +
+                                             k5 * (-5103.0 / 18656.0)) *
+                          h;
+    const V3<double> k7 = f(t + 1.0 * h, mt + k1 * (35.0 / 384.0) + k3 * (500.0 / 1113.0) +
+                                             k4 * (125.0 / 192.0) + k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0)) *
+                          h;
+    V3<double> m5 = mt + k1 * (35.0 / 384.0) + k3 * (500.0 / 1113.0) + k4 * (125.0 / 192.0) +
+                    k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0);
+    const V3<double> m4 = mt + k1 * (5179.0 / 57600.0) + k3 * (7571.0 / 16695.0) + k4 * (393.0 / 640.0) +
+                          k5 * (-92097.0 / 339200.0) + k6 * (187.0 / 2100.0) + k7 * (1.0 / 40.0);
+    const double err = vlen(m5 - m4);
+    const double tol = abs_tol + rel_tol * vlen(mt);
+    double ratio = err / tol;
+    if (tol < 1e-15 || isnan(ratio) || isinf(ratio)) ratio = 10.0;
+    double factor = safety * pow(1.0 / ratio, 0.2);
+    if (isnan(factor) || isinf(factor)) factor = min_factor;
+    factor = fmin(factor, max_factor);
+    factor = fmax(factor, min_factor);
+    double nh = h * factor;
+    if (isnan(nh) || isinf(nh) || nh <= 0) nh = h * min_factor;
+    bool acc;
+    if (ratio <= 1.0) {
+      m5 = normalize_exact(m5);
+      if (isnan(m5.x)) {  // the reference throws "NAN magnetisation"
+        m[0] = m5;
+        code = CMTJ_ERR_NAN;
+        nan_layer = 0;
+        break;
+      }
+      m[0] = m5;
+      h = nh;
+      acc = true;
+    } else {
+      h = nh;
+      acc = false;
+    }
+    if (!acc && (h < 1e-15 || rejections >= 10)) acc = true;  // forced acceptance (:2017-2025)
+    if (acc) {
+      t += h;
+      ++accepted_steps;
+      if (t >= next_write) {
+        if (k >= NE) {
+          code = CMTJ_ERR_LIMIT;
+          break;
+        }
+        if (logging) {
+          char* p = lp + (uint64_t)k * esize;
+          write_entry<double, 1, FM, false>(g, J, m, er, p, cs, J->Rp, J->Rap);
+          log_put(g.log_f32, p + tch * cs, t);
+        }
+        ++k;
+        next_write = (double)((uint32_t)(t / wf) + 1u) * wf;
+      }
+    }
+  }
+  if (g.mag_out) {
+    double* o = g.mag_out + (size_t)jid * CMTJ_MAX_LAYERS * 3;
+    o[0] = m[0].x;
+    o[1] = m[0].y;
+    o[2] = m[0].z;
+  }
+  if (g.status) {
+    cmtj_status s;
+    s.code = code;
+    s.steps_done = accepted_steps;
+    s.entries_done = k;
+    s.nan_layer = nan_layer;
+    g.status[jid] = s;
+  }
+}
+
+}  // namespace cmtj

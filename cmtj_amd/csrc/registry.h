@@ -44,6 +44,7 @@ KTable kernels_ll_spec();
 KTable kernels_exact();
 KTable kernels_cl();
 KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stacks only
+KTable kernels_dp();     // Dormand-Prince (single-layer, FP64)
 
 }  // namespace cmtj
 

@@ -177,6 +177,8 @@ class Server:
         a Result (or raises RuntimeError("NAN magnetisation"))."""
         jd, L, tabs, meta = JunctionBatch._pack(junction, totalTime, timeStep, writeFrequency, solverMode)
         jd = np.array(jd, dtype=_lib.JUNCTION_DTYPE).reshape(1)
+        if int(jd["solver"][0]) == _lib.DORMAND_PRINCE:
+            raise NotImplementedError("Dormand-Prince runs are not served in slices (their step count is adaptive)")
         mask = _LOG_MODES[log] if isinstance(log, str) else int(log)
         if mask & _lib.LOG_ENERGY:
             raise ValueError("energy channels are a drop-in (Junction.runSimulation) feature")

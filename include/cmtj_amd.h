@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define CMTJ_ABI_VERSION 3
+#define CMTJ_ABI_VERSION 4
 #define CMTJ_MAX_LAYERS 8
 #define CMTJ_MAX_STACK 64 /* junctions of one coupled stack (one wave) */
 
@@ -96,9 +96,19 @@ enum cmtj_mr_mode { CMTJ_MR_NONE = 0, CMTJ_MR_CLASSIC = 1, CMTJ_MR_STRIP = 2 };
  *   for each layer l: 23 energy channels    (LOG_ENERGY, FP64 runs only):
  *     K K2 Jbottom Jtop I Hext(xyz) Hiec(xyz) Hanis(xyz) Hdemag(xyz)
  *     Hfl(xyz) Hdl(xyz)
+ *   time                                   (LOG_TIME: Dormand-Prince runs,
+ *                                            whose entry times are adaptive)
  * Entries are written every write_every steps (entry k after step k*W,
- * reference core/junction.hpp:1973-1981). */
-enum cmtj_log_bits { CMTJ_LOG_MX = 1, CMTJ_LOG_MYZ = 2, CMTJ_LOG_MR = 4, CMTJ_LOG_ENERGY = 8 };
+ * reference core/junction.hpp:1973-1981); Dormand-Prince runs write one
+ * after each accepted step that reaches the next multiple of the write
+ * frequency (:2031-2039). */
+enum cmtj_log_bits {
+  CMTJ_LOG_MX = 1,
+  CMTJ_LOG_MYZ = 2,
+  CMTJ_LOG_MR = 4,
+  CMTJ_LOG_ENERGY = 8,
+  CMTJ_LOG_TIME = 16
+};
 #define CMTJ_LOG_FULL (CMTJ_LOG_MX | CMTJ_LOG_MYZ | CMTJ_LOG_MR)
 #define CMTJ_ENERGY_CHANNELS 23
 
@@ -138,6 +148,9 @@ typedef struct cmtj_layer {
   int32_t alternative_stt;
   int32_t temperature_set; /* thermal (stochastic) field active */
   cmtj_driver drv[CMTJ_NSLOT];
+  /* Dormand-Prince step control (AdaptiveIntegrationParams,
+   * core/junction.hpp:125-139; the standard controller) */
+  double dp_abs_tol, dp_rel_tol, dp_max_factor, dp_min_factor, dp_safety;
 } cmtj_layer;
 
 /* One junction instance of a run.  n_steps and write_every are the FP64
@@ -167,6 +180,12 @@ typedef struct cmtj_junction {
   double SMR_X[CMTJ_MAX_LAYERS];
   double SMR_Y[CMTJ_MAX_LAYERS];
   double AHE[CMTJ_MAX_LAYERS];
+  /* Dormand-Prince runs (solver CMTJ_DORMAND_PRINCE, one layer): totalTime
+   * and writeFrequency of runSimulation; dt is the initial step, n_steps
+   * the cap on step attempts (a run that needs more ends with
+   * CMTJ_ERR_LIMIT) and n_entries the log capacity, floor(T / wf) + 4. */
+  double total_time;
+  double write_freq;
 } cmtj_junction;
 
 /* Snapshot of reference PhysicalConstants (core/constants.hpp:6-53). */
@@ -184,7 +203,8 @@ enum cmtj_status_code {
   CMTJ_OK = 0,
   CMTJ_ERR_INVALID = 1,
   CMTJ_ERR_HIP = 2,
-  CMTJ_ERR_NAN = 3 /* reference: runtime_error("NAN magnetisation"), core/junction.hpp:1037-1039 */
+  CMTJ_ERR_NAN = 3, /* reference: runtime_error("NAN magnetisation"), core/junction.hpp:1037-1039 */
+  CMTJ_ERR_LIMIT = 4 /* Dormand-Prince: step-attempt cap or log capacity reached */
 };
 typedef struct cmtj_status {
   int32_t code;

@@ -354,6 +354,7 @@ int oracle_n_channels(int n_layers, int mr_mode, int log_mask) {
   if (log_mask & CMTJ_LOG_MYZ) n += 2 * n_layers;
   if (log_mask & CMTJ_LOG_MR) n += (mr_mode == CMTJ_MR_CLASSIC) ? 1 : (mr_mode == CMTJ_MR_STRIP ? 2 : 0);
   if (log_mask & CMTJ_LOG_ENERGY) n += CMTJ_ENERGY_CHANNELS * n_layers;
+  if (log_mask & CMTJ_LOG_TIME) n += 1;
   return n;
 }
 
@@ -495,6 +496,107 @@ static void store_mag(const cmtj_junction* j, const olayer* Ls, double* mag_out)
   }
 }
 
+/* Layer::dormand_prince_step (core/junction.hpp:866-1002) with the standard
+ * controller, neighbours zero (one layer).  Returns 1 if accepted; *h is
+ * replaced by the proposed next step either way; *nan set on a NaN. */
+static int dp_step(olayer* L, double time, double* h_io, int* nan) {
+  const double h = *h_io;
+  const vec3 zero = v3(0, 0, 0);
+  const vec3 m_t = L->mag;
+  const vec3 k1 = vmul(llg(L, time, 0, 0, m_t, zero, zero), h);
+  const vec3 k2 = vmul(llg(L, time + (1.0 / 5.0) * h, 0, 0, vadd(m_t, vmul(k1, 1.0 / 5.0)), zero, zero), h);
+  const vec3 k3 = vmul(llg(L, time + (3.0 / 10.0) * h, 0, 0,
+                           vadd(vadd(m_t, vmul(k1, 3.0 / 40.0)), vmul(k2, 9.0 / 40.0)), zero, zero), h);
+  const vec3 k4 = vmul(llg(L, time + (4.0 / 5.0) * h, 0, 0,
+                           vadd(vadd(vadd(m_t, vmul(k1, 44.0 / 45.0)), vmul(k2, -56.0 / 15.0)),
+                                vmul(k3, 32.0 / 9.0)), zero, zero), h);
+  const vec3 k5 = vmul(llg(L, time + (8.0 / 9.0) * h, 0, 0,
+                           vadd(vadd(vadd(vadd(m_t, vmul(k1, 19372.0 / 6561.0)), vmul(k2, -25360.0 / 2187.0)),
+                                     vmul(k3, 64448.0 / 6561.0)), vmul(k4, -212.0 / 729.0)), zero, zero), h);
+  const vec3 k6 = vmul(llg(L, time + 1.0 * h, 0, 0,
+                           vadd(vadd(vadd(vadd(vadd(m_t, vmul(k1, 9017.0 / 3168.0)), vmul(k2, -355.0 / 33.0)),
+                                          vmul(k3, 46732.0 / 5247.0)), vmul(k4, 49.0 / 176.0)),
+                                vmul(k5, -5103.0 / 18656.0)), zero, zero), h);
+  const vec3 m5s = vadd(vadd(vadd(vadd(vadd(m_t, vmul(k1, 35.0 / 384.0)), vmul(k3, 500.0 / 1113.0)),
+                                  vmul(k4, 125.0 / 192.0)), vmul(k5, -2187.0 / 6784.0)), vmul(k6, 11.0 / 84.0));
+  const vec3 k7 = vmul(llg(L, time + 1.0 * h, 0, 0, m5s, zero, zero), h);
+  vec3 m_t5 = m5s;
+  const vec3 m_t4 = vadd(vadd(vadd(vadd(vadd(vadd(m_t, vmul(k1, 5179.0 / 57600.0)), vmul(k3, 7571.0 / 16695.0)),
+                                        vmul(k4, 393.0 / 640.0)), vmul(k5, -92097.0 / 339200.0)),
+                              vmul(k6, 187.0 / 2100.0)), vmul(k7, 1.0 / 40.0));
+  const vec3 e = vsub(m_t5, m_t4);
+  const double error_norm = sqrt(e.x * e.x + e.y * e.y + e.z * e.z);
+  const double mlen = sqrt(m_t.x * m_t.x + m_t.y * m_t.y + m_t.z * m_t.z);
+  const double tol = L->p->dp_abs_tol + L->p->dp_rel_tol * mlen;
+  double error_ratio = error_norm / tol;
+  if (tol < 1e-15 || isnan(error_ratio) || isinf(error_ratio)) error_ratio = 10.0;
+  double factor = L->p->dp_safety * pow(1.0 / error_ratio, 0.2);
+  if (isnan(factor) || isinf(factor)) factor = L->p->dp_min_factor;
+  factor = fmin(factor, L->p->dp_max_factor);
+  factor = fmax(factor, L->p->dp_min_factor);
+  double new_h = h * factor;
+  if (isnan(new_h) || isinf(new_h) || new_h <= 0) new_h = h * L->p->dp_min_factor;
+  *h_io = new_h;
+  if (error_ratio <= 1.0) {
+    m_t5 = vnormalize(m_t5);
+    L->mag = m_t5;
+    if (isnan(L->mag.x)) *nan = 1;
+    return 1;
+  }
+  return 0;
+}
+
+/* The adaptive branch of Junction::runSimulation (:1995-2044): the clock
+ * advances by the step proposed for the next step; a log entry, labelled
+ * with the new time, after each accepted step that reaches the next
+ * multiple of the write frequency.  At most n_steps attempts. */
+static void dp_run(const cmtj_junction* j, olayer* Ls, double* log, cmtj_status* st) {
+  const double T = j->total_time, wf = j->write_freq, h0 = j->dt;
+  const size_t E = j->n_entries;
+  const size_t tch = (size_t)oracle_n_channels(1, j->mr_mode, j->log_mask) - 1;
+  double t = 0.0, h = h0, next_write = 0.0;
+  uint32_t it = 0, k = 0, accepted = 0;
+  while (t < T) {
+    if (it >= j->n_steps) {
+      st->code = CMTJ_ERR_LIMIT;
+      break;
+    }
+    ++it;
+    if (t + h > T) h = T - t;
+    int rejections = 0;
+    if (isnan(h) || isinf(h) || h <= 0) {
+      h = h0;
+      ++rejections;
+    }
+    int nan = 0;
+    int acc = dp_step(&Ls[0], t, &h, &nan);
+    if (nan) {
+      st->code = CMTJ_ERR_NAN;
+      st->nan_layer = 0;
+      break;
+    }
+    if (!acc && (h < 1e-15 || rejections >= 10)) acc = 1;
+    if (acc) {
+      t += h;
+      ++accepted;
+      if (t >= next_write) {
+        if (k >= E) {
+          st->code = CMTJ_ERR_LIMIT;
+          break;
+        }
+        if (j->log_mask) {
+          log_entry(j, Ls, log, k);
+          log[tch * E + k] = t;
+        }
+        ++k;
+        next_write = (double)((unsigned int)(t / wf) + 1u) * wf;
+      }
+    }
+  }
+  st->steps_done = accepted;
+  st->entries_done = k;
+}
+
 int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
                const double* tables, size_t n_table, double* log, double* mag_out, cmtj_status* st) {
   (void)n_table;
@@ -504,6 +606,15 @@ int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_cons
   st->steps_done = 0;
   st->entries_done = 0;
   st->nan_layer = -1;
+  if (j->solver == CMTJ_DORMAND_PRINCE) {
+    if (j->n_layers != 1) {
+      st->code = CMTJ_ERR_INVALID;
+    } else {
+      dp_run(j, Ls, log, st);
+    }
+    store_mag(j, Ls, mag_out);
+    return 0;
+  }
   uint32_t k = 0;
   for (uint32_t i = 0; i < j->n_steps; ++i) {
     if (junction_step(j, Ls, i, st)) break;

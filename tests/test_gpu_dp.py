@@ -1,0 +1,135 @@
+"""Dormand-Prince on the GPU (k_dp: one adaptive lane per single-layer
+instance) against the oracle: the same accepted steps, entry times and
+magnetisations for an SOT-driven switch with second-order anisotropy, a sine
+drive, an exact equilibrium (the step-size collapse of the reference's
+controller) and a batch of mixed instances; at a tight tolerance, both on
+the closed-form damped precession."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cmtj_amd import (AdaptiveIntegrationParams, AxialDriver, CVector, Junction, Layer, SolverMode, constantDriver,
+                      sineDriver, stepDriver)
+from cmtj_amd.batch import JunctionBatch
+from oracle_backend import OracleJunction
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, J):
+    demag = [CVector(0.017, 0, 0), CVector(0, 0.017, 0), CVector(0, 0, math.sqrt(1 - 2 * 0.017 ** 2))]
+    if name == "precession":
+        l = Layer("free", mag=CVector(math.sin(1.2), 0, math.cos(1.2)), anis=CVector(1, 0, 0), Ms=1.0,
+                  thickness=1e-9, cellSurface=1e-16, demagTensor=[CVector(0, 0, 0)] * 3, damping=0.02)
+        p = AdaptiveIntegrationParams()
+        p.abs_tol = p.rel_tol = 1e-10
+        l.setAdaptiveParams(p)
+        j = J([l])
+        j.setLayerExternalFieldDriver("all", AxialDriver(0, 0, 1e5))
+        return j, (0.5e-9, 1e-13, 1e-12)
+    if name == "sot_switch":  # the stability-diagram example of the reference's curated tests
+        l = Layer("free", mag=CVector(0.1, 0.1, 0.99), anis=CVector(0, 0, 1), Ms=1.12, thickness=1e-9,
+                  cellSurface=math.pi * (300e-9) ** 2, demagTensor=demag, damping=0.01)
+        l.setReferenceLayer(CVector(0, 0, 1))
+        l.setSecondOrderAnisotropyDriver(constantDriver(-30e3))
+        l.setAnisotropyDriver(constantDriver(563e3))
+        j = J([l])
+        j.setLayerExternalFieldDriver("free", AxialDriver(2e4, 0, 1e5))
+        j.setLayerFieldLikeTorqueDriver("free", stepDriver(0, 0.7 * 795774.715459 * 1e-3 * 0.04, 0, 1e-9))
+        j.setLayerDampingLikeTorqueDriver("free", stepDriver(0, 0.7 * 795774.715459 * 16e-3 * 0.2, 0, 1e-9))
+        return j, (2e-9, 1e-12, 1e-11)
+    if name == "sine":
+        l = Layer("free", mag=CVector(1, 0.1, 0), anis=CVector(1, 0, 0), Ms=1.65, thickness=3e-9,
+                  cellSurface=1e-16, demagTensor=demag, damping=0.01)
+        j = J([l])
+        j.setLayerAnisotropyDriver("free", constantDriver(1e4))
+        j.setLayerExternalFieldDriver("all", AxialDriver(1e5, 0, 0))
+        j.setLayerOerstedFieldDriver("all", AxialDriver(constantDriver(0), sineDriver(0, 500, 7e9, 0),
+                                                        constantDriver(0)))
+        return j, (1e-9, 1e-13, 1e-12)
+    if name == "equilibrium":  # zero torque: the controller shrinks the step to nothing and resets
+        l = Layer("free", mag=CVector(0, 0, 1), anis=CVector(1, 0, 0), Ms=1.0, thickness=1e-9,
+                  cellSurface=1e-16, demagTensor=[CVector(0, 0, 0)] * 3, damping=0.02)
+        j = J([l])
+        j.setLayerExternalFieldDriver("all", AxialDriver(0, 0, 1e5))
+        return j, (3e-12, 1e-13, 1e-12)
+    raise KeyError(name)
+
+
+def _compare(a, b):
+    la, lb = a.getLog(), b.getLog()
+    assert la.keys() == lb.keys()
+    assert len(la["time"]) == len(lb["time"])
+    worst = 0.0
+    for k in la:
+        x, y = np.asarray(la[k]), np.asarray(lb[k])
+        scale = float(np.max(np.abs(y))) if y.size else 1.0
+        worst = max(worst, float(np.max(np.abs(x - y))) / max(scale, 1e-300) if y.size else 0.0)
+    return worst
+
+
+@pytest.mark.parametrize("name", ["sot_switch", "sine", "equilibrium"])
+def test_dp_matches_oracle(name):
+    g, args = _case(name, Junction)
+    o, _ = _case(name, OracleJunction)
+    g.runSimulation(*args, solverMode=SolverMode.DormandPrince)
+    o.runSimulation(*args, solverMode=SolverMode.DormandPrince)
+    worst = _compare(g, o)
+    print(f"[dp] {name}: {len(g.getLog()['time'])} entries, max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+    np.testing.assert_allclose(np.array(g.getLayerMagnetisation("free").tolist()),
+                               np.array(o.getLayerMagnetisation("free").tolist()), atol=1e-8)
+
+
+def test_dp_tight_tolerance_stays_on_the_exact_trajectory():
+    """At tolerance 1e-10 the controller reads |m5 - m4| ~ 1e-10, so one-ulp
+    differences between two builds move the proposed step by ~1e-7 relative
+    and the two runs sample the same trajectory at slightly different times.
+    Both stay on the closed-form damped precession, entry for entry."""
+    from test_dp_cpu import _off_trajectory
+
+    g, args = _case("precession", Junction)
+    o, _ = _case("precession", OracleJunction)
+    g.runSimulation(*args, solverMode=SolverMode.DormandPrince)
+    o.runSimulation(*args, solverMode=SolverMode.DormandPrince)
+    lg, lo = g.getLog(), o.getLog()
+    assert len(lg["time"]) == len(lo["time"])
+    for lg_ in (lg, lo):
+        m = np.stack([lg_[f"free_m{c}"] for c in "xyz"], axis=1)
+        off = float(_off_trajectory(m, 1.2, 1e5, 0.02)[0].max())
+        print(f"[dp] precession: {len(lg_['time'])} entries, off the exact trajectory by {off:.3e} rad", flush=True)
+        assert off < 1e-6
+    np.testing.assert_allclose(lg["time"], lo["time"], rtol=1e-7)
+
+
+def test_dp_batch_matches_oracle():
+    """Instances of different lengths and dynamics in one launch, each lane
+    with its own step history."""
+    js, os_ = [], []
+    for name in ("sot_switch", "sine", "equilibrium", "sine"):
+        for J, out in ((Junction, js), (OracleJunction, os_)):
+            j, _ = _case(name, J)
+            out.append(j)
+    T = [1e-9, 0.5e-9, 3e-12, 0.1e-9]
+    b = JunctionBatch.from_junctions(js, T, 1e-13, 1e-12, SolverMode.DormandPrince, precision="fp64", log="full",
+                                     log_dtype="f64")
+    logs = run_and_split(b, js)
+    for q, o in enumerate(os_):
+        o.runSimulation(T[q], 1e-13, 1e-12, solverMode=SolverMode.DormandPrince)
+        ol = o.getLog()
+        assert len(ol["time"]) == len(logs[q]["time"]), q
+        for k in ol:
+            np.testing.assert_allclose(logs[q][k], ol[k], rtol=1e-8, atol=1e-12)
+
+
+def run_and_split(b, js):
+    from cmtj_amd.batch import run_batch  # noqa: F401  (same packing as run_batch)
+
+    r = b.run()
+    out = []
+    for q in range(len(js)):
+        out.append(r.instance_log(q))
+    return out
