@@ -242,6 +242,58 @@ def _bench_stack(n_members: int = 8, total_time: float = 10e-9) -> dict:
             "what": "host-to-host Stack.runSimulation / run_stacks (packing, upload, kernel, log copy back)"}
 
 
+def _bench_dp(n_fields: int = 64, n_currents: int = 64, total_time: float = 3e-9) -> dict:
+    """Dormand-Prince stability-diagram sweep: single-layer SOT junctions
+    (perpendicular, second-order anisotropy, 1 ns field- and damping-like
+    torque pulses) over fields x current densities, default step control,
+    initial step 1e-12 s, one entry per 10 ps; the whole grid in one launch
+    against the oracle on the host (a sample, 16 threads)."""
+    import math
+
+    import numpy as np
+
+    import oracle
+    from cmtj_amd import AxialDriver, CVector, Junction, Layer, SolverMode, _lib, constantDriver, stepDriver
+    from cmtj_amd.batch import JunctionBatch
+
+    p = 0.017
+    demag = [CVector(p, 0, 0), CVector(0, p, 0), CVector(0, 0, math.sqrt(1 - 2 * p * p))]
+    TtoAm = 795774.715459
+    js = []
+    for Hz in np.linspace(-4e5, 4e5, n_fields):
+        for jden in np.linspace(0.0, 0.3, n_currents):
+            l = Layer("free", mag=CVector(0.1, 0.1, 0.99), anis=CVector(0, 0, 1), Ms=1.12, thickness=1e-9,
+                      cellSurface=math.pi * (300e-9) ** 2, demagTensor=demag, damping=0.01)
+            l.setReferenceLayer(CVector(0, 0, 1))
+            l.setSecondOrderAnisotropyDriver(constantDriver(-30e3))
+            l.setAnisotropyDriver(constantDriver(563e3))
+            j = Junction([l])
+            j.setLayerExternalFieldDriver("free", AxialDriver(0.1 * abs(Hz) + 1e3, 0, float(Hz)))
+            j.setLayerFieldLikeTorqueDriver("free", stepDriver(0, 0.7 * TtoAm * 1e-3 * jden ** 2, 0, 1e-9))
+            j.setLayerDampingLikeTorqueDriver("free", stepDriver(0, 0.7 * TtoAm * 16e-3 * jden, 0, 1e-9))
+            js.append(j)
+    b = JunctionBatch.from_junctions(js, total_time, 1e-12, 1e-11, SolverMode.DormandPrince, precision="fp64",
+                                     log="full", log_dtype="f64")
+    r, ms = _timed_run(b)
+    st = r.status()
+    steps = int(st["steps_done"].astype(np.int64).sum())
+    idx = np.arange(0, len(js), max(1, len(js) // 256))
+    sub = b.subset(idx)
+    t0 = time.perf_counter()
+    olog, omag, ost = oracle.run_batch(sub.js, sub.layers, _lib.constants_get(), None, sub.log_elements,
+                                       n_threads=min(16, os.cpu_count() or 1))
+    cpu_s = time.perf_counter() - t0
+    cpu_steps = int(ost["steps_done"].astype(np.int64).sum())
+    same = bool(np.all(ost["steps_done"] == st["steps_done"][idx]) and np.all(ost["entries_done"] ==
+                                                                               st["entries_done"][idx]))
+    dm = float(np.max(np.abs(r.final_m.cpu().numpy()[idx, 0] - omag[:, 0])))
+    return {"workload": f"dormand_prince_sot_stability_{n_fields}x{n_currents}", "instances": len(js),
+            "accepted_steps": steps, "gpu_ms": ms, "gpu_accepted_steps_per_s": steps / (ms * 1e-3),
+            "cpu_oracle_accepted_steps_per_s": cpu_steps / cpu_s, "cpu_threads": min(16, os.cpu_count() or 1),
+            "cpu_sample": f"{len(idx)} instances", "same_step_and_entry_counts_on_sample": same,
+            "max_dm_final_gpu_vs_oracle_sample": dm}
+
+
 def _bench_cims(n_currents: int = 256, n_seeds: int = 4096) -> dict:
     """CIMS / p-bit ensemble: 256 currents x 4096 seeds, 2 layers, static SOT
     at 300 K, Euler-Heun dt 1e-14 s, 5 ns; switching probability per current
@@ -341,7 +393,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving", "stack"],
+    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving", "stack", "dp"],
                     help="pimm: the headline sweep (default); rh, cims, sto: the other sweep workloads, "
                          "one JSON line each")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -363,7 +415,7 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if args.workload != "pimm":
-        fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto, "stack": _bench_stack,
+        fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto, "stack": _bench_stack, "dp": _bench_dp,
               "serving": lambda: _serving(args.serving_qps, args.serving_seconds, lanes=args.serving_lanes)}[args.workload]
         print(json.dumps(fn()))
         return 0
