@@ -51,10 +51,11 @@ def install_as_cmtj() -> None:
     mod = _sys.modules[__name__]
     _sys.modules.setdefault("cmtj", mod)
     _sys.modules.setdefault("cmtj.constants", constants)
-    from . import stack, utils
+    from . import reservoir, stack, utils
 
     _sys.modules.setdefault("cmtj.utils", utils)
     _sys.modules.setdefault("cmtj.stack", stack)
+    _sys.modules.setdefault("cmtj.reservoir", reservoir)
 
 
 __all__ = [

@@ -14,9 +14,11 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libcmtj_amd.so"
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_LAYERS = 8
 MAX_STACK = 64
+MAX_GROUP = 64
+INTERACTION_NULL, INTERACTION_KANAO, INTERACTION_NOMURA = 0, 1, 2
 NSLOT = 30
 ENERGY_CHANNELS = 23
 
@@ -153,6 +155,16 @@ STACK_DTYPE = np.dtype(
     align=True,
 )
 
+GROUP_DTYPE = np.dtype(
+    [
+        ("n_elements", "<i4"),
+        ("interaction", "<i4"),
+        ("top", "<i4", (MAX_GROUP,)),
+        ("pos", "<f8", (MAX_GROUP, 3)),
+    ],
+    align=True,
+)
+
 STATE_DTYPE = np.dtype(
     [("m", "<f8", (MAX_LAYERS, 3)), ("step", "<u4"), ("entries", "<u4"), ("code", "<i4"), ("nan_layer", "<i4")],
     align=True,
@@ -173,6 +185,7 @@ class _AbiInfo(ctypes.Structure):
             "sizeof_status",
             "sizeof_state",
             "sizeof_stack",
+            "sizeof_group",
             "offsetof_layer_drv",
             "offsetof_layer_ref_type",
             "offsetof_junction_dt",
@@ -218,6 +231,7 @@ EXPORTS = (
     "cmtj_plan_destroy",
     "cmtj_vsd_mix",
     "cmtj_run_stacks",
+    "cmtj_run_groups",
 )
 
 _P = ctypes.c_void_p
@@ -265,6 +279,7 @@ def _load():
     lib.cmtj_vsd_mix.argtypes = [_P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _P, ctypes.c_double, _P, _P,
                                  ctypes.c_int32, _P, _P]
     lib.cmtj_run_stacks.argtypes = [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P]
+    lib.cmtj_run_groups.argtypes = [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P]
     info = _AbiInfo()
     lib.cmtj_abi(ctypes.byref(info))
     checks = {
@@ -278,6 +293,7 @@ def _load():
         "status": (info.sizeof_status, STATUS_DTYPE.itemsize),
         "state": (info.sizeof_state, STATE_DTYPE.itemsize),
         "stack": (info.sizeof_stack, STACK_DTYPE.itemsize),
+        "group": (info.sizeof_group, GROUP_DTYPE.itemsize),
         "layer.drv": (info.offsetof_layer_drv, LAYER_DTYPE.fields["drv"][1]),
         "layer.ref_type": (info.offsetof_layer_ref_type, LAYER_DTYPE.fields["ref_type"][1]),
         "junction.dt": (info.offsetof_junction_dt, JUNCTION_DTYPE.fields["dt"][1]),
@@ -380,6 +396,26 @@ def run_stacks(stacks: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: n
     st = np.zeros(n, dtype=STATUS_DTYPE)
     check(lib().cmtj_run_stacks(ptr(stacks), stacks.size, ptr(js), ptr(layers), layers.size, ptr(consts),
                                 ptr(log) if log.size else None, log.size, ptr(mag), ptr(st)), "cmtj_run_stacks")
+    return log, mag, st
+
+
+def run_groups(groups: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray):
+    """Dipole-coupled groups through cmtj_run_groups (js[g * elements + e]:
+    element e of group g); js[q]["log_offset"] must place each element's
+    block of n_channels channels.  Returns (log, mag [n, 8, 3], status [n])."""
+    groups = np.ascontiguousarray(groups, dtype=GROUP_DTYPE).reshape(-1)
+    js = np.ascontiguousarray(js, dtype=JUNCTION_DTYPE)
+    layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
+    n = js.size
+    need = 0
+    for q in range(n):
+        nch = n_channels(int(js[q]["n_layers"]), int(js[q]["mr_mode"]), int(js[q]["log_mask"]))
+        need = max(need, int(js[q]["log_offset"]) + nch * int(js[q]["n_entries"]))
+    log = np.zeros(need, dtype=np.float64)
+    mag = np.zeros((n, MAX_LAYERS, 3), dtype=np.float64)
+    st = np.zeros(n, dtype=STATUS_DTYPE)
+    check(lib().cmtj_run_groups(ptr(groups), groups.size, ptr(js), ptr(layers), layers.size, ptr(consts),
+                                ptr(log) if log.size else None, log.size, ptr(mag), ptr(st)), "cmtj_run_groups")
     return log, mag, st
 
 

@@ -590,6 +590,7 @@ int cmtj_abi(cmtj_abi_info* out) {
   out->sizeof_status = (int32_t)sizeof(cmtj_status);
   out->sizeof_state = (int32_t)sizeof(cmtj_state);
   out->sizeof_stack = (int32_t)sizeof(cmtj_stack);
+  out->sizeof_group = (int32_t)sizeof(cmtj_group);
   out->offsetof_layer_drv = (int32_t)offsetof(cmtj_layer, drv);
   out->offsetof_layer_ref_type = (int32_t)offsetof(cmtj_layer, ref_type);
   out->offsetof_junction_dt = (int32_t)offsetof(cmtj_junction, dt);
@@ -748,6 +749,110 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
   G.a.stack = (const cmtj_stack*)(out + o_stk);
   G.a.stack_n = (uint32_t)n_stacks;
   G.a.stack_pad = pad;
+  r = launch_plan(&P, need ? out : nullptr, (double*)(out + o_mag), (cmtj_status*)(out + o_st), g_plain.stream);
+  if (r) return r;
+  if (need) HIP_TRY(hipMemcpyAsync(log_host, out, need * sizeof(double), hipMemcpyDeviceToHost, g_plain.stream));
+  HIP_TRY(hipMemcpyAsync(mag_out_host, out + o_mag, sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj,
+                         hipMemcpyDeviceToHost, g_plain.stream));
+  HIP_TRY(hipMemcpyAsync(st_host, out + o_st, sizeof(cmtj_status) * nj, hipMemcpyDeviceToHost, g_plain.stream));
+  HIP_TRY(hipStreamSynchronize(g_plain.stream));
+  return 0;
+}
+
+int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_junction* js, const cmtj_layer* layers,
+                    size_t n_layer_rows, const cmtj_constants* c, double* log_host, size_t log_cap,
+                    double* mag_out_host, cmtj_status* st_host) {
+  if (!groups || !js || !layers || !c || !mag_out_host || !st_host) return fail(CMTJ_ERR_INVALID, "null pointer");
+  if (n_groups == 0) return 0;
+  if (n_groups > (1u << 24)) return fail(CMTJ_ERR_INVALID, "too many groups");
+  const int n = groups[0].n_elements;
+  if (n < 1 || n > CMTJ_MAX_GROUP)
+    return fail(CMTJ_ERR_INVALID, "a group has 1.." + std::to_string(CMTJ_MAX_GROUP) + " elements");
+  const size_t nj = n_groups * (size_t)n;
+  const cmtj_junction& j0 = js[0];
+  if (j0.solver != CMTJ_RK4 && j0.solver != CMTJ_EULER_HEUN)
+    return fail(CMTJ_ERR_INVALID, "groups step with RK4 or Euler-Heun");
+  std::vector<cmtj_layer> rows(layers, layers + n_layer_rows);
+  size_t need = 0;
+  for (size_t q = 0; q < n_groups; ++q) {
+    const cmtj_group& G = groups[q];
+    if (G.n_elements != n) return fail(CMTJ_ERR_INVALID, "every group of a batch has the same element count");
+    if (G.interaction < CMTJ_INTERACTION_NULL || G.interaction > CMTJ_INTERACTION_NOMURA)
+      return fail(CMTJ_ERR_INVALID, "unknown interaction");
+    if (G.interaction == CMTJ_INTERACTION_NULL || j0.n_steps == 0) continue;
+    // computeDipoleInteraction / getDipoleTensorFromRelPositions throw on
+    // the first step (core/reservoir.hpp:83-86, :39-42)
+    const double lim = (G.interaction == CMTJ_INTERACTION_KANAO) ? 1e-10 : 1e-14;
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b < n; ++b) {
+        if (a == b) continue;
+        const double dx = G.pos[a][0] - G.pos[b][0], dy = G.pos[a][1] - G.pos[b][1], dz = G.pos[a][2] - G.pos[b][2];
+        if (!(std::sqrt(dx * dx + dy * dy + dz * dz) >= lim))
+          return fail(CMTJ_ERR_INVALID, "Points are too close for stable dipole calculation");
+      }
+  }
+  for (size_t q = 0; q < nj; ++q) {
+    const cmtj_junction& j = js[q];
+    const cmtj_group& G = groups[q / (size_t)n];
+    const int e = (int)(q % (size_t)n);
+    // equal step counts keep the plan's instance order the caller's
+    if (j.solver != j0.solver || j.n_steps != j0.n_steps || j.write_every != j0.write_every ||
+        j.n_entries != j0.n_entries || j.dt != j0.dt)
+      return fail(CMTJ_ERR_INVALID, "group elements must share solver, step count, time step and log cadence");
+    if (j.n_layers < 1 || j.n_layers > 4) return fail(CMTJ_ERR_INVALID, "group elements have 1..4 layers");
+    if (G.top[e] < 0 || G.top[e] >= j.n_layers) return fail(CMTJ_ERR_INVALID, "top layer index out of range");
+    const size_t row = (size_t)j.layer_offset + (size_t)G.top[e];
+    if (row >= n_layer_rows) return fail(CMTJ_ERR_INVALID, "layer_offset out of range");
+    // setLayerReservedInteractionField replaces the top layer's driver
+    // (reservoir.hpp:144-145); the kernel adds the group field itself
+    for (int s = 0; s < 3; ++s) rows[row].drv[CMTJ_SLOT_HRES + s] = cmtj_driver{};
+    const size_t nch = (size_t)n_channels(j.n_layers, j.mr_mode, j.log_mask);
+    need = std::max<size_t>(need, j.log_offset + nch * j.n_entries);
+  }
+  if (need > 0 && (!log_host || log_cap < need)) return fail(CMTJ_ERR_INVALID, "log buffer too small");
+  const KTable kt = kernels_group();
+  int maxl = 1;
+  for (size_t q = 0; q < nj; ++q) maxl = std::max(maxl, (int)js[q].n_layers);
+  const KEntry* ke = nullptr;
+  for (size_t q = 0; q < kt.n; ++q)
+    if (kt.e[q].solver == j0.solver && kt.e[q].lm >= maxl &&
+        (!ke || kt.e[q].lm < ke->lm))
+      ke = &kt.e[q];
+  if (!ke) return fail(CMTJ_ERR_INVALID, "no group kernel for this solver and layer count");
+
+  std::lock_guard<std::mutex> lk(g_plain.mu);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (g_plain.stream == nullptr || g_plain.device != dev) {
+    if (g_plain.stream) (void)hipStreamDestroy(g_plain.stream);
+    if (g_plain.arena) (void)hipFree(g_plain.arena);
+    if (g_plain.out) (void)hipFree(g_plain.out);
+    g_plain.arena = g_plain.out = nullptr;
+    g_plain.arena_bytes = g_plain.out_bytes = 0;
+    HIP_TRY(hipStreamCreateWithFlags(&g_plain.stream, hipStreamNonBlocking));
+    g_plain.device = dev;
+  }
+  cmtj_plan P;
+  P.owns = false;
+  int r = build_plan(&P, js, nj, rows.data(), rows.size(), c, nullptr, 0, CMTJ_FP64, CMTJ_LOG_F64, CMTJ_LANES_ONE,
+                     plain_arena, &g_plain, g_plain.stream);
+  if (r) return r;
+  if (P.groups.size() != 1)
+    return fail(CMTJ_ERR_INVALID,
+                "the junctions of a group must share one structure (layers, modes, driver kinds)");
+  const size_t o_mag = align_up(need * sizeof(double));
+  const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj);
+  const size_t o_grp = align_up(o_st + sizeof(cmtj_status) * nj);
+  const size_t out_bytes = o_grp + sizeof(cmtj_group) * n_groups;
+  char* out = (char*)grow(g_plain.out, g_plain.out_bytes, out_bytes);
+  if (!out) return CMTJ_ERR_HIP;
+  HIP_TRY(hipMemcpyAsync(out + o_grp, groups, sizeof(cmtj_group) * n_groups, hipMemcpyHostToDevice,
+                         g_plain.stream));
+  Group& G = P.groups[0];
+  G.k = ke;
+  G.blocks = (uint32_t)n_groups;  // group g on workgroup g, element e on lane e
+  G.a.group = (const cmtj_group*)(out + o_grp);
+  G.a.group_n = (uint32_t)n_groups;
   r = launch_plan(&P, need ? out : nullptr, (double*)(out + o_mag), (cmtj_status*)(out + o_st), g_plain.stream);
   if (r) return r;
   if (need) HIP_TRY(hipMemcpyAsync(log_host, out, need * sizeof(double), hipMemcpyDeviceToHost, g_plain.stream));

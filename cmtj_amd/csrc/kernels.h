@@ -44,6 +44,8 @@ struct GroupArgs {
   const cmtj_stack* stack;  // coupled stacks (k_junction<..., STK>): [stack_n]
   uint32_t stack_n;         // stacks of the launch; member j of stack s is instance s * members + j
   uint32_t stack_pad;       // lanes per stack: a power of two >= members, <= 64 (a stack never straddles a wave)
+  const cmtj_group* group;  // dipole-coupled groups (k_junction<..., GRP>): group g on workgroup g
+  uint32_t group_n;
   uint32_t slice;     // steps per launch when state is set
   uint32_t n;       // lanes of this group
   uint32_t flags;   // feature bits of the group
@@ -880,15 +882,80 @@ __device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n
   }
 }
 
+// A dipole-coupled group in LDS: the top-layer magnetisations of the step,
+// positions, source prefactors and the pair constants of (source j, element
+// e) -- Kanao: pa = |r|^3, pb = |r|^5; Nomura: pa = |r|^2, pb = 3 / (4 pi
+// |r|^5).  pa/pb are [j][e] so the lanes of one source read consecutive
+// words; m, pos and pref of a source are one broadcast read.
+struct GroupLds {
+  double m[CMTJ_MAX_GROUP][3];
+  double pos[CMTJ_MAX_GROUP][3];
+  double pref[CMTJ_MAX_GROUP];
+  double pa[CMTJ_MAX_GROUP][CMTJ_MAX_GROUP];
+  double pb[CMTJ_MAX_GROUP][CMTJ_MAX_GROUP];
+};
+
+// Field of the group on element e from the top-layer magnetisations in LDS
+// (stepFunctionalSolver, core/reservoir.hpp:126-147: j ascending, j != e).
+__device__ __forceinline__ V3<double> group_field(const GroupLds& L, int kind, int n, int e) {
+  V3<double> H = mk<double>(0, 0, 0);
+  if (kind == CMTJ_INTERACTION_NULL) return H;
+  const V3<double> ri = mk<double>(L.pos[e][0], L.pos[e][1], L.pos[e][2]);
+  for (int j = 0; j < n; ++j) {
+    if (j == e) continue;
+    const V3<double> rij = ri - mk<double>(L.pos[j][0], L.pos[j][1], L.pos[j][2]);
+    const V3<double> m1 = mk<double>(L.m[j][0], L.m[j][1], L.m[j][2]);
+    if (kind == CMTJ_INTERACTION_KANAO) {
+      // computeDipoleInteraction (:79-96): pref (3 (m . r) r / r^5 - m / r^3)
+      const double dt3 = 3 * (m1.x * rij.x + m1.y * rij.y + m1.z * rij.z);
+      const double r3 = L.pa[j][e], r5 = L.pb[j][e];
+      const V3<double> a = mk<double>(dt3 * rij.x / r5, dt3 * rij.y / r5, dt3 * rij.z / r5);
+      const V3<double> b = mk<double>(m1.x / r3, m1.y / r3, m1.z / r3);
+      H = H + (a - b) * L.pref[j];
+    } else {
+      // computeDipoleInteractionNoumra (:106-119): (T m) (Ms V / mu0) with
+      // T = (r r^T - |r|^2/3) 3 / (4 pi |r|^5) (getDipoleTensorFromRelPositions, :36-56)
+      const double rm3 = L.pa[j][e] / 3, mult = L.pb[j][e];
+      const V3<double> t0 = mk<double>(rij.x * rij.x - rm3, rij.x * rij.y, rij.x * rij.z) * mult;
+      const V3<double> t1 = mk<double>(rij.x * rij.y, rij.y * rij.y - rm3, rij.y * rij.z) * mult;
+      const V3<double> t2 = mk<double>(rij.x * rij.z, rij.y * rij.z, rij.z * rij.z - rm3) * mult;
+      const V3<double> Tm = mk<double>(t0.x * m1.x + t0.y * m1.y + t0.z * m1.z, t1.x * m1.x + t1.y * m1.y + t1.z * m1.z,
+                                       t2.x * m1.x + t2.y * m1.y + t2.z * m1.z);
+      H = H + Tm * L.pref[j];
+    }
+  }
+  return H;
+}
+
+template <class R, int LM>
+__device__ __forceinline__ void add_field_to(DV<R> (&dv)[LM], int layer, V3<double> H) {
+#pragma unroll
+  for (int l = 0; l < LM; ++l)
+    if (l == layer) dv[l].H = dv[l].H + cvt<R>(H);
+}
+
 template <class R, int LM>
 __device__ __forceinline__ void set_current(DV<R> (&dv)[LM], double I) {
 #pragma unroll
   for (int l = 0; l < LM; ++l) dv[l].I = (R)I;
 }
 
-template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false>
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false, bool GRP = false>
 __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  [[maybe_unused]] const cmtj_group* gp = nullptr;
+  [[maybe_unused]] int g_n = 0, g_e = 0, g_top = 0;
+  if constexpr (GRP) {
+    // group blockIdx.x on one wave, element e on lane e; lanes past the
+    // group's elements leave at once (a one-wave barrier needs no lane)
+    if (blockIdx.x >= g.group_n) return;
+    gp = g.group + blockIdx.x;
+    g_n = gp->n_elements;
+    g_e = (int)threadIdx.x;
+    if (g_e >= g_n) return;
+    g_top = gp->top[g_e];
+    lane = blockIdx.x * (uint32_t)g_n + (uint32_t)g_e;
+  }
   [[maybe_unused]] const cmtj_stack* sk = nullptr;
   [[maybe_unused]] int s_j = 0, s_base = 0;
   if constexpr (STK) {
@@ -983,8 +1050,45 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     s_ti = sk->top[s_j];
     s_bi = sk->bottom[s_j];
   }
+  [[maybe_unused]] GroupLds* gl = nullptr;
+  if constexpr (GRP) {
+    __shared__ GroupLds lds;
+    gl = &lds;
+    const cmtj_layer& Ly = g.layers[J->layer_offset + g_top];
+    const double V = Ly.thickness * Ly.cell_surface;
+    gl->pos[g_e][0] = gp->pos[g_e][0];
+    gl->pos[g_e][1] = gp->pos[g_e][1];
+    gl->pos[g_e][2] = gp->pos[g_e][2];
+    gl->pref[g_e] = (gp->interaction == CMTJ_INTERACTION_KANAO) ? (Ly.Ms / g.c.mu0) * V : Ly.Ms * V / g.c.mu0;
+    __syncthreads();
+    for (int j = 0; j < g_n; ++j) {
+      if (j == g_e) continue;
+      const double dx = gl->pos[g_e][0] - gl->pos[j][0], dy = gl->pos[g_e][1] - gl->pos[j][1],
+                   dz = gl->pos[g_e][2] - gl->pos[j][2];
+      const double len = sqrt(dx * dx + dy * dy + dz * dz);  // CVector::length
+      if (gp->interaction == CMTJ_INTERACTION_KANAO) {
+        gl->pa[j][g_e] = pow(len, 3);
+        gl->pb[j][g_e] = pow(len, 5);
+      } else {
+        gl->pa[j][g_e] = pow(len, 2);
+        gl->pb[j][g_e] = 3 / (4 * M_PI * pow(len, 5));
+      }
+    }
+  }
   for (; i < iend; ++i) {
     const double t = (double)i * dt;
+    [[maybe_unused]] V3<double> gH = mk<double>(0, 0, 0);
+    if constexpr (GRP) {
+      V3<double> mt = m[0];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) mt = (l == g_top) ? m[l] : mt;
+      __syncthreads();  // the previous step's reads are done
+      gl->m[g_e][0] = mt.x;
+      gl->m[g_e][1] = mt.y;
+      gl->m[g_e][2] = mt.z;
+      __syncthreads();
+      gH = group_field(*gl, gp->interaction, g_n, g_e);
+    }
     [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
     [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
     if constexpr (STK) stack_current<LM>(sk, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, sI, sIlog, sClog);
@@ -1069,6 +1173,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       }
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t, i, 0, sps);
       if constexpr (STK) set_current<R, LM>(dv, sI[0]);
+      if constexpr (GRP) add_field_to<R, LM>(dv, g_top, gH);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -1079,6 +1184,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       }
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t + 0.5 * dt, i, 1, sps);
       if constexpr (STK) set_current<R, LM>(dv, sI[1]);
+      if constexpr (GRP) add_field_to<R, LM>(dv, g_top, gH);
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
@@ -1097,6 +1203,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       }
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t + dt, i, 2, sps);
       if constexpr (STK) set_current<R, LM>(dv, sI[2]);
+      if constexpr (GRP) add_field_to<R, LM>(dv, g_top, gH);
       V3<double> mn[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
@@ -1119,7 +1226,14 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
           if (isnan(mn[l].x)) nan_layer = l;
         }
       }
-      if constexpr (STK) {
+      if constexpr (GRP) {
+        // a NaN ends the group's run (the reference throws): the group is
+        // one wave, its lanes leave the loop together
+        if (__any(nan_layer >= 0)) {
+          if (nan_layer >= 0) code = CMTJ_ERR_NAN;
+          break;
+        }
+      } else if constexpr (STK) {
         // a member's NaN ends its stack's run (the reference throws): every
         // member of that stack stops at this step, the lanes of the stack
         // leave the loop together, other stacks go on
@@ -1141,6 +1255,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       for (int l = 0; l < LM; ++l) mr_[l] = cvt<R>(m[l]);
       eval_drivers<R, LM, FM, LOGE>(dv, base, dr, fw, g, lane, fl, t, i, 0, sps);
       if constexpr (STK) set_current<R, LM>(dv, sI[0]);
+      if constexpr (GRP) add_field_to<R, LM>(dv, g_top, gH);
       V3<double> mn[LM];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
@@ -1252,9 +1367,9 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
 
 // One instance per lane.  (A variant compiled for >= 4 waves per SIMD spilled
 // 88-300 VGPRs and ran 1.6-5x slower on full-chip batches; not kept.)
-template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false>
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false, bool GRP = false>
 __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
-  junction_body<R, LM, SOLVER, FM, LOGE, STK>(g);
+  junction_body<R, LM, SOLVER, FM, LOGE, STK, GRP>(g);
 }
 
 }  // namespace cmtj

@@ -45,6 +45,7 @@ KTable kernels_exact();
 KTable kernels_cl();
 KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stacks only
 KTable kernels_dp();     // Dormand-Prince (single-layer, FP64)
+KTable kernels_group();  // dipole-coupled groups: chosen by cmtj_run_groups only
 
 }  // namespace cmtj
 

@@ -20,9 +20,10 @@
 extern "C" {
 #endif
 
-#define CMTJ_ABI_VERSION 4
+#define CMTJ_ABI_VERSION 5
 #define CMTJ_MAX_LAYERS 8
 #define CMTJ_MAX_STACK 64 /* junctions of one coupled stack (one wave) */
+#define CMTJ_MAX_GROUP 64 /* elements of one dipole-coupled group (one workgroup) */
 
 /* ScalarDriver waveform kinds; numbering of reference UpdateType
  * (core/drivers.hpp:18-29).  CMTJ_TABLE is this library's tabulated form of
@@ -237,6 +238,7 @@ typedef struct cmtj_abi_info {
   int32_t sizeof_status;
   int32_t sizeof_state;
   int32_t sizeof_stack;
+  int32_t sizeof_group;
   int32_t offsetof_layer_drv;
   int32_t offsetof_layer_ref_type;
   int32_t offsetof_junction_dt;
@@ -319,6 +321,39 @@ typedef struct cmtj_stack {
  * replaces Stack<T>::runSimulation (:296-300); a batch replaces a loop of
  * them over a parameter sweep. */
 int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_junction* js,
+                    const cmtj_layer* layers, size_t n_layer_rows, const cmtj_constants* c,
+                    double* log_host, size_t log_cap, double* mag_out_host, cmtj_status* st_host);
+
+/* ---- dipole-coupled groups (FP64, host buffers) ------------------------- */
+
+enum cmtj_interaction {
+  CMTJ_INTERACTION_NULL = 0,   /* nullDipoleInteraction (core/reservoir.hpp:63-68) */
+  CMTJ_INTERACTION_KANAO = 1,  /* computeDipoleInteraction (:79-96) */
+  CMTJ_INTERACTION_NOMURA = 2  /* computeDipoleInteractionNoumra (:106-119) */
+};
+
+/* A GroupInteraction (core/reservoir.hpp:121-230): junctions at fixed
+ * positions whose top layers act on each other.  Each step, with the top
+ * layers' magnetisations at the start of the step, element i's top layer
+ * gets the reserved interaction field sum_{j != i} f(r_i, r_j, layer_i,
+ * layer_j), j ascending (stepFunctionalSolver, :126-147), then every element
+ * takes one step of its solver. */
+typedef struct cmtj_group {
+  int32_t n_elements;              /* 1..CMTJ_MAX_GROUP */
+  int32_t interaction;             /* enum cmtj_interaction */
+  int32_t top[CMTJ_MAX_GROUP];     /* layer index of topId in element j */
+  double pos[CMTJ_MAX_GROUP][3];   /* coordinates (m) */
+} cmtj_group;
+
+/* Runs n_groups groups of the same element count in one launch, group g on
+ * workgroup g (element j on lane j).  js[g * elements + j] is element j of
+ * group g: one structure, the solver (RK4 or Euler-Heun), n_steps,
+ * write_every and dt shared; the top layer's reserved-interaction drivers
+ * are replaced by the group field.  js[q].log_offset places element q's
+ * cmtj_run-style log block; mag_out_host [n][8][3], st_host [n].  A NaN in
+ * an element stops its group at that step.  One group replaces
+ * GroupInteraction::runSimulation (:181-221); a batch, a loop of them. */
+int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_junction* js,
                     const cmtj_layer* layers, size_t n_layer_rows, const cmtj_constants* c,
                     double* log_host, size_t log_cap, double* mag_out_host, cmtj_status* st_host);
 

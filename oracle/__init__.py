@@ -53,6 +53,7 @@ def _load():
                                                ctypes.c_uint32, P]
         lib.oracle_n_channels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.oracle_run_stack.argtypes = [P, P, P, P, P, P, P]
+        lib.oracle_run_group.argtypes = [P, P, P, P, P, P, P]
         _lib = (lib, abi)
     return _lib
 
@@ -114,6 +115,33 @@ def run_stack(stacks, js, layers, consts):
                                   _p(mag[s * n:(s + 1) * n]), _p(st[s * n:(s + 1) * n]))
         if rc:
             raise ValueError("oracle_run_stack: bad stack descriptor")
+    return log[:need], mag, st
+
+
+def run_group(groups, js, layers, consts):
+    """Dipole-coupled groups, same layout as cmtj_run_groups (js[g * elements
+    + e]: element e of group g), one after another: returns (log, mag [n, 8,
+    3], status [n])."""
+    lib, abi = _load()
+    groups = np.ascontiguousarray(groups, dtype=abi.GROUP_DTYPE).reshape(-1)
+    js = np.ascontiguousarray(js, dtype=abi.JUNCTION_DTYPE)
+    layers = np.ascontiguousarray(layers, dtype=abi.LAYER_DTYPE)
+    need = 0
+    for q in range(js.size):
+        nch = lib.oracle_n_channels(int(js[q]["n_layers"]), int(js[q]["mr_mode"]), int(js[q]["log_mask"]))
+        need = max(need, int(js[q]["log_offset"]) + nch * int(js[q]["n_entries"]))
+    log = np.zeros(max(need, 1), dtype=np.float64)
+    mag = np.zeros((js.size, abi.MAX_LAYERS, 3), dtype=np.float64)
+    st = np.zeros(js.size, dtype=abi.STATUS_DTYPE)
+    consts = np.ascontiguousarray(consts)
+    n = int(groups[0]["n_elements"]) if groups.size else 0
+    for g in range(groups.size):
+        rc = lib.oracle_run_group(_p(groups[g:g + 1]), _p(js[g * n:(g + 1) * n]), _p(layers), _p(consts), _p(log),
+                                  _p(mag[g * n:(g + 1) * n]), _p(st[g * n:(g + 1) * n]))
+        if rc == 2:
+            raise RuntimeError("Points are too close for stable dipole calculation")
+        if rc:
+            raise ValueError("oracle_run_group: bad group descriptor")
     return log[:need], mag, st
 
 

@@ -729,6 +729,122 @@ int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_la
   return 0;
 }
 
+/* ---- dipole-coupled groups (core/reservoir.hpp) ------------------------ */
+
+/* computeDipoleInteraction, core/reservoir.hpp:79-96 (Kanao et al.): the
+ * field at r1 of the moment of layer2 at r2.  Returns 1 when the points are
+ * too close (the reference throws). */
+static int dipole_kanao(vec3 r1, vec3 r2, const cmtj_layer* layer2, vec3 m1, double mu0, vec3* out) {
+  const vec3 rij = vsub(r1, r2);
+  if (vlength(rij) < 1e-10) return 1;
+  const double r3 = pow(vlength(rij), 3);
+  const double r5 = pow(vlength(rij), 5);
+  const double V = layer2->thickness * layer2->cell_surface;
+  const double prefactor = (layer2->Ms / mu0) * V;
+  *out = smul(prefactor, vsub(vdiv(smul(3 * vdot(m1, rij), rij), r5), vdiv(m1, r3)));
+  return 0;
+}
+
+/* computeDipoleInteractionNoumra, :106-119 (Nomura et al.), with the tensor
+ * of getDipoleTensorFromRelPositions (:36-56) applied by
+ * calculate_tensor_interaction (core/junction.hpp:64-72). */
+static int dipole_nomura(vec3 r1, vec3 r2, const cmtj_layer* layer2, vec3 m1, double mu0, vec3* out) {
+  const vec3 rij = vsub(r1, r2);
+  if (vlength(rij) < 1e-14) return 1;
+  const double r_mag = pow(vlength(rij), 2);
+  const double mult = 3 / (4 * M_PI * pow(vlength(rij), 5));
+  double T[9];
+  const vec3 t0 = vmul(v3(pow(rij.x, 2) - (r_mag / 3), rij.x * rij.y, rij.x * rij.z), mult);
+  const vec3 t1 = vmul(v3(rij.x * rij.y, pow(rij.y, 2) - (r_mag / 3), rij.y * rij.z), mult);
+  const vec3 t2 = vmul(v3(rij.x * rij.z, rij.y * rij.z, pow(rij.z, 2) - (r_mag / 3)), mult);
+  T[0] = t0.x, T[1] = t0.y, T[2] = t0.z;
+  T[3] = t1.x, T[4] = t1.y, T[5] = t1.z;
+  T[6] = t2.x, T[7] = t2.y, T[8] = t2.z;
+  const double V = layer2->thickness * layer2->cell_surface;
+  *out = tensor_interaction(m1, T, layer2->Ms * V, mu0);
+  return 0;
+}
+
+/* GroupInteraction::runSimulation (core/reservoir.hpp:181-221): each step,
+ * with the top layers' magnetisations at its start, element i's top layer
+ * gets the reserved interaction field sum_{j != i} f(r_i, r_j) (j
+ * ascending, stepFunctionalSolver :126-147, AxialDriver of constants), then
+ * the elements step one after another; entries every write_every steps.
+ * Same descriptors and log layout as cmtj_run_groups for one group; mag_out
+ * [n][CMTJ_MAX_LAYERS][3], st [n].  Returns 0, 1 for a bad descriptor, 2
+ * when two points are too close. */
+int oracle_run_group(const cmtj_group* g, const cmtj_junction* js, const cmtj_layer* layers,
+                     const cmtj_constants* c, double* log, double* mag_out, cmtj_status* st) {
+  const int n = g->n_elements;
+  if (n < 1 || n > CMTJ_MAX_GROUP) return 1;
+  cmtj_layer* own = (cmtj_layer*)malloc(sizeof(cmtj_layer) * CMTJ_MAX_LAYERS * (size_t)n);
+  olayer* Ls = (olayer*)malloc(sizeof(olayer) * CMTJ_MAX_LAYERS * (size_t)n);
+  if (!own || !Ls) {
+    free(own);
+    free(Ls);
+    return 1;
+  }
+  for (int q = 0; q < n; ++q) {
+    const cmtj_junction* j = &js[q];
+    memcpy(&own[q * CMTJ_MAX_LAYERS], layers + j->layer_offset, sizeof(cmtj_layer) * (size_t)j->n_layers);
+    load_layers(j, &own[q * CMTJ_MAX_LAYERS], c, NULL, &Ls[q * CMTJ_MAX_LAYERS]);
+    st[q].code = CMTJ_OK;
+    st[q].steps_done = 0;
+    st[q].entries_done = 0;
+    st[q].nan_layer = -1;
+  }
+  const cmtj_junction* j0 = &js[0];
+  uint32_t k = 0;
+  int stop = 0, rc = 0;
+  for (uint32_t i = 0; i < j0->n_steps && !stop; ++i) {
+    vec3 H[CMTJ_MAX_GROUP];
+    for (int a = 0; a < n && !rc; ++a) {
+      const vec3 ra = vload(g->pos[a]);
+      H[a] = v3(0, 0, 0);
+      for (int b = 0; b < n && !rc; ++b) {
+        if (a == b) continue;
+        const cmtj_layer* lb = &own[b * CMTJ_MAX_LAYERS + g->top[b]];
+        const vec3 mb = Ls[b * CMTJ_MAX_LAYERS + g->top[b]].mag;
+        vec3 f = v3(0, 0, 0); /* nullDipoleInteraction, :63-68 */
+        if (g->interaction == CMTJ_INTERACTION_KANAO)
+          rc = dipole_kanao(ra, vload(g->pos[b]), lb, mb, c->mu0, &f);
+        else if (g->interaction == CMTJ_INTERACTION_NOMURA)
+          rc = dipole_nomura(ra, vload(g->pos[b]), lb, mb, c->mu0, &f);
+        H[a] = vadd(H[a], f);
+      }
+    }
+    if (rc) {
+      rc = 2;
+      break;
+    }
+    for (int q = 0; q < n; ++q) {
+      cmtj_driver* d = own[q * CMTJ_MAX_LAYERS + g->top[q]].drv + CMTJ_SLOT_HRES;
+      const double hv[3] = {H[q].x, H[q].y, H[q].z};
+      for (int s = 0; s < 3; ++s) {
+        memset(&d[s], 0, sizeof(cmtj_driver));
+        d[s].type = CMTJ_CONSTANT;
+        d[s].constant = hv[s];
+      }
+    }
+    for (int q = 0; q < n && !stop; ++q)
+      if (junction_step(&js[q], &Ls[q * CMTJ_MAX_LAYERS], i, &st[q])) stop = 1;
+    if (stop) break;
+    for (int q = 0; q < n; ++q) st[q].steps_done = i + 1;
+    if (i % j0->write_every == 0) { /* logLayerParams of every element, :215-218 */
+      for (int q = 0; q < n; ++q) {
+        log_entry(&js[q], &Ls[q * CMTJ_MAX_LAYERS], log + js[q].log_offset, k);
+        st[q].entries_done = k + 1;
+      }
+      ++k;
+    }
+  }
+  for (int q = 0; q < n; ++q)
+    store_mag(&js[q], &Ls[q * CMTJ_MAX_LAYERS], mag_out + (size_t)q * 3 * CMTJ_MAX_LAYERS);
+  free(own);
+  free(Ls);
+  return rc;
+}
+
 /* ---- threaded batch (CPU baseline) ------------------------------------ */
 
 typedef struct {

@@ -45,6 +45,14 @@ int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js,
                      const cmtj_layer* layers, const cmtj_constants* c,
                      double* log, double* mag_out, cmtj_status* st);
 
+/* A dipole-coupled group (core/reservoir.hpp GroupInteraction), same
+ * descriptors and log layout as cmtj_run_groups for one group; mag_out
+ * [n][CMTJ_MAX_LAYERS][3], st [n].  Returns 0, 1 (bad descriptor) or 2
+ * (two points too close for the interaction). */
+int oracle_run_group(const cmtj_group* g, const cmtj_junction* js,
+                     const cmtj_layer* layers, const cmtj_constants* c,
+                     double* log, double* mag_out, cmtj_status* st);
+
 /* Channels of an instance for a log mask (same rule as the engine). */
 int oracle_n_channels(int n_layers, int mr_mode, int log_mask);
 

@@ -23,3 +23,11 @@ class OracleSeriesStack(SeriesStack):
 class OracleParallelStack(ParallelStack):
     def _execute(self, stack, js, layers, consts):
         return oracle.run_stack(stack, js, layers, consts)
+
+
+from cmtj_amd.reservoir import GroupInteraction  # noqa: E402
+
+
+class OracleGroupInteraction(GroupInteraction):
+    def _execute(self, g, js, layers, consts):
+        return oracle.run_group(g, js, layers, consts)

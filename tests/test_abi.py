@@ -35,6 +35,7 @@ def test_struct_layout_matches_binding():
     assert _lib.LAYER_DTYPE.itemsize == 3096
     assert _lib.JUNCTION_DTYPE.itemsize == 544
     assert _lib.STACK_DTYPE.itemsize == 1160
+    assert _lib.GROUP_DTYPE.itemsize == 1800
 
 
 def test_host_entry_points_need_no_gpu():

@@ -1,0 +1,115 @@
+"""Dipole-coupled groups on the GPU (cmtj_run_groups: one group per
+workgroup, the group field built in LDS every step) against the oracle's
+group runner: Kanao and Nomura coupling, one- and two-layer elements, the
+thermal Euler-Heun path, a full 64-element group, many groups in one launch
+and a NaN that stops its group only."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cmtj_amd import AxialDriver, CVector, Junction, constantDriver, stepDriver
+from cmtj_amd import reservoir
+from cmtj_amd.reservoir import GroupInteraction, run_groups
+from oracle_backend import OracleGroupInteraction
+from test_group_cpu import POS, _elements
+
+pytestmark = pytest.mark.gpu
+
+_FN = {"kanao": reservoir.computeDipoleInteraction, "nomura": reservoir.computeDipoleInteractionNoumra,
+       "null": reservoir.nullDipoleInteraction}
+
+
+def _compare(a, b, n):
+    worst = 0.0
+    for q in range(n):
+        la, lb = a.getLog(q), b.getLog(q)
+        assert la.keys() == lb.keys()
+        for k in la:
+            x, y = np.asarray(la[k]), np.asarray(lb[k])
+            assert x.shape == y.shape, k
+            scale = max(1.0, float(np.max(np.abs(y)))) if y.size else 1.0
+            worst = max(worst, float(np.max(np.abs(x - y))) / scale if y.size else 0.0)
+    return worst
+
+
+def _grid(n, pitch=55e-9):
+    side = int(math.ceil(math.sqrt(n)))
+    return [CVector((q % side) * pitch, (q // side) * pitch, 0.0) for q in range(n)]
+
+
+@pytest.mark.parametrize("kind", ["kanao", "nomura", "null"])
+@pytest.mark.parametrize("two", [False, True])
+def test_group_matches_oracle(kind, two):
+    js = _elements(Junction, 4, two)
+    pos = [CVector(*p) for p in POS[:4]]
+    g, o = GroupInteraction(pos, js), OracleGroupInteraction(pos, js)
+    for x in (g, o):
+        x.setInteractionFunction(_FN[kind])
+        x.runSimulation(0.3e-9, 1e-13, 1e-12)
+    worst = _compare(g, o, 4)
+    print(f"[group] {kind} two_layer={two}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+    for q in range(4):
+        np.testing.assert_allclose(np.array(g.junctionList[q].getLayerMagnetisation("free").tolist()),
+                                   np.array(o.junctionList[q].getLayerMagnetisation("free").tolist()), atol=1e-8)
+
+
+def test_thermal_group_matches_oracle_pathwise():
+    """Elements at 300 K step with Euler-Heun; the engine and the oracle share
+    the Philox stream, so the paths agree."""
+    js = _elements(Junction, 3)
+    for q, j in enumerate(js):
+        j.setLayerTemperatureDriver("all", constantDriver(300.0))
+        j.setNoiseSeed(11 + q)
+    pos = [CVector(*p) for p in POS[:3]]
+    g, o = GroupInteraction(pos, js), OracleGroupInteraction(pos, js)
+    for x in (g, o):
+        x.runSimulation(0.2e-9, 1e-13, 1e-12)
+    worst = _compare(g, o, 3)
+    print(f"[group] Euler-Heun 300 K: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_full_group_of_64():
+    js = _elements(Junction, 64)
+    pos = _grid(64)
+    g, o = GroupInteraction(pos, js), OracleGroupInteraction(pos, js)
+    for x in (g, o):
+        x.runSimulation(0.1e-9, 1e-13, 1e-12)
+    worst = _compare(g, o, 64)
+    print(f"[group] 64 elements: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_nan_element_stops_its_group_only():
+    """Many 5-element groups in one launch (one workgroup each); an element
+    whose field steps to infinity ends its group's run at that step."""
+    def build(k, J, G, bad=False):
+        js = _elements(J, 5)
+        for q, j in enumerate(js):
+            amp = float("inf") if (bad and q == 2) else 1e3 * (k % 4)
+            j.setLayerExternalFieldDriver("all", AxialDriver(stepDriver(0, amp, 5e-12, 1e-9), constantDriver(0),
+                                                             constantDriver(5e3)))
+        grp = G(_grid(5, pitch=50e-9 + 2e-9 * k), js)
+        grp.setInteractionFunction(_FN["nomura" if k % 2 else "kanao"])
+        return grp
+
+    K = 24
+    g = [build(k, Junction, GroupInteraction, bad=(k == 5)) for k in range(K)]
+    o = [build(k, Junction, OracleGroupInteraction, bad=(k == 5)) for k in range(K)]
+    with pytest.raises(RuntimeError, match="NAN magnetisation"):
+        run_groups(g, 0.2e-9, 1e-13, 1e-12)
+    worst = 0.0
+    for k, b in enumerate(o):
+        if k == 5:
+            with pytest.raises(RuntimeError, match="NAN magnetisation"):
+                b.runSimulation(0.2e-9, 1e-13, 1e-12)
+        else:
+            b.runSimulation(0.2e-9, 1e-13, 1e-12)
+        worst = max(worst, _compare(g[k], b, 5))
+    print(f"[group] {K} groups in one launch: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+    assert 0 < len(g[5].getLog(0)["time"]) < len(g[4].getLog(0)["time"])
