@@ -242,6 +242,101 @@ def _bench_stack(n_members: int = 8, total_time: float = 10e-9) -> dict:
             "what": "host-to-host Stack.runSimulation / run_stacks (packing, upload, kernel, log copy back)"}
 
 
+def _bench_group(side: int = 8, total_time: float = 2e-9, n_sweep: int = 1024) -> dict:
+    """Dipole-coupled reservoir: a side x side square array (pitch 60 nm) of
+    perpendicular single-layer junctions with Kanao point-dipole coupling,
+    each in its own in-plane field; RK4 dt 1e-13, one entry per ps.  One
+    group on the GPU (one workgroup) against the oracle's group runner on one
+    host core (a sample of the same run), then n_sweep groups (an input-field
+    sweep, as a reservoir's readout training would run) in one launch."""
+    import math
+
+    import numpy as np
+
+    import oracle
+    from cmtj_amd import AxialDriver, CVector, Junction, Layer, constantDriver
+    from cmtj_amd.reservoir import GroupInteraction, run_groups
+
+    n = side * side
+    demag = [CVector(0.04, 0, 0), CVector(0, 0.04, 0), CVector(0, 0, 0.92)]
+    pos = [CVector((q % side) * 60e-9, (q // side) * 60e-9, 0.0) for q in range(n)]
+
+    def elements(h0):
+        js = []
+        for q in range(n):
+            th = 0.2 + 0.01 * q
+            l = Layer("free", mag=CVector(math.sin(th), 0.0, math.cos(th)), anis=CVector(0, 0, 1), Ms=1.2,
+                      thickness=1.5e-9, cellSurface=math.pi * (25e-9) ** 2, demagTensor=demag, damping=0.01)
+            j = Junction([l])
+            j.setLayerAnisotropyDriver("free", constantDriver(8e5))
+            j.setLayerExternalFieldDriver("all", AxialDriver(h0 + 500.0 * q, 0.0, 0.0))
+            js.append(j)
+        return js
+
+    class _OracleGroup(GroupInteraction):
+        def _execute(self, g, js, layers, consts):
+            return oracle.run_group(g, js, layers, consts)
+
+    base = elements(2e4)
+    out = {}
+    sample = total_time / 20
+    for name, cls, T in (("gpu", GroupInteraction, total_time), ("cpu_oracle", _OracleGroup, sample)):
+        g = cls(pos, base)
+        g.runSimulation(1e-12, 1e-13, 1e-12)  # warm-up (code objects, buffers)
+        g = cls(pos, base)
+        t0 = time.perf_counter()
+        g.runSimulation(T, 1e-13, 1e-12)
+        out[name] = (time.perf_counter() - t0, g, n * int(T / 1e-13))
+    # the GPU run against the oracle over the sample's span
+    ref = out["cpu_oracle"][1]
+    gpu_g = out["gpu"][1]
+    k = len(ref.getLog(0)["time"])
+    dm = max(float(np.max(np.abs(np.asarray(gpu_g.getLog(q)["free_mx"][:k]) -
+                                 np.asarray(ref.getLog(q)["free_mx"])))) for q in range(n))
+    groups = []
+    for k_ in range(n_sweep):
+        grp = GroupInteraction(pos, elements(1e4 + 40.0 * k_))
+        groups.append(grp)
+    run_groups(groups[:4], 1e-12, 1e-13, 1e-12)  # warm-up
+    for grp in groups[:4]:
+        grp.clearLogs()
+    sweep_T = 1e-9
+    t0 = time.perf_counter()
+    run_groups(groups, sweep_T, 1e-13, 1e-12)
+    sweep_s = time.perf_counter() - t0
+    sweep_steps = n_sweep * n * int(sweep_T / 1e-13)
+    # the engine call alone on the packed sweep (upload, kernel, log copy back)
+    from cmtj_amd import _lib
+
+    packs = [grp._pack(sweep_T, 1e-13, 1e-12) for grp in groups]
+    recs = np.stack([p_[0] for p_ in packs])
+    js_all, rows_all, rows, off = [], [], 0, 0
+    for p_ in packs:
+        jq = p_[1].copy()
+        jq["layer_offset"] += rows
+        jq["log_offset"] += off
+        rows += len(p_[2])
+        off += p_[6]
+        js_all.append(jq)
+        rows_all.append(p_[2])
+    js_all, rows_all = np.concatenate(js_all), np.concatenate(rows_all)
+    consts = _lib.constants_get()
+    t0 = time.perf_counter()
+    _lib.run_groups(recs, js_all, rows_all, consts)
+    engine_s = time.perf_counter() - t0
+    g_s, o_s = out["gpu"][0], out["cpu_oracle"][0]
+    return {"workload": f"dipole_reservoir_{side}x{side}_kanao", "elements": n, "gpu_total_time": total_time,
+            "gpu_junction_steps": out["gpu"][2], "gpu_s": g_s, "gpu_junction_steps_per_s": out["gpu"][2] / g_s,
+            "cpu_oracle_total_time": sample, "cpu_oracle_junction_steps": out["cpu_oracle"][2],
+            "cpu_oracle_s": o_s, "cpu_oracle_junction_steps_per_s": out["cpu_oracle"][2] / o_s, "cpu_threads": 1,
+            "max_dm_gpu_vs_oracle": dm,
+            "sweep": {"groups": n_sweep, "elements": n, "total_time": sweep_T, "junction_steps": sweep_steps,
+                      "s": sweep_s, "junction_steps_per_s": sweep_steps / sweep_s, "engine_s": engine_s,
+                      "engine_junction_steps_per_s": sweep_steps / engine_s},
+            "what": "host-to-host GroupInteraction.runSimulation / run_groups (packing, upload, kernel, log copy "
+                    "back); engine_s: cmtj_run_groups on the packed sweep"}
+
+
 def _bench_dp(n_fields: int = 64, n_currents: int = 64, total_time: float = 3e-9) -> dict:
     """Dormand-Prince stability-diagram sweep: single-layer SOT junctions
     (perpendicular, second-order anisotropy, 1 ns field- and damping-like
@@ -393,7 +488,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving", "stack", "dp"],
+    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving", "stack", "dp", "group"],
                     help="pimm: the headline sweep (default); rh, cims, sto: the other sweep workloads, "
                          "one JSON line each")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -416,6 +511,7 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.workload != "pimm":
         fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto, "stack": _bench_stack, "dp": _bench_dp,
+              "group": _bench_group,
               "serving": lambda: _serving(args.serving_qps, args.serving_seconds, lanes=args.serving_lanes)}[args.workload]
         print(json.dumps(fn()))
         return 0

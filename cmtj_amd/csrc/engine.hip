@@ -810,16 +810,6 @@ int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_juncti
     need = std::max<size_t>(need, j.log_offset + nch * j.n_entries);
   }
   if (need > 0 && (!log_host || log_cap < need)) return fail(CMTJ_ERR_INVALID, "log buffer too small");
-  const KTable kt = kernels_group();
-  int maxl = 1;
-  for (size_t q = 0; q < nj; ++q) maxl = std::max(maxl, (int)js[q].n_layers);
-  const KEntry* ke = nullptr;
-  for (size_t q = 0; q < kt.n; ++q)
-    if (kt.e[q].solver == j0.solver && kt.e[q].lm >= maxl &&
-        (!ke || kt.e[q].lm < ke->lm))
-      ke = &kt.e[q];
-  if (!ke) return fail(CMTJ_ERR_INVALID, "no group kernel for this solver and layer count");
-
   std::lock_guard<std::mutex> lk(g_plain.mu);
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -840,6 +830,16 @@ int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_juncti
   if (P.groups.size() != 1)
     return fail(CMTJ_ERR_INVALID,
                 "the junctions of a group must share one structure (layers, modes, driver kinds)");
+  // the smallest group kernel that covers the structure
+  const KTable kt = kernels_group();
+  const KEntry* ke = nullptr;
+  for (size_t q = 0; q < kt.n; ++q) {
+    const KEntry& e = kt.e[q];
+    if (e.solver == j0.solver && e.lm >= P.groups[0].a.nl &&
+        covers(e, P.groups[0].a.flags, 0, P.groups[0].a.ndyn) && (!ke || rank(e) < rank(*ke)))
+      ke = &e;
+  }
+  if (!ke) return fail(CMTJ_ERR_INVALID, "no group kernel for this solver and layer count");
   const size_t o_mag = align_up(need * sizeof(double));
   const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj);
   const size_t o_grp = align_up(o_st + sizeof(cmtj_status) * nj);

@@ -9,6 +9,12 @@
 namespace cmtj {
 constexpr uint32_t FM_GROUP = F_ALL;
 static const KEntry kG[] = {
+    // the usual reservoir element: anisotropy, demagnetisation, fields
+    {CMTJ_FP64, 1, 1, CMTJ_RK4, FM_LEAN, false,
+     (const void*)&k_junction<double, 1, CMTJ_RK4, FM_LEAN, false, false, true>, "k_junction<double,1,RK4,lean,group>"},
+    {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_LEAN, false,
+     (const void*)&k_junction<double, 2, CMTJ_RK4, FM_LEAN, false, false, true>, "k_junction<double,2,RK4,lean,group>"},
+    // every feature
     {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_GROUP, false,
      (const void*)&k_junction<double, 2, CMTJ_RK4, FM_GROUP, false, false, true>, "k_junction<double,2,RK4,group>"},
     {CMTJ_FP64, 4, 1, CMTJ_RK4, FM_GROUP, false,

@@ -883,46 +883,39 @@ __device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n
 }
 
 // A dipole-coupled group in LDS: the top-layer magnetisations of the step,
-// positions, source prefactors and the pair constants of (source j, element
-// e) -- Kanao: pa = |r|^3, pb = |r|^5; Nomura: pa = |r|^2, pb = 3 / (4 pi
-// |r|^5).  pa/pb are [j][e] so the lanes of one source read consecutive
-// words; m, pos and pref of a source are one broadcast read.
+// positions, source strengths s and the inverse distances il[j][e] = 1/|r_e
+// - r_j| (0 on the diagonal, so a lane's own term vanishes without a
+// branch).  il is [j][e]: the lanes of one source read consecutive words;
+// m, pos and s of a source are broadcast reads.  36 KB at 64 elements, so
+// four groups share a CU.
 struct GroupLds {
   double m[CMTJ_MAX_GROUP][3];
   double pos[CMTJ_MAX_GROUP][3];
-  double pref[CMTJ_MAX_GROUP];
-  double pa[CMTJ_MAX_GROUP][CMTJ_MAX_GROUP];
-  double pb[CMTJ_MAX_GROUP][CMTJ_MAX_GROUP];
+  double s[CMTJ_MAX_GROUP];
+  double il[CMTJ_MAX_GROUP][CMTJ_MAX_GROUP];
 };
 
 // Field of the group on element e from the top-layer magnetisations in LDS
-// (stepFunctionalSolver, core/reservoir.hpp:126-147: j ascending, j != e).
-__device__ __forceinline__ V3<double> group_field(const GroupLds& L, int kind, int n, int e) {
+// (stepFunctionalSolver, core/reservoir.hpp:126-147, sources j ascending):
+//   sum_j s_j (3 (m_j . r) r / |r|^5 - m_j / |r|^3),  r = r_e - r_j,
+// which is computeDipoleInteraction (:79-96) with s_j = (Ms_j / mu0) V_j,
+// and computeDipoleInteractionNoumra (:106-119) with s_j = Ms_j V_j / mu0 /
+// (4 pi): its tensor (r r^T - |r|^2/3) 3 / (4 pi |r|^5) applied to m_j is
+// the same point-dipole field over 4 pi.  Powers of the stored 1/|r| replace
+// the reference's pow() and divisions (a few ulp apart).
+__device__ __forceinline__ V3<double> group_field(const GroupLds& L, int n, int e) {
   V3<double> H = mk<double>(0, 0, 0);
-  if (kind == CMTJ_INTERACTION_NULL) return H;
   const V3<double> ri = mk<double>(L.pos[e][0], L.pos[e][1], L.pos[e][2]);
+#pragma unroll 4
   for (int j = 0; j < n; ++j) {
-    if (j == e) continue;
-    const V3<double> rij = ri - mk<double>(L.pos[j][0], L.pos[j][1], L.pos[j][2]);
-    const V3<double> m1 = mk<double>(L.m[j][0], L.m[j][1], L.m[j][2]);
-    if (kind == CMTJ_INTERACTION_KANAO) {
-      // computeDipoleInteraction (:79-96): pref (3 (m . r) r / r^5 - m / r^3)
-      const double dt3 = 3 * (m1.x * rij.x + m1.y * rij.y + m1.z * rij.z);
-      const double r3 = L.pa[j][e], r5 = L.pb[j][e];
-      const V3<double> a = mk<double>(dt3 * rij.x / r5, dt3 * rij.y / r5, dt3 * rij.z / r5);
-      const V3<double> b = mk<double>(m1.x / r3, m1.y / r3, m1.z / r3);
-      H = H + (a - b) * L.pref[j];
-    } else {
-      // computeDipoleInteractionNoumra (:106-119): (T m) (Ms V / mu0) with
-      // T = (r r^T - |r|^2/3) 3 / (4 pi |r|^5) (getDipoleTensorFromRelPositions, :36-56)
-      const double rm3 = L.pa[j][e] / 3, mult = L.pb[j][e];
-      const V3<double> t0 = mk<double>(rij.x * rij.x - rm3, rij.x * rij.y, rij.x * rij.z) * mult;
-      const V3<double> t1 = mk<double>(rij.x * rij.y, rij.y * rij.y - rm3, rij.y * rij.z) * mult;
-      const V3<double> t2 = mk<double>(rij.x * rij.z, rij.y * rij.z, rij.z * rij.z - rm3) * mult;
-      const V3<double> Tm = mk<double>(t0.x * m1.x + t0.y * m1.y + t0.z * m1.z, t1.x * m1.x + t1.y * m1.y + t1.z * m1.z,
-                                       t2.x * m1.x + t2.y * m1.y + t2.z * m1.z);
-      H = H + Tm * L.pref[j];
-    }
+    const V3<double> r = ri - mk<double>(L.pos[j][0], L.pos[j][1], L.pos[j][2]);
+    const V3<double> mj = mk<double>(L.m[j][0], L.m[j][1], L.m[j][2]);
+    const double il = L.il[j][e], il2 = il * il;
+    const double b = L.s[j] * (il2 * il);                                          // s / |r|^3
+    const double a = 3 * (mj.x * r.x + mj.y * r.y + mj.z * r.z) * (b * il2);      // 3 s (m . r) / |r|^5
+    H.x = fma(a, r.x, fma(-b, mj.x, H.x));
+    H.y = fma(a, r.y, fma(-b, mj.y, H.y));
+    H.z = fma(a, r.z, fma(-b, mj.z, H.z));
   }
   return H;
 }
@@ -1059,20 +1052,16 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     gl->pos[g_e][0] = gp->pos[g_e][0];
     gl->pos[g_e][1] = gp->pos[g_e][1];
     gl->pos[g_e][2] = gp->pos[g_e][2];
-    gl->pref[g_e] = (gp->interaction == CMTJ_INTERACTION_KANAO) ? (Ly.Ms / g.c.mu0) * V : Ly.Ms * V / g.c.mu0;
+    const int kind = gp->interaction;
+    gl->s[g_e] = (kind == CMTJ_INTERACTION_KANAO)    ? (Ly.Ms / g.c.mu0) * V
+                 : (kind == CMTJ_INTERACTION_NOMURA) ? Ly.Ms * V / g.c.mu0 / (4 * M_PI)
+                                                     : 0.0;  // nullDipoleInteraction
     __syncthreads();
     for (int j = 0; j < g_n; ++j) {
-      if (j == g_e) continue;
       const double dx = gl->pos[g_e][0] - gl->pos[j][0], dy = gl->pos[g_e][1] - gl->pos[j][1],
                    dz = gl->pos[g_e][2] - gl->pos[j][2];
-      const double len = sqrt(dx * dx + dy * dy + dz * dz);  // CVector::length
-      if (gp->interaction == CMTJ_INTERACTION_KANAO) {
-        gl->pa[j][g_e] = pow(len, 3);
-        gl->pb[j][g_e] = pow(len, 5);
-      } else {
-        gl->pa[j][g_e] = pow(len, 2);
-        gl->pb[j][g_e] = 3 / (4 * M_PI * pow(len, 5));
-      }
+      // the host has checked |r| against the reference's limits
+      gl->il[j][g_e] = (j == g_e) ? 0.0 : 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
     }
   }
   for (; i < iend; ++i) {
@@ -1087,7 +1076,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       gl->m[g_e][1] = mt.y;
       gl->m[g_e][2] = mt.z;
       __syncthreads();
-      gH = group_field(*gl, gp->interaction, g_n, g_e);
+      if (gp->interaction != CMTJ_INTERACTION_NULL) gH = group_field(*gl, g_n, g_e);
     }
     [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
     [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
