@@ -113,3 +113,25 @@ def test_nan_element_stops_its_group_only():
     print(f"[group] {K} groups in one launch: max relative difference {worst:.3e}", flush=True)
     assert worst <= 1e-8
     assert 0 < len(g[5].getLog(0)["time"]) < len(g[4].getLog(0)["time"])
+
+
+def test_single_element_group_equals_the_junction_alone():
+    """One element: the sum has no source (its own inverse distance is 0), so
+    the run is the junction's own run."""
+    (j,) = _elements(Junction, 1, True)
+    g = GroupInteraction([CVector(0, 0, 0)], [j])
+    g.runSimulation(0.2e-9, 1e-13, 1e-12)
+    j.runSimulation(0.2e-9, 1e-13, 1e-12)
+    a, b = g.getLog(0), j.getLog()
+    assert a.keys() == b.keys()
+    for k in a:
+        np.testing.assert_allclose(a[k], b[k], rtol=0, atol=1e-13)
+
+
+def test_elements_of_different_structure_are_rejected():
+    js = _elements(Junction, 2)
+    js[1].setLayerOerstedFieldDriver("all", AxialDriver(stepDriver(0, 1e3, 1e-12, 2e-12), constantDriver(0),
+                                                        constantDriver(0)))
+    g = GroupInteraction([CVector(*p) for p in POS[:2]], js)
+    with pytest.raises(RuntimeError, match="one structure"):
+        g.runSimulation(1e-12, 1e-13, 1e-13)
