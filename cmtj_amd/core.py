@@ -480,23 +480,16 @@ def gaussianStepDriver(constantValue: float, amplitude: float, t0: float, sigma:
     return ScalarDriver.getGaussianStepDriver(constantValue, amplitude, t0, sigma)
 
 
-def _fill_driver(rec: np.ndarray, d: ScalarDriver, table_index: int = -1) -> None:
+def _driver_row(d: ScalarDriver, table_index: int = -1) -> tuple:
+    """One cmtj_driver record as a tuple in field order."""
     if d.update == UpdateType.custom:
-        rec["type"] = _lib.TABLE
-        rec["table_index"] = table_index
-        return
-    rec["type"] = int(d.update)
-    rec["table_index"] = 0
-    rec["constant"] = d.constantValue
-    rec["amplitude"] = d.amplitude
-    rec["frequency"] = d.frequency
-    rec["phase"] = d.phase
-    rec["period"] = d.period
-    rec["cycle"] = d.cycle
-    rec["t_start"] = d.timeStart
-    rec["t_stop"] = d.timeStop
-    rec["edge_time"] = d.edgeTime
-    rec["steady_time"] = d.steadyTime
+        return (_lib.TABLE, table_index, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    return (int(d.update), 0, d.constantValue, d.amplitude, d.frequency, d.phase, d.period, d.cycle, d.timeStart,
+            d.timeStop, d.edgeTime, d.steadyTime)
+
+
+def _fill_driver(rec: np.ndarray, d: ScalarDriver, table_index: int = -1) -> None:
+    rec[()] = _driver_row(d, table_index)
 
 
 # ---------------------------------------------------------------------------
@@ -844,13 +837,14 @@ class Layer:
         rec["dynamic_sot"] = int(self.dynamicSOT)
         rec["alternative_stt"] = int(self.alternativeSTT)
         rec["temperature_set"] = int(self.temperatureSet)
-        drv = rec["drv"]
+        rows = [None] * _lib.NSLOT
         for name, slot in _SCALAR_SLOTS.items():
             d = self._scalar[name]
-            _fill_driver(drv[slot], d, tables.add(d, run_info) if d.update == UpdateType.custom else -1)
+            rows[slot] = _driver_row(d, tables.add(d, run_info) if d.update == UpdateType.custom else -1)
         for name, slot in _AXIAL_SLOTS.items():
             for q, d in enumerate(self._axial[name].drivers):
-                _fill_driver(drv[slot + q], d, tables.add(d, run_info) if d.update == UpdateType.custom else -1)
+                rows[slot + q] = _driver_row(d, tables.add(d, run_info) if d.update == UpdateType.custom else -1)
+        rec["drv"] = np.array(rows, dtype=_lib.DRIVER_DTYPE)
 
 
 class _Tables:
