@@ -17,7 +17,13 @@ Pinning (what fixes each oracle function independently of its own formula):
   * thermal field: Philox4x32-10 known-answer vectors (Random123) and the
     Boltzmann (Langevin-function) equilibrium of an ensemble;
   * energy channels: key sets of tests/test_basic.py and closed forms of the
-    logged fields.
+    logged fields;
+  * coupled stacks: zero coupling equals each member alone, logged currents
+    and couplings recomputed from the logged magnetisations;
+  * Dormand-Prince: scipy's RK45 tableau on a textbook LLG right-hand side
+    and the closed-form damped precession;
+  * dipole-coupled groups: single-junction runs carrying the textbook
+    point-dipole field as their reserved field, step by step.
 """
 from __future__ import annotations
 
