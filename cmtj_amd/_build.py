@@ -16,7 +16,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libcmtj_amd.so"
 UNITS = ["engine.hip", "k_f64.hip", "k_f64e.hip", "k_f32.hip", "k_spec.hip", "k_ll_f64.hip", "k_ll_f32.hip",
-         "k_ll_spec.hip", "k_exact.hip", "k_cl.hip", "k_stack.hip", "k_dp.hip", "k_group.hip"]
+         "k_ll_spec.hip", "k_exact.hip", "k_cl.hip", "k_cl64.hip", "k_stack.hip", "k_dp.hip", "k_group.hip"]
 ARCH = os.environ.get("CMTJ_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}",

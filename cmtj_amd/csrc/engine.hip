@@ -77,7 +77,7 @@ static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, u
   static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),
                                    kernels_spec(),   kernels_ll_f64(),  kernels_ll_f32(),
                                    kernels_ll_spec(), kernels_exact(),  kernels_cl(),
-                                   kernels_dp()};
+                                   kernels_cl64(),    kernels_dp()};
   // [0] one lane per instance, [1] one lane per layer, [2] one lane per component
   const KEntry* best[3] = {nullptr, nullptr, nullptr};
   for (const KTable& t : kTables)

@@ -1,0 +1,10 @@
+// One-component-per-lane kernels in binary64 (every RK4 stage, the
+// drop-in's arithmetic): the same feature sets as the binary32 ones.
+#include "registry.h"
+namespace cmtj {
+static const KEntry kC64[] = {
+    CMTJ_KC64(2, FM_PIMM, 1),  CMTJ_KC64(2, FM_PIMM, -1), CMTJ_KC64(4, FM_PIMM, -1), CMTJ_KC64(1, FM_RH, 0),
+    CMTJ_KC64(1, FM_VSD, 1),   CMTJ_KC64(1, FM_LEAN, -1), CMTJ_KC64(2, FM_LEAN, -1), CMTJ_KC64(4, FM_LEAN, -1),
+};
+KTable kernels_cl64() { return {kC64, sizeof(kC64) / sizeof(kC64[0])}; }
+}  // namespace cmtj

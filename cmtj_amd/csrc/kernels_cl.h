@@ -91,8 +91,9 @@ struct CLConst {
 };
 
 // ND: number of time-dependent records when fixed at compile time (exact
-// kernels), -1 to read it from the group.
-template <int L, uint32_t FM, int ND>
+// kernels), -1 to read it from the group.  F64: every stage in binary64
+// (the drop-in's arithmetic) instead of the anchored binary32 increments.
+template <int L, uint32_t FM, int ND, bool F64 = false>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
   constexpr int LPI = 4 * L;
@@ -270,22 +271,66 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       }
       return dd;
     };
-    const float dmid = drive_delta(t + 0.5 * dt, 1);
-    const float d2 = (float)(k1 * 0.5);
-    float dk = delta(d2, dmid);
-    float acc = dk * 2.f;
-    const float d3 = d2 + dk * 0.5f;
-    dk = delta(d3, dmid);
-    acc = acc + dk * 2.f;
-    const float dend = drive_delta(t + dt, 2);
-    const float d4 = (float)k1 + dk;
-    acc = acc + delta(d4, dend);
-    double mn = m + k1 + (double)(acc * sixth_inv);
-    // CVector::normalize (core/cvector.hpp:185-192) over the quad: binary32
-    // reciprocal square root and one Newton step in binary64 (|m| = 1 to
-    // ~2e-14, far inside the increments' binary32 rounding); a zero vector
-    // stays as it is
-    {
+    double mn;
+    if constexpr (F64) {
+      // stages 2-4 of rk4_step (core/junction.hpp:1017-1041) in binary64:
+      // the field drives at t + dt/2 (stages 2 and 3) and t + dt (stage 4)
+      auto hdrive = [&](double ts, int stage) -> double {
+        double Hd = K.hb;
+        if (!CMTJ_HAS(F_DYNH)) return Hd;
+        if constexpr ((FM & F_KSTEP) != 0) {
+          // step records are 0 outside [min t_start, max t_stop] (inclusive)
+          if (!(ts >= win_lo && ts <= win_hi)) return Hd;
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          if (ND >= 0 || r < g.ndyn) {
+            const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, ts, i, stage, sps);
+            Hd = Hd + (sel[r] ? v : 0.0);
+          }
+        }
+        for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
+          const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+          const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, rr, g, ts, i, stage, sps);
+          Hd = Hd + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
+        }
+        return Hd;
+      };
+      // dm/dt dt at (ms, Hd): the anchor's terms at a stage magnetisation
+      auto f = [&](double ms, double Hs) -> double {
+        const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
+        if (CMTJ_HAS(F_ANIS)) Hs = fma(K.akn, fma(K.an2, s2, fma(K.an1, s1, K.an * ms)), Hs);
+        if (CMTJ_HAS(F_IEC)) Hs = fma(top, K.jt, fma(bot, K.jb, Hs));
+        Hs = fma(-K.nms, ms, Hs);
+        const double Pc = cross_c(s1, s2, Hs);
+        const double Qc = cross_c(s1, s2, Pc);
+        return fma(Qc, K.alpha, Pc) * K.gdt;
+      };
+      const double Hmid = hdrive(t + 0.5 * dt, 1);
+      const double k2 = f(m + k1 * 0.5, Hmid);
+      const double k3 = f(m + k2 * 0.5, Hmid);
+      const double k4 = f(m + k3, hdrive(t + dt, 2));
+      mn = m + (k1 + (k2 * 2.0) + (k3 * 2.0) + k4) / 6.0;
+      // CVector::normalize (core/cvector.hpp:185-192): the quad sum adds
+      // (x^2 + y^2) + z^2 in the reference's order; a zero vector stays
+      const double ss = qsum(mn * mn);
+      mn = (ss != 0.0) ? mn / sqrt(ss) : mn;
+    } else {
+      const float dmid = drive_delta(t + 0.5 * dt, 1);
+      const float d2 = (float)(k1 * 0.5);
+      float dk = delta(d2, dmid);
+      float acc = dk * 2.f;
+      const float d3 = d2 + dk * 0.5f;
+      dk = delta(d3, dmid);
+      acc = acc + dk * 2.f;
+      const float dend = drive_delta(t + dt, 2);
+      const float d4 = (float)k1 + dk;
+      acc = acc + delta(d4, dend);
+      mn = m + k1 + (double)(acc * sixth_inv);
+      // CVector::normalize (core/cvector.hpp:185-192) over the quad: binary32
+      // reciprocal square root and one Newton step in binary64 (|m| = 1 to
+      // ~2e-14, far inside the increments' binary32 rounding); a zero vector
+      // stays as it is
       const double s = qsum(mn * mn);
       double y = (double)__frsqrt_rn((float)s);
       y = y * (1.5 - (0.5 * s) * (y * y));

@@ -43,6 +43,7 @@ KTable kernels_ll_f32();
 KTable kernels_ll_spec();
 KTable kernels_exact();
 KTable kernels_cl();
+KTable kernels_cl64();
 KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stacks only
 KTable kernels_dp();     // Dormand-Prince (single-layer, FP64)
 KTable kernels_group();  // dipole-coupled groups: chosen by cmtj_run_groups only
@@ -60,6 +61,10 @@ KTable kernels_group();  // dipole-coupled groups: chosen by cmtj_run_groups onl
 #define CMTJ_KC(L, FM, ND)                                                                     \
   {CMTJ_FP32, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND>,  \
    "k_junction_cl<" #L "," #FM "," #ND ">", (ND) + 1}
+
+#define CMTJ_KC64(L, FM, ND)                                                                        \
+  {CMTJ_FP64, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND, true>,  \
+   "k_junction_cl<double," #L "," #FM "," #ND ">", (ND) + 1}
 
 #define CMTJ_KL_SOLVERS(R, PREC, L, FM) \
   CMTJ_KL(R, PREC, L, CMTJ_RK4, FM), CMTJ_KL(R, PREC, L, CMTJ_HEUN, FM), CMTJ_KL(R, PREC, L, CMTJ_EULER_HEUN, FM)

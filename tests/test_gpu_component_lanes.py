@@ -1,5 +1,6 @@
-"""The one-component-per-lane kernels (anchored binary32 RK4) against the
-oracle (binary64): workloads and multilayer cases of the lean feature set,
+"""The one-component-per-lane kernels (anchored binary32 RK4, and the
+binary64 variant) against the oracle (binary64): workloads and multilayer
+cases of the lean feature set,
 the magnetoresistance channels, the NaN semantics of a multilayer step and
 the automatic lane choice."""
 from __future__ import annotations
@@ -50,40 +51,40 @@ def _trilayer_sine():
     return j
 
 
-def _batch(name):
+def _batch(name, precision="fp32"):
     if name == "pimm":
-        return workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp32", log="full",
+        return workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision=precision, log="full",
                               log_dtype="f64")
     if name == "pimm_delayed":
-        b = workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp32", log="mx", log_dtype="f64")
+        b = workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision=precision, log="mx", log_dtype="f64")
         n = len(b.js)
         b.set_driver("all", "oersted", "t_start", np.full((n, 3), 5.05e-12))
         b.set_driver("all", "oersted", "t_stop", np.full((n, 3), 2.0e-11))
         return b
     if name == "r_h_loop":
-        b = workloads.r_h_loop(n_fields=11, total_time=0.5e-9, precision="fp32", log="full")
+        b = workloads.r_h_loop(n_fields=11, total_time=0.5e-9, precision=precision, log="full")
         b.log_dtype = _lib.LOG_F64
         return b
     if name == "vsd":
         b = JunctionBatch.replicate(workloads.vsd_junction(), 8, 1e-9, 1e-13, 1e-12, SolverMode.RK4,
-                                    precision="fp32", log="full", log_dtype="f64")
+                                    precision=precision, log="full", log_dtype="f64")
         f = np.zeros((8, 3))
         f[:, 1] = np.linspace(1e9, 20e9, 8)
         b.set_driver("free", "oersted", "frequency", f)
         return b
     if name == "rh_strip":
         j, kw = parity_cases.rh_strip(Junction)
-        return JunctionBatch.from_junctions([j] * 5, 0.5e-9, 1e-13, 1e-12, SolverMode.RK4, precision="fp32",
+        return JunctionBatch.from_junctions([j] * 5, 0.5e-9, 1e-13, 1e-12, SolverMode.RK4, precision=precision,
                                             log="full", log_dtype="f64")
     if name == "chain2_strip":
         return JunctionBatch.from_junctions([_chain(2)] * 3, 0.5e-9, 1e-13, 1e-12, SolverMode.RK4,
-                                            precision="fp32", log="full", log_dtype="f64")
+                                            precision=precision, log="full", log_dtype="f64")
     if name == "chain4_strip":
         return JunctionBatch.from_junctions([_chain(4)] * 3, 0.5e-9, 1e-13, 1e-12, SolverMode.RK4,
-                                            precision="fp32", log="full", log_dtype="f64")
+                                            precision=precision, log="full", log_dtype="f64")
     if name == "trilayer_sine":
         return JunctionBatch.from_junctions([_trilayer_sine()] * 3, 1e-9, 1e-13, 3e-13, SolverMode.RK4,
-                                            precision="fp32", log="full", log_dtype="f64")
+                                            precision=precision, log="full", log_dtype="f64")
     raise KeyError(name)
 
 
@@ -123,6 +124,34 @@ def test_component_lanes_match_oracle(name):
     assert wm <= FP32_TOL and fm <= FP32_TOL and wr <= FP32_TOL
 
 
+@pytest.mark.parametrize("name", ["pimm", "pimm_delayed", "r_h_loop", "vsd", "rh_strip", "chain2_strip",
+                                  "chain4_strip", "trilayer_sine"])
+def test_fp64_component_lanes_match_oracle(name):
+    """The binary64 variant (every RK4 stage in binary64): the drop-in's
+    tolerance against the oracle."""
+    b = _batch(name, precision="fp64").set_lanes("per_component")
+    kname = b.plan().info.kernel_name.decode()
+    print(f"[cl64] {name}: {kname}", flush=True)
+    assert kname.startswith("k_junction_cl<double")
+    r = b.run()
+    olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
+    st = r.status()
+    assert (st["code"] == 0).all()
+    assert (st["steps_done"] == ost["steps_done"]).all() and (st["entries_done"] == ost["entries_done"]).all()
+    glog = r.log.cpu().numpy()
+    worst = 0.0
+    for i in range(len(b.js)):
+        g, o = _channels(b, glog, i), _channels(b, olog, i)
+        assert g.keys() == o.keys()
+        for k in o:
+            if len(o[k]):
+                worst = max(worst, float(np.max(np.abs(g[k] - o[k]))) / max(1.0, float(np.max(np.abs(o[k])))))
+    nl = int(b.js[0]["n_layers"])
+    fm = float(np.max(np.abs(r.final_m.cpu().numpy()[:, :nl] - omag[:, :nl])))
+    print(f"[cl64] {name}: max relative difference {worst:.3e}, final |dm| {fm:.3e}", flush=True)
+    assert worst <= 1e-8 and fm <= 1e-8
+
+
 @pytest.mark.parametrize("bad_layer", [0, 1])
 def test_component_lanes_nan_commits_like_the_reference(bad_layer):
     """A NaN in layer b stops the run: layers up to and including it take
@@ -157,7 +186,7 @@ def test_component_lanes_chosen_automatically_on_small_batches():
     big = workloads.pimm(n_points=16384, t_min=1e-11, t_max=2e-11, precision="fp32", log="mx")
     assert "k_junction_ll" in big.plan().info.kernel_name.decode()
     fp64 = workloads.pimm(n_points=64, t_min=1e-11, t_max=2e-11, precision="fp64", log="mx")
-    assert "k_junction_cl" not in fp64.plan().info.kernel_name.decode()
+    assert fp64.plan().info.kernel_name.decode().startswith("k_junction_cl<double")
 
 
 def test_component_lanes_sliced_run_equals_uninterrupted():
