@@ -707,11 +707,6 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
     need = std::max<size_t>(need, j.log_offset + nch * j.n_entries);
   }
   if (need > 0 && (!log_host || log_cap < need)) return fail(CMTJ_ERR_INVALID, "log buffer too small");
-  const KTable kt = kernels_stack();
-  const KEntry* ke = nullptr;
-  for (size_t q = 0; q < kt.n; ++q)
-    if (kt.e[q].solver == j0.solver) ke = &kt.e[q];
-  if (!ke) return fail(CMTJ_ERR_INVALID, "no stack kernel for this solver");
   uint32_t pad = 1;
   while (pad < (uint32_t)n) pad <<= 1;
 
@@ -735,6 +730,16 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
   if (P.groups.size() != 1)
     return fail(CMTJ_ERR_INVALID,
                 "the junctions of a stack must share one structure (layers, modes, driver kinds)");
+  // the smallest stack kernel that covers the structure
+  const KTable kt = kernels_stack();
+  const KEntry* ke = nullptr;
+  for (size_t q = 0; q < kt.n; ++q) {
+    const KEntry& e = kt.e[q];
+    if (e.solver == j0.solver && e.lm >= P.groups[0].a.nl &&
+        covers(e, P.groups[0].a.flags, 0, P.groups[0].a.ndyn) && (!ke || rank(e) < rank(*ke)))
+      ke = &e;
+  }
+  if (!ke) return fail(CMTJ_ERR_INVALID, "no stack kernel for this solver");
   const size_t o_mag = align_up(need * sizeof(double));
   const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj);
   const size_t o_stk = align_up(o_st + sizeof(cmtj_status) * nj);

@@ -14,6 +14,13 @@ static const KEntry kG[] = {
      (const void*)&k_junction<double, 1, CMTJ_RK4, FM_LEAN, false, false, true>, "k_junction<double,1,RK4,lean,group>"},
     {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_LEAN, false,
      (const void*)&k_junction<double, 2, CMTJ_RK4, FM_LEAN, false, false, true>, "k_junction<double,2,RK4,lean,group>"},
+    // spin torques, scalar drives, temperature
+    {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_CPL, false,
+     (const void*)&k_junction<double, 2, CMTJ_RK4, FM_CPL, false, false, true>,
+     "k_junction<double,2,RK4,coupled,group>"},
+    {CMTJ_FP64, 2, 1, CMTJ_EULER_HEUN, FM_CPL, false,
+     (const void*)&k_junction<double, 2, CMTJ_EULER_HEUN, FM_CPL, false, false, true>,
+     "k_junction<double,2,EulerHeun,coupled,group>"},
     // every feature
     {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_GROUP, false,
      (const void*)&k_junction<double, 2, CMTJ_RK4, FM_GROUP, false, false, true>, "k_junction<double,2,RK4,group>"},

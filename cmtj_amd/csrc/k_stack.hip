@@ -7,6 +7,15 @@
 namespace cmtj {
 constexpr uint32_t FM_STACK = F_ALL;
 static const KEntry kS[] = {
+    // the usual member: anisotropy, IEC, spin torques, drives, temperature
+    {CMTJ_FP64, 1, 1, CMTJ_RK4, FM_CPL, false,
+     (const void*)&k_junction<double, 1, CMTJ_RK4, FM_CPL, false, true>, "k_junction<double,1,RK4,coupled,stack>"},
+    {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_CPL, false,
+     (const void*)&k_junction<double, 2, CMTJ_RK4, FM_CPL, false, true>, "k_junction<double,2,RK4,coupled,stack>"},
+    {CMTJ_FP64, 2, 1, CMTJ_EULER_HEUN, FM_CPL, false,
+     (const void*)&k_junction<double, 2, CMTJ_EULER_HEUN, FM_CPL, false, true>,
+     "k_junction<double,2,EulerHeun,coupled,stack>"},
+    // every feature
     {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_STACK, false,
      (const void*)&k_junction<double, 2, CMTJ_RK4, FM_STACK, false, true>, "k_junction<double,2,RK4,stack>"},
     {CMTJ_FP64, 2, 1, CMTJ_EULER_HEUN, FM_STACK, false,

@@ -25,6 +25,8 @@ struct KTable {
 constexpr uint32_t FM_LEAN = F_ANIS | F_IEC | F_DYNH;
 constexpr uint32_t FM_STT = F_ANIS | F_IEC | F_DYNH | F_STT;
 constexpr uint32_t FM_SOTT = F_ANIS | F_IEC | F_DYN | F_DYNH | F_SOT | F_THERMAL;
+// coupled runs (stacks, groups): the usual member's features
+constexpr uint32_t FM_CPL = F_ANIS | F_IEC | F_DYN | F_DYNH | F_STT | F_SOT | F_THERMAL;
 
 // exact feature sets of the benchmark workloads (see workloads.py)
 constexpr uint32_t FM_PIMM = F_ANIS | F_IEC | F_DYNH | F_EXACT | F_KSTEP;          // 2-layer MTJ, Hoe step
