@@ -839,8 +839,33 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
 // the step (storeMagnetisations, :65-76) and travel between the lanes of the
 // wave.  I[s]: the current at stage time s (RK4: t, t + dt/2, t + dt);
 // Ilog, Clog: the stack log's I_j and C_j_{j+1} of this step.
+// The stack's constants of one member, read once before the step loop (the
+// loop's log stores may alias the descriptor, so loads inside it would be
+// repeated every step).
+struct StackLane {
+  cmtj_driver current;  // the stack current driver
+  double pol[3];
+  double cj;      // coupling of pair (j, j+1); 0 for the last member
+  double phase;   // phase shift of member j (non-KCL)
+  int two_layer, series, kcl;
+};
+
+__device__ __forceinline__ StackLane stack_lane(const cmtj_stack* sk, int j, int n) {
+  StackLane s;
+  s.current = sk->current;
+  s.pol[0] = sk->pol[0];
+  s.pol[1] = sk->pol[1];
+  s.pol[2] = sk->pol[2];
+  s.cj = (j + 1 < n) ? sk->coupling[j] : 0.0;
+  s.two_layer = sk->two_layer;
+  s.series = sk->series;
+  s.kcl = sk->kcl;
+  s.phase = s.series ? sk->phase_offset * j : sk->phase_offset;
+  return s;
+}
+
 template <int LM>
-__device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n, int base, int ti, int bi,
+__device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n, int base, int ti, int bi,
                                               const V3<double> (&m)[LM], double t, double dt, double (&I)[3],
                                               double& Ilog, double& Clog) {
   V3<double> mt = m[0], mb = m[0];
@@ -849,14 +874,14 @@ __device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n
     mt = (l == ti) ? m[l] : mt;
     mb = (l == bi) ? m[l] : mb;
   }
-  const V3<double> p = sk->two_layer ? mb : mk<double>(sk->pol[0], sk->pol[1], sk->pol[2]);
+  const V3<double> p = sk.two_layer ? mb : mk<double>(sk.pol[0], sk.pol[1], sk.pol[2]);
   const double d = mt.x * p.x + mt.y * p.y + mt.z * p.z;  // c_dot, core/junction.hpp:82-84
   // base: the wave lane of member 0 of this stack
   const double dn = __shfl(d, base + ((j + 1 < n) ? j + 1 : j), 64);
-  const double Cj = (j + 1 < n) ? sk->coupling[j] * (sk->series ? d + dn : d - dn) : 0.0;
-  if (sk->kcl) {
+  const double Cj = (j + 1 < n) ? sk.cj * (sk.series ? d + dn : d - dn) : 0.0;
+  if (sk.kcl) {
     // totalCurrent = I0 + sum_j C_j I0, the same constant for every member
-    const double I0 = driver_value(sk->current, t);
+    const double I0 = driver_value(sk.current, t);
     double total = I0;
     for (int q = 0; q + 1 < n; ++q) total += __shfl(Cj, base + q, 64) * I0;
     I[0] = I[1] = I[2] = total;
@@ -871,10 +896,10 @@ __device__ __forceinline__ void stack_current(const cmtj_stack* sk, int j, int n
       if (q < j) eff *= (1 + cq);
     }
     Clog = eff * (1 + Cj);
-    cmtj_driver dl = sk->current;
+    cmtj_driver dl = sk.current;
     dl.constant *= eff;
     dl.amplitude *= eff;
-    dl.phase += sk->series ? sk->phase_offset * j : sk->phase_offset;
+    dl.phase += sk.phase;
     I[0] = driver_value(dl, t);
     I[1] = driver_value(dl, t + 0.5 * dt);
     I[2] = driver_value(dl, t + dt);
@@ -937,7 +962,7 @@ template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false,
 __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
   [[maybe_unused]] const cmtj_group* gp = nullptr;
-  [[maybe_unused]] int g_n = 0, g_e = 0, g_top = 0;
+  [[maybe_unused]] int g_n = 0, g_e = 0, g_top = 0, g_kind = 0;
   if constexpr (GRP) {
     // group blockIdx.x on one wave, element e on lane e; lanes past the
     // group's elements leave at once (a one-wave barrier needs no lane)
@@ -947,6 +972,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     g_e = (int)threadIdx.x;
     if (g_e >= g_n) return;
     g_top = gp->top[g_e];
+    g_kind = gp->interaction;
     lane = blockIdx.x * (uint32_t)g_n + (uint32_t)g_e;
   }
   [[maybe_unused]] const cmtj_stack* sk = nullptr;
@@ -1038,10 +1064,12 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   }
   uint64_t next_log = (uint64_t)k * W;
   [[maybe_unused]] int s_n = 0, s_ti = 0, s_bi = 0;
+  [[maybe_unused]] StackLane sl;
   if constexpr (STK) {
     s_n = sk->n_junctions;
     s_ti = sk->top[s_j];
     s_bi = sk->bottom[s_j];
+    sl = stack_lane(sk, s_j, s_n);
   }
   [[maybe_unused]] GroupLds* gl = nullptr;
   if constexpr (GRP) {
@@ -1076,11 +1104,11 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       gl->m[g_e][1] = mt.y;
       gl->m[g_e][2] = mt.z;
       __syncthreads();
-      if (gp->interaction != CMTJ_INTERACTION_NULL) gH = group_field(*gl, g_n, g_e);
+      if (g_kind != CMTJ_INTERACTION_NULL) gH = group_field(*gl, g_n, g_e);
     }
     [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
     [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
-    if constexpr (STK) stack_current<LM>(sk, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, sI, sIlog, sClog);
+    if constexpr (STK) stack_current<LM>(sl, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, sI, sIlog, sClog);
     if constexpr (ANCHORED) {
       // RK4 of the reference (core/junction.hpp:1017-1041, neighbours frozen
       // at the start of the step) in anchored binary32: k1 and the anchor
