@@ -628,11 +628,9 @@ int cmtj_driver_eval(const cmtj_driver* d, double t, double* out) {
   return 0;
 }
 
-int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
-             const double* tables, size_t n_table, double* log_host, size_t log_cap,
-             double* mag_out_host, cmtj_status* st) {
-  if (!j || !layers || !c || !mag_out_host || !st) return fail(CMTJ_ERR_INVALID, "null pointer");
-  std::lock_guard<std::mutex> lk(g_plain.mu);
+// The host-buffer entry points share one stream and buffers per process,
+// made again when the current device changes; the caller holds g_plain.mu.
+static int plain_bind() {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   if (g_plain.stream == nullptr || g_plain.device != dev) {
@@ -644,6 +642,15 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_consta
     HIP_TRY(hipStreamCreateWithFlags(&g_plain.stream, hipStreamNonBlocking));
     g_plain.device = dev;
   }
+  return 0;
+}
+
+int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
+             const double* tables, size_t n_table, double* log_host, size_t log_cap,
+             double* mag_out_host, cmtj_status* st) {
+  if (!j || !layers || !c || !mag_out_host || !st) return fail(CMTJ_ERR_INVALID, "null pointer");
+  std::lock_guard<std::mutex> lk(g_plain.mu);
+  if (const int rb = plain_bind()) return rb;
   cmtj_junction jj = *j;
   jj.layer_offset = 0;
   jj.log_offset = 0;
@@ -711,17 +718,7 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
   while (pad < (uint32_t)n) pad <<= 1;
 
   std::lock_guard<std::mutex> lk(g_plain.mu);
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev));
-  if (g_plain.stream == nullptr || g_plain.device != dev) {
-    if (g_plain.stream) (void)hipStreamDestroy(g_plain.stream);
-    if (g_plain.arena) (void)hipFree(g_plain.arena);
-    if (g_plain.out) (void)hipFree(g_plain.out);
-    g_plain.arena = g_plain.out = nullptr;
-    g_plain.arena_bytes = g_plain.out_bytes = 0;
-    HIP_TRY(hipStreamCreateWithFlags(&g_plain.stream, hipStreamNonBlocking));
-    g_plain.device = dev;
-  }
+  if (const int rb = plain_bind()) return rb;
   cmtj_plan P;
   P.owns = false;
   int r = build_plan(&P, js, nj, layers, n_layer_rows, c, nullptr, 0, CMTJ_FP64, CMTJ_LOG_F64, CMTJ_LANES_ONE,
@@ -816,17 +813,7 @@ int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_juncti
   }
   if (need > 0 && (!log_host || log_cap < need)) return fail(CMTJ_ERR_INVALID, "log buffer too small");
   std::lock_guard<std::mutex> lk(g_plain.mu);
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev));
-  if (g_plain.stream == nullptr || g_plain.device != dev) {
-    if (g_plain.stream) (void)hipStreamDestroy(g_plain.stream);
-    if (g_plain.arena) (void)hipFree(g_plain.arena);
-    if (g_plain.out) (void)hipFree(g_plain.out);
-    g_plain.arena = g_plain.out = nullptr;
-    g_plain.arena_bytes = g_plain.out_bytes = 0;
-    HIP_TRY(hipStreamCreateWithFlags(&g_plain.stream, hipStreamNonBlocking));
-    g_plain.device = dev;
-  }
+  if (const int rb = plain_bind()) return rb;
   cmtj_plan P;
   P.owns = false;
   int r = build_plan(&P, js, nj, rows.data(), rows.size(), c, nullptr, 0, CMTJ_FP64, CMTJ_LOG_F64, CMTJ_LANES_ONE,
