@@ -261,6 +261,7 @@ struct Group {
   const KEntry* k;
   GroupArgs a;
   uint32_t blocks;
+  uint32_t threads = 64;  // per workgroup
 };
 
 struct cmtj_plan {
@@ -460,7 +461,7 @@ static int launch_plan(cmtj_plan* P, void* d_log, double* d_mag, cmtj_status* d_
     a.slice = slice;
     void* args[] = {&a};
     const hipStream_t st = n_streams > 0 ? (hipStream_t)streams[q % (size_t)n_streams] : s;
-    HIP_TRY(hipLaunchKernel(G.k->fn, dim3(G.blocks), dim3(64), args, 0, st));
+    HIP_TRY(hipLaunchKernel(G.k->fn, dim3(G.blocks), dim3(G.threads), args, 0, st));
   }
   return 0;
 }
@@ -822,8 +823,13 @@ int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_juncti
   if (P.groups.size() != 1)
     return fail(CMTJ_ERR_INVALID,
                 "the junctions of a group must share one structure (layers, modes, driver kinds)");
-  // the smallest group kernel that covers the structure
-  const KTable kt = kernels_group();
+  // the smallest group kernel that covers the structure; four waves per
+  // group while the groups leave SIMDs idle (the sum over the sources then
+  // takes a quarter of the step), one wave each beyond
+  bool any_field = false;
+  for (size_t q = 0; q < n_groups; ++q) any_field |= groups[q].interaction != CMTJ_INTERACTION_NULL;
+  const uint32_t gw = (any_field && n > 8 && n_groups * 4 <= 1024) ? 4u : 1u;
+  const KTable kt = gw == 4 ? kernels_group4() : kernels_group();
   const KEntry* ke = nullptr;
   for (size_t q = 0; q < kt.n; ++q) {
     const KEntry& e = kt.e[q];
@@ -842,7 +848,8 @@ int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_juncti
                          g_plain.stream));
   Group& G = P.groups[0];
   G.k = ke;
-  G.blocks = (uint32_t)n_groups;  // group g on workgroup g, element e on lane e
+  G.blocks = (uint32_t)n_groups;  // group g on workgroup g, element e on lane e of its first wave
+  G.threads = 64 * gw;
   G.a.group = (const cmtj_group*)(out + o_grp);
   G.a.group_n = (uint32_t)n_groups;
   r = launch_plan(&P, need ? out : nullptr, (double*)(out + o_mag), (cmtj_status*)(out + o_st), g_plain.stream);

@@ -911,28 +911,35 @@ __device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n,
 // positions, source strengths s and the inverse distances il[j][e] = 1/|r_e
 // - r_j| (0 on the diagonal, so a lane's own term vanishes without a
 // branch).  il is [j][e]: the lanes of one source read consecutive words;
-// m, pos and s of a source are broadcast reads.  36 KB at 64 elements, so
-// four groups share a CU.
+// m, pos and s of a source are broadcast reads.  With GW waves per group,
+// waves 1..GW-1 sum their share of the sources into part[w - 1] and ctl
+// tells them whether another step follows.  About 38 KB at 64 elements and
+// one wave (four groups per CU), 42 KB with four waves.
+template <int GW>
 struct GroupLds {
   double m[CMTJ_MAX_GROUP][3];
   double pos[CMTJ_MAX_GROUP][3];
   double s[CMTJ_MAX_GROUP];
   double il[CMTJ_MAX_GROUP][CMTJ_MAX_GROUP];
+  double part[GW > 1 ? GW - 1 : 1][CMTJ_MAX_GROUP][3];
+  int ctl;
 };
 
-// Field of the group on element e from the top-layer magnetisations in LDS
-// (stepFunctionalSolver, core/reservoir.hpp:126-147, sources j ascending):
+// Field on element e of the sources j0 <= j < j1, from the top-layer
+// magnetisations in LDS (stepFunctionalSolver, core/reservoir.hpp:126-147,
+// sources ascending):
 //   sum_j s_j (3 (m_j . r) r / |r|^5 - m_j / |r|^3),  r = r_e - r_j,
 // which is computeDipoleInteraction (:79-96) with s_j = (Ms_j / mu0) V_j,
 // and computeDipoleInteractionNoumra (:106-119) with s_j = Ms_j V_j / mu0 /
 // (4 pi): its tensor (r r^T - |r|^2/3) 3 / (4 pi |r|^5) applied to m_j is
 // the same point-dipole field over 4 pi.  Powers of the stored 1/|r| replace
 // the reference's pow() and divisions (a few ulp apart).
-__device__ __forceinline__ V3<double> group_field(const GroupLds& L, int n, int e) {
+template <int GW>
+__device__ __forceinline__ V3<double> group_field(const GroupLds<GW>& L, int j0, int j1, int e) {
   V3<double> H = mk<double>(0, 0, 0);
   const V3<double> ri = mk<double>(L.pos[e][0], L.pos[e][1], L.pos[e][2]);
 #pragma unroll 4
-  for (int j = 0; j < n; ++j) {
+  for (int j = j0; j < j1; ++j) {
     const V3<double> r = ri - mk<double>(L.pos[j][0], L.pos[j][1], L.pos[j][2]);
     const V3<double> mj = mk<double>(L.m[j][0], L.m[j][1], L.m[j][2]);
     const double il = L.il[j][e], il2 = il * il;
@@ -943,6 +950,39 @@ __device__ __forceinline__ V3<double> group_field(const GroupLds& L, int n, int 
     H.z = fma(a, r.z, fma(-b, mj.z, H.z));
   }
   return H;
+}
+
+// Sources of wave w of a group of n elements on GW waves.
+template <int GW>
+__device__ __forceinline__ void group_share(int n, int w, int& j0, int& j1) {
+  const int chunk = (n + GW - 1) / GW;
+  j0 = min(n, w * chunk);
+  j1 = min(n, j0 + chunk);
+}
+
+// Waves 1..GW-1 of a group: per step, wait for the magnetisations (two
+// barriers, as wave 0 writes them between its first two), sum the field of
+// this wave's sources on every element, publish it (third barrier).  Wave 0
+// sets ctl to 0 between the first two barriers of the round after its last
+// step, so every wave passes the same barriers and leaves together.
+template <int GW>
+__device__ __noinline__ void group_helper(GroupLds<GW>& L, int n, int w) {
+  const int e = (int)(threadIdx.x & 63);
+  int j0, j1;
+  group_share<GW>(n, w, j0, j1);
+  __syncthreads();  // positions and strengths are written
+  for (;;) {
+    __syncthreads();  // the previous step's reads are done
+    __syncthreads();  // m and ctl are written
+    if (L.ctl == 0) break;
+    if (e < n) {
+      const V3<double> h = group_field<GW>(L, j0, j1, e);
+      L.part[w - 1][e][0] = h.x;
+      L.part[w - 1][e][1] = h.y;
+      L.part[w - 1][e][2] = h.z;
+    }
+    __syncthreads();  // partial fields are written
+  }
 }
 
 template <class R, int LM>
@@ -958,17 +998,27 @@ __device__ __forceinline__ void set_current(DV<R> (&dv)[LM], double I) {
   for (int l = 0; l < LM; ++l) dv[l].I = (R)I;
 }
 
-template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false, bool GRP = false>
+template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false, bool GRP = false, int GW = 1>
 __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
   [[maybe_unused]] const cmtj_group* gp = nullptr;
   [[maybe_unused]] int g_n = 0, g_e = 0, g_top = 0, g_kind = 0;
+  [[maybe_unused]] GroupLds<GW>* gl = nullptr;
   if constexpr (GRP) {
-    // group blockIdx.x on one wave, element e on lane e; lanes past the
-    // group's elements leave at once (a one-wave barrier needs no lane)
+    // group blockIdx.x on a workgroup of GW waves, element e on lane e of
+    // wave 0; lanes of wave 0 past the group's elements leave at once (a
+    // barrier waits for waves, not lanes)
+    __shared__ GroupLds<GW> lds;
+    gl = &lds;
     if (blockIdx.x >= g.group_n) return;
     gp = g.group + blockIdx.x;
     g_n = gp->n_elements;
+    if constexpr (GW > 1) {
+      if (threadIdx.x >= 64) {
+        group_helper<GW>(lds, g_n, (int)(threadIdx.x >> 6));
+        return;
+      }
+    }
     g_e = (int)threadIdx.x;
     if (g_e >= g_n) return;
     g_top = gp->top[g_e];
@@ -1071,10 +1121,10 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     s_bi = sk->bottom[s_j];
     sl = stack_lane(sk, s_j, s_n);
   }
-  [[maybe_unused]] GroupLds* gl = nullptr;
+  [[maybe_unused]] int g_j1 = 0;
   if constexpr (GRP) {
-    __shared__ GroupLds lds;
-    gl = &lds;
+    int j0_;
+    group_share<GW>(g_n, 0, j0_, g_j1);
     const cmtj_layer& Ly = g.layers[J->layer_offset + g_top];
     const double V = Ly.thickness * Ly.cell_surface;
     gl->pos[g_e][0] = gp->pos[g_e][0];
@@ -1103,8 +1153,15 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       gl->m[g_e][0] = mt.x;
       gl->m[g_e][1] = mt.y;
       gl->m[g_e][2] = mt.z;
-      __syncthreads();
-      if (g_kind != CMTJ_INTERACTION_NULL) gH = group_field(*gl, g_n, g_e);
+      if (g_e == 0) gl->ctl = 1;
+      __syncthreads();  // m and ctl are written
+      if (g_kind != CMTJ_INTERACTION_NULL) gH = group_field<GW>(*gl, 0, g_j1, g_e);
+      if constexpr (GW > 1) {
+        __syncthreads();  // the other waves' partial fields are written
+        if (g_kind != CMTJ_INTERACTION_NULL)
+          for (int w = 1; w < GW; ++w)
+            gH = gH + mk<double>(gl->part[w - 1][g_e][0], gl->part[w - 1][g_e][1], gl->part[w - 1][g_e][2]);
+      }
     }
     [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
     [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
@@ -1244,8 +1301,8 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
         }
       }
       if constexpr (GRP) {
-        // a NaN ends the group's run (the reference throws): the group is
-        // one wave, its lanes leave the loop together
+        // a NaN ends the group's run (the reference throws): the elements
+        // are one wave, its lanes leave the loop together
         if (__any(nan_layer >= 0)) {
           if (nan_layer >= 0) code = CMTJ_ERR_NAN;
           break;
@@ -1347,6 +1404,13 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       next_log += W;
     }
   }
+  if constexpr (GRP && GW > 1) {
+    // release the other waves: one more round of the first two barriers
+    // with ctl = 0
+    __syncthreads();
+    if (g_e == 0) gl->ctl = 0;
+    __syncthreads();
+  }
   if (g.mag_out) {
 #pragma unroll
     for (int l = 0; l < LM; ++l) {
@@ -1387,6 +1451,13 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
 template <class R, int LM, int SOLVER, uint32_t FM, bool LOGE, bool STK = false, bool GRP = false>
 __global__ __launch_bounds__(64) void k_junction(GroupArgs g) {
   junction_body<R, LM, SOLVER, FM, LOGE, STK, GRP>(g);
+}
+
+// Dipole-coupled groups on GW waves each: wave 0 integrates the elements,
+// waves 1..GW-1 share the sum over the sources.
+template <class R, int LM, int SOLVER, uint32_t FM, int GW>
+__global__ __launch_bounds__(64 * GW) void k_group(GroupArgs g) {
+  junction_body<R, LM, SOLVER, FM, false, false, true, GW>(g);
 }
 
 }  // namespace cmtj

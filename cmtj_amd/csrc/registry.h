@@ -48,7 +48,8 @@ KTable kernels_cl();
 KTable kernels_cl64();
 KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stacks only
 KTable kernels_dp();     // Dormand-Prince (single-layer, FP64)
-KTable kernels_group();  // dipole-coupled groups: chosen by cmtj_run_groups only
+KTable kernels_group();   // dipole-coupled groups, one wave each: chosen by cmtj_run_groups only
+KTable kernels_group4();  // the same on four waves per group
 
 }  // namespace cmtj
 

@@ -135,3 +135,51 @@ def test_elements_of_different_structure_are_rejected():
     g = GroupInteraction([CVector(*p) for p in POS[:2]], js)
     with pytest.raises(RuntimeError, match="one structure"):
         g.runSimulation(1e-12, 1e-13, 1e-13)
+
+
+@pytest.mark.parametrize("n", [9, 12, 33])
+def test_four_wave_groups_match_oracle(n):
+    """More than 8 elements in at most 256 groups: four waves per group, the
+    sum over the sources split four ways (uneven shares included); two-layer
+    elements, Nomura coupling, several groups in one launch."""
+    def build(k, G):
+        js = _elements(Junction, n, two=True)
+        grp = G(_grid(n, pitch=70e-9 + 3e-9 * k), js)
+        grp.setInteractionFunction(_FN["nomura" if k % 2 else "kanao"])
+        return grp
+
+    g = [build(k, GroupInteraction) for k in range(6)]
+    o = [build(k, OracleGroupInteraction) for k in range(6)]
+    run_groups(g, 0.1e-9, 1e-13, 1e-12)
+    worst = 0.0
+    for a, b in zip(g, o):
+        b.runSimulation(0.1e-9, 1e-13, 1e-12)
+        worst = max(worst, _compare(a, b, n))
+    print(f"[group] {n} elements x 6 groups, four waves each: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_four_wave_group_nan_releases_every_wave():
+    """A NaN in one element of a 12-element group (four waves) stops that
+    group; the other waves of its workgroup leave with it and the other
+    groups finish."""
+    def build(k, G, bad):
+        js = _elements(Junction, 12)
+        for q, j in enumerate(js):
+            amp = float("inf") if (bad and q == 7) else 1e3 * (k % 3)
+            j.setLayerExternalFieldDriver("all", AxialDriver(stepDriver(0, amp, 5e-12, 1e-9), constantDriver(0),
+                                                             constantDriver(5e3)))
+        return G(_grid(12, pitch=60e-9), js)
+
+    g = [build(k, GroupInteraction, k == 2) for k in range(4)]
+    o = [build(k, OracleGroupInteraction, k == 2) for k in range(4)]
+    with pytest.raises(RuntimeError, match="NAN magnetisation"):
+        run_groups(g, 0.1e-9, 1e-13, 1e-12)
+    for k, b in enumerate(o):
+        if k == 2:
+            with pytest.raises(RuntimeError, match="NAN magnetisation"):
+                b.runSimulation(0.1e-9, 1e-13, 1e-12)
+        else:
+            b.runSimulation(0.1e-9, 1e-13, 1e-12)
+        assert _compare(g[k], b, 12) <= 1e-8
+    assert 0 < len(g[2].getLog(0)["time"]) < len(g[1].getLog(0)["time"])
