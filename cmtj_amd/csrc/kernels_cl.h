@@ -88,6 +88,10 @@ __device__ __forceinline__ bool feeds(int meta, int ly, int c) {
 struct CLConst {
   double hb, an, an1, an2, nms, alpha, gdt, akn, jb, jt, ref;  // binary64 anchor
   float anf, an1f, an2f, nmsf, alphaf, gdtf, aknf;             // binary32 increments
+  // spin-transfer torque (binary64 kernels, constant current)
+  double sttI, kap, bet, ng, cdt, Pp, lam, lam2;  // gamma hbar I / (e Ms t), kappa, beta, -gamma, dt / (1 + a^2), P, L, L^2
+  int rt;                                          // reference: none, fixed, top, bottom
+  bool stt, alt;                                   // Slonczewski torque on this layer; alternative form
 };
 
 // ND: number of time-dependent records when fixed at compile time (exact
@@ -96,6 +100,7 @@ struct CLConst {
 template <int L, uint32_t FM, int ND, bool F64 = false>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
+  static_assert(F64 || !(FM & F_STT), "spin-transfer torque: binary64 kernels only");
   constexpr int LPI = 4 * L;
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t inst = gl / LPI;
@@ -153,6 +158,18 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.alphaf = (float)K.alpha;
     K.gdtf = (float)P.gconv * (float)dt;
     K.aknf = (float)K.akn;
+    const int bits = g.lbits[lyr];
+    K.rt = bits & 3;
+    K.stt = live && ((bits >> 2) & 3) == 1;
+    K.alt = ((bits >> 5) & 1) != 0;
+    K.sttI = P.stt_pref * base.I;
+    K.kap = P.kappa;
+    K.bet = P.beta;
+    K.ng = P.neg_gamma;
+    K.cdt = P.conv * dt;
+    K.Pp = P.P;
+    K.lam = P.lam;
+    K.lam2 = P.lam2;
   }
   constexpr int NR = (ND >= 0) ? ND : DREG;  // records kept in registers
   static_assert(NR <= DREG, "at most DREG records in registers");
@@ -222,6 +239,32 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
       }
     }
+    // spin-transfer torque (solveLLG, core/junction.hpp:743-769) with the
+    // reference vector of the layer: fixed, or the start-of-step top/bottom
+    // neighbour; computed for every lane and selected, so the quad algebra
+    // never runs under a divergent mask
+    double refc = 0.0;
+    if constexpr (F64 && (FM & F_STT) != 0) {
+      if (CMTJ_HAS(F_STT))
+        refc = (K.rt == CMTJ_REF_FIXED) ? K.ref : (K.rt == CMTJ_REF_TOP) ? top : (K.rt == CMTJ_REF_BOTTOM) ? bot : 0.0;
+    }
+    // dm/dt dt from m (ms, its rotations s1, s2) and P = m x H, Q = m x (m x H)
+    auto torque = [&](double ms, double s1, double s2, double Pc, double Qc) -> double {
+      const double plain = fma(Qc, K.alpha, Pc) * K.gdt;
+      if constexpr (F64 && (FM & F_STT) != 0) {
+        if (CMTJ_HAS(F_STT)) {
+          const double mp = qsum(ms * refc);
+          const double den = K.alt ? fma(K.lam, mp, 1.0) : fma(K.lam2 - 1.0, mp, K.lam2 + 1.0);
+          const double eta = (K.alt ? K.Pp : K.Pp * K.lam2) / den;
+          const double stt = K.sttI * eta;
+          const double FLc = cross_c(s1, s2, refc);  // m x p
+          const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
+          const double r = (fma(Qc, K.alpha, Pc) * K.ng + DLc * (-stt * K.kap) + FLc * (stt * K.bet)) * K.cdt;
+          return K.stt ? r : plain;
+        }
+      }
+      return plain;
+    };
     // m of the other two components (dot products below sum the three
     // products of each lane's own view: the quad's lanes may differ in the
     // last bit of a dot product, never more)
@@ -234,7 +277,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     // argument, not the stored vector), so m (m . H) - H is no substitute
     const double P0 = cross_c(m1, m2, H);
     const double Q0 = cross_c(m1, m2, P0);
-    const double k1 = fma(Q0, K.alpha, P0) * K.gdt;
+    const double k1 = torque(m, m1, m2, P0, Q0);
 
     // ---- stages 2-4: binary32 increments about the anchor
     const float m0a = (float)m1, m0b = (float)m2;
@@ -304,7 +347,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         Hs = fma(-K.nms, ms, Hs);
         const double Pc = cross_c(s1, s2, Hs);
         const double Qc = cross_c(s1, s2, Pc);
-        return fma(Qc, K.alpha, Pc) * K.gdt;
+        return torque(ms, s1, s2, Pc, Qc);
       };
       const double Hmid = hdrive(t + 0.5 * dt, 1);
       const double k2 = f(m + k1 * 0.5, Hmid);

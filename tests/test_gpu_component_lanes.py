@@ -10,8 +10,8 @@ import pytest
 
 import oracle
 import parity_cases
-from cmtj_amd import (AxialDriver, CVector, Junction, Layer, NullDriver, SolverMode, _lib, constantDriver,
-                      sineDriver, stepDriver, workloads)
+from cmtj_amd import (AxialDriver, CVector, Junction, Layer, NullDriver, Reference, SolverMode, _lib,
+                      constantDriver, sineDriver, stepDriver, workloads)
 from cmtj_amd.batch import JunctionBatch, _channel_names
 
 pytestmark = pytest.mark.gpu
@@ -150,6 +150,56 @@ def test_fp64_component_lanes_match_oracle(name):
     fm = float(np.max(np.abs(r.final_m.cpu().numpy()[:, :nl] - omag[:, :nl])))
     print(f"[cl64] {name}: max relative difference {worst:.3e}, final |dm| {fm:.3e}", flush=True)
     assert worst <= 1e-8 and fm <= 1e-8
+
+
+def _stt_constant(alternative):
+    """STT at a constant current against an unnormalised fixed polariser
+    (kappa, beta, alternative form), anisotropy and constant field."""
+    l = Layer.createSTTLayer("free", mag=CVector(0.1, 0.2, 1.0), anis=CVector(0, 0, 1.0), Ms=1.0, thickness=1.4e-9,
+                             cellSurface=7e-10 * 7e-10, demagTensor=parity_cases.DEMAG, damping=0.05,
+                             SlonczewskiSpacerLayerParameter=1.5, beta=0.3, spinPolarisation=0.6)
+    l.setReferenceLayer(CVector(0, 1.0, 1.0))
+    l.setKappa(0.9)
+    l.setAlternativeSTT(alternative)
+    j = Junction([l], 100, 200)
+    j.setLayerAnisotropyDriver("free", constantDriver(300e3))
+    j.setLayerCurrentDriver("free", constantDriver(-6e9))
+    j.setLayerExternalFieldDriver("all", AxialDriver(5e3, 1e3, 2e3))
+    return j
+
+
+@pytest.mark.parametrize("name", ["sto", "stt_fixed", "stt_alternative", "stt_ref_top", "stt_ref_bottom"])
+def test_fp64_component_lanes_spin_transfer_torque(name):
+    """Spin-transfer torque at a constant current in the binary64
+    component-lane kernels: fixed, top and bottom polarisers, both eta
+    forms, against the oracle at the drop-in's tolerance."""
+    if name == "sto":
+        b = workloads.sto(n_currents=8, total_time=0.5e-9, precision="fp64", log="full")
+        b.log_dtype = _lib.LOG_F64
+    else:
+        if name.startswith("stt_ref"):
+            j, _ = parity_cases.stt_neighbour_reference(Junction, Reference.top if name.endswith("top")
+                                                        else Reference.bottom)
+        else:
+            j = _stt_constant(name == "stt_alternative")
+        b = JunctionBatch.from_junctions([j] * 3, 1e-9, 1e-13, 1e-12, SolverMode.RK4, precision="fp64", log="full",
+                                         log_dtype="f64")
+    b.set_lanes("per_component")
+    kname = b.plan().info.kernel_name.decode()
+    print(f"[cl64-stt] {name}: {kname}", flush=True)
+    assert kname.startswith("k_junction_cl<double")
+    r = b.run()
+    olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
+    assert (r.status()["code"] == 0).all()
+    glog = r.log.cpu().numpy()
+    worst = 0.0
+    for i in range(len(b.js)):
+        g, o = _channels(b, glog, i), _channels(b, olog, i)
+        for k in o:
+            if len(o[k]):
+                worst = max(worst, float(np.max(np.abs(g[k] - o[k]))) / max(1.0, float(np.max(np.abs(o[k])))))
+    print(f"[cl64-stt] {name}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
 
 
 @pytest.mark.parametrize("bad_layer", [0, 1])
