@@ -88,10 +88,11 @@ __device__ __forceinline__ bool feeds(int meta, int ly, int c) {
 struct CLConst {
   double hb, an, an1, an2, nms, alpha, gdt, akn, jb, jt, ref;  // binary64 anchor
   float anf, an1f, an2f, nmsf, alphaf, gdtf, aknf;             // binary32 increments
-  // spin-transfer torque (binary64 kernels, constant current)
+  // spin torques (binary64 kernels, constant current)
   double sttI, kap, bet, ng, cdt, Pp, lam, lam2;  // gamma hbar I / (e Ms t), kappa, beta, -gamma, dt / (1 + a^2), P, L, L^2
+  double sa1, sb1;                                // SOT: Hfl - a Hdl, Hdl + a Hfl
   int rt;                                          // reference: none, fixed, top, bottom
-  bool stt, alt;                                   // Slonczewski torque on this layer; alternative form
+  bool stt, alt, sot;                              // Slonczewski torque on this layer; alternative form; SOT
 };
 
 // ND: number of time-dependent records when fixed at compile time (exact
@@ -100,7 +101,7 @@ struct CLConst {
 template <int L, uint32_t FM, int ND, bool F64 = false>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
-  static_assert(F64 || !(FM & F_STT), "spin-transfer torque: binary64 kernels only");
+  static_assert(F64 || !(FM & (F_STT | F_SOT)), "spin torques: binary64 kernels only");
   constexpr int LPI = 4 * L;
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t inst = gl / LPI;
@@ -170,6 +171,14 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.Pp = P.P;
     K.lam = P.lam;
     K.lam2 = P.lam2;
+    // SOT of solveLLG (core/junction.hpp:770-812): dynamic coefficients are
+    // the drivers' (constant here), static ones scale the current
+    K.sot = live && ((bits >> 2) & 3) == 2;
+    const bool dyn_sot = ((bits >> 4) & 1) != 0;
+    const double Hdl = dyn_sot ? base.DL : P.dl_t * base.I;
+    const double Hfl = dyn_sot ? base.FL : P.fl_t * base.I;
+    K.sa1 = Hfl - P.alpha * Hdl;
+    K.sb1 = Hdl + P.alpha * Hfl;
   }
   constexpr int NR = (ND >= 0) ? ND : DREG;  // records kept in registers
   static_assert(NR <= DREG, "at most DREG records in registers");
@@ -239,13 +248,13 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
       }
     }
-    // spin-transfer torque (solveLLG, core/junction.hpp:743-769) with the
-    // reference vector of the layer: fixed, or the start-of-step top/bottom
-    // neighbour; computed for every lane and selected, so the quad algebra
-    // never runs under a divergent mask
+    // spin torques (solveLLG, core/junction.hpp:743-812) with the reference
+    // vector of the layer: fixed, or the start-of-step top/bottom neighbour;
+    // computed for every lane and selected, so the quad algebra never runs
+    // under a divergent mask
     double refc = 0.0;
-    if constexpr (F64 && (FM & F_STT) != 0) {
-      if (CMTJ_HAS(F_STT))
+    if constexpr (F64 && (FM & (F_STT | F_SOT)) != 0) {
+      if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT))
         refc = (K.rt == CMTJ_REF_FIXED) ? K.ref : (K.rt == CMTJ_REF_TOP) ? top : (K.rt == CMTJ_REF_BOTTOM) ? bot : 0.0;
     }
     // dm/dt dt from m (ms, its rotations s1, s2) and P = m x H, Q = m x (m x H)
@@ -260,7 +269,18 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           const double FLc = cross_c(s1, s2, refc);  // m x p
           const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
           const double r = (fma(Qc, K.alpha, Pc) * K.ng + DLc * (-stt * K.kap) + FLc * (stt * K.bet)) * K.cdt;
-          return K.stt ? r : plain;
+          if constexpr ((FM & F_SOT) == 0) return K.stt ? r : plain;
+          if (!CMTJ_HAS(F_SOT)) return K.stt ? r : plain;
+          const double rs = fma(DLc, K.sb1, fma(FLc, K.sa1, fma(Qc, K.alpha, Pc))) * K.gdt;
+          return K.stt ? r : (K.sot ? rs : plain);
+        }
+      }
+      if constexpr (F64 && (FM & F_SOT) != 0) {
+        if (CMTJ_HAS(F_SOT)) {
+          const double FLc = cross_c(s1, s2, refc);  // m x p
+          const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
+          const double rs = fma(DLc, K.sb1, fma(FLc, K.sa1, fma(Qc, K.alpha, Pc))) * K.gdt;
+          return K.sot ? rs : plain;
         }
       }
       return plain;

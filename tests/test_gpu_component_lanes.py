@@ -168,11 +168,37 @@ def _stt_constant(alternative):
     return j
 
 
-@pytest.mark.parametrize("name", ["sto", "stt_fixed", "stt_alternative", "stt_ref_top", "stt_ref_bottom"])
+def _sot_constant(dynamic):
+    """SOT at a constant current (static coefficients) or with constant
+    field-like / damping-like drivers (dynamic), two layers with IEC."""
+    kw = dict(mag=CVector(0.001, 0.99, 0.1), anis=CVector(0, 0, 1.0), Ms=1.45, thickness=1.7e-9, cellSurface=1e-16,
+              demagTensor=parity_cases.DEMAG, damping=0.0064)
+    if dynamic:  # the reference refuses dynamic drivers on a static SOT layer
+        a = Layer("free", **kw)
+    else:
+        a = Layer.createSOTLayer("free", dampingLikeTorque=0.000371 * 795774.715459 / 6e10,
+                                 fieldLikeTorque=-0.000433 * 795774.715459 / 6e10, **kw)
+    b = Layer("pinned", mag=CVector(0, 0, 1), anis=CVector(0, 0, 1), Ms=1.2, thickness=2e-9, cellSurface=1e-16,
+              demagTensor=parity_cases.DEMAG, damping=0.02)
+    j = Junction([a, b], 100, 200)
+    j.setLayerAnisotropyDriver("free", constantDriver(0.3e6))
+    j.setLayerAnisotropyDriver("pinned", constantDriver(1e6))
+    j.setIECDriver("free", "pinned", constantDriver(-1e-4))
+    j.setLayerReferenceLayer("free", CVector(0, 1, 0))
+    if dynamic:
+        j.setLayerPrimaryTorqueDrivers("free", constantDriver(-2e3), constantDriver(5e3))
+    else:
+        j.setLayerCurrentDriver("free", constantDriver(0.08e12))
+    return j
+
+
+@pytest.mark.parametrize("name", ["sto", "stt_fixed", "stt_alternative", "stt_ref_top", "stt_ref_bottom",
+                                  "sot_static", "sot_dynamic"])
 def test_fp64_component_lanes_spin_transfer_torque(name):
-    """Spin-transfer torque at a constant current in the binary64
-    component-lane kernels: fixed, top and bottom polarisers, both eta
-    forms, against the oracle at the drop-in's tolerance."""
+    """Spin torques at a constant current in the binary64 component-lane
+    kernels: STT with fixed, top and bottom polarisers and both eta forms,
+    SOT with static and (constant) dynamic coefficients, against the oracle
+    at the drop-in's tolerance."""
     if name == "sto":
         b = workloads.sto(n_currents=8, total_time=0.5e-9, precision="fp64", log="full")
         b.log_dtype = _lib.LOG_F64
@@ -180,6 +206,8 @@ def test_fp64_component_lanes_spin_transfer_torque(name):
         if name.startswith("stt_ref"):
             j, _ = parity_cases.stt_neighbour_reference(Junction, Reference.top if name.endswith("top")
                                                         else Reference.bottom)
+        elif name.startswith("sot"):
+            j = _sot_constant(name == "sot_dynamic")
         else:
             j = _stt_constant(name == "stt_alternative")
         b = JunctionBatch.from_junctions([j] * 3, 1e-9, 1e-13, 1e-12, SolverMode.RK4, precision="fp64", log="full",
