@@ -1,12 +1,12 @@
 // One-component-per-lane kernels in binary64 (every RK4 stage, the
-// drop-in's arithmetic): the feature sets of the binary32 ones, and
-// spin-transfer torque at a constant current.
+// drop-in's arithmetic): the feature sets of the binary32 ones, and a set
+// with spin torques and time-dependent scalar drives.
 #include "registry.h"
 namespace cmtj {
 static const KEntry kC64[] = {
     CMTJ_KC64(2, FM_PIMM, 1),  CMTJ_KC64(2, FM_PIMM, -1), CMTJ_KC64(4, FM_PIMM, -1), CMTJ_KC64(1, FM_RH, 0),
     CMTJ_KC64(1, FM_VSD, 1),   CMTJ_KC64(1, FM_LEAN, -1), CMTJ_KC64(2, FM_LEAN, -1), CMTJ_KC64(4, FM_LEAN, -1),
-    // spin-transfer torque at a constant current: the STO set and the lean set
+    // spin torques, scalar drives: the STO set and the lean set with both
     CMTJ_KC64(2, FM_STO, -1),  CMTJ_KC64(1, FM_STTC, -1), CMTJ_KC64(2, FM_STTC, -1), CMTJ_KC64(4, FM_STTC, -1),
 };
 KTable kernels_cl64() { return {kC64, sizeof(kC64) / sizeof(kC64[0])}; }

@@ -88,12 +88,40 @@ __device__ __forceinline__ bool feeds(int meta, int ly, int c) {
 struct CLConst {
   double hb, an, an1, an2, nms, alpha, gdt, akn, jb, jt, ref;  // binary64 anchor
   float anf, an1f, an2f, nmsf, alphaf, gdtf, aknf;             // binary32 increments
-  // spin torques (binary64 kernels, constant current)
-  double sttI, kap, bet, ng, cdt, Pp, lam, lam2;  // gamma hbar I / (e Ms t), kappa, beta, -gamma, dt / (1 + a^2), P, L, L^2
-  double sa1, sb1;                                // SOT: Hfl - a Hdl, Hdl + a Hfl
-  int rt;                                          // reference: none, fixed, top, bottom
-  bool stt, alt, sot;                              // Slonczewski torque on this layer; alternative form; SOT
+  // binary64 kernels: the layer's scalar drives (constant parts) and spin
+  // torques
+  double bK, bJt, bJb, bI, bFL, bDL;        // anisotropy, IEC top/bottom, current, SOT field-/damping-like
+  double inv_ms, inv_mst;                   // 1 / Ms, 1 / (Ms t)
+  double sttp, kap, bet, ng, cdt, Pp, lam, lam2;  // gamma hbar / (e Ms t), kappa, beta, -gamma, dt / (1 + a^2), P, L, L^2
+  double dlt, flt;                          // static SOT coefficients
+  int rt;                                   // reference: none, fixed, top, bottom
+  bool stt, alt, sot, dyn_sot;              // Slonczewski torque; alternative form; SOT; SOT from drivers
 };
+
+// The drives of one stage time as a binary64 lane sees them: its field
+// component and its layer's scalar slots.
+struct CLDrive {
+  double H, K, Jt, Jb, I, FL, DL;
+};
+
+// Which scalar slot of layer ly a time-dependent record feeds: 1 anisotropy,
+// 2 IEC top, 3 IEC bottom, 4 current, 5 field-like, 6 damping-like torque;
+// 0 none (field slots, other slots, other layers).
+__device__ __forceinline__ int scalar_slot(int meta, int ly) {
+  const int slot = (meta >> 8) & 0xff;
+  if ((meta & 0xff) == CMTJ_CONSTANT || !(((meta >> 16) & 0xff) >> ly & 1)) return 0;
+  return slot == CMTJ_SLOT_K ? 1 : slot == CMTJ_SLOT_J_TOP ? 2 : slot == CMTJ_SLOT_J_BOTTOM ? 3
+       : slot == CMTJ_SLOT_CURRENT ? 4 : slot == CMTJ_SLOT_FL ? 5 : slot == CMTJ_SLOT_DL ? 6 : 0;
+}
+
+__device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
+  d.K += kind == 1 ? v : 0.0;
+  d.Jt += kind == 2 ? v : 0.0;
+  d.Jb += kind == 3 ? v : 0.0;
+  d.I += kind == 4 ? v : 0.0;
+  d.FL += kind == 5 ? v : 0.0;
+  d.DL += kind == 6 ? v : 0.0;
+}
 
 // ND: number of time-dependent records when fixed at compile time (exact
 // kernels), -1 to read it from the group.  F64: every stage in binary64
@@ -159,11 +187,19 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.alphaf = (float)K.alpha;
     K.gdtf = (float)P.gconv * (float)dt;
     K.aknf = (float)K.akn;
+    K.bK = base.K;
+    K.bJt = base.Jt;
+    K.bJb = base.Jb;
+    K.bI = base.I;
+    K.bFL = base.FL;
+    K.bDL = base.DL;
+    K.inv_ms = P.inv_ms;
+    K.inv_mst = P.inv_mst;
     const int bits = g.lbits[lyr];
     K.rt = bits & 3;
     K.stt = live && ((bits >> 2) & 3) == 1;
     K.alt = ((bits >> 5) & 1) != 0;
-    K.sttI = P.stt_pref * base.I;
+    K.sttp = P.stt_pref;
     K.kap = P.kappa;
     K.bet = P.beta;
     K.ng = P.neg_gamma;
@@ -171,19 +207,16 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.Pp = P.P;
     K.lam = P.lam;
     K.lam2 = P.lam2;
-    // SOT of solveLLG (core/junction.hpp:770-812): dynamic coefficients are
-    // the drivers' (constant here), static ones scale the current
     K.sot = live && ((bits >> 2) & 3) == 2;
-    const bool dyn_sot = ((bits >> 4) & 1) != 0;
-    const double Hdl = dyn_sot ? base.DL : P.dl_t * base.I;
-    const double Hfl = dyn_sot ? base.FL : P.fl_t * base.I;
-    K.sa1 = Hfl - P.alpha * Hdl;
-    K.sb1 = Hdl + P.alpha * Hfl;
+    K.dyn_sot = ((bits >> 4) & 1) != 0;
+    K.dlt = P.dl_t;
+    K.flt = P.fl_t;
   }
   constexpr int NR = (ND >= 0) ? ND : DREG;  // records kept in registers
   static_assert(NR <= DREG, "at most DREG records in registers");
   DynRec dr[DREG];
   bool sel[DREG];
+  [[maybe_unused]] int skind[DREG];  // binary64 kernels: the scalar slot each record feeds
   double win_lo = __builtin_inf(), win_hi = -__builtin_inf();
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
@@ -192,9 +225,11 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       win_lo = fmin(win_lo, dr[k].p1);
       win_hi = fmax(win_hi, dr[k].p2);
       sel[k] = live && feeds(g.dmeta[k], ly, c);
+      skind[k] = lact ? scalar_slot(g.dmeta[k], ly) : 0;
     } else {
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
       sel[k] = false;
+      skind[k] = 0;
     }
   }
   const float sixth_inv = 1.0f / 6.0f;
@@ -228,157 +263,164 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
 
   for (; i < iend; ++i) {
     const double t = (double)i * dt;
-    // ---- anchor: dm/dt at (m, t) in binary64 (calculateHeffDipoleInjection,
-    // core/junction.hpp:590-622; solveLLG, :716-815)
-    double v0[DREG] = {0.0, 0.0};
-    double H = K.hb;
-    bool inwin = true;
-    if constexpr ((FM & F_KSTEP) != 0) inwin = t >= win_lo && t <= win_hi;
-    if (CMTJ_HAS(F_DYNH) && inwin) {
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        if (ND >= 0 || r < g.ndyn) {
-          v0[r] = wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, t, i, 0, sps);
-          H = H + (sel[r] ? v0[r] : 0.0);
-        }
-      }
-      for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
-        const DynRec rr = g.dyn[(size_t)r * g.n + inst];
-        const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, rr, g, t, i, 0, sps);
-        H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
-      }
-    }
-    // spin torques (solveLLG, core/junction.hpp:743-812) with the reference
-    // vector of the layer: fixed, or the start-of-step top/bottom neighbour;
-    // computed for every lane and selected, so the quad algebra never runs
-    // under a divergent mask
-    double refc = 0.0;
-    if constexpr (F64 && (FM & (F_STT | F_SOT)) != 0) {
-      if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT))
-        refc = (K.rt == CMTJ_REF_FIXED) ? K.ref : (K.rt == CMTJ_REF_TOP) ? top : (K.rt == CMTJ_REF_BOTTOM) ? bot : 0.0;
-    }
-    // dm/dt dt from m (ms, its rotations s1, s2) and P = m x H, Q = m x (m x H)
-    auto torque = [&](double ms, double s1, double s2, double Pc, double Qc) -> double {
-      const double plain = fma(Qc, K.alpha, Pc) * K.gdt;
-      if constexpr (F64 && (FM & F_STT) != 0) {
-        if (CMTJ_HAS(F_STT)) {
-          const double mp = qsum(ms * refc);
-          const double den = K.alt ? fma(K.lam, mp, 1.0) : fma(K.lam2 - 1.0, mp, K.lam2 + 1.0);
-          const double eta = (K.alt ? K.Pp : K.Pp * K.lam2) / den;
-          const double stt = K.sttI * eta;
-          const double FLc = cross_c(s1, s2, refc);  // m x p
-          const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
-          const double r = (fma(Qc, K.alpha, Pc) * K.ng + DLc * (-stt * K.kap) + FLc * (stt * K.bet)) * K.cdt;
-          if constexpr ((FM & F_SOT) == 0) return K.stt ? r : plain;
-          if (!CMTJ_HAS(F_SOT)) return K.stt ? r : plain;
-          const double rs = fma(DLc, K.sb1, fma(FLc, K.sa1, fma(Qc, K.alpha, Pc))) * K.gdt;
-          return K.stt ? r : (K.sot ? rs : plain);
-        }
-      }
-      if constexpr (F64 && (FM & F_SOT) != 0) {
-        if (CMTJ_HAS(F_SOT)) {
-          const double FLc = cross_c(s1, s2, refc);  // m x p
-          const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
-          const double rs = fma(DLc, K.sb1, fma(FLc, K.sa1, fma(Qc, K.alpha, Pc))) * K.gdt;
-          return K.sot ? rs : plain;
-        }
-      }
-      return plain;
-    };
-    // m of the other two components (dot products below sum the three
-    // products of each lane's own view: the quad's lanes may differ in the
-    // last bit of a dot product, never more)
-    const double m1 = dpp_d<QP_ROT1>(m), m2 = dpp_d<QP_ROT2>(m);
-    if (CMTJ_HAS(F_ANIS)) H = fma(K.akn, fma(K.an2, m2, fma(K.an1, m1, K.an * m)), H);
-    if (CMTJ_HAS(F_IEC)) H = fma(top, K.jt, fma(bot, K.jb, H));
-    H = fma(-K.nms, m, H);
-    // m x H and m x (m x H) as cross products: the initial m of a run need
-    // not have unit length (the reference normalises its constructor
-    // argument, not the stored vector), so m (m . H) - H is no substitute
-    const double P0 = cross_c(m1, m2, H);
-    const double Q0 = cross_c(m1, m2, P0);
-    const double k1 = torque(m, m1, m2, P0, Q0);
-
-    // ---- stages 2-4: binary32 increments about the anchor
-    const float m0a = (float)m1, m0b = (float)m2;
-    const float H0 = (float)H, Pf = (float)P0;
-    auto delta = [&](float d, float ddH) -> float {
-      // f(m0 + d, ts) - f(m0, t), component c (rhs_delta, kernels.h)
-      const float d1 = dpp_f<QP_ROT1>(d), d2 = dpp_f<QP_ROT2>(d);
-      float dH = fmaf(-K.nmsf, d, ddH);
-      if (CMTJ_HAS(F_ANIS)) dH = fmaf(K.aknf, fmaf(K.an2f, d2, fmaf(K.an1f, d1, K.anf * d)), dH);
-      const float Hs = H0 + dH;
-      const float dP = cross_c(m0a, m0b, dH) + cross_c(d1, d2, Hs);
-      const float dQ = cross_c(m0a, m0b, dP) + cross_c(d1, d2, Pf + dP);
-      return fmaf(dQ, K.alphaf, dP) * K.gdtf;
-    };
-    auto drive_delta = [&](double ts, int stage) -> float {
-      float dd = 0.f;
-      if (!CMTJ_HAS(F_DYNH)) return dd;
-      if constexpr ((FM & F_KSTEP) != 0) {
-        if (!(ts >= win_lo && t <= win_hi)) return dd;
-      }
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        if (ND >= 0 || r < g.ndyn) {
-          const float v = (float)(wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, ts, i, stage, sps) - v0[r]);
-          dd = dd + (sel[r] ? v : 0.f);
-        }
-      }
-      for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
-        const DynRec rr = g.dyn[(size_t)r * g.n + inst];
-        const int ty = g.dmeta[r] & 0xff;
-        const float v = (float)(wave_value<double, true, FM>(ty, rr, g, ts, i, stage, sps) -
-                                wave_value<double, true, FM>(ty, rr, g, t, i, 0, sps));
-        dd = dd + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.f);
-      }
-      return dd;
-    };
     double mn;
     if constexpr (F64) {
-      // stages 2-4 of rk4_step (core/junction.hpp:1017-1041) in binary64:
-      // the field drives at t + dt/2 (stages 2 and 3) and t + dt (stage 4)
-      auto hdrive = [&](double ts, int stage) -> double {
-        double Hd = K.hb;
-        if (!CMTJ_HAS(F_DYNH)) return Hd;
+      // rk4_step (core/junction.hpp:1017-1041) in binary64: every stage
+      // evaluates the same terms (calculateHeffDipoleInjection, :590-622;
+      // solveLLG, :716-815) at its time and magnetisation.  Spin torques use
+      // the layer's reference vector: fixed, or the start-of-step top/bottom
+      // neighbour; they are computed on every lane and selected per layer,
+      // so no quad operation runs under a divergent mask.
+      double refc = 0.0;
+      if constexpr ((FM & (F_STT | F_SOT)) != 0) {
+        if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT))
+          refc = (K.rt == CMTJ_REF_FIXED) ? K.ref : (K.rt == CMTJ_REF_TOP) ? top : (K.rt == CMTJ_REF_BOTTOM) ? bot : 0.0;
+      }
+      // drives at ts: this component's field slots and the layer's scalar
+      // slots, transcendental waveforms in full binary64 (the binary32 path
+      // below takes the faster single-precision ones)
+      auto drive = [&](double ts, int stage) -> CLDrive {
+        CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL};
+        if (!(CMTJ_HAS(F_DYNH) || CMTJ_HAS(F_DYN))) return d;
         if constexpr ((FM & F_KSTEP) != 0) {
           // step records are 0 outside [min t_start, max t_stop] (inclusive)
-          if (!(ts >= win_lo && ts <= win_hi)) return Hd;
+          if (!(ts >= win_lo && ts <= win_hi)) return d;
         }
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
           if (ND >= 0 || r < g.ndyn) {
-            const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, ts, i, stage, sps);
-            Hd = Hd + (sel[r] ? v : 0.0);
+            const double v = wave_value<double, false, FM>(g.dmeta[r] & 0xff, dr[r], g, ts, i, stage, sps);
+            d.H = d.H + (sel[r] ? v : 0.0);
+            if constexpr ((FM & F_DYN) != 0) scalar_add(d, skind[r], v);
           }
         }
         for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
           const DynRec rr = g.dyn[(size_t)r * g.n + inst];
-          const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, rr, g, ts, i, stage, sps);
-          Hd = Hd + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
+          const double v = wave_value<double, false, FM>(g.dmeta[r] & 0xff, rr, g, ts, i, stage, sps);
+          d.H = d.H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
+          if constexpr ((FM & F_DYN) != 0) scalar_add(d, lact ? scalar_slot(g.dmeta[r], ly) : 0, v);
         }
-        return Hd;
+        return d;
       };
-      // dm/dt dt at (ms, Hd): the anchor's terms at a stage magnetisation
-      auto f = [&](double ms, double Hs) -> double {
+      // dm/dt dt at (ms, drives d)
+      auto f = [&](double ms, const CLDrive& d) -> double {
         const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
-        if (CMTJ_HAS(F_ANIS)) Hs = fma(K.akn, fma(K.an2, s2, fma(K.an1, s1, K.an * ms)), Hs);
-        if (CMTJ_HAS(F_IEC)) Hs = fma(top, K.jt, fma(bot, K.jb, Hs));
+        double Hs = d.H;
+        if (CMTJ_HAS(F_ANIS)) Hs = fma(K.an * ((2.0 * d.K) * K.inv_ms), fma(K.an2, s2, fma(K.an1, s1, K.an * ms)), Hs);
+        if (CMTJ_HAS(F_IEC)) Hs = fma(top, d.Jt * K.inv_mst, fma(bot, d.Jb * K.inv_mst, Hs));
         Hs = fma(-K.nms, ms, Hs);
+        // m x H and m x (m x H) as cross products: the initial m of a run
+        // need not have unit length
         const double Pc = cross_c(s1, s2, Hs);
         const double Qc = cross_c(s1, s2, Pc);
-        return torque(ms, s1, s2, Pc, Qc);
+        double r = fma(Qc, K.alpha, Pc) * K.gdt;
+        if constexpr ((FM & (F_STT | F_SOT)) != 0) {
+          if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT)) {
+            const double FLc = cross_c(s1, s2, refc);  // m x p
+            const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
+            if (CMTJ_HAS(F_SOT)) {
+              // SOT (:770-812): coefficients from the drivers, or static
+              // ones scaling the current
+              const double Hdl = K.dyn_sot ? d.DL : K.dlt * d.I;
+              const double Hfl = K.dyn_sot ? d.FL : K.flt * d.I;
+              const double rs =
+                  fma(DLc, Hdl + K.alpha * Hfl, fma(FLc, Hfl - K.alpha * Hdl, fma(Qc, K.alpha, Pc))) * K.gdt;
+              r = K.sot ? rs : r;
+            }
+            if (CMTJ_HAS(F_STT)) {
+              // Slonczewski STT (:743-769)
+              const double mp = qsum(ms * refc);
+              const double den = K.alt ? fma(K.lam, mp, 1.0) : fma(K.lam2 - 1.0, mp, K.lam2 + 1.0);
+              const double eta = (K.alt ? K.Pp : K.Pp * K.lam2) / den;
+              const double stt = (K.sttp * d.I) * eta;
+              const double rt =
+                  (fma(Qc, K.alpha, Pc) * K.ng + DLc * (-stt * K.kap) + FLc * (stt * K.bet)) * K.cdt;
+              r = K.stt ? rt : r;
+            }
+          }
+        }
+        return r;
       };
-      const double Hmid = hdrive(t + 0.5 * dt, 1);
-      const double k2 = f(m + k1 * 0.5, Hmid);
-      const double k3 = f(m + k2 * 0.5, Hmid);
-      const double k4 = f(m + k3, hdrive(t + dt, 2));
+      const double k1 = f(m, drive(t, 0));
+      const CLDrive dmid = drive(t + 0.5 * dt, 1);
+      const double k2 = f(m + k1 * 0.5, dmid);
+      const double k3 = f(m + k2 * 0.5, dmid);
+      const double k4 = f(m + k3, drive(t + dt, 2));
       mn = m + (k1 + (k2 * 2.0) + (k3 * 2.0) + k4) / 6.0;
       // CVector::normalize (core/cvector.hpp:185-192): the quad sum adds
       // (x^2 + y^2) + z^2 in the reference's order; a zero vector stays
       const double ss = qsum(mn * mn);
       mn = (ss != 0.0) ? mn / sqrt(ss) : mn;
     } else {
+      // ---- anchor: dm/dt at (m, t) in binary64 (calculateHeffDipoleInjection,
+      // core/junction.hpp:590-622; solveLLG, :716-815)
+      double v0[DREG] = {0.0, 0.0};
+      double H = K.hb;
+      bool inwin = true;
+      if constexpr ((FM & F_KSTEP) != 0) inwin = t >= win_lo && t <= win_hi;
+      if (CMTJ_HAS(F_DYNH) && inwin) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          if (ND >= 0 || r < g.ndyn) {
+            v0[r] = wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, t, i, 0, sps);
+            H = H + (sel[r] ? v0[r] : 0.0);
+          }
+        }
+        for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
+          const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+          const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, rr, g, t, i, 0, sps);
+          H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
+        }
+      }
+      // m of the other two components (dot products below sum the three
+      // products of each lane's own view: the quad's lanes may differ in the
+      // last bit of a dot product, never more)
+      const double m1 = dpp_d<QP_ROT1>(m), m2 = dpp_d<QP_ROT2>(m);
+      if (CMTJ_HAS(F_ANIS)) H = fma(K.akn, fma(K.an2, m2, fma(K.an1, m1, K.an * m)), H);
+      if (CMTJ_HAS(F_IEC)) H = fma(top, K.jt, fma(bot, K.jb, H));
+      H = fma(-K.nms, m, H);
+      // m x H and m x (m x H) as cross products: the initial m of a run need
+      // not have unit length (the reference normalises its constructor
+      // argument, not the stored vector), so m (m . H) - H is no substitute
+      const double P0 = cross_c(m1, m2, H);
+      const double Q0 = cross_c(m1, m2, P0);
+      const double k1 = fma(Q0, K.alpha, P0) * K.gdt;
+
+      // ---- stages 2-4: binary32 increments about the anchor
+      const float m0a = (float)m1, m0b = (float)m2;
+      const float H0 = (float)H, Pf = (float)P0;
+      auto delta = [&](float d, float ddH) -> float {
+        // f(m0 + d, ts) - f(m0, t), component c (rhs_delta, kernels.h)
+        const float d1 = dpp_f<QP_ROT1>(d), d2 = dpp_f<QP_ROT2>(d);
+        float dH = fmaf(-K.nmsf, d, ddH);
+        if (CMTJ_HAS(F_ANIS)) dH = fmaf(K.aknf, fmaf(K.an2f, d2, fmaf(K.an1f, d1, K.anf * d)), dH);
+        const float Hs = H0 + dH;
+        const float dP = cross_c(m0a, m0b, dH) + cross_c(d1, d2, Hs);
+        const float dQ = cross_c(m0a, m0b, dP) + cross_c(d1, d2, Pf + dP);
+        return fmaf(dQ, K.alphaf, dP) * K.gdtf;
+      };
+      auto drive_delta = [&](double ts, int stage) -> float {
+        float dd = 0.f;
+        if (!CMTJ_HAS(F_DYNH)) return dd;
+        if constexpr ((FM & F_KSTEP) != 0) {
+          if (!(ts >= win_lo && t <= win_hi)) return dd;
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          if (ND >= 0 || r < g.ndyn) {
+            const float v = (float)(wave_value<double, true, FM>(g.dmeta[r] & 0xff, dr[r], g, ts, i, stage, sps) - v0[r]);
+            dd = dd + (sel[r] ? v : 0.f);
+          }
+        }
+        for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
+          const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+          const int ty = g.dmeta[r] & 0xff;
+          const float v = (float)(wave_value<double, true, FM>(ty, rr, g, ts, i, stage, sps) -
+                                  wave_value<double, true, FM>(ty, rr, g, t, i, 0, sps));
+          dd = dd + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.f);
+        }
+        return dd;
+      };
       const float dmid = drive_delta(t + 0.5 * dt, 1);
       const float d2 = (float)(k1 * 0.5);
       float dk = delta(d2, dmid);

@@ -26,8 +26,8 @@ constexpr uint32_t FM_LEAN = F_ANIS | F_IEC | F_DYNH;
 constexpr uint32_t FM_STT = F_ANIS | F_IEC | F_DYNH | F_STT;
 constexpr uint32_t FM_SOTT = F_ANIS | F_IEC | F_DYN | F_DYNH | F_SOT | F_THERMAL;
 // one component per lane, binary64: the lean set with spin torques (STT,
-// SOT) at a constant current
-constexpr uint32_t FM_STTC = F_ANIS | F_IEC | F_DYNH | F_STT | F_SOT;
+// SOT) and time-dependent anisotropy, IEC, current and torque drivers
+constexpr uint32_t FM_STTC = F_ANIS | F_IEC | F_DYNH | F_DYN | F_STT | F_SOT;
 // coupled runs (stacks, groups): the usual member's features
 constexpr uint32_t FM_CPL = F_ANIS | F_IEC | F_DYN | F_DYNH | F_STT | F_SOT | F_THERMAL;
 

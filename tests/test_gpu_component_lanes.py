@@ -192,13 +192,22 @@ def _sot_constant(dynamic):
     return j
 
 
+def _iec_dynamic():
+    j = _chain(2, mr="classic")
+    j.setIECDriver("L0", "L1", sineDriver(-2e-4, 5e-5, 1.3e9, 0.2))
+    j.setLayerAnisotropyDriver("L1", sineDriver(30e3, 10e3, 0.7e9, 0.0))
+    return j
+
+
 @pytest.mark.parametrize("name", ["sto", "stt_fixed", "stt_alternative", "stt_ref_top", "stt_ref_bottom",
-                                  "sot_static", "sot_dynamic"])
+                                  "sot_static", "sot_dynamic", "stt_drivers", "stt_drivers_alternative",
+                                  "sot_step_current", "iec_dynamic"])
 def test_fp64_component_lanes_spin_transfer_torque(name):
-    """Spin torques at a constant current in the binary64 component-lane
-    kernels: STT with fixed, top and bottom polarisers and both eta forms,
-    SOT with static and (constant) dynamic coefficients, against the oracle
-    at the drop-in's tolerance."""
+    """Spin torques and time-dependent scalar drives in the binary64
+    component-lane kernels: STT with fixed, top and bottom polarisers and
+    both eta forms, SOT with static and dynamic coefficients, sine and step
+    currents, trapezoid and sine anisotropy, a sine IEC coupling, against the
+    oracle at the drop-in's tolerance."""
     if name == "sto":
         b = workloads.sto(n_currents=8, total_time=0.5e-9, precision="fp64", log="full")
         b.log_dtype = _lib.LOG_F64
@@ -206,8 +215,14 @@ def test_fp64_component_lanes_spin_transfer_torque(name):
         if name.startswith("stt_ref"):
             j, _ = parity_cases.stt_neighbour_reference(Junction, Reference.top if name.endswith("top")
                                                         else Reference.bottom)
-        elif name.startswith("sot"):
+        elif name in ("sot_static", "sot_dynamic"):
             j = _sot_constant(name == "sot_dynamic")
+        elif name.startswith("stt_drivers"):
+            j, _ = parity_cases.stt_drivers(Junction, alternative=name.endswith("alternative"))
+        elif name == "sot_step_current":
+            j, _ = parity_cases.sot_static(Junction)
+        elif name == "iec_dynamic":
+            j = _iec_dynamic()
         else:
             j = _stt_constant(name == "stt_alternative")
         b = JunctionBatch.from_junctions([j] * 3, 1e-9, 1e-13, 1e-12, SolverMode.RK4, precision="fp64", log="full",
