@@ -8,6 +8,9 @@ static const KEntry kC64[] = {
     CMTJ_KC64(1, FM_VSD, 1),   CMTJ_KC64(1, FM_LEAN, -1), CMTJ_KC64(2, FM_LEAN, -1), CMTJ_KC64(4, FM_LEAN, -1),
     // spin torques, scalar drives: the STO set and the lean set with both
     CMTJ_KC64(2, FM_STO, -1),  CMTJ_KC64(1, FM_STTC, -1), CMTJ_KC64(2, FM_STTC, -1), CMTJ_KC64(4, FM_STTC, -1),
+    // Euler-Heun (stochastic runs): the same set with temperature
+    CMTJ_KC64EH(1, FM_STTC | F_THERMAL, -1), CMTJ_KC64EH(2, FM_STTC | F_THERMAL, -1),
+    CMTJ_KC64EH(4, FM_STTC | F_THERMAL, -1),
 };
 KTable kernels_cl64() { return {kC64, sizeof(kC64) / sizeof(kC64[0])}; }
 }  // namespace cmtj

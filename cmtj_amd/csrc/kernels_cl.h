@@ -90,28 +90,30 @@ struct CLConst {
   float anf, an1f, an2f, nmsf, alphaf, gdtf, aknf;             // binary32 increments
   // binary64 kernels: the layer's scalar drives (constant parts) and spin
   // torques
-  double bK, bJt, bJb, bI, bFL, bDL;        // anisotropy, IEC top/bottom, current, SOT field-/damping-like
+  double bK, bJt, bJb, bI, bFL, bDL, bT;    // anisotropy, IEC top/bottom, current, SOT field-/damping-like, T
+  double sig2, gconv;                       // 2 alpha kB / (Ms V gamma), -gamma / (1 + alpha^2)
   double inv_ms, inv_mst;                   // 1 / Ms, 1 / (Ms t)
   double sttp, kap, bet, ng, cdt, Pp, lam, lam2;  // gamma hbar / (e Ms t), kappa, beta, -gamma, dt / (1 + a^2), P, L, L^2
   double dlt, flt;                          // static SOT coefficients
   int rt;                                   // reference: none, fixed, top, bottom
-  bool stt, alt, sot, dyn_sot;              // Slonczewski torque; alternative form; SOT; SOT from drivers
+  bool stt, alt, sot, dyn_sot, therm;       // Slonczewski torque; alternative form; SOT; SOT from drivers; thermal
 };
 
 // The drives of one stage time as a binary64 lane sees them: its field
 // component and its layer's scalar slots.
 struct CLDrive {
-  double H, K, Jt, Jb, I, FL, DL;
+  double H, K, Jt, Jb, I, FL, DL, T;
 };
 
 // Which scalar slot of layer ly a time-dependent record feeds: 1 anisotropy,
-// 2 IEC top, 3 IEC bottom, 4 current, 5 field-like, 6 damping-like torque;
-// 0 none (field slots, other slots, other layers).
+// 2 IEC top, 3 IEC bottom, 4 current, 5 field-like, 6 damping-like torque,
+// 7 temperature; 0 none (field slots, other slots, other layers).
 __device__ __forceinline__ int scalar_slot(int meta, int ly) {
   const int slot = (meta >> 8) & 0xff;
   if ((meta & 0xff) == CMTJ_CONSTANT || !(((meta >> 16) & 0xff) >> ly & 1)) return 0;
   return slot == CMTJ_SLOT_K ? 1 : slot == CMTJ_SLOT_J_TOP ? 2 : slot == CMTJ_SLOT_J_BOTTOM ? 3
-       : slot == CMTJ_SLOT_CURRENT ? 4 : slot == CMTJ_SLOT_FL ? 5 : slot == CMTJ_SLOT_DL ? 6 : 0;
+       : slot == CMTJ_SLOT_CURRENT ? 4 : slot == CMTJ_SLOT_FL ? 5 : slot == CMTJ_SLOT_DL ? 6
+       : slot == CMTJ_SLOT_TEMPERATURE ? 7 : 0;
 }
 
 __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
@@ -121,15 +123,17 @@ __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
   d.I += kind == 4 ? v : 0.0;
   d.FL += kind == 5 ? v : 0.0;
   d.DL += kind == 6 ? v : 0.0;
+  d.T += kind == 7 ? v : 0.0;
 }
 
 // ND: number of time-dependent records when fixed at compile time (exact
 // kernels), -1 to read it from the group.  F64: every stage in binary64
 // (the drop-in's arithmetic) instead of the anchored binary32 increments.
-template <int L, uint32_t FM, int ND, bool F64 = false>
+template <int L, uint32_t FM, int ND, bool F64 = false, int SOLVER = CMTJ_RK4>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
-  static_assert(F64 || !(FM & (F_STT | F_SOT)), "spin torques: binary64 kernels only");
+  static_assert(F64 || !(FM & (F_STT | F_SOT | F_DYN | F_THERMAL)), "binary64 kernels only");
+  static_assert(SOLVER == CMTJ_RK4 || (F64 && SOLVER == CMTJ_EULER_HEUN), "RK4, or Euler-Heun in binary64");
   constexpr int LPI = 4 * L;
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t inst = gl / LPI;
@@ -154,6 +158,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   char* const lp = logging ? (char*)g.log + lbase * esize : nullptr;
   const uint64_t cs = (uint64_t)NE * esize;
   const double Rp = J->Rp, Rap = J->Rap;
+  [[maybe_unused]] const uint64_t seed = J->seed, stream = J->stream;
+  [[maybe_unused]] const double sdt = sqrt(dt);
   constexpr int sps = 3;
   const bool hasB = lact && ly > 0, hasT = lact && ly + 1 < nl;
   const uint32_t gshift = threadIdx.x & ~(uint32_t)(LPI - 1);
@@ -193,6 +199,9 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.bI = base.I;
     K.bFL = base.FL;
     K.bDL = base.DL;
+    K.bT = base.T;
+    K.sig2 = P.sig2;
+    K.gconv = P.gconv;
     K.inv_ms = P.inv_ms;
     K.inv_mst = P.inv_mst;
     const int bits = g.lbits[lyr];
@@ -211,6 +220,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.dyn_sot = ((bits >> 4) & 1) != 0;
     K.dlt = P.dl_t;
     K.flt = P.fl_t;
+    K.therm = live && ((bits >> 6) & 1) != 0;
   }
   constexpr int NR = (ND >= 0) ? ND : DREG;  // records kept in registers
   static_assert(NR <= DREG, "at most DREG records in registers");
@@ -280,7 +290,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // slots, transcendental waveforms in full binary64 (the binary32 path
       // below takes the faster single-precision ones)
       auto drive = [&](double ts, int stage) -> CLDrive {
-        CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL};
+        CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL, K.bT};
         if (!(CMTJ_HAS(F_DYNH) || CMTJ_HAS(F_DYN))) return d;
         if constexpr ((FM & F_KSTEP) != 0) {
           // step records are 0 outside [min t_start, max t_stop] (inclusive)
@@ -341,12 +351,35 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         }
         return r;
       };
-      const double k1 = f(m, drive(t, 0));
-      const CLDrive dmid = drive(t + 0.5 * dt, 1);
-      const double k2 = f(m + k1 * 0.5, dmid);
-      const double k3 = f(m + k2 * 0.5, dmid);
-      const double k4 = f(m + k3, drive(t + dt, 2));
-      mn = m + (k1 + (k2 * 2.0) + (k3 * 2.0) + k4) / 6.0;
+      if constexpr (SOLVER == CMTJ_EULER_HEUN) {
+        // eulerHeunSolverStep (core/junction.hpp:1709-1754): drift at t,
+        // then the stochastic torque g of stochasticTorque (:824-831) at m
+        // and at m + g sqrt(dt), with one dW per layer and step (the
+        // layer's three normals of the shared Philox stream, this lane's
+        // component)
+        const CLDrive d0 = drive(t, 0);
+        mn = m + f(m, d0);
+        if (CMTJ_HAS(F_THERMAL)) {
+          const V3<double> xi = thermal_normals<double>(seed, stream, i, (uint32_t)lyr);
+          const double dW = comp(xi, c) * sqrt(K.sig2 * d0.T);
+          auto stoch_c = [&](double ms) -> double {
+            const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
+            const double c1 = cross_c(s1, s2, dW);  // m x dW
+            const double c2 = cross_c(s1, s2, c1);  // m x (m x dW)
+            return fma(c2, K.alpha, c1) * K.gconv;
+          };
+          const double gn = stoch_c(m);
+          const double gp = stoch_c(m + gn * sdt);
+          mn = K.therm ? mn + ((gn + gp) * 0.5) * sdt : mn;
+        }
+      } else {
+        const double k1 = f(m, drive(t, 0));
+        const CLDrive dmid = drive(t + 0.5 * dt, 1);
+        const double k2 = f(m + k1 * 0.5, dmid);
+        const double k3 = f(m + k2 * 0.5, dmid);
+        const double k4 = f(m + k3, drive(t + dt, 2));
+        mn = m + (k1 + (k2 * 2.0) + (k3 * 2.0) + k4) / 6.0;
+      }
       // CVector::normalize (core/cvector.hpp:185-192): the quad sum adds
       // (x^2 + y^2) + z^2 in the reference's order; a zero vector stays
       const double ss = qsum(mn * mn);
@@ -442,8 +475,9 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       mn = (s != 0.0) ? mn * y : mn;
     }
     // the reference updates layers in turn and stops at the first NaN
-    // layer; layers up to and including it take the new value
-    const unsigned long long bad = __ballot(live && c == 0 && isnan(mn));
+    // layer; layers up to and including it take the new value (RK4 only:
+    // the Euler-Heun step of the reference has no such test)
+    const unsigned long long bad = (SOLVER == CMTJ_RK4) ? __ballot(live && c == 0 && isnan(mn)) : 0ull;
     if (bad) {  // wave-uniform test first: the per-instance work only on the rare path
       const uint32_t gb = (uint32_t)((bad >> gshift) & ((1ull << LPI) - 1));
       if (gb) {

@@ -72,6 +72,11 @@ KTable kernels_group4();  // the same on four waves per group
   {CMTJ_FP64, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND, true>,  \
    "k_junction_cl<double," #L "," #FM "," #ND ">", (ND) + 1}
 
+#define CMTJ_KC64EH(L, FM, ND)                                                                     \
+  {CMTJ_FP64, L, 4 * (L), CMTJ_EULER_HEUN, FM, false,                                                \
+   (const void*)&cmtj::k_junction_cl<L, FM, ND, true, CMTJ_EULER_HEUN>,                              \
+   "k_junction_cl<double," #L ",EulerHeun," #FM "," #ND ">", (ND) + 1}
+
 #define CMTJ_KL_SOLVERS(R, PREC, L, FM) \
   CMTJ_KL(R, PREC, L, CMTJ_RK4, FM), CMTJ_KL(R, PREC, L, CMTJ_HEUN, FM), CMTJ_KL(R, PREC, L, CMTJ_EULER_HEUN, FM)
 

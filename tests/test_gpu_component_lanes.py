@@ -245,6 +245,47 @@ def test_fp64_component_lanes_spin_transfer_torque(name):
     assert worst <= 1e-8
 
 
+@pytest.mark.parametrize("name", ["sot_step_300K", "chain2_300K", "sto_300K", "stt_sine_T_ramp"])
+def test_fp64_component_lanes_euler_heun(name):
+    """Stochastic runs (Euler-Heun) in the binary64 component-lane kernels:
+    the engine and the oracle share the Philox stream, so the paths agree
+    sample by sample; a constant temperature on one layer or all, and a
+    temperature that changes in time."""
+    if name == "sot_step_300K":
+        j, _ = parity_cases.sot_static(Junction)
+        j.getLayer("free").cellSurface = 1e-16
+        j.getLayer("free").cellVolume = 1e-16 * j.getLayer("free").thickness
+        j.setLayerTemperatureDriver("all", constantDriver(300.0))
+    elif name == "chain2_300K":
+        j = _chain(2, mr="classic")
+        j.setLayerTemperatureDriver("L1", constantDriver(300.0))
+    elif name == "sto_300K":
+        j = workloads.sto_junction()
+        j.setLayerCurrentDriver("free", constantDriver(-1e11))
+        j.setLayerTemperatureDriver("all", constantDriver(300.0))
+    else:
+        j, _ = parity_cases.stt_drivers(Junction)
+        j.setLayerTemperatureDriver("all", sineDriver(200.0, 100.0, 1e9, 0.0))
+    j.setNoiseSeed(1234)
+    b = JunctionBatch.from_junctions([j] * 3, 0.5e-9, 1e-13, 1e-12, SolverMode.EulerHeun, precision="fp64",
+                                     log="full", log_dtype="f64")
+    b.set_lanes("per_component")
+    kname = b.plan().info.kernel_name.decode()
+    print(f"[cl64-eh] {name}: {kname}", flush=True)
+    assert kname.startswith("k_junction_cl<double") and "EulerHeun" in kname
+    r = b.run()
+    olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
+    glog = r.log.cpu().numpy()
+    worst = 0.0
+    for i in range(len(b.js)):
+        g, o = _channels(b, glog, i), _channels(b, olog, i)
+        for k in o:
+            if len(o[k]):
+                worst = max(worst, float(np.max(np.abs(g[k] - o[k]))) / max(1.0, float(np.max(np.abs(o[k])))))
+    print(f"[cl64-eh] {name}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
 @pytest.mark.parametrize("bad_layer", [0, 1])
 def test_component_lanes_nan_commits_like_the_reference(bad_layer):
     """A NaN in layer b stops the run: layers up to and including it take
