@@ -439,7 +439,9 @@ def _bench_sto(n_currents: int = 512) -> dict:
     import oracle
     from cmtj_amd import _lib, analysis, workloads
 
-    b32 = workloads.sto(n_currents=n_currents, precision="fp32", log="full")
+    # the binary32 leg on one lane per layer: left to the automatic choice,
+    # an FP32 batch this small runs the binary64 component-lane kernel
+    b32 = workloads.sto(n_currents=n_currents, precision="fp32", log="full").set_lanes("per_layer")
     b32.log_dtype = _lib.LOG_F64
     b64 = workloads.sto(n_currents=n_currents, precision="fp64", log="full")
     r32, ms32 = _timed_run(b32)

@@ -78,17 +78,33 @@ static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, u
                                    kernels_spec(),   kernels_ll_f64(),  kernels_ll_f32(),
                                    kernels_ll_spec(), kernels_exact(),  kernels_cl(),
                                    kernels_cl64(),    kernels_dp()};
-  // [0] one lane per instance, [1] one lane per layer, [2] one lane per component
+  // [0] one lane per instance, [1] one lane per layer, [2] one lane per
+  // component; cl64: the binary64 one-lane-per-component kernel of an FP32
+  // group that no binary32 one covers
   const KEntry* best[3] = {nullptr, nullptr, nullptr};
+  const KEntry* cl64 = nullptr;
   for (const KTable& t : kTables)
     for (size_t q = 0; q < t.n; ++q) {
       const KEntry& k = t.e[q];
-      if (k.prec != prec || k.solver != solver || k.loge != loge || k.lm < nl) continue;
+      if (k.solver != solver || k.loge != loge || k.lm < nl) continue;
       if (!covers(k, flags, kinds, ndyn)) continue;
       const int cat = (k.lpi == 1) ? 0 : (k.lpi == k.lm ? 1 : 2);
+      if (k.prec != prec) {
+        if (prec == CMTJ_FP32 && k.prec == CMTJ_FP64 && cat == 2 && solver == CMTJ_RK4 &&
+            (!cl64 || rank(k) < rank(*cl64)))
+          cl64 = &k;
+        continue;
+      }
       const KEntry*& b = best[cat];
       if (!b || rank(k) < rank(*b)) b = &k;
     }
+  // FP32 mode, RK4: where only the binary64 component kernel covers a group
+  // (spin torques, scalar drives), it stands in for the missing binary32
+  // one: more accurate, and on batches that leave SIMDs idle faster than
+  // binary32 one lane per layer (a 512-point STO sweep: 5.9e8 vs 3.2e8
+  // junction-steps/s).  Not for Euler-Heun: its binary64 normals cost more
+  // than the binary32 ones of one lane per layer.
+  if (!best[2] && cl64) best[2] = cl64;
   if (lanes == CMTJ_LANES_ONE && best[0]) return best[0];
   if (lanes == CMTJ_LANES_PER_COMPONENT && best[2]) return best[2];
   // automatic choice: on a batch too small to fill the chip, the time of a
