@@ -91,6 +91,8 @@ struct CLConst {
   // binary64 kernels: the layer's scalar drives (constant parts) and spin
   // torques
   double bK, bJt, bJb, bI, bFL, bDL, bT;    // anisotropy, IEC top/bottom, current, SOT field-/damping-like, T
+  double bK2, bJ2t, bJ2b;                   // second-order anisotropy, biquadratic IEC top/bottom
+  double nd1, nd2;                          // off-diagonal demag row entries (components c + 1, c + 2) x Ms / mu0
   double sig2, gconv;                       // 2 alpha kB / (Ms V gamma), -gamma / (1 + alpha^2)
   double inv_ms, inv_mst;                   // 1 / Ms, 1 / (Ms t)
   double sttp, kap, bet, ng, cdt, Pp, lam, lam2;  // gamma hbar / (e Ms t), kappa, beta, -gamma, dt / (1 + a^2), P, L, L^2
@@ -102,18 +104,20 @@ struct CLConst {
 // The drives of one stage time as a binary64 lane sees them: its field
 // component and its layer's scalar slots.
 struct CLDrive {
-  double H, K, Jt, Jb, I, FL, DL, T;
+  double H, K, Jt, Jb, I, FL, DL, T, K2, J2t, J2b;
 };
 
 // Which scalar slot of layer ly a time-dependent record feeds: 1 anisotropy,
 // 2 IEC top, 3 IEC bottom, 4 current, 5 field-like, 6 damping-like torque,
-// 7 temperature; 0 none (field slots, other slots, other layers).
+// 7 temperature, 8 second-order anisotropy, 9/10 biquadratic IEC top/bottom;
+// 0 none (field slots, other slots, other layers).
 __device__ __forceinline__ int scalar_slot(int meta, int ly) {
   const int slot = (meta >> 8) & 0xff;
   if ((meta & 0xff) == CMTJ_CONSTANT || !(((meta >> 16) & 0xff) >> ly & 1)) return 0;
   return slot == CMTJ_SLOT_K ? 1 : slot == CMTJ_SLOT_J_TOP ? 2 : slot == CMTJ_SLOT_J_BOTTOM ? 3
        : slot == CMTJ_SLOT_CURRENT ? 4 : slot == CMTJ_SLOT_FL ? 5 : slot == CMTJ_SLOT_DL ? 6
-       : slot == CMTJ_SLOT_TEMPERATURE ? 7 : 0;
+       : slot == CMTJ_SLOT_TEMPERATURE ? 7 : slot == CMTJ_SLOT_K2 ? 8 : slot == CMTJ_SLOT_J2_TOP ? 9
+       : slot == CMTJ_SLOT_J2_BOTTOM ? 10 : 0;
 }
 
 __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
@@ -124,6 +128,9 @@ __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
   d.FL += kind == 5 ? v : 0.0;
   d.DL += kind == 6 ? v : 0.0;
   d.T += kind == 7 ? v : 0.0;
+  d.K2 += kind == 8 ? v : 0.0;
+  d.J2t += kind == 9 ? v : 0.0;
+  d.J2b += kind == 10 ? v : 0.0;
 }
 
 // ND: number of time-dependent records when fixed at compile time (exact
@@ -132,7 +139,8 @@ __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
 template <int L, uint32_t FM, int ND, bool F64 = false, int SOLVER = CMTJ_RK4>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
-  static_assert(F64 || !(FM & (F_STT | F_SOT | F_DYN | F_THERMAL)), "binary64 kernels only");
+  static_assert(F64 || !(FM & (F_STT | F_SOT | F_DYN | F_THERMAL | F_ANIS2 | F_IEC2 | F_DEMAG_OFF)),
+                "binary64 kernels only");
   static_assert(SOLVER == CMTJ_RK4 || (F64 && SOLVER == CMTJ_EULER_HEUN), "RK4, or Euler-Heun in binary64");
   constexpr int LPI = 4 * L;
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,6 +208,12 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     K.bFL = base.FL;
     K.bDL = base.DL;
     K.bT = base.T;
+    K.bK2 = base.K2;
+    K.bJ2t = base.J2t;
+    K.bJ2b = base.J2b;
+    // demag row c against components c + 1 and c + 2 (the rotations)
+    K.nd1 = live ? P.N[3 * c + (c == 2 ? 0 : c + 1)] * P.ms_mu0 : z;
+    K.nd2 = live ? P.N[3 * c + (c == 0 ? 2 : c - 1)] * P.ms_mu0 : z;
     K.sig2 = P.sig2;
     K.gconv = P.gconv;
     K.inv_ms = P.inv_ms;
@@ -290,7 +304,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // slots, transcendental waveforms in full binary64 (the binary32 path
       // below takes the faster single-precision ones)
       auto drive = [&](double ts, int stage) -> CLDrive {
-        CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL, K.bT};
+        CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL, K.bT, K.bK2, K.bJ2t, K.bJ2b};
         if (!(CMTJ_HAS(F_DYNH) || CMTJ_HAS(F_DYN))) return d;
         if constexpr ((FM & F_KSTEP) != 0) {
           // step records are 0 outside [min t_start, max t_stop] (inclusive)
@@ -316,8 +330,20 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       auto f = [&](double ms, const CLDrive& d) -> double {
         const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
         double Hs = d.H;
-        if (CMTJ_HAS(F_ANIS)) Hs = fma(K.an * ((2.0 * d.K) * K.inv_ms), fma(K.an2, s2, fma(K.an1, s1, K.an * ms)), Hs);
-        if (CMTJ_HAS(F_IEC)) Hs = fma(top, d.Jt * K.inv_mst, fma(bot, d.Jb * K.inv_mst, Hs));
+        const double u = fma(K.an2, s2, fma(K.an1, s1, K.an * ms));  // anis . m
+        if (CMTJ_HAS(F_ANIS)) Hs = fma(K.an * ((2.0 * d.K) * K.inv_ms), u, Hs);
+        // calculateSecondOrderAnisotropy (core/junction.hpp:645-652)
+        if (CMTJ_HAS(F_ANIS2)) Hs = fma(K.an, ((4.0 * d.K2) * (u * u * u)) * K.inv_ms, Hs);
+        if (CMTJ_HAS(F_IEC) || CMTJ_HAS(F_IEC2)) {
+          // calculateIEC_ (:654-675): J + 2 J2 (m_neighbour . m)
+          double st = d.Jt, sb = d.Jb;
+          if (CMTJ_HAS(F_IEC2)) {
+            st += 2.0 * d.J2t * qsum(top * ms);
+            sb += 2.0 * d.J2b * qsum(bot * ms);
+          }
+          Hs = fma(top, st * K.inv_mst, fma(bot, sb * K.inv_mst, Hs));
+        }
+        if (CMTJ_HAS(F_DEMAG_OFF)) Hs = fma(-K.nd1, s1, fma(-K.nd2, s2, Hs));
         Hs = fma(-K.nms, ms, Hs);
         // m x H and m x (m x H) as cross products: the initial m of a run
         // need not have unit length

@@ -192,6 +192,28 @@ def _sot_constant(dynamic):
     return j
 
 
+def _exotic(dynamic):
+    """Full demag tensor, second-order anisotropy and biquadratic IEC on two
+    layers (constant, or time-dependent second-order and biquadratic terms)."""
+    N = [CVector(0.02, 0.01, 0.0), CVector(0.01, 0.03, 0.005), CVector(0.0, 0.005, 0.95)]
+    a = Layer("free", mag=CVector(0.3, 0.2, 0.9), anis=CVector(0.1, 0, 1.0), Ms=1.1, thickness=1.2e-9,
+              cellSurface=1e-16, demagTensor=N, damping=0.02)
+    b = Layer("ref", mag=CVector(0.9, 0.1, 0.2), anis=CVector(1, 0, 0), Ms=1.4, thickness=2e-9, cellSurface=1e-16,
+              demagTensor=N, damping=0.03)
+    j = Junction([a, b], 100, 200)
+    j.setLayerAnisotropyDriver("free", constantDriver(4e5))
+    j.setLayerAnisotropyDriver("ref", constantDriver(1e5))
+    j.setIECDriver("free", "ref", constantDriver(-1e-4))
+    if dynamic:
+        j.setLayerSecondOrderAnisotropyDriver("free", sineDriver(-2e4, 1e4, 0.8e9, 0.1))
+        j.setQuadIECDriver("free", "ref", sineDriver(-3e-5, 1e-5, 1.1e9, 0.0))
+    else:
+        j.setLayerSecondOrderAnisotropyDriver("free", constantDriver(-2e4))
+        j.setQuadIECDriver("free", "ref", constantDriver(-3e-5))
+    j.setLayerExternalFieldDriver("all", AxialDriver(2e4, 1e4, 5e4))
+    return j
+
+
 def _iec_dynamic():
     j = _chain(2, mr="classic")
     j.setIECDriver("L0", "L1", sineDriver(-2e-4, 5e-5, 1.3e9, 0.2))
@@ -201,7 +223,7 @@ def _iec_dynamic():
 
 @pytest.mark.parametrize("name", ["sto", "stt_fixed", "stt_alternative", "stt_ref_top", "stt_ref_bottom",
                                   "sot_static", "sot_dynamic", "stt_drivers", "stt_drivers_alternative",
-                                  "sot_step_current", "iec_dynamic"])
+                                  "sot_step_current", "iec_dynamic", "exotic", "exotic_dynamic", "bilayer_iec2"])
 def test_fp64_component_lanes_spin_transfer_torque(name):
     """Spin torques and time-dependent scalar drives in the binary64
     component-lane kernels: STT with fixed, top and bottom polarisers and
@@ -223,6 +245,10 @@ def test_fp64_component_lanes_spin_transfer_torque(name):
             j, _ = parity_cases.sot_static(Junction)
         elif name == "iec_dynamic":
             j = _iec_dynamic()
+        elif name.startswith("exotic"):
+            j = _exotic(name.endswith("dynamic"))
+        elif name == "bilayer_iec2":
+            j, _ = parity_cases.bilayer_iec(Junction)
         else:
             j = _stt_constant(name == "stt_alternative")
         b = JunctionBatch.from_junctions([j] * 3, 1e-9, 1e-13, 1e-12, SolverMode.RK4, precision="fp64", log="full",
