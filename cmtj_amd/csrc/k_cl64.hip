@@ -11,6 +11,9 @@ static const KEntry kC64[] = {
     // Euler-Heun (stochastic runs): the same set with temperature
     CMTJ_KC64EH(1, FM_STTC | F_THERMAL, -1), CMTJ_KC64EH(2, FM_STTC | F_THERMAL, -1),
     CMTJ_KC64EH(4, FM_STTC | F_THERMAL, -1),
+    // Heun
+    CMTJ_KC64H(1, FM_STTC | F_THERMAL, -1), CMTJ_KC64H(2, FM_STTC | F_THERMAL, -1),
+    CMTJ_KC64H(4, FM_STTC | F_THERMAL, -1),
 };
 KTable kernels_cl64() { return {kC64, sizeof(kC64) / sizeof(kC64[0])}; }
 }  // namespace cmtj

@@ -141,7 +141,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
   static_assert(F64 || !(FM & (F_STT | F_SOT | F_DYN | F_THERMAL | F_ANIS2 | F_IEC2 | F_DEMAG_OFF)),
                 "binary64 kernels only");
-  static_assert(SOLVER == CMTJ_RK4 || (F64 && SOLVER == CMTJ_EULER_HEUN), "RK4, or Euler-Heun in binary64");
+  static_assert(SOLVER == CMTJ_RK4 || (F64 && (SOLVER == CMTJ_EULER_HEUN || SOLVER == CMTJ_HEUN)),
+                "RK4, or Euler-Heun / Heun in binary64");
   constexpr int LPI = 4 * L;
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t inst = gl / LPI;
@@ -295,11 +296,6 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // the layer's reference vector: fixed, or the start-of-step top/bottom
       // neighbour; they are computed on every lane and selected per layer,
       // so no quad operation runs under a divergent mask.
-      double refc = 0.0;
-      if constexpr ((FM & (F_STT | F_SOT)) != 0) {
-        if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT))
-          refc = (K.rt == CMTJ_REF_FIXED) ? K.ref : (K.rt == CMTJ_REF_TOP) ? top : (K.rt == CMTJ_REF_BOTTOM) ? bot : 0.0;
-      }
       // drives at ts: this component's field slots and the layer's scalar
       // slots, transcendental waveforms in full binary64 (the binary32 path
       // below takes the faster single-precision ones)
@@ -326,8 +322,9 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         }
         return d;
       };
-      // dm/dt dt at (ms, drives d)
-      auto f = [&](double ms, const CLDrive& d) -> double {
+      // dm/dt dt at (ms, drives d) with neighbour components tp (top) and
+      // bt (bottom)
+      auto f = [&](double ms, const CLDrive& d, double tp, double bt) -> double {
         const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
         double Hs = d.H;
         const double u = fma(K.an2, s2, fma(K.an1, s1, K.an * ms));  // anis . m
@@ -338,10 +335,10 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           // calculateIEC_ (:654-675): J + 2 J2 (m_neighbour . m)
           double st = d.Jt, sb = d.Jb;
           if (CMTJ_HAS(F_IEC2)) {
-            st += 2.0 * d.J2t * qsum(top * ms);
-            sb += 2.0 * d.J2b * qsum(bot * ms);
+            st += 2.0 * d.J2t * qsum(tp * ms);
+            sb += 2.0 * d.J2b * qsum(bt * ms);
           }
-          Hs = fma(top, st * K.inv_mst, fma(bot, sb * K.inv_mst, Hs));
+          Hs = fma(tp, st * K.inv_mst, fma(bt, sb * K.inv_mst, Hs));
         }
         if (CMTJ_HAS(F_DEMAG_OFF)) Hs = fma(-K.nd1, s1, fma(-K.nd2, s2, Hs));
         Hs = fma(-K.nms, ms, Hs);
@@ -352,6 +349,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         double r = fma(Qc, K.alpha, Pc) * K.gdt;
         if constexpr ((FM & (F_STT | F_SOT)) != 0) {
           if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT)) {
+            const double refc = (K.rt == CMTJ_REF_FIXED) ? K.ref : (K.rt == CMTJ_REF_TOP) ? tp
+                              : (K.rt == CMTJ_REF_BOTTOM) ? bt : 0.0;
             const double FLc = cross_c(s1, s2, refc);  // m x p
             const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
             if (CMTJ_HAS(F_SOT)) {
@@ -377,33 +376,57 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         }
         return r;
       };
-      if constexpr (SOLVER == CMTJ_EULER_HEUN) {
+      // stochasticTorque (core/junction.hpp:824-831) with this lane's
+      // component dW of the layer's three normals (shared Philox stream)
+      [[maybe_unused]] auto stoch_c = [&](double ms, double dW) -> double {
+        const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
+        const double c1 = cross_c(s1, s2, dW);  // m x dW
+        const double c2 = cross_c(s1, s2, c1);  // m x (m x dW)
+        return fma(c2, K.alpha, c1) * K.gconv;
+      };
+      if constexpr (SOLVER == CMTJ_HEUN) {
+        // heunSolverStep (core/junction.hpp:1756-1815): a predictor m + f dt
+        // + g sqrt(dt), whose neighbours feed the corrector's second
+        // evaluation at t + dt; the same dW in both
+        const CLDrive d0 = drive(t, 0);
+        const double k1 = f(m, d0, top, bot);
+        double dW = 0.0, g1 = 0.0;
+        if (CMTJ_HAS(F_THERMAL)) {
+          const V3<double> xi = thermal_normals<double>(seed, stream, i, (uint32_t)lyr);
+          dW = K.therm ? comp(xi, c) * sqrt(K.sig2 * d0.T) : 0.0;
+          g1 = K.therm ? stoch_c(m, dW) : 0.0;
+        }
+        const double mN = m + k1 + g1 * sdt;
+        double bN = dpp_d<DPP_ROW_SHR4>(mN), tN = dpp_d<DPP_ROW_SHL4>(mN);
+        bN = hasB ? bN : 0.0;
+        tN = hasT ? tN : 0.0;
+        const double k2 = f(mN, drive(t + dt, 1), tN, bN);
+        mn = m + (k1 + k2) * 0.5;
+        if (CMTJ_HAS(F_THERMAL)) {
+          const double g2 = K.therm ? stoch_c(mN, dW) : 0.0;
+          mn = K.therm ? mn + ((g1 + g2) * 0.5) * sdt : mn;
+        }
+      } else if constexpr (SOLVER == CMTJ_EULER_HEUN) {
         // eulerHeunSolverStep (core/junction.hpp:1709-1754): drift at t,
         // then the stochastic torque g of stochasticTorque (:824-831) at m
         // and at m + g sqrt(dt), with one dW per layer and step (the
         // layer's three normals of the shared Philox stream, this lane's
         // component)
         const CLDrive d0 = drive(t, 0);
-        mn = m + f(m, d0);
+        mn = m + f(m, d0, top, bot);
         if (CMTJ_HAS(F_THERMAL)) {
           const V3<double> xi = thermal_normals<double>(seed, stream, i, (uint32_t)lyr);
           const double dW = comp(xi, c) * sqrt(K.sig2 * d0.T);
-          auto stoch_c = [&](double ms) -> double {
-            const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
-            const double c1 = cross_c(s1, s2, dW);  // m x dW
-            const double c2 = cross_c(s1, s2, c1);  // m x (m x dW)
-            return fma(c2, K.alpha, c1) * K.gconv;
-          };
-          const double gn = stoch_c(m);
-          const double gp = stoch_c(m + gn * sdt);
+          const double gn = stoch_c(m, dW);
+          const double gp = stoch_c(m + gn * sdt, dW);
           mn = K.therm ? mn + ((gn + gp) * 0.5) * sdt : mn;
         }
       } else {
-        const double k1 = f(m, drive(t, 0));
+        const double k1 = f(m, drive(t, 0), top, bot);
         const CLDrive dmid = drive(t + 0.5 * dt, 1);
-        const double k2 = f(m + k1 * 0.5, dmid);
-        const double k3 = f(m + k2 * 0.5, dmid);
-        const double k4 = f(m + k3, drive(t + dt, 2));
+        const double k2 = f(m + k1 * 0.5, dmid, top, bot);
+        const double k3 = f(m + k2 * 0.5, dmid, top, bot);
+        const double k4 = f(m + k3, drive(t + dt, 2), top, bot);
         mn = m + (k1 + (k2 * 2.0) + (k3 * 2.0) + k4) / 6.0;
       }
       // CVector::normalize (core/cvector.hpp:185-192): the quad sum adds

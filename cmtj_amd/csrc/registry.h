@@ -78,6 +78,10 @@ KTable kernels_group4();  // the same on four waves per group
    (const void*)&cmtj::k_junction_cl<L, FM, ND, true, CMTJ_EULER_HEUN>,                              \
    "k_junction_cl<double," #L ",EulerHeun," #FM "," #ND ">", (ND) + 1}
 
+#define CMTJ_KC64H(L, FM, ND)                                                                      \
+  {CMTJ_FP64, L, 4 * (L), CMTJ_HEUN, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND, true, CMTJ_HEUN>, \
+   "k_junction_cl<double," #L ",Heun," #FM "," #ND ">", (ND) + 1}
+
 #define CMTJ_KL_SOLVERS(R, PREC, L, FM) \
   CMTJ_KL(R, PREC, L, CMTJ_RK4, FM), CMTJ_KL(R, PREC, L, CMTJ_HEUN, FM), CMTJ_KL(R, PREC, L, CMTJ_EULER_HEUN, FM)
 

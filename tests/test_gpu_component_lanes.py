@@ -271,12 +271,15 @@ def test_fp64_component_lanes_spin_transfer_torque(name):
     assert worst <= 1e-8
 
 
-@pytest.mark.parametrize("name", ["sot_step_300K", "chain2_300K", "sto_300K", "stt_sine_T_ramp"])
-def test_fp64_component_lanes_euler_heun(name):
-    """Stochastic runs (Euler-Heun) in the binary64 component-lane kernels:
-    the engine and the oracle share the Philox stream, so the paths agree
-    sample by sample; a constant temperature on one layer or all, and a
-    temperature that changes in time."""
+@pytest.mark.parametrize("solver", [SolverMode.EulerHeun, SolverMode.Heun])
+@pytest.mark.parametrize("name", ["sot_step_300K", "chain2_300K", "sto_300K", "stt_sine_T_ramp", "stt_ref_top_300K",
+                                  "chain2"])
+def test_fp64_component_lanes_stochastic_solvers(name, solver):
+    """Stochastic runs (Euler-Heun, Heun) in the binary64 component-lane
+    kernels: the engine and the oracle share the Philox stream, so the paths
+    agree sample by sample; a constant temperature on one layer or all, a
+    temperature that changes in time, a polariser taken from the (predicted)
+    neighbour, and a deterministic run."""
     if name == "sot_step_300K":
         j, _ = parity_cases.sot_static(Junction)
         j.getLayer("free").cellSurface = 1e-16
@@ -289,16 +292,21 @@ def test_fp64_component_lanes_euler_heun(name):
         j = workloads.sto_junction()
         j.setLayerCurrentDriver("free", constantDriver(-1e11))
         j.setLayerTemperatureDriver("all", constantDriver(300.0))
+    elif name == "stt_ref_top_300K":
+        j, _ = parity_cases.stt_neighbour_reference(Junction, Reference.top)
+        j.setLayerTemperatureDriver("all", constantDriver(300.0))
+    elif name == "chain2":
+        j = _chain(2, mr="classic")
     else:
         j, _ = parity_cases.stt_drivers(Junction)
         j.setLayerTemperatureDriver("all", sineDriver(200.0, 100.0, 1e9, 0.0))
     j.setNoiseSeed(1234)
-    b = JunctionBatch.from_junctions([j] * 3, 0.5e-9, 1e-13, 1e-12, SolverMode.EulerHeun, precision="fp64",
-                                     log="full", log_dtype="f64")
+    b = JunctionBatch.from_junctions([j] * 3, 0.5e-9, 1e-13, 1e-12, solver, precision="fp64", log="full",
+                                     log_dtype="f64")
     b.set_lanes("per_component")
     kname = b.plan().info.kernel_name.decode()
-    print(f"[cl64-eh] {name}: {kname}", flush=True)
-    assert kname.startswith("k_junction_cl<double") and "EulerHeun" in kname
+    print(f"[cl64-eh] {name} {solver.name}: {kname}", flush=True)
+    assert kname.startswith("k_junction_cl<double") and f",{solver.name}," in kname
     r = b.run()
     olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
     glog = r.log.cpu().numpy()
@@ -308,7 +316,7 @@ def test_fp64_component_lanes_euler_heun(name):
         for k in o:
             if len(o[k]):
                 worst = max(worst, float(np.max(np.abs(g[k] - o[k]))) / max(1.0, float(np.max(np.abs(o[k])))))
-    print(f"[cl64-eh] {name}: max relative difference {worst:.3e}", flush=True)
+    print(f"[cl64-eh] {name} {solver.name}: max relative difference {worst:.3e}", flush=True)
     assert worst <= 1e-8
 
 
