@@ -93,6 +93,8 @@ struct CLConst {
   double bK, bJt, bJb, bI, bFL, bDL, bT;    // anisotropy, IEC top/bottom, current, SOT field-/damping-like, T
   double bK2, bJ2t, bJ2b;                   // second-order anisotropy, biquadratic IEC top/bottom
   double nd1, nd2;                          // off-diagonal demag row entries (components c + 1, c + 2) x Ms / mu0
+  double nb0, nb1, nb2, nt0, nt1, nt2;      // dipole tensor rows (bottom, top) x Ms / mu0
+  double bDt, bDb, bFL2, bDL2, ref2;        // IDMI vectors (this component), secondary SOT drives and reference
   double sig2, gconv;                       // 2 alpha kB / (Ms V gamma), -gamma / (1 + alpha^2)
   double inv_ms, inv_mst;                   // 1 / Ms, 1 / (Ms t)
   double sttp, kap, bet, ng, cdt, Pp, lam, lam2;  // gamma hbar / (e Ms t), kappa, beta, -gamma, dt / (1 + a^2), P, L, L^2
@@ -104,20 +106,23 @@ struct CLConst {
 // The drives of one stage time as a binary64 lane sees them: its field
 // component and its layer's scalar slots.
 struct CLDrive {
-  double H, K, Jt, Jb, I, FL, DL, T, K2, J2t, J2b;
+  double H, K, Jt, Jb, I, FL, DL, T, K2, J2t, J2b, FL2, DL2, Dt, Db;
 };
 
 // Which scalar slot of layer ly a time-dependent record feeds: 1 anisotropy,
 // 2 IEC top, 3 IEC bottom, 4 current, 5 field-like, 6 damping-like torque,
-// 7 temperature, 8 second-order anisotropy, 9/10 biquadratic IEC top/bottom;
-// 0 none (field slots, other slots, other layers).
-__device__ __forceinline__ int scalar_slot(int meta, int ly) {
+// 7 temperature, 8 second-order anisotropy, 9/10 biquadratic IEC top/bottom,
+// 11/12 secondary field-/damping-like torque, 13/14 component c of the IDMI
+// vector towards the top/bottom neighbour; 0 none (field slots, other
+// slots, other layers, other components).
+__device__ __forceinline__ int scalar_slot(int meta, int ly, int c) {
   const int slot = (meta >> 8) & 0xff;
   if ((meta & 0xff) == CMTJ_CONSTANT || !(((meta >> 16) & 0xff) >> ly & 1)) return 0;
   return slot == CMTJ_SLOT_K ? 1 : slot == CMTJ_SLOT_J_TOP ? 2 : slot == CMTJ_SLOT_J_BOTTOM ? 3
        : slot == CMTJ_SLOT_CURRENT ? 4 : slot == CMTJ_SLOT_FL ? 5 : slot == CMTJ_SLOT_DL ? 6
        : slot == CMTJ_SLOT_TEMPERATURE ? 7 : slot == CMTJ_SLOT_K2 ? 8 : slot == CMTJ_SLOT_J2_TOP ? 9
-       : slot == CMTJ_SLOT_J2_BOTTOM ? 10 : 0;
+       : slot == CMTJ_SLOT_J2_BOTTOM ? 10 : slot == CMTJ_SLOT_FL2 ? 11 : slot == CMTJ_SLOT_DL2 ? 12
+       : slot == CMTJ_SLOT_IDMI_TOP + c ? 13 : slot == CMTJ_SLOT_IDMI_BOTTOM + c ? 14 : 0;
 }
 
 __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
@@ -131,6 +136,10 @@ __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
   d.K2 += kind == 8 ? v : 0.0;
   d.J2t += kind == 9 ? v : 0.0;
   d.J2b += kind == 10 ? v : 0.0;
+  d.FL2 += kind == 11 ? v : 0.0;
+  d.DL2 += kind == 12 ? v : 0.0;
+  d.Dt += kind == 13 ? v : 0.0;
+  d.Db += kind == 14 ? v : 0.0;
 }
 
 // ND: number of time-dependent records when fixed at compile time (exact
@@ -139,7 +148,8 @@ __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
 template <int L, uint32_t FM, int ND, bool F64 = false, int SOLVER = CMTJ_RK4>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
-  static_assert(F64 || !(FM & (F_STT | F_SOT | F_DYN | F_THERMAL | F_ANIS2 | F_IEC2 | F_DEMAG_OFF)),
+  static_assert(F64 || !(FM & (F_STT | F_SOT | F_SOT2 | F_DYN | F_THERMAL | F_ANIS2 | F_IEC2 | F_DEMAG_OFF |
+                                F_IDMI | F_DIPOLE)),
                 "binary64 kernels only");
   static_assert(SOLVER == CMTJ_RK4 || (F64 && (SOLVER == CMTJ_EULER_HEUN || SOLVER == CMTJ_HEUN)),
                 "RK4, or Euler-Heun / Heun in binary64");
@@ -215,6 +225,18 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     // demag row c against components c + 1 and c + 2 (the rotations)
     K.nd1 = live ? P.N[3 * c + (c == 2 ? 0 : c + 1)] * P.ms_mu0 : z;
     K.nd2 = live ? P.N[3 * c + (c == 0 ? 2 : c - 1)] * P.ms_mu0 : z;
+    const int c1 = (c == 2) ? 0 : c + 1, c2 = (c == 0) ? 2 : c - 1;
+    K.nb0 = live ? P.Nb[3 * c + c] * P.ms_mu0 : z;
+    K.nb1 = live ? P.Nb[3 * c + c1] * P.ms_mu0 : z;
+    K.nb2 = live ? P.Nb[3 * c + c2] * P.ms_mu0 : z;
+    K.nt0 = live ? P.Nt[3 * c + c] * P.ms_mu0 : z;
+    K.nt1 = live ? P.Nt[3 * c + c1] * P.ms_mu0 : z;
+    K.nt2 = live ? P.Nt[3 * c + c2] * P.ms_mu0 : z;
+    K.bDt = live ? comp(base.Dt, c) : z;
+    K.bDb = live ? comp(base.Db, c) : z;
+    K.bFL2 = base.FL2;
+    K.bDL2 = base.DL2;
+    K.ref2 = live ? comp(mk<double>(Ly.ref2[0], Ly.ref2[1], Ly.ref2[2]), c) : z;
     K.sig2 = P.sig2;
     K.gconv = P.gconv;
     K.inv_ms = P.inv_ms;
@@ -250,7 +272,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       win_lo = fmin(win_lo, dr[k].p1);
       win_hi = fmax(win_hi, dr[k].p2);
       sel[k] = live && feeds(g.dmeta[k], ly, c);
-      skind[k] = lact ? scalar_slot(g.dmeta[k], ly) : 0;
+      skind[k] = lact ? scalar_slot(g.dmeta[k], ly, c) : 0;
     } else {
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
       sel[k] = false;
@@ -300,7 +322,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // slots, transcendental waveforms in full binary64 (the binary32 path
       // below takes the faster single-precision ones)
       auto drive = [&](double ts, int stage) -> CLDrive {
-        CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL, K.bT, K.bK2, K.bJ2t, K.bJ2b};
+        CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL, K.bT, K.bK2, K.bJ2t, K.bJ2b, K.bFL2, K.bDL2, K.bDt, K.bDb};
         if (!(CMTJ_HAS(F_DYNH) || CMTJ_HAS(F_DYN))) return d;
         if constexpr ((FM & F_KSTEP) != 0) {
           // step records are 0 outside [min t_start, max t_stop] (inclusive)
@@ -318,7 +340,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           const DynRec rr = g.dyn[(size_t)r * g.n + inst];
           const double v = wave_value<double, false, FM>(g.dmeta[r] & 0xff, rr, g, ts, i, stage, sps);
           d.H = d.H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
-          if constexpr ((FM & F_DYN) != 0) scalar_add(d, lact ? scalar_slot(g.dmeta[r], ly) : 0, v);
+          if constexpr ((FM & F_DYN) != 0) scalar_add(d, lact ? scalar_slot(g.dmeta[r], ly, c) : 0, v);
         }
         return d;
       };
@@ -340,8 +362,21 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           }
           Hs = fma(tp, st * K.inv_mst, fma(bt, sb * K.inv_mst, Hs));
         }
+        if (CMTJ_HAS(F_IDMI)) {
+          // calculateIDMI_ (:677-693): -(D x m_neighbour) / (Ms t) per neighbour
+          const double dt1 = dpp_d<QP_ROT1>(d.Dt), dt2 = dpp_d<QP_ROT2>(d.Dt);
+          const double db1 = dpp_d<QP_ROT1>(d.Db), db2 = dpp_d<QP_ROT2>(d.Db);
+          Hs = Hs - (cross_c(db1, db2, bt) + cross_c(dt1, dt2, tp)) * K.inv_mst;
+        }
         if (CMTJ_HAS(F_DEMAG_OFF)) Hs = fma(-K.nd1, s1, fma(-K.nd2, s2, Hs));
         Hs = fma(-K.nms, ms, Hs);
+        if (CMTJ_HAS(F_DIPOLE)) {
+          // calculate_tensor_interaction of the neighbours' dipole tensors
+          // (:579-586), with this layer's Ms
+          const double b1 = dpp_d<QP_ROT1>(bt), b2 = dpp_d<QP_ROT2>(bt);
+          const double t1 = dpp_d<QP_ROT1>(tp), t2 = dpp_d<QP_ROT2>(tp);
+          Hs = Hs - (fma(K.nb0, bt, fma(K.nb1, b1, K.nb2 * b2)) + fma(K.nt0, tp, fma(K.nt1, t1, K.nt2 * t2)));
+        }
         // m x H and m x (m x H) as cross products: the initial m of a run
         // need not have unit length
         const double Pc = cross_c(s1, s2, Hs);
@@ -358,9 +393,15 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
               // ones scaling the current
               const double Hdl = K.dyn_sot ? d.DL : K.dlt * d.I;
               const double Hfl = K.dyn_sot ? d.FL : K.flt * d.I;
-              const double rs =
-                  fma(DLc, Hdl + K.alpha * Hfl, fma(FLc, Hfl - K.alpha * Hdl, fma(Qc, K.alpha, Pc))) * K.gdt;
-              r = K.sot ? rs : r;
+              double br = fma(DLc, Hdl + K.alpha * Hfl, fma(FLc, Hfl - K.alpha * Hdl, fma(Qc, K.alpha, Pc)));
+              if (CMTJ_HAS(F_SOT2)) {
+                // the secondary reference (dynamic SOT drivers only)
+                const double Hdl2 = K.dyn_sot ? d.DL2 : 0.0, Hfl2 = K.dyn_sot ? d.FL2 : 0.0;
+                const double F2 = cross_c(s1, s2, K.ref2);  // m x p2
+                const double D2 = cross_c(s1, s2, F2);      // m x (m x p2)
+                br = br + F2 * (Hfl2 - K.alpha * Hdl2) + D2 * (Hdl2 + K.alpha * Hfl2);
+              }
+              r = K.sot ? br * K.gdt : r;
             }
             if (CMTJ_HAS(F_STT)) {
               // Slonczewski STT (:743-769)

@@ -25,10 +25,11 @@ struct KTable {
 constexpr uint32_t FM_LEAN = F_ANIS | F_IEC | F_DYNH;
 constexpr uint32_t FM_STT = F_ANIS | F_IEC | F_DYNH | F_STT;
 constexpr uint32_t FM_SOTT = F_ANIS | F_IEC | F_DYN | F_DYNH | F_SOT | F_THERMAL;
-// one component per lane, binary64: the lean set with a full demag tensor,
-// second-order anisotropy, biquadratic IEC, spin torques (STT, SOT) and
-// time-dependent scalar drivers
-constexpr uint32_t FM_STTC = F_ANIS | F_ANIS2 | F_IEC | F_IEC2 | F_DEMAG_OFF | F_DYNH | F_DYN | F_STT | F_SOT;
+// one component per lane, binary64: every term of the field and torque
+// (full demag and dipole tensors, second-order anisotropy, biquadratic IEC,
+// IDMI, STT, SOT with a secondary reference) and time-dependent drivers
+constexpr uint32_t FM_STTC = F_ANIS | F_ANIS2 | F_IEC | F_IEC2 | F_DEMAG_OFF | F_IDMI | F_DIPOLE | F_DYNH | F_DYN |
+                             F_STT | F_SOT | F_SOT2;
 // coupled runs (stacks, groups): the usual member's features
 constexpr uint32_t FM_CPL = F_ANIS | F_IEC | F_DYN | F_DYNH | F_STT | F_SOT | F_THERMAL;
 
