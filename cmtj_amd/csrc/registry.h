@@ -39,6 +39,7 @@ constexpr uint32_t FM_RH = F_ANIS | F_EXACT | F_KSTEP;                          
 constexpr uint32_t FM_VSD = F_ANIS | F_DYNH | F_EXACT | F_KSINE;                   // sine Oersted drive
 constexpr uint32_t FM_CIMS = F_ANIS | F_DYN | F_SOT | F_THERMAL | F_EXACT | F_KSTEP;  // SOT pulse at T
 constexpr uint32_t FM_STO = F_ANIS | F_STT | F_EXACT | F_KSTEP;                    // DC spin-torque oscillator
+constexpr uint32_t FM_SOTK = F_ANIS | F_DYN | F_SOT | F_EXACT | F_KSTEP;            // SOT switching by a current step
 
 // one table per instantiation unit
 KTable kernels_f64();
