@@ -50,7 +50,11 @@ static std::mutex g_consts_mu;
 // record, bit 2 = a sine record.
 constexpr uint32_t KIND_OTHER = 1, KIND_STEP = 2, KIND_SINE = 4;
 
-static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds, int ndyn) {
+// Log layout the LOGF = 1 kernels compile in: log_mask | mr_mode << 8 | f32 << 16.
+constexpr uint32_t kLogPimm = (CMTJ_LOG_MX | CMTJ_LOG_MR) | (CMTJ_MR_CLASSIC << 8) | (1u << 16);
+
+static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds, int ndyn, uint32_t logcfg = 0xffffffffu) {
+  if (k.logf == 1 && logcfg != kLogPimm) return false;
   const uint32_t feat = k.fm & F_ALL;
   if ((feat & flags) != flags) return false;
   if ((k.fm & F_EXACT) && (feat != flags || ndyn > DREG)) return false;  // exact kernels keep every record in registers
@@ -65,7 +69,7 @@ static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds, int ndyn) {
 // compiled-in features.
 static int rank(const KEntry& k) {
   return k.lm * 1000 + ((k.fm & F_EXACT) ? 0 : 200) + ((k.fm & (F_KSTEP | F_KSINE)) ? 0 : 100) +
-         (k.nd1 ? 0 : 50) + __builtin_popcount(k.fm & F_ALL);
+         (k.nd1 ? 0 : 50) + (k.logf ? 0 : 25) + __builtin_popcount(k.fm & F_ALL);
 }
 
 // SIMDs of the target (MI355X: 256 CUs x 4): a batch whose waves number at
@@ -73,7 +77,7 @@ static int rank(const KEntry& k) {
 constexpr uint64_t kSimds = 1024;
 
 static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, uint32_t kinds, int ndyn, bool loge,
-                                 int lanes, uint64_t n_inst) {
+                                 int lanes, uint64_t n_inst, uint32_t logcfg) {
   static const KTable kTables[] = {kernels_f64(),    kernels_f64e(),    kernels_f32(),
                                    kernels_spec(),   kernels_ll_f64(),  kernels_ll_f32(),
                                    kernels_ll_spec(), kernels_exact(),  kernels_cl(),
@@ -87,7 +91,7 @@ static const KEntry* pick_kernel(int prec, int nl, int solver, uint32_t flags, u
     for (size_t q = 0; q < t.n; ++q) {
       const KEntry& k = t.e[q];
       if (k.solver != solver || k.loge != loge || k.lm < nl) continue;
-      if (!covers(k, flags, kinds, ndyn)) continue;
+      if (!covers(k, flags, kinds, ndyn, logcfg)) continue;
       const int cat = (k.lpi == 1) ? 0 : (k.lpi == k.lm ? 1 : 2);
       if (k.prec != prec) {
         if (prec == CMTJ_FP32 && k.prec == CMTJ_FP64 && cat == 2 && solver == CMTJ_RK4 &&
@@ -419,7 +423,9 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
       kinds |= (ty == CMTJ_STEP) ? KIND_STEP : (ty == CMTJ_SINE) ? KIND_SINE : KIND_OTHER;
     }
     const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], kinds,
-                                  (int)recs_of[orders[g][0]].size(), loge, lanes, orders[g].size());
+                                  (int)recs_of[orders[g][0]].size(), loge, lanes, orders[g].size(),
+                                  (uint32_t)j0.log_mask | ((uint32_t)j0.mr_mode << 8) |
+                                      (log_dtype == CMTJ_LOG_F32 ? (1u << 16) : 0u));
     if (!k) return fail(CMTJ_ERR_INVALID, "no kernel for this junction configuration");
     Group Gr;
     Gr.k = k;

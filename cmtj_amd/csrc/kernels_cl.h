@@ -145,7 +145,9 @@ __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
 // ND: number of time-dependent records when fixed at compile time (exact
 // kernels), -1 to read it from the group.  F64: every stage in binary64
 // (the drop-in's arithmetic) instead of the anchored binary32 increments.
-template <int L, uint32_t FM, int ND, bool F64 = false, int SOLVER = CMTJ_RK4>
+// LOGF: 0 reads the log layout from the group; 1 compiles in m_x of every
+// layer, the classic resistance and binary32 entries (the PIMM sweep's log).
+template <int L, uint32_t FM, int ND, bool F64 = false, int SOLVER = CMTJ_RK4, int LOGF = 0>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
   static_assert(F64 || !(FM & (F_STT | F_SOT | F_SOT2 | F_DYN | F_THERMAL | F_ANIS2 | F_IEC2 | F_DEMAG_OFF |
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   const double dt = J->dt;
   const uint64_t lbase = J->log_offset;
   const bool logging = (g.log != nullptr) && g.log_mask != 0;
-  const bool f32log = g.log_f32;
+  const bool f32log = (LOGF == 1) ? true : g.log_f32;
   const uint32_t esize = f32log ? 4u : 8u;
   char* const lp = logging ? (char*)g.log + lbase * esize : nullptr;
   const uint64_t cs = (uint64_t)NE * esize;
@@ -283,7 +285,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   // log layout of this lane (logLayerParams, core/junction.hpp:1580-1636):
   // its m channel, if it writes one, and the resistance channels after the
   // m channels of all layers
-  const int lmask = logging ? g.log_mask : 0, mrm = g.mr_mode;
+  const int lmask = logging ? ((LOGF == 1) ? (CMTJ_LOG_MX | CMTJ_LOG_MR) : g.log_mask) : 0;
+  const int mrm = (LOGF == 1) ? CMTJ_MR_CLASSIC : g.mr_mode;
   const int nm = ((lmask & CMTJ_LOG_MX) ? 1 : 0) + ((lmask & CMTJ_LOG_MYZ) ? 2 : 0);
   const int mch = (lmask & CMTJ_LOG_MX) ? c : c - 1;
   const bool wm = live && nm && mch >= 0 && ((lmask & CMTJ_LOG_MYZ) || c == 0);

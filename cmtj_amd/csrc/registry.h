@@ -15,6 +15,7 @@ struct KEntry {
   const void* fn;
   const char* name;
   int nd1;  // 1 + the number of time-dependent records the kernel is compiled for; 0: any
+  int logf;  // 0: any log layout; 1: m_x of every layer + classic R, binary32 entries
 };
 
 struct KTable {
@@ -70,6 +71,10 @@ KTable kernels_group4();  // the same on four waves per group
 #define CMTJ_KC(L, FM, ND)                                                                     \
   {CMTJ_FP32, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND>,  \
    "k_junction_cl<" #L "," #FM "," #ND ">", (ND) + 1}
+
+#define CMTJ_KCLOG(L, FM, ND)                                                                      \
+  {CMTJ_FP32, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND, false, CMTJ_RK4, 1>, \
+   "k_junction_cl<" #L "," #FM "," #ND ",log m_x+R f32>", (ND) + 1, 1}
 
 #define CMTJ_KC64(L, FM, ND)                                                                        \
   {CMTJ_FP64, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND, true>,  \

@@ -352,6 +352,34 @@ def test_component_lanes_nan_commits_like_the_reference(bad_layer):
     assert float(np.max(np.abs(gm[:, :2][ok] - omag[:, :2][ok]))) <= FP32_TOL
 
 
+def test_pimm_log_layout_kernel_matches_oracle():
+    """The PIMM sweep's kernel with its log layout compiled in (m_x of both
+    layers and the classic resistance as binary32 entries) against the
+    oracle: the same entries as the general layout, at the binary32 mode's
+    tolerance (R relative, as the log entries are binary32)."""
+    b = workloads.pimm(n_points=16, t_min=0.2e-9, t_max=0.5e-9, precision="fp32", log="mx", log_dtype="f32")
+    kname = b.plan().info.kernel_name.decode()
+    print(f"[cl] pimm log layout: {kname}", flush=True)
+    assert "log m_x+R f32" in kname
+    r = b.run()
+    olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
+    st = r.status()
+    assert (st["entries_done"] == ost["entries_done"]).all()
+    glog = r.log.cpu().numpy().astype(np.float64)
+    wm = wr = 0.0
+    for i in range(len(b.js)):
+        g, o = _channels(b, glog, i), _channels(b, olog, i)
+        assert g.keys() == o.keys() and len(o) == 3
+        for k in o:
+            d = float(np.max(np.abs(g[k] - o[k])))
+            if "_m" in k:
+                wm = max(wm, d)
+            else:
+                wr = max(wr, d / float(np.max(np.abs(o[k]))))
+    print(f"[cl] pimm log layout: max|dm| {wm:.3e}, R rel {wr:.3e}", flush=True)
+    assert wm <= FP32_TOL and wr <= FP32_TOL
+
+
 def test_component_lanes_chosen_automatically_on_small_batches():
     small = workloads.pimm(n_points=64, t_min=1e-11, t_max=2e-11, precision="fp32", log="mx")
     assert "k_junction_cl" in small.plan().info.kernel_name.decode()
