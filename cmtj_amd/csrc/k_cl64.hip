@@ -7,7 +7,9 @@ static const KEntry kC64[] = {
     CMTJ_KC64(2, FM_PIMM, 1),  CMTJ_KC64(2, FM_PIMM, -1), CMTJ_KC64(4, FM_PIMM, -1), CMTJ_KC64(1, FM_RH, 0),
     CMTJ_KC64(1, FM_VSD, 1),   CMTJ_KC64(1, FM_LEAN, -1), CMTJ_KC64(2, FM_LEAN, -1), CMTJ_KC64(4, FM_LEAN, -1),
     // spin torques, scalar drives: the STO set and the lean set with both
-    CMTJ_KC64(2, FM_STO, -1),  CMTJ_KC64(1, FM_STTC, -1), CMTJ_KC64(2, FM_STTC, -1), CMTJ_KC64(4, FM_STTC, -1),
+    CMTJ_KC64(2, FM_STO, -1),
+    // RK4 also reads callback drivers sampled at its three stage times
+    CMTJ_KC64(1, FM_STTC | F_TABLE, -1), CMTJ_KC64(2, FM_STTC | F_TABLE, -1), CMTJ_KC64(4, FM_STTC | F_TABLE, -1),
     // SOT switching by a current step, deterministic and at temperature
     CMTJ_KC64(1, FM_SOTK, 1), CMTJ_KC64(2, FM_SOTK, 1), CMTJ_KC64EH(1, FM_CIMS, 1), CMTJ_KC64EH(2, FM_CIMS, 1),
     // Euler-Heun (stochastic runs): the same set with temperature

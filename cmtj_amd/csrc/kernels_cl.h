@@ -155,6 +155,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
                 "binary64 kernels only");
   static_assert(SOLVER == CMTJ_RK4 || (F64 && (SOLVER == CMTJ_EULER_HEUN || SOLVER == CMTJ_HEUN)),
                 "RK4, or Euler-Heun / Heun in binary64");
+  // callback-driver samples are laid out for RK4's three stage times
+  static_assert(!(FM & F_TABLE) || (F64 && SOLVER == CMTJ_RK4), "tabulated drivers: binary64 RK4 only");
   constexpr int LPI = 4 * L;
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t inst = gl / LPI;

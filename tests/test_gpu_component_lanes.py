@@ -224,7 +224,7 @@ def _iec_dynamic():
 @pytest.mark.parametrize("name", ["sto", "stt_fixed", "stt_alternative", "stt_ref_top", "stt_ref_bottom",
                                   "sot_static", "sot_dynamic", "stt_drivers", "stt_drivers_alternative",
                                   "sot_step_current", "iec_dynamic", "exotic", "exotic_dynamic", "bilayer_iec2",
-                                  "exotic_terms", "sot_secondary"])
+                                  "exotic_terms", "sot_secondary", "callback_drivers"])
 def test_fp64_component_lanes_spin_transfer_torque(name):
     """Spin torques and time-dependent scalar drives in the binary64
     component-lane kernels: STT with fixed, top and bottom polarisers and
@@ -254,6 +254,8 @@ def test_fp64_component_lanes_spin_transfer_torque(name):
             j, _ = parity_cases.exotic_terms(Junction)
         elif name == "sot_secondary":
             j, _ = parity_cases.sot_dynamic(Junction)
+        elif name == "callback_drivers":
+            j, _ = parity_cases.custom_driver(Junction)
         else:
             j = _stt_constant(name == "stt_alternative")
         b = JunctionBatch.from_junctions([j] * 3, 1e-9, 1e-13, 1e-12, SolverMode.RK4, precision="fp64", log="full",
