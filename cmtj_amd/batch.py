@@ -314,8 +314,9 @@ class JunctionBatch:
     def set_lanes(self, lanes: str) -> "JunctionBatch":
         """Lane mapping of the instances: "one" (all layers of an instance in
         one GPU lane), "per_layer" (one lane per layer), "per_component" (one
-        lane per vector component of each layer; RK4 groups of the lean
-        feature set, binary32 or binary64) or "auto" (per component where such a kernel covers
+        lane per vector component of each layer: binary32 RK4 groups of the
+        lean feature set; binary64 groups of every feature but energy
+        channels) or "auto" (per component where such a kernel covers
         the group and the batch leaves SIMDs idle, else per layer for
         multilayer instances)."""
         if lanes not in _LANES:

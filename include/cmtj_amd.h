@@ -392,7 +392,8 @@ typedef struct cmtj_plan_info {
  * lanes_per_instance: CMTJ_LANES_ONE (all layers of an instance in one
  * lane), CMTJ_LANES_PER_LAYER (one lane per layer, for multilayer
  * instances), CMTJ_LANES_PER_COMPONENT (four lanes per layer, one per vector
- * component; RK4 groups of the lean feature set, binary32 or binary64) or CMTJ_LANES_AUTO
+ * component: binary32 RK4 groups of the lean feature set; binary64 groups of
+ * every feature but energy channels, with RK4, Heun or Euler-Heun) or CMTJ_LANES_AUTO
  * (one lane per component where such a kernel covers the group and its waves
  * fit one per SIMD, else one lane per layer for multilayer instances).
  * Returns NULL on error (see cmtj_last_error). */
