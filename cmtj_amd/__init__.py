@@ -47,15 +47,32 @@ __version__ = "0.1.0"
 
 def install_as_cmtj() -> None:
     """Registers this package under the module name ``cmtj`` so that scripts
-    written for the reference import it unchanged."""
+    written for the reference import it unchanged: ``cmtj`` and its compiled
+    core ``cmtj._cmtj`` (also bare ``_cmtj``, which the reference's package
+    imports), ``cmtj.constants``, ``cmtj.stack``, ``cmtj.reservoir``,
+    ``cmtj.utils`` and its submodules ``procedures``, ``resistance``,
+    ``linear``, ``filters`` and ``constants``."""
     mod = _sys.modules[__name__]
-    _sys.modules.setdefault("cmtj", mod)
-    _sys.modules.setdefault("cmtj.constants", constants)
     from . import reservoir, stack, utils
+    from .utils import constants as uconstants
+    from .utils import filters, linear, procedures, resistance
 
-    _sys.modules.setdefault("cmtj.utils", utils)
-    _sys.modules.setdefault("cmtj.stack", stack)
-    _sys.modules.setdefault("cmtj.reservoir", reservoir)
+    names = {
+        "cmtj": mod,
+        "cmtj._cmtj": mod,
+        "_cmtj": mod,
+        "cmtj.constants": constants,
+        "cmtj.stack": stack,
+        "cmtj.reservoir": reservoir,
+        "cmtj.utils": utils,
+        "cmtj.utils.procedures": procedures,
+        "cmtj.utils.resistance": resistance,
+        "cmtj.utils.linear": linear,
+        "cmtj.utils.filters": filters,
+        "cmtj.utils.constants": uconstants,
+    }
+    for name, module in names.items():
+        _sys.modules.setdefault(name, module)
 
 
 __all__ = [

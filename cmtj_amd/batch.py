@@ -130,8 +130,13 @@ class JunctionBatch:
     # --- construction ---------------------------------------------------------
     @staticmethod
     def _pack(j: Junction, totalTime, timeStep, writeFrequency, solverMode):
+        """Packs one run of `j`.  Every packing takes the junction's next
+        thermal-noise stream, as every runSimulation call does: a junction
+        packed twice (submitted twice, or listed twice in a batch) runs two
+        independent noise paths."""
         jd, L, tabs, _, _ = j._descriptor(totalTime, timeStep, writeFrequency, False, solverMode)
         jd["stream"] = j._runs
+        j._runs += 1
         meta = _Meta(j.getLayerIds(), j.MR_mode, j.Rtag if j.layerNo > 1 else "R")
         return jd, L, tabs, meta
 
@@ -177,7 +182,10 @@ class JunctionBatch:
         js = np.tile(np.ascontiguousarray(jd.reshape(1)).view(np.uint8), n).view(_lib.JUNCTION_DTYPE)
         layers = np.tile(np.ascontiguousarray(L).view(np.uint8), n).view(_lib.LAYER_DTYPE)
         js["layer_offset"] = np.arange(n, dtype=np.uint32) * nl
-        js["stream"] = np.arange(n, dtype=np.uint64)
+        # instance i runs the junction's stream base + i; the junction moves past them
+        base = int(jd["stream"])
+        js["stream"] = base + np.arange(n, dtype=np.uint64)
+        junction._runs = base + n
         b = cls(js, layers, None, [meta], np.zeros(n, dtype=np.int32), precision, log, log_dtype)
         b.set_times(totalTime, timeStep, writeFrequency)
         return b
@@ -421,7 +429,6 @@ def run_batch(junctions, totalTime, timeStep=1e-13, writeFrequency=1e-11, solver
             j.logLength += int(st[i]["entries_done"])
             for q, l in enumerate(j.layers):
                 l.mag = CVector(*mags[i, q])
-            j._runs += 1
     bad = [i for i in range(len(junctions)) if int(st[i]["code"]) == _lib.ERR_NAN]
     if bad:
         raise RuntimeError(f"NAN magnetisation (instances {bad[:8]})")

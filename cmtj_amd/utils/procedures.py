@@ -3,12 +3,15 @@
 The reference's procedures (cmtj/utils/procedures.py:42-196, 199-331) run
 one ``runSimulation`` per field point (and per frequency for VSD), each
 point starting from the final state of the previous one, then post-process
-every log on the host.  Here, with ``independent=True`` (the default), every
-point starts from the junction's present magnetisation plus its own random
-disturbance, all points run as one batch, and the spectra are taken on the
-device (one batched FFT); ``independent=False`` keeps the reference's chained
-semantics (point k starts where point k-1 ended), one drop-in run per point.
-Signatures, defaults, validation messages and outputs follow the reference.
+every log on the host.  The default here, ``independent=False``, keeps
+those chained semantics (point k starts where point k-1 ended, so hysteretic
+sweeps follow their branch), one drop-in run per point.  With
+``independent=True`` (opt-in) every point starts from the junction's present
+magnetisation plus its own random disturbance, all points run as one batch,
+and the spectra are taken on the device (one batched FFT); a sweep that
+crosses a hysteretic region then gives different results from the chained
+one.  Signatures, defaults, validation messages and outputs follow the
+reference.
 """
 from __future__ import annotations
 
@@ -18,9 +21,9 @@ from typing import Any, Callable
 
 import numpy as np
 
-from .batch import JunctionBatch
-from .core import AxialDriver, Axis, CVector, NullDriver, ScalarDriver, SolverMode
-from .utils import calculate_resistance_series
+from ..batch import JunctionBatch
+from ..core import AxialDriver, Axis, CVector, NullDriver, ScalarDriver, SolverMode
+from .resistance import calculate_resistance_series
 
 
 @dataclass
@@ -77,7 +80,7 @@ def PIMM_procedure(junction, Hvecs: np.ndarray, int_step: float, resistance_para
                    simulation_duration: float = 5e-9, wait_time: float = 0e-9, max_frequency: float = 80e9,
                    resistance_fn: Callable = calculate_resistance_series, disturbance: float = 1e-3,
                    take_last_n: int = 100, full_output: bool = False, disable_tqdm: bool = False,
-                   static_only: bool = False, *, independent: bool = True, seed=None,
+                   static_only: bool = False, *, independent: bool = False, seed=None,
                    precision: str = "fp64") -> tuple[np.ndarray, np.ndarray, dict[str, Any]]:
     """Pulse-induced microwave magnetometry over the field points Hvecs
     (procedures.py:42-196): a step Oersted pulse of Hoe_excitation A/m for
@@ -118,8 +121,9 @@ def PIMM_procedure(junction, Hvecs: np.ndarray, int_step: float, resistance_para
             junction.clearLog()
             junction.setLayerExternalFieldDriver("all", AxialDriver(*[ScalarDriver.getConstantDriver(float(h))
                                                                       for h in H]))
-            for lid, v in _disturbed(junction, 1, disturbance, rng).items():
-                junction.setLayerMagnetisation(lid, CVector(*v[0]))
+            if disturbance:
+                for lid, v in _disturbed(junction, 1, disturbance, rng).items():
+                    junction.setLayerMagnetisation(lid, CVector(*v[0]))
             junction.runSimulation(simulation_duration, int_step, int_step)
             lg = junction.getLog()
             trajs.append([[lg[f"{lid}_m{a}"] for a in "xyz"] for lid in layer_ids])
@@ -171,13 +175,14 @@ def VSD_procedure(junction, Hvecs: np.ndarray, frequencies: np.ndarray, int_step
                   resistance_params: list = None, Hoe_direction: Axis = Axis.yaxis, Hoe_excitation: float = 50,
                   simulation_duration: float = 30e-9, disturbance: float = 1e-3, Rtype: str = "Rz",
                   resistance_fn: Callable = calculate_resistance_series, disable_tqdm: bool = False, *,
-                  independent: bool = True, seed=None, precision: str = "fp64") -> np.ndarray:
+                  independent: bool = False, seed=None, precision: str = "fp64") -> np.ndarray:
     """Voltage spin-diode spectrum (procedures.py:199-331): a sine Oersted
     drive at each frequency for each field point; V_mix = mean of the 10 MHz
     low-passed -sin(2 pi f t) * R(t) (resistance.py:12-22).  Rtype "Rz" uses
     the junction's own resistance log (R for one layer, R_a_b for two),
     "Rx"/"Ry" resistance_fn over the m logs.  With independent=True all
-    (field, frequency) pairs run as one batch.  Returns [fields, freqs]."""
+    (field, frequency) pairs run as one batch; by default they chain, each
+    run starting from the previous run's final state.  Returns [fields, freqs]."""
     if resistance_params is None:
         resistance_params = []
     layer_ids = junction.getLayerIds()
@@ -234,8 +239,9 @@ def VSD_procedure(junction, Hvecs: np.ndarray, frequencies: np.ndarray, int_step
                     "all", AxialDriver(*[ScalarDriver.getConstantDriver(float(h)) for h in H]))
                 junction.setLayerOerstedFieldDriver(
                     "all", _on_axis(Hoe_direction, ScalarDriver.getSineDriver(0, Hoe_excitation, float(f), 0)))
-                for lid, v in _disturbed(junction, 1, disturbance, rng).items():
-                    junction.setLayerMagnetisation(lid, CVector(*v[0]))
+                if disturbance:
+                    for lid, v in _disturbed(junction, 1, disturbance, rng).items():
+                        junction.setLayerMagnetisation(lid, CVector(*v[0]))
                 junction.runSimulation(simulation_duration, int_step, int_step)
                 lg = junction.getLog()
                 if Rtype == "Rz":
@@ -257,7 +263,7 @@ def _mixing_voltage(R, freqs, int_step: float) -> np.ndarray:
     import torch
     from scipy.signal import butter
 
-    from . import _lib
+    from .. import _lib
 
     fs = 1.0 / int_step
     nyq = 0.5 * fs

@@ -5,8 +5,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from cmtj_amd import AxialDriver, Axis, CVector, Junction, Layer, constantDriver
-from cmtj_amd.procedures import PIMM_procedure, ResistanceParameters, VSD_procedure
+from cmtj_amd import AxialDriver, Axis, CVector, Junction, Layer, NullDriver, ScalarDriver, constantDriver
+from cmtj_amd.utils.procedures import PIMM_procedure, ResistanceParameters, VSD_procedure
 
 pytestmark = pytest.mark.gpu
 
@@ -83,3 +83,55 @@ def test_device_mixing_voltage_matches_scipy():
     wide[:, 1, :] = Rd
     got2 = _lib.vsd_mix(wide[:, 1, :], f, dt, b, a).cpu().numpy()
     np.testing.assert_array_equal(got2, got)
+
+
+def _easy_axis_film():
+    # in-plane easy axis x, H_K = 2K/Ms = 2e4 A/m; strong damping so a point settles in 2 ns
+    l = Layer("free", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=1.0, thickness=2e-9, cellSurface=1e-16,
+              demagTensor=THIN, damping=0.5)
+    j = Junction([l])
+    j.setLayerAnisotropyDriver("free", constantDriver(1e4))
+    return j
+
+
+def _field(amplitude, phi_deg=5.0):
+    p = np.deg2rad(phi_deg)
+    return [amplitude * np.cos(p), amplitude * np.sin(p), 0.0]
+
+
+def test_default_pimm_chains_points_like_the_reference():
+    """A hysteretic sweep: -4 H_K switches the film to -x, then -0.5 H_K
+    (below the coercive field) keeps it there only when the point starts
+    from the previous point's end.  The default call is imported through the
+    reference's path and must reproduce an explicit runSimulation chain; the
+    opt-in batched mode starts each point from +x and stays on the other
+    branch."""
+    import cmtj_amd
+
+    cmtj_amd.install_as_cmtj()
+    from cmtj.utils.procedures import PIMM_procedure as ref_path_pimm
+
+    H = np.array([_field(-8e4), _field(-1e4)])
+    res = [ResistanceParameters(Rxx0=100, Ramr=1, l=1, w=1)]
+    kw = dict(Hoe_direction=Axis.zaxis, simulation_duration=2e-9, disturbance=0.0)
+    s_chain, f_chain, o_chain = ref_path_pimm(_easy_axis_film(), H, 1e-12, res, **kw)
+    _, _, o_ind = PIMM_procedure(_easy_axis_film(), H, 1e-12, res, independent=True, **kw)
+    mx_chain = [float(m[0]) for m in o_chain["m_avg"]]
+    mx_ind = [float(m[0]) for m in o_ind["m_avg"]]
+    print(f"[pimm chain] m_x chained {mx_chain}, independent {mx_ind}", flush=True)
+    assert mx_chain[0] < -0.9 and mx_chain[1] < -0.9   # stays on the switched branch
+    assert mx_ind[0] < -0.9 and mx_ind[1] > 0.9        # second point starts again from +x
+
+    # the same chain written out as the reference's loop (procedures.py:115-141)
+    j = _easy_axis_film()
+    oe = AxialDriver(NullDriver(), NullDriver(), ScalarDriver.getStepDriver(0, 50, 0, 1e-12 * 3))
+    for p, h in enumerate(H):
+        j.clearLog()
+        j.setLayerExternalFieldDriver("all", AxialDriver(*[ScalarDriver.getConstantDriver(float(v)) for v in h]))
+        j.setLayerOerstedFieldDriver("all", oe)
+        j.runSimulation(2e-9, 1e-12, 1e-12)
+        lg = j.getLog()
+        m_end = np.array([lg["free_mx"][-1], lg["free_my"][-1], lg["free_mz"][-1]])
+        np.testing.assert_allclose(o_chain["m_avg"][p], m_end, rtol=0, atol=1e-12)
+        spec = np.abs(np.fft.fft(np.asarray(lg["free_mz"])))[: len(lg["free_mz"]) // 2]
+        np.testing.assert_allclose(s_chain[p], spec[: s_chain.shape[1]], rtol=1e-9, atol=1e-12)

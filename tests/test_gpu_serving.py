@@ -21,13 +21,16 @@ def test_sliced_run_equals_uninterrupted(precision):
     for n in ("bilayer_iec_rk4", "trilayer_heun", "sot_dynamic_heun", "stt_drivers", "five_layer_chain",
               "rh_strip"):
         j, kw = parity_cases.CASES[n](Junction)
-        j.setNoiseSeed(11)
         js.append(j)
         kws.append(kw)
-    mk = lambda: JunctionBatch.from_junctions(js, [k["totalTime"] for k in kws], [k["timeStep"] for k in kws],
-                                              [k["writeFrequency"] for k in kws],
-                                              [k.get("solverMode", SolverMode.RK4) for k in kws],
-                                              precision=precision, log="full").set_lanes("per_layer")
+    def mk():
+        for j in js:  # each packing takes the next noise stream: start both batches from the same one
+            j.setNoiseSeed(11)
+        return JunctionBatch.from_junctions(js, [k["totalTime"] for k in kws], [k["timeStep"] for k in kws],
+                                            [k["writeFrequency"] for k in kws],
+                                            [k.get("solverMode", SolverMode.RK4) for k in kws],
+                                            precision=precision, log="full").set_lanes("per_layer")
+
     a = mk().run()
     b = mk().run_sliced(slice_steps=777, n_streams=3)
     la, lb = a.log.cpu().numpy(), b.log.cpu().numpy()
@@ -51,13 +54,15 @@ def test_results_are_the_same_alone_and_in_the_mix(precision):
     with Server(precision=precision, slice_steps=1500, n_streams=4, log_elements=1 << 22) as srv:
         futs = []
         for q, (kind, j, T, dt, wf, sm, log) in enumerate(mix):
+            j.setNoiseSeed(1000 + q)  # the same (seed, stream) as the run alone below
             futs.append(srv.submit(j, T, dt, wf, sm, log=log))
             if q % 6 == 5:
                 time.sleep(0.05)  # later requests join while earlier ones run
         res = [f.result(timeout=300) for f in futs]
         ticks = srv.ticks
     worst = 0.0
-    for spec, r in zip(mix, res):
+    for q, (spec, r) in enumerate(zip(mix, res)):
+        spec[1].setNoiseSeed(1000 + q)
         lg, m = _alone(spec, precision)
         assert set(lg) == set(r.log)
         for k in lg:

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from cmtj_amd import AxialDriver, Axis, CVector, Layer, constantDriver
-from cmtj_amd.procedures import PIMM_procedure, ResistanceParameters, VSD_procedure, compute_spectrum_strip
+from cmtj_amd.utils.procedures import PIMM_procedure, ResistanceParameters, VSD_procedure, compute_spectrum_strip
 from cmtj_amd.utils import FieldScan
 from oracle_backend import OracleJunction
 

@@ -73,3 +73,24 @@ def test_group_key_follows_the_structure():
     assert key(a) == key(b) == key(a, T=2e-9, dt=2e-13)
     assert key(a) != key(workloads.chain_junction(3)) != key(workloads.chain_junction(4))
     assert key(workloads.vsd_junction()) != key(a)  # sine drive, one layer
+
+
+def test_each_submission_takes_its_own_noise_stream():
+    """Two submissions of one 300 K junction run two independent thermal
+    paths, as two runSimulation calls do; so do repeated entries of a batch."""
+    from cmtj_amd import constantDriver
+    from cmtj_amd.batch import JunctionBatch
+
+    j = workloads.pimm_junction()
+    j.setLayerTemperatureDriver("all", constantDriver(300.0))
+    j.setNoiseSeed(42)
+    s = Server(log_elements=1 << 20)  # not started: requests stay queued, no device work
+    s.submit(j, 1e-10, 1e-13, 1e-13, log="mx")
+    s.submit(j, 1e-10, 1e-13, 1e-13, log="mx")
+    pairs = [(int(r.jd["seed"][0]), int(r.jd["stream"][0])) for r in s._pending]
+    assert pairs[0] != pairs[1] and pairs[0][0] == pairs[1][0] == 42
+    b = JunctionBatch.from_junctions([j, j, j], 1e-10, 1e-13, 1e-13)
+    assert len(set(b.js["stream"].tolist())) == 3
+    assert min(b.js["stream"].tolist()) > max(p[1] for p in pairs)
+    r = JunctionBatch.replicate(j, 4, 1e-10, 1e-13, 1e-13)
+    assert r.js["stream"].tolist() == list(range(5, 9)) and j._runs == 9
