@@ -161,7 +161,7 @@ def _bench_rh() -> dict:
     plain_rate = int(base["n_steps"]) / float(np.median(t))
     t0 = time.perf_counter()
     olog, omag, ost = oracle.run_batch(b64.js, b64.layers, _lib.constants_get(), None, b64.log_elements,
-                                       n_threads=min(16, os.cpu_count() or 1))
+                                       n_threads=_host_cpus()[0])
     cpu_s = time.perf_counter() - t0
     glog = r64.log.cpu().numpy()
     dm64 = float(np.max(np.abs(r64.final_m.cpu().numpy()[:, 0] - omag[:, 0])))
@@ -171,7 +171,7 @@ def _bench_rh() -> dict:
             "fp64_batched_junction_steps_per_s": steps / (ms64 * 1e-3), "fp64_batched_ms": ms64,
             "fp32_batched_junction_steps_per_s": steps / (ms32 * 1e-3), "fp32_batched_ms": ms32,
             "fp64_plain_junction_steps_per_s": plain_rate,
-            "cpu_oracle_junction_steps_per_s": steps / cpu_s, "cpu_threads": min(16, os.cpu_count() or 1),
+            "cpu_oracle_junction_steps_per_s": steps / cpu_s, "cpu_threads": _host_cpus()[0],
             "max_dm_fp64_vs_oracle_final": dm64, "max_dlog_fp32_vs_fp64": dm32,
             "log_max_abs_difference_fp64_vs_oracle": float(np.max(np.abs(glog - olog))),
             "final_Rx_first_mid_last": [float(rx[0]), float(rx[50]), float(rx[-1])]}
@@ -376,7 +376,7 @@ def _bench_dp(n_fields: int = 64, n_currents: int = 64, total_time: float = 3e-9
     sub = b.subset(idx)
     t0 = time.perf_counter()
     olog, omag, ost = oracle.run_batch(sub.js, sub.layers, _lib.constants_get(), None, sub.log_elements,
-                                       n_threads=min(16, os.cpu_count() or 1))
+                                       n_threads=_host_cpus()[0])
     cpu_s = time.perf_counter() - t0
     cpu_steps = int(ost["steps_done"].astype(np.int64).sum())
     same = bool(np.all(ost["steps_done"] == st["steps_done"][idx]) and np.all(ost["entries_done"] ==
@@ -384,7 +384,7 @@ def _bench_dp(n_fields: int = 64, n_currents: int = 64, total_time: float = 3e-9
     dm = float(np.max(np.abs(r.final_m.cpu().numpy()[idx, 0] - omag[:, 0])))
     return {"workload": f"dormand_prince_sot_stability_{n_fields}x{n_currents}", "instances": len(js),
             "accepted_steps": steps, "gpu_ms": ms, "gpu_accepted_steps_per_s": steps / (ms * 1e-3),
-            "cpu_oracle_accepted_steps_per_s": cpu_steps / cpu_s, "cpu_threads": min(16, os.cpu_count() or 1),
+            "cpu_oracle_accepted_steps_per_s": cpu_steps / cpu_s, "cpu_threads": _host_cpus()[0],
             "cpu_sample": f"{len(idx)} instances", "same_step_and_entry_counts_on_sample": same,
             "max_dm_final_gpu_vs_oracle_sample": dm}
 
@@ -413,7 +413,7 @@ def _bench_cims(n_currents: int = 256, n_seeds: int = 4096) -> dict:
     bo = b.subset(idxo, precision="fp64")
     t0 = time.perf_counter()
     _, omag, _ = oracle.run_batch(bo.js, bo.layers, _lib.constants_get(), None, 0,
-                                  n_threads=min(16, os.cpu_count() or 1))
+                                  n_threads=_host_cpus()[0])
     cpu_s = time.perf_counter() - t0
     ro = b64.subset(np.searchsorted(idx64, idxo)).run()  # the GPU FP64 of the oracle's sample
     dmo = float(np.max(np.abs(ro.final_m.cpu().numpy()[:, :2] - omag[:, :2])))
@@ -457,7 +457,7 @@ def _bench_sto(n_currents: int = 512) -> dict:
     bo = b64.subset(idx)
     t0 = time.perf_counter()
     olog, omag, _ = oracle.run_batch(bo.js, bo.layers, _lib.constants_get(), None, bo.log_elements,
-                                     n_threads=min(16, os.cpu_count() or 1))
+                                     n_threads=_host_cpus()[0])
     cpu_s = time.perf_counter() - t0
     ro = bo.run()
     dmo = float(np.max(np.abs(ro.log.cpu().numpy() - olog)))
@@ -474,6 +474,63 @@ def _bench_sto(n_currents: int = 512) -> dict:
             "fp32_vs_fp64_max_dm": dm_m, "fp32_vs_fp64_max_dlog": dm,
             "fp64_gpu_vs_oracle_max_dlog": dmo, "oracle_sample": f"{len(idx)} currents",
             "cpu_oracle_junction_steps_per_s": bo.total_steps / cpu_s}
+
+
+def _host_cpus():
+    """(threads to use, nproc, CPUs this process may run on).  The box gives a
+    GPU's process a share of 16 host CPUs, whatever nproc shows."""
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    return min(16, avail), nproc, avail
+
+
+def _cpu_baseline(b, args):
+    """FP64 oracle (-O3 -march=native build of oracle.c, no FP contraction) on
+    a stratified sample of the sweep at full durations: all the process's host
+    CPUs (at most the box's 16-CPU share), then one core on a smaller sample."""
+    import tempfile
+
+    import oracle
+
+    ncores, nproc, avail = _host_cpus()
+    try:
+        lib_path, build = oracle.build_native(tempfile.mkdtemp(prefix="oracle_native_")), "-O3 -march=native"
+    except Exception as e:  # no compiler: the shipped generic build
+        lib_path, build = None, f"-O3 generic (native build failed: {type(e).__name__})"
+
+    def sample(n_pts):
+        idx = np.linspace(0, len(b.js) - 1, n_pts).astype(int)
+        bs = _pimm("fp64", args.points, log_dtype="f64")
+        js = bs.js[idx].copy()
+        layers = np.concatenate([bs.layers[int(js[q]["layer_offset"]): int(js[q]["layer_offset"]) + 2]
+                                 for q in range(len(idx))])
+        js["layer_offset"] = np.arange(len(idx), dtype=np.uint32) * 2
+        sizes = 3 * js["n_entries"].astype(np.uint64)
+        js["log_offset"] = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint64)
+        return js, layers, int(sizes.sum())
+
+    def timed(n_pts, threads):
+        js, layers, nlog = sample(n_pts)
+        t0 = time.perf_counter()
+        oracle.run_batch(js, layers, _lib.constants_get(), None, nlog, n_threads=threads, lib_path=lib_path)
+        s = time.perf_counter() - t0
+        return int(js["n_steps"].astype(np.int64).sum()), s
+
+    steps, cs = timed(args.cpu_sample, ncores)
+    n1 = max(8, args.cpu_sample // 32)
+    steps1, cs1 = timed(n1, 1)
+    cpu = {"value": steps / cs, "unit": "junction-steps/s", "cores": ncores, "kind": "port",
+           "host_nproc": nproc, "host_cpus_available": avail, "build": build,
+           "single_core": steps1 / cs1,
+           "sample": f"{args.cpu_sample} of {len(b.js)} sweep points (stratified), full durations, "
+                     f"{steps:.3e} junction-steps, {cs:.1f} s on {ncores} threads; single core: {n1} points, "
+                     f"{steps1:.3e} junction-steps, {cs1:.1f} s"}
+    print(f"[bench] cpu oracle: {cpu['value']:.4e} junction-steps/s on {ncores} threads "
+          f"(nproc {nproc}, {avail} available), single core {cpu['single_core']:.4e}", file=sys.stderr, flush=True)
+    return cpu
 
 
 def main() -> int:
@@ -590,26 +647,7 @@ def main() -> int:
     # ---- CPU baseline: FP64 oracle, bounded stratified sample ----------------
     cpu = None
     if not args.no_cpu:
-        import oracle
-
-        ncores = min(16, os.cpu_count() or 1)
-        idx = np.linspace(0, len(b.js) - 1, args.cpu_sample).astype(int)
-        bs = _pimm("fp64", args.points, log_dtype="f64")
-        js = bs.js[idx].copy()
-        layers = np.concatenate([bs.layers[int(js[q]["layer_offset"]): int(js[q]["layer_offset"]) + 2]
-                                 for q in range(len(idx))])
-        js["layer_offset"] = np.arange(len(idx), dtype=np.uint32) * 2
-        sizes = 3 * js["n_entries"].astype(np.uint64)
-        js["log_offset"] = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint64)
-        t2 = time.perf_counter()
-        oracle.run_batch(js, layers, _lib.constants_get(), None, int(sizes.sum()), n_threads=ncores)
-        cs = time.perf_counter() - t2
-        csteps = int(js["n_steps"].astype(np.int64).sum())
-        cpu = {"value": csteps / cs, "unit": "junction-steps/s", "cores": ncores, "kind": "port",
-               "sample": f"{len(idx)} of {len(b.js)} sweep points (every {len(b.js) // len(idx)}th), "
-                         f"full durations, {csteps:.3e} junction-steps, {cs:.1f} s"}
-        print(f"[bench] cpu oracle: {cpu['value']:.4e} junction-steps/s on {ncores} threads", file=sys.stderr,
-              flush=True)
+        cpu = _cpu_baseline(b, args)
 
     serving = None
     if not args.no_serving:

@@ -3,7 +3,8 @@
   r_h_loop      1-layer R(H) loop, 101 fields, STRIP Rx, RK4 dt=1e-13, 10 ns
   pimm          2-layer MTJ pulse-induced spectra, fields 0..600 kA/m at 30 deg
                 in-plane, per-point duration U(2, 20) ns, 1 kA/m Hoe_y step
-  cims_pbit     2-layer static-SOT switching ensemble at 300 K, Euler-Heun
+  cims_pbit     2-layer static-SOT switching ensemble at 300 K, Euler-Heun,
+                trapezoid current pulses, perpendicular free layer
   sto           2-layer STT oscillator, DC current sweep, RK4
 
 Parameters the workload descriptions leave open are fixed here and listed in
@@ -76,15 +77,24 @@ def pimm(n_points: int = 4096, seed: int = 0, t_min: float = 2e-9, t_max: float 
 
 
 def cims_junction(T: float = 300.0) -> Junction:
-    """Free: Ms 1.1 T, t 1.2 nm, Ku 3e5 J/m^3 along z, alpha 0.02, static SOT
-    with tau_DL/j = 1e-12 T per A/m^2 (x TtoAm -> A/m) and FL/DL = 0.3,
-    polarisation along y; pinned layer: Ms 1.2 T, t 3 nm, Ku 1e5 J/m^3 along
-    x, alpha 0.02; S = (50 nm)^2 pi; thermal field on both layers."""
+    """SOT switching device.  Free: Ms 1.1 T, t 1.2 nm, Ku 3e5 J/m^3 along z
+    taken as the effective perpendicular anisotropy (shape anisotropy folded
+    in, so the free layer's demagnetisation tensor is zero), alpha 0.02,
+    static SOT with tau_DL/j = 1e-12 T per A/m^2 (x TtoAm -> A/m) and
+    FL/DL = 0.3, polarisation along y, starting at +z.  Anisotropy field
+    2K/Ms = 545 kA/m; thermal stability K V / kT = 171 at 300 K, so no
+    switching without current.  Pinned (the TMR reference): Ms 1.2 T, t 3 nm,
+    Ku 1e5 J/m^3 along x, thin-film demagnetisation, alpha 0.02.
+    S = (50 nm)^2 pi.  In-plane bias -20 kA/m along x (the current's axis)
+    sets which way a damping-like torque switches the free layer; thermal
+    field on both layers.  A damping-like field above about half the
+    anisotropy field (j ~ 3.4e11 A/m^2) switches it to -z."""
     TtoAm = 795774.715459
     hdl = 1e-12 * TtoAm
     S = (50e-9) ** 2 * math.pi
-    free = Layer.createSOTLayer("free", mag=CVector(0.01, 0.0, 1.0), anis=CVector(0, 0, 1), Ms=1.1,
-                                thickness=1.2e-9, cellSurface=S, demagTensor=THIN, damping=0.02,
+    zero = [CVector(0, 0, 0), CVector(0, 0, 0), CVector(0, 0, 0)]
+    free = Layer.createSOTLayer("free", mag=CVector(0.0, 0.0, 1.0), anis=CVector(0, 0, 1), Ms=1.1,
+                                thickness=1.2e-9, cellSurface=S, demagTensor=zero, damping=0.02,
                                 fieldLikeTorque=0.3 * hdl, dampingLikeTorque=hdl)
     free.setReferenceLayer(CVector(0, 1, 0))
     pinned = Layer("pinned", mag=CVector(1, 0, 0), anis=CVector(1, 0, 0), Ms=1.2, thickness=3e-9, cellSurface=S,
@@ -93,22 +103,29 @@ def cims_junction(T: float = 300.0) -> Junction:
     j.setLayerAnisotropyDriver("free", constantDriver(3e5))
     j.setLayerAnisotropyDriver("pinned", constantDriver(1e5))
     j.setLayerTemperatureDriver("all", constantDriver(T))
-    j.setLayerExternalFieldDriver("all", AxialDriver(5e3, 0, 0))
+    j.setLayerExternalFieldDriver("all", AxialDriver(-20e3, 0, 0))
     return j
 
 
 def cims_pbit(n_currents: int = 256, n_seeds: int = 4096, total_time: float = 5e-9, dt: float = 1e-14,
-              pulse: float = 5e-9, precision: str = "fp32", seed: int = 1) -> JunctionBatch:
-    """Current pulses 1e10..1e12 A/m^2 (log-spaced) x seeds; final state only."""
-    j = cims_junction()
+              edge: float = 1e-9, plateau: float = 1.5e-9, precision: str = "fp32", seed: int = 1,
+              j_min: float = 1e10, j_max: float = 1e12, T: float = 300.0) -> JunctionBatch:
+    """Current pulses j_min..j_max A/m^2 (log-spaced, default 1e10..1e12) x
+    seeds; final state only.  Each pulse is a trapezoid from t = 0: linear
+    rise over `edge`, `plateau` at j, linear fall over `edge`, then zero
+    current to total_time (default 1 + 1.5 + 1 ns, then 1.5 ns of
+    relaxation).  Instance q = current (q // n_seeds), seed (q % n_seeds);
+    a run switched when its final free-layer m_z < 0."""
+    j = cims_junction(T)
     n = n_currents * n_seeds
     b = JunctionBatch.replicate(j, n, total_time, dt, total_time, SolverMode.EulerHeun, precision=precision,
                                 log="none")
-    jcur = np.repeat(np.logspace(10, 12, n_currents), n_seeds)
-    b.set_driver("free", "current", "type", np.full(n, 3, dtype=np.int32))  # step
+    jcur = np.repeat(np.logspace(math.log10(j_min), math.log10(j_max), n_currents), n_seeds)
+    b.set_driver("free", "current", "type", np.full(n, 6, dtype=np.int32))  # trapezoid
     b.set_driver("free", "current", "amplitude", jcur)
     b.set_driver("free", "current", "t_start", np.zeros(n))
-    b.set_driver("free", "current", "t_stop", np.full(n, pulse))
+    b.set_driver("free", "current", "edge_time", np.full(n, edge))
+    b.set_driver("free", "current", "steady_time", np.full(n, plateau))
     rng = np.random.default_rng(seed)
     b.set_seeds(rng.integers(0, 2**63, n, dtype=np.uint64))
     return b
@@ -209,7 +226,7 @@ def serving_mix(n: int, seed: int = 0, t_min: float = 1e-9, t_max: float = 50e-9
         elif k == "cims":
             j = cims_junction()
             jc = float(10 ** rng.uniform(10, 12))
-            j.setLayerCurrentDriver("free", ScalarDriver.getStepDriver(0, jc, 0, T / 2))
+            j.setLayerCurrentDriver("free", ScalarDriver.getTrapezoidDriver(0, jc, 0, T / 5, T / 5))
             j.setNoiseSeed(int(rng.integers(0, 2**62)))
             out.append((k, j, T, 1e-14, max(1e-12, 1e-14), SolverMode.EulerHeun, "m"))
         else:

@@ -41,27 +41,43 @@ REF_DRIVERS = HERE / "_ref" / "ref_drivers"
 _lib = None
 
 
+def _bind(path):
+    from cmtj_amd import _lib as abi  # struct dtypes only
+
+    lib = ctypes.CDLL(str(path))
+    P = ctypes.c_void_p
+    lib.oracle_run.argtypes = [P, P, P, P, ctypes.c_size_t, P, P, P]
+    lib.oracle_run_batch.argtypes = [P, ctypes.c_size_t, P, P, P, ctypes.c_size_t, P, P, P, ctypes.c_int]
+    lib.oracle_driver_value.argtypes = [P, ctypes.c_double, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P]
+    lib.oracle_driver_value.restype = ctypes.c_double
+    lib.oracle_philox4x32_10.argtypes = [P, P, P]
+    lib.oracle_thermal_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.c_uint32, P]
+    lib.oracle_n_channels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.oracle_run_stack.argtypes = [P, P, P, P, P, P, P]
+    lib.oracle_run_group.argtypes = [P, P, P, P, P, P, P]
+    return (lib, abi)
+
+
 def _load():
     global _lib
     if _lib is None:
         if not LIB.exists():
             raise ImportError(f"{LIB} missing: run `make -C oracle`")
-        from cmtj_amd import _lib as abi  # struct dtypes only
-
-        lib = ctypes.CDLL(str(LIB))
-        P = ctypes.c_void_p
-        lib.oracle_run.argtypes = [P, P, P, P, ctypes.c_size_t, P, P, P]
-        lib.oracle_run_batch.argtypes = [P, ctypes.c_size_t, P, P, P, ctypes.c_size_t, P, P, P, ctypes.c_int]
-        lib.oracle_driver_value.argtypes = [P, ctypes.c_double, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, P]
-        lib.oracle_driver_value.restype = ctypes.c_double
-        lib.oracle_philox4x32_10.argtypes = [P, P, P]
-        lib.oracle_thermal_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
-                                               ctypes.c_uint32, P]
-        lib.oracle_n_channels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        lib.oracle_run_stack.argtypes = [P, P, P, P, P, P, P]
-        lib.oracle_run_group.argtypes = [P, P, P, P, P, P, P]
-        _lib = (lib, abi)
+        _lib = _bind(LIB)
     return _lib
+
+
+_native = {}
+
+
+def build_native(out_dir) -> Path:
+    """Builds the oracle tuned for this machine (-march=native, still without
+    FP contraction or fast-math) into out_dir, for CPU-baseline timing only."""
+    out = Path(out_dir) / "liboracle.so"
+    subprocess.run(["make", "-s", "-C", str(HERE), "native", f"NATIVE_OUT={out}"], check=True,
+                   capture_output=True)
+    return out
 
 
 def _p(a):
@@ -83,9 +99,15 @@ def run(j, layers, consts, tables=None):
     return log, mag, st
 
 
-def run_batch(js, layers, consts, tables=None, log_elements=0, n_threads=None):
-    """Many junctions over host threads; log blocks at each log_offset (FP64)."""
-    lib, abi = _load()
+def run_batch(js, layers, consts, tables=None, log_elements=0, n_threads=None, lib_path=None):
+    """Many junctions over host threads; log blocks at each log_offset (FP64).
+    lib_path: another build of oracle.c (build_native) to run instead."""
+    if lib_path is None:
+        lib, abi = _load()
+    else:
+        if str(lib_path) not in _native:
+            _native[str(lib_path)] = _bind(lib_path)
+        lib, abi = _native[str(lib_path)]
     js = np.ascontiguousarray(js, dtype=abi.JUNCTION_DTYPE)
     layers = np.ascontiguousarray(layers, dtype=abi.LAYER_DTYPE)
     log = np.zeros(max(int(log_elements), 1), dtype=np.float64)

@@ -113,8 +113,8 @@ def _workload(name):
         return _vsd_sweep("fp64", n=8)
     if name == "sto":
         return workloads.sto(n_currents=8, total_time=0.5e-9, precision="fp64")
-    return workloads.cims_pbit(n_currents=4, n_seeds=4, total_time=0.1e-9, dt=1e-14, pulse=0.05e-9,
-                               precision="fp64")
+    return workloads.cims_pbit(n_currents=4, n_seeds=4, total_time=0.1e-9, dt=1e-14, edge=0.02e-9,
+                               plateau=0.03e-9, precision="fp64")
 
 
 @pytest.mark.parametrize("lanes", ["one", "per_layer"])
