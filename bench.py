@@ -99,12 +99,50 @@ def _serving(qps: float, seconds: float, seed: int = 0, lanes: str = "per_compon
     lat = np.array([r.latency for r in res])
     steps = sum(r.steps for r in res)
     wall = max(r.t_done for r in res) - t0
+    iso = np.array([_isolated_s(mix[q], lanes) for q in range(len(res))])
+    slow = lat / iso
+    by_kind = {}
+    for k in sorted(set(kinds)):
+        sel = np.array([kk == k for kk in kinds])
+        by_kind[k] = {"n": int(sel.sum()), "p50": float(np.percentile(slow[sel], 50)),
+                      "p99": float(np.percentile(slow[sel], 99))}
     return {"qps": qps, "arrival_window_s": seconds, "requests": len(res),
             "mix": {k: kinds.count(k) for k in sorted(set(kinds))},
             "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "max": float(lat.max())},
+            "slowdown": {"p50": float(np.percentile(slow, 50)), "p99": float(np.percentile(slow, 99)),
+                         "by_kind": by_kind,
+                         "definition": "latency / the request's kernel time alone (its own junction, "
+                                       "lane mapping and log, up to 20000 steps timed with HIP events, "
+                                       "scaled to its step count)"},
             "junction_steps": int(steps), "junction_steps_per_s": steps / wall, "wall_s": wall,
             "slices": ticks, "slice_ms_target": 20.0, "host_s": host_s, "precision": "fp32", "lanes": lanes}
+
+
+def _isolated_s(spec, lanes: str, cap: int = 20000) -> float:
+    """Kernel time of one served request run alone: its own junction, lane
+    mapping and log, truncated to `cap` steps, timed with HIP events on the
+    launch stream and scaled to the request's full step count."""
+    import torch
+
+    from cmtj_amd.batch import JunctionBatch
+
+    kind, j, T, dt, wf, sm, log = spec
+    n_full = int(T / dt)
+    Tc = min(T, cap * dt)
+    b = JunctionBatch.from_junctions([j], Tc, dt, max(dt, min(wf, Tc)), sm, precision="fp32",
+                                     log=log).set_lanes(lanes)
+    b.plan()
+    b._buffers("cuda")
+    stream = torch.cuda.current_stream()
+    b.execute(stream)  # warm-up
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    b.execute(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    n_c = max(1, int(b.js[0]["n_steps"]))
+    return e0.elapsed_time(e1) * 1e-3 * n_full / n_c
 
 
 def _timed_run(b, reps: int = 1):

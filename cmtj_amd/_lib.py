@@ -46,7 +46,7 @@ LOG_FULL = LOG_MX | LOG_MYZ | LOG_MR
 FP64, FP32 = 0, 1
 LOG_F64, LOG_F32 = 0, 1
 
-OK, ERR_INVALID, ERR_HIP, ERR_NAN, ERR_LIMIT = 0, 1, 2, 3, 4
+OK, ERR_INVALID, ERR_HIP, ERR_NAN, ERR_LIMIT, ERR_DP_REJECTED = 0, 1, 2, 3, 4, 5
 
 DRIVER_DTYPE = np.dtype(
     [

@@ -1153,12 +1153,9 @@ class Junction:
             if l.scaleNoise != 0 or l.oneFNoise is not None:
                 raise NotImplementedError("coloured (1/f, alpha) noise is not provided by the GPU engine")
         if local == SolverMode.DormandPrince:
-            if self.layerNo != 1:
-                # the reference's multi-layer Dormand-Prince sets the step to
-                # -1 after any rejected step (layers' hopt, junction.hpp:1699-1704)
-                # and then runs its clock backwards
-                raise NotImplementedError("DormandPrince runs single-layer junctions on the GPU engine")
-            if self.layers[0].adaptiveParams.use_pid_control:
+            if self.layerNo > 4:
+                raise NotImplementedError("DormandPrince runs junctions of 1 to 4 layers on the GPU engine")
+            if any(l.adaptiveParams.use_pid_control for l in self.layers):
                 raise NotImplementedError("the PID step controller of DormandPrince is not provided")
         return int(local)
 
@@ -1295,6 +1292,13 @@ class Junction:
             raise RuntimeError("NAN magnetisation")
         if int(st["code"]) == _lib.ERR_LIMIT:
             raise RuntimeError("DormandPrince: the step-attempt or log-entry limit of the run was reached")
+        if int(st["code"]) == _lib.ERR_DP_REJECTED:
+            # runMultiLayerSolver (core/junction.hpp:1699-1704) takes the
+            # smallest Layer::hopt after a rejection; hopt stays -1, so the
+            # reference's clock runs backwards from here and never finishes
+            raise RuntimeError("DormandPrince: a layer rejected its step; the reference then sets the step to "
+                               "Layer::hopt = -1 and its clock runs backwards (the run cannot finish). "
+                               "Start from a smaller timeStep or loosen the tolerances")
         if int(st["code"]) != _lib.OK:
             raise RuntimeError(f"engine status {int(st['code'])}")
         if verbose:

@@ -187,11 +187,10 @@ static int validate(const cmtj_junction& j, size_t idx, const cmtj_layer* layers
     return fail(CMTJ_ERR_INVALID, "unknown solver" + at);
   if (!(j.dt > 0.0) || !std::isfinite(j.dt)) return fail(CMTJ_ERR_INVALID, "time step must be > 0" + at);
   if (dp) {
-    // adaptive steps: one layer (the reference's multi-layer variant sets
-    // the step to -1 after any rejection and does not terminate), binary64,
-    // closed-form drivers only (callback tables are sampled at fixed stage
-    // times), no energy channels, the time logged per entry
-    if (j.n_layers != 1) return fail(CMTJ_ERR_INVALID, "Dormand-Prince runs take one layer" + at);
+    // adaptive steps: 1-4 layers, binary64, closed-form drivers only
+    // (callback tables are sampled at fixed stage times), no energy
+    // channels, the time logged per entry
+    if (j.n_layers > 4) return fail(CMTJ_ERR_INVALID, "Dormand-Prince runs take 1 to 4 layers" + at);
     if (prec != CMTJ_FP64) return fail(CMTJ_ERR_INVALID, "Dormand-Prince runs are FP64" + at);
     if (j.log_mask & CMTJ_LOG_ENERGY)
       return fail(CMTJ_ERR_INVALID, "no energy channels with Dormand-Prince" + at);
@@ -199,9 +198,10 @@ static int validate(const cmtj_junction& j, size_t idx, const cmtj_layer* layers
       return fail(CMTJ_ERR_INVALID, "a Dormand-Prince log needs the time channel" + at);
     if (!(j.total_time >= 0.0) || !std::isfinite(j.total_time) || !(j.write_freq > 0.0))
       return fail(CMTJ_ERR_INVALID, "total_time >= 0 and write_freq > 0 needed" + at);
-    for (int s = 0; s < CMTJ_NSLOT; ++s)
-      if (layers[j.layer_offset].drv[s].type == CMTJ_TABLE)
-        return fail(CMTJ_ERR_INVALID, "callback drivers need fixed stage times (not Dormand-Prince)" + at);
+    for (int l = 0; l < j.n_layers; ++l)
+      for (int s = 0; s < CMTJ_NSLOT; ++s)
+        if (layers[j.layer_offset + l].drv[s].type == CMTJ_TABLE)
+          return fail(CMTJ_ERR_INVALID, "callback drivers need fixed stage times (not Dormand-Prince)" + at);
   } else {
     if (j.log_mask & CMTJ_LOG_TIME) return fail(CMTJ_ERR_INVALID, "the time channel is for Dormand-Prince" + at);
     if (j.write_every < 1) return fail(CMTJ_ERR_INVALID, "write_every must be >= 1" + at);

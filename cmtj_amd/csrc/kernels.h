@@ -757,12 +757,47 @@ __device__ __forceinline__ void log_put(bool f32, char* p, double v) {
     *(double*)p = v;
 }
 
+// Resistance coefficients of one layer, read from the descriptors once per
+// run: the descriptors may sit in host-mapped memory (a serving loop's
+// plans), where every read crosses the host link.  ahe2 = 0.5 AHE and
+// xy = AMR_Y + SMR_Y are the first products of the reference's expressions
+// (core/junction.hpp:1840-1846), so the entries round as before.
+struct MRCoef {
+  double ref[3];  // CLASSIC with one layer: the fixed reference vector
+  double rx0, amrx, smrx, ry0, ahe2, xy;  // STRIP
+};
+
+__device__ __forceinline__ MRCoef mr_coef(const GroupArgs& g, const cmtj_junction* J, int l, bool on) {
+  MRCoef c = {{0.0, 0.0, 0.0}, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (!on || !(g.log_mask & CMTJ_LOG_MR)) return c;
+  if (g.mr_mode == CMTJ_MR_CLASSIC && g.nl == 1) {
+    const double* rf = g.layers[J->layer_offset].ref;
+    c.ref[0] = rf[0];
+    c.ref[1] = rf[1];
+    c.ref[2] = rf[2];
+  } else if (g.mr_mode == CMTJ_MR_STRIP) {
+    c.rx0 = J->Rx0[l];
+    c.amrx = J->AMR_X[l];
+    c.smrx = J->SMR_X[l];
+    c.ry0 = J->Ry0[l];
+    c.ahe2 = 0.5 * J->AHE[l];
+    c.xy = J->AMR_Y[l] + J->SMR_Y[l];
+  }
+  return c;
+}
+
+// Layers of capacity whose STRIP coefficients a one-lane-per-instance kernel
+// keeps in registers; larger kernels read the descriptors per entry.
+constexpr int kMrRegLayers = 2;
+
 // logLayerParams (core/junction.hpp:1580-1636) for entry k.  p points at
-// channel 0 of entry k; consecutive channels are cs bytes apart.
+// channel 0 of entry k; consecutive channels are cs bytes apart.  mc: the
+// resistance coefficients of layers 0..min(LM, kMrRegLayers)-1.
 template <class R, int LM, uint32_t FM, bool LOGE>
 __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junction* J,
                                             const V3<double> (&m)[LM], const ERec<R> (&er)[LM], char* p,
-                                            uint64_t cs, double Rp, double Rap) {
+                                            uint64_t cs, double Rp, double Rap,
+                                            const MRCoef (&mc)[LM < kMrRegLayers ? LM : kMrRegLayers]) {
   const int nl = g.nl, lmask = g.log_mask, mr = g.mr_mode;
   const bool f32 = g.log_f32;
 #pragma unroll
@@ -786,7 +821,7 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
       // reference vector, 2 layers against each other (:1623-1629)
       double cosT;
       if (nl == 1) {
-        const double* rf = g.layers[J->layer_offset].ref;
+        const double* rf = mc[0].ref;
         cosT = m[0].x * rf[0] + m[0].y * rf[1] + m[0].z * rf[2];
       } else {
         const V3<double>& m1 = m[LM > 1 ? 1 : 0];
@@ -800,9 +835,13 @@ __device__ __forceinline__ void write_entry(const GroupArgs& g, const cmtj_junct
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
         if (l < nl) {
-          const double Rx = J->Rx0[l] + J->AMR_X[l] * (m[l].x * m[l].x) + J->SMR_X[l] * (m[l].y * m[l].y);
-          const double Ry = J->Ry0[l] + 0.5 * J->AHE[l] * m[l].z +
-                            (J->AMR_Y[l] + J->SMR_Y[l]) * m[l].x * m[l].y;
+          MRCoef q;
+          if (l < kMrRegLayers)
+            q = mc[l < kMrRegLayers ? l : 0];
+          else
+            q = mr_coef(g, J, l, true);
+          const double Rx = q.rx0 + q.amrx * (m[l].x * m[l].x) + q.smrx * (m[l].y * m[l].y);
+          const double Ry = q.ry0 + q.ahe2 * m[l].z + q.xy * m[l].x * m[l].y;
           rx += 1. / Rx;
           ry += 1. / Ry;
         }
@@ -1054,6 +1093,9 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   char* const lp = logging ? (char*)g.log + lbase * esize : nullptr;
   const uint64_t cs = (uint64_t)NE * esize;
   const double Rp = J->Rp, Rap = J->Rap;
+  MRCoef mcoef[LM < kMrRegLayers ? LM : kMrRegLayers];
+#pragma unroll
+  for (int l = 0; l < (LM < kMrRegLayers ? LM : kMrRegLayers); ++l) mcoef[l] = mr_coef(g, J, l, logging && l < nl);
   const int sps = (SOLVER == CMTJ_RK4) ? 3 : (SOLVER == CMTJ_HEUN ? 2 : 1);
 
   constexpr bool ANCHORED = (sizeof(R) == 4) && (SOLVER == CMTJ_RK4);
@@ -1391,7 +1433,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       }
     }
     if ((uint64_t)i == next_log) {
-      if (logging) write_entry<R, LM, FM, LOGE>(g, J, m, er, lp + (uint64_t)k * esize, cs, Rp, Rap);
+      if (logging) write_entry<R, LM, FM, LOGE>(g, J, m, er, lp + (uint64_t)k * esize, cs, Rp, Rap, mcoef);
       if constexpr (STK) {
         if (logging) {  // the member's stack channels follow its junction channels
           const uint64_t nch = (uint64_t)log_channels(nl, g.mr_mode, g.log_mask);

@@ -296,6 +296,18 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   const uint64_t orr = (uint64_t)(nl * nm) * cs;
   const bool classic = (lmask & CMTJ_LOG_MR) && mrm == CMTJ_MR_CLASSIC;
   const bool strip = (lmask & CMTJ_LOG_MR) && mrm == CMTJ_MR_STRIP;
+  // STRIP coefficients of the lane's layer, read once (the descriptors may
+  // sit in host-mapped memory); 0.5 AHE and AMR_Y + SMR_Y are the reference's
+  // own first products, so the entries round as before
+  double sRx0 = 0.0, sAMRX = 0.0, sSMRX = 0.0, sRy0 = 0.0, sAHE2 = 0.0, sXY = 0.0;
+  if (strip && lact && c == 0) {
+    sRx0 = J->Rx0[ly];
+    sAMRX = J->AMR_X[ly];
+    sSMRX = J->SMR_X[ly];
+    sRy0 = J->Ry0[ly];
+    sAHE2 = 0.5 * J->AHE[ly];
+    sXY = J->AMR_Y[ly] + J->SMR_Y[ly];
+  }
   uint32_t k = 0;
   int code = CMTJ_OK, nan_layer = -1;
   uint32_t i = 0, iend = nsteps;
@@ -600,8 +612,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
         double rx = 0.0, ry = 0.0;
         if (lact && c == 0) {
-          const double Rx = J->Rx0[ly] + J->AMR_X[ly] * (mx * mx) + J->SMR_X[ly] * (my * my);
-          const double Ry = J->Ry0[ly] + 0.5 * J->AHE[ly] * mz + (J->AMR_Y[ly] + J->SMR_Y[ly]) * mx * my;
+          const double Rx = sRx0 + sAMRX * (mx * mx) + sSMRX * (my * my);
+          const double Ry = sRy0 + sAHE2 * mz + sXY * mx * my;
           rx = 1. / Rx;
           ry = 1. / Ry;
         }

@@ -1,5 +1,5 @@
-// kernels_dp.h -- Dormand-Prince 5(4) with adaptive steps, one single-layer
-// instance per lane (FP64).
+// kernels_dp.h -- Dormand-Prince 5(4) with adaptive steps, one instance per
+// lane (FP64): k_dp for one layer, k_dp_ml for 2..LM layers.
 //
 // Each lane runs its own step-size history: Layer::dormand_prince_step
 // (core/junction.hpp:866-1002, the standard controller) inside the adaptive
@@ -58,6 +58,7 @@ __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
   const double max_factor = Ly.dp_max_factor, min_factor = Ly.dp_min_factor, safety = Ly.dp_safety;
   const uint32_t cap = J->n_steps, NE = J->n_entries;
   const bool logging = (g.log != nullptr) && g.log_mask != 0;
+  const MRCoef mcoef[1] = {mr_coef(g, J, 0, logging)};
   const uint32_t esize = g.log_f32 ? 4u : 8u;
   char* const lp = logging ? (char*)g.log + J->log_offset * esize : nullptr;
   const uint64_t cs = (uint64_t)NE * esize;
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
         }
         if (logging) {
           char* p = lp + (uint64_t)k * esize;
-          write_entry<double, 1, FM, false>(g, J, m, er, p, cs, J->Rp, J->Rap);
+          write_entry<double, 1, FM, false>(g, J, m, er, p, cs, J->Rp, J->Rap, mcoef);
           log_put(g.log_f32, p + tch * cs, t);
         }
         ++k;
@@ -149,6 +150,182 @@ __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
     o[0] = m[0].x;
     o[1] = m[0].y;
     o[2] = m[0].z;
+  }
+  if (g.status) {
+    cmtj_status s;
+    s.code = code;
+    s.steps_done = accepted_steps;
+    s.entries_done = k;
+    s.nan_layer = nan_layer;
+    g.status[jid] = s;
+  }
+}
+
+// Several layers (Junction::runMultiLayerSolver, core/junction.hpp:1682-1707
+// inside the adaptive loop :1995-2044).  Each attempt freezes the
+// neighbours at the start-of-attempt magnetisations; the layers step in
+// order, each with the step size the previous layer proposed (every
+// dormand_prince_step writes its proposal into the one shared step), and a
+// layer that accepts keeps its new m whatever the others do.  The clock
+// advances by the last layer's proposal.  A rejection anywhere makes the
+// reference set the step to the smallest Layer::hopt, which stays at its
+// initial -1 (:259): the attempt is forced through with a step of -1 s and
+// the clock runs backwards, after which that run does not finish.  The lane
+// stops there with CMTJ_ERR_DP_REJECTED, keeping the attempt's
+// magnetisations.  Layer parameters stay in registers (the per-layer loops
+// unroll over LM); driver values are evaluated for every layer at each
+// stage time (the layers' stage times differ).
+template <int LM, uint32_t FM>
+__global__ __launch_bounds__(64) void k_dp_ml(GroupArgs g) {
+  const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= g.n) return;
+  const uint32_t fl = g.flags;
+  const int nl = g.nl;
+  const uint32_t jid = g.order[lane];
+  const cmtj_junction* J = g.js + jid;
+  LP<double> P[LM];
+  DV<double> base[LM];
+  V3<double> m[LM];
+  double atol[LM], rtol[LM], fmaxv[LM], fminv[LM], safety[LM];
+#pragma unroll
+  for (int l = 0; l < LM; ++l) {
+    if (l < nl) {
+      const cmtj_layer& Ly = g.layers[J->layer_offset + l];
+      load_layer<double>(Ly, g.c, P[l], base[l], m[l]);
+      atol[l] = Ly.dp_abs_tol;
+      rtol[l] = Ly.dp_rel_tol;
+      fmaxv[l] = Ly.dp_max_factor;
+      fminv[l] = Ly.dp_min_factor;
+      safety[l] = Ly.dp_safety;
+    } else {
+      dv_zero(base[l]);
+      m[l] = mk<double>(0, 0, 0);
+      P[l] = P[0];
+      atol[l] = rtol[l] = fmaxv[l] = fminv[l] = safety[l] = 1.0;
+    }
+  }
+  DynRec dr[DREG];
+#pragma unroll
+  for (int k = 0; k < DREG; ++k) {
+    if (k < g.ndyn)
+      dr[k] = g.dyn[(size_t)k * g.n + lane];
+    else
+      dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
+  }
+  FieldWeights<double, LM> fw;
+  field_weights<double, LM>(fw, g, 0, dr);
+  ERec<double> er[LM];
+  const V3<double> zero = mk<double>(0, 0, 0);
+  const double T = J->total_time, wf = J->write_freq, h0 = J->dt;
+  const uint32_t cap = J->n_steps, NE = J->n_entries;
+  const bool logging = (g.log != nullptr) && g.log_mask != 0;
+  MRCoef mcoef[LM < kMrRegLayers ? LM : kMrRegLayers];
+#pragma unroll
+  for (int l = 0; l < (LM < kMrRegLayers ? LM : kMrRegLayers); ++l) mcoef[l] = mr_coef(g, J, l, logging && l < nl);
+  const uint32_t esize = g.log_f32 ? 4u : 8u;
+  char* const lp = logging ? (char*)g.log + J->log_offset * esize : nullptr;
+  const uint64_t cs = (uint64_t)NE * esize;
+  const uint64_t tch = (uint64_t)(log_channels(nl, g.mr_mode, g.log_mask) - 1);  // the time channel is last
+
+  double t = 0.0, h = h0, next_write = 0.0;
+  uint32_t it = 0, k = 0, accepted_steps = 0;
+  int code = CMTJ_OK, nan_layer = -1;
+  while (t < T) {
+    if (it >= cap) {
+      code = CMTJ_ERR_LIMIT;
+      break;
+    }
+    ++it;
+    if (t + h > T) h = T - t;
+    if (isnan(h) || isinf(h) || h <= 0) h = h0;
+    V3<double> cp[LM];  // start-of-attempt magnetisations (magCopies)
+#pragma unroll
+    for (int l = 0; l < LM; ++l) cp[l] = m[l];
+    bool acc = true;
+#pragma unroll
+    for (int l = 0; l < LM; ++l) {
+      if (l < nl && code == CMTJ_OK) {
+        const V3<double> bot = (l > 0) ? cp[l > 0 ? l - 1 : 0] : zero;
+        const V3<double> top = (l + 1 < nl) ? cp[l + 1 < LM ? l + 1 : l] : zero;
+        // dm/dt of layer l at (ts, mm) (calculateLLGWithFieldTorque, :851-864)
+        auto f = [&](double ts, V3<double> mm) -> V3<double> {
+          DV<double> dv[LM];
+          eval_drivers<double, LM, FM, false>(dv, base, dr, fw, g, lane, fl, ts, 0, 0, 1);
+          return rhs<double, FM, false>(P[l], g.lbits[l], fl, mm, dv[l], bot, top, er[l]);
+        };
+        const V3<double> mt = m[l];
+        const V3<double> k1 = f(t, mt) * h;
+        const V3<double> k2 = f(t + (1.0 / 5.0) * h, mt + k1 * (1.0 / 5.0)) * h;
+        const V3<double> k3 = f(t + (3.0 / 10.0) * h, mt + k1 * (3.0 / 40.0) + k2 * (9.0 / 40.0)) * h;
+        const V3<double> k4 =
+            f(t + (4.0 / 5.0) * h, mt + k1 * (44.0 / 45.0) + k2 * (-56.0 / 15.0) + k3 * (32.0 / 9.0)) * h;
+        const V3<double> k5 = f(t + (8.0 / 9.0) * h, mt + k1 * (19372.0 / 6561.0) + k2 * (-25360.0 / 2187.0) +
+                                                          k3 * (64448.0 / 6561.0) + k4 * (-212.0 / 729.0)) *
+                              h;
+        const V3<double> k6 = f(t + 1.0 * h, mt + k1 * (9017.0 / 3168.0) + k2 * (-355.0 / 33.0) +
+                                                 k3 * (46732.0 / 5247.0) + k4 * (49.0 / 176.0) +
+                                                 k5 * (-5103.0 / 18656.0)) *
+                              h;
+        const V3<double> k7 = f(t + 1.0 * h, mt + k1 * (35.0 / 384.0) + k3 * (500.0 / 1113.0) +
+                                                 k4 * (125.0 / 192.0) + k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0)) *
+                              h;
+        V3<double> m5 = mt + k1 * (35.0 / 384.0) + k3 * (500.0 / 1113.0) + k4 * (125.0 / 192.0) +
+                        k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0);
+        const V3<double> m4 = mt + k1 * (5179.0 / 57600.0) + k3 * (7571.0 / 16695.0) + k4 * (393.0 / 640.0) +
+                              k5 * (-92097.0 / 339200.0) + k6 * (187.0 / 2100.0) + k7 * (1.0 / 40.0);
+        const double err = vlen(m5 - m4);
+        const double tol = atol[l] + rtol[l] * vlen(mt);
+        double ratio = err / tol;
+        if (tol < 1e-15 || isnan(ratio) || isinf(ratio)) ratio = 10.0;
+        double factor = safety[l] * pow(1.0 / ratio, 0.2);
+        if (isnan(factor) || isinf(factor)) factor = fminv[l];
+        factor = fmin(factor, fmaxv[l]);
+        factor = fmax(factor, fminv[l]);
+        double nh = h * factor;
+        if (isnan(nh) || isinf(nh) || nh <= 0) nh = h * fminv[l];
+        h = nh;  // the next layer steps with this layer's proposal
+        if (ratio <= 1.0) {
+          m5 = normalize_exact(m5);
+          m[l] = m5;
+          if (isnan(m5.x)) {  // the reference throws "NAN magnetisation"
+            code = CMTJ_ERR_NAN;
+            nan_layer = l;
+          }
+        } else {
+          acc = false;
+        }
+      }
+    }
+    if (code != CMTJ_OK) break;
+    if (!acc) {
+      code = CMTJ_ERR_DP_REJECTED;
+      break;
+    }
+    t += h;
+    ++accepted_steps;
+    if (t >= next_write) {
+      if (k >= NE) {
+        code = CMTJ_ERR_LIMIT;
+        break;
+      }
+      if (logging) {
+        char* p = lp + (uint64_t)k * esize;
+        write_entry<double, LM, FM, false>(g, J, m, er, p, cs, J->Rp, J->Rap, mcoef);
+        log_put(g.log_f32, p + tch * cs, t);
+      }
+      ++k;
+      next_write = (double)((uint32_t)(t / wf) + 1u) * wf;
+    }
+  }
+  if (g.mag_out) {
+    double* o = g.mag_out + (size_t)jid * CMTJ_MAX_LAYERS * 3;
+#pragma unroll
+    for (int l = 0; l < LM; ++l)
+      if (l < nl) {
+        o[3 * l + 0] = m[l].x;
+        o[3 * l + 1] = m[l].y;
+        o[3 * l + 2] = m[l].z;
+      }
   }
   if (g.status) {
     cmtj_status s;

@@ -62,7 +62,7 @@ __device__ __forceinline__ double group_sum(double v) {
 template <class R, int L, uint32_t FM, bool LOGE>
 __device__ __forceinline__ void write_entry_ll(const GroupArgs& g, const cmtj_junction* J, int ly, bool act,
                                                V3<double> m, V3<double> top, const ERec<R>& er, char* p,
-                                               uint64_t cs, double Rp, double Rap) {
+                                               uint64_t cs, double Rp, double Rap, const MRCoef& mc) {
   const int nl = g.nl, lmask = g.log_mask, mr = g.mr_mode;
   const bool f32 = g.log_f32;
   const int nm = ((lmask & CMTJ_LOG_MX) ? 1 : 0) + ((lmask & CMTJ_LOG_MYZ) ? 2 : 0);
@@ -86,7 +86,7 @@ __device__ __forceinline__ void write_entry_ll(const GroupArgs& g, const cmtj_ju
       if (ly == 0) {
         double cosT;
         if (nl == 1) {
-          const double* rf = g.layers[J->layer_offset].ref;
+          const double* rf = mc.ref;
           cosT = m.x * rf[0] + m.y * rf[1] + m.z * rf[2];
         } else {
           cosT = m.x * top.x + m.y * top.y + m.z * top.z;
@@ -98,9 +98,8 @@ __device__ __forceinline__ void write_entry_ll(const GroupArgs& g, const cmtj_ju
       // stripMagnetoResistance (:1835-1857), parallel sum of the layers
       double rx = 0.0, ry = 0.0;
       if (act) {
-        const int l = ly;
-        const double Rx = J->Rx0[l] + J->AMR_X[l] * (m.x * m.x) + J->SMR_X[l] * (m.y * m.y);
-        const double Ry = J->Ry0[l] + 0.5 * J->AHE[l] * m.z + (J->AMR_Y[l] + J->SMR_Y[l]) * m.x * m.y;
+        const double Rx = mc.rx0 + mc.amrx * (m.x * m.x) + mc.smrx * (m.y * m.y);
+        const double Ry = mc.ry0 + mc.ahe2 * m.z + mc.xy * m.x * m.y;
         rx = 1. / Rx;
         ry = 1. / Ry;
       }
@@ -153,6 +152,7 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
   char* const lp = logging ? (char*)g.log + lbase * esize : nullptr;
   const uint64_t cs = (uint64_t)NE * esize;
   const double Rp = J->Rp, Rap = J->Rap;
+  const MRCoef mcoef = mr_coef(g, J, lyr, logging && act);
   const int sps = (SOLVER == CMTJ_RK4) ? 3 : (SOLVER == CMTJ_HEUN ? 2 : 1);
   const bool hasB = act && ly > 0, hasT = act && ly + 1 < nl;
   const uint32_t gshift = threadIdx.x & ~(uint32_t)(L - 1);  // first lane of the instance in the wave
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
     top = from_next(m, hasT);
     if ((uint64_t)i == next_log) {
       if (logging)
-        write_entry_ll<R, L, FM, LOGE>(g, J, ly, act, m, top, er, lp + (uint64_t)k * esize, cs, Rp, Rap);
+        write_entry_ll<R, L, FM, LOGE>(g, J, ly, act, m, top, er, lp + (uint64_t)k * esize, cs, Rp, Rap, mcoef);
       ++k;
       next_log += W;
     }

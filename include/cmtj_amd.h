@@ -181,7 +181,7 @@ typedef struct cmtj_junction {
   double SMR_X[CMTJ_MAX_LAYERS];
   double SMR_Y[CMTJ_MAX_LAYERS];
   double AHE[CMTJ_MAX_LAYERS];
-  /* Dormand-Prince runs (solver CMTJ_DORMAND_PRINCE, one layer): totalTime
+  /* Dormand-Prince runs (solver CMTJ_DORMAND_PRINCE, 1-4 layers): totalTime
    * and writeFrequency of runSimulation; dt is the initial step, n_steps
    * the cap on step attempts (a run that needs more ends with
    * CMTJ_ERR_LIMIT) and n_entries the log capacity, floor(T / wf) + 4. */
@@ -205,7 +205,14 @@ enum cmtj_status_code {
   CMTJ_ERR_INVALID = 1,
   CMTJ_ERR_HIP = 2,
   CMTJ_ERR_NAN = 3, /* reference: runtime_error("NAN magnetisation"), core/junction.hpp:1037-1039 */
-  CMTJ_ERR_LIMIT = 4 /* Dormand-Prince: step-attempt cap or log capacity reached */
+  CMTJ_ERR_LIMIT = 4, /* Dormand-Prince: step-attempt cap or log capacity reached */
+  /* Dormand-Prince with several layers: a layer rejected its step.  The
+   * reference then sets the shared step to the smallest Layer::hopt, which
+   * is never assigned and stays -1 (core/junction.hpp:259, 1699-1704), forces
+   * the attempt through and moves its clock back by 1 s, after which its run
+   * does not finish; the run stops here instead, with the log and the
+   * magnetisations of that attempt (layers that accepted keep their step). */
+  CMTJ_ERR_DP_REJECTED = 5
 };
 typedef struct cmtj_status {
   int32_t code;

@@ -497,29 +497,29 @@ static void store_mag(const cmtj_junction* j, const olayer* Ls, double* mag_out)
 }
 
 /* Layer::dormand_prince_step (core/junction.hpp:866-1002) with the standard
- * controller, neighbours zero (one layer).  Returns 1 if accepted; *h is
- * replaced by the proposed next step either way; *nan set on a NaN. */
-static int dp_step(olayer* L, double time, double* h_io, int* nan) {
+ * controller and the neighbours bottom/top (zero vectors at the stack ends,
+ * :1688-1691).  Returns 1 if accepted; *h is replaced by the proposed next
+ * step either way; *nan set on a NaN. */
+static int dp_step(olayer* L, double time, double* h_io, vec3 bottom, vec3 top, int* nan) {
   const double h = *h_io;
-  const vec3 zero = v3(0, 0, 0);
   const vec3 m_t = L->mag;
-  const vec3 k1 = vmul(llg(L, time, 0, 0, m_t, zero, zero), h);
-  const vec3 k2 = vmul(llg(L, time + (1.0 / 5.0) * h, 0, 0, vadd(m_t, vmul(k1, 1.0 / 5.0)), zero, zero), h);
+  const vec3 k1 = vmul(llg(L, time, 0, 0, m_t, bottom, top), h);
+  const vec3 k2 = vmul(llg(L, time + (1.0 / 5.0) * h, 0, 0, vadd(m_t, vmul(k1, 1.0 / 5.0)), bottom, top), h);
   const vec3 k3 = vmul(llg(L, time + (3.0 / 10.0) * h, 0, 0,
-                           vadd(vadd(m_t, vmul(k1, 3.0 / 40.0)), vmul(k2, 9.0 / 40.0)), zero, zero), h);
+                           vadd(vadd(m_t, vmul(k1, 3.0 / 40.0)), vmul(k2, 9.0 / 40.0)), bottom, top), h);
   const vec3 k4 = vmul(llg(L, time + (4.0 / 5.0) * h, 0, 0,
                            vadd(vadd(vadd(m_t, vmul(k1, 44.0 / 45.0)), vmul(k2, -56.0 / 15.0)),
-                                vmul(k3, 32.0 / 9.0)), zero, zero), h);
+                                vmul(k3, 32.0 / 9.0)), bottom, top), h);
   const vec3 k5 = vmul(llg(L, time + (8.0 / 9.0) * h, 0, 0,
                            vadd(vadd(vadd(vadd(m_t, vmul(k1, 19372.0 / 6561.0)), vmul(k2, -25360.0 / 2187.0)),
-                                     vmul(k3, 64448.0 / 6561.0)), vmul(k4, -212.0 / 729.0)), zero, zero), h);
+                                     vmul(k3, 64448.0 / 6561.0)), vmul(k4, -212.0 / 729.0)), bottom, top), h);
   const vec3 k6 = vmul(llg(L, time + 1.0 * h, 0, 0,
                            vadd(vadd(vadd(vadd(vadd(m_t, vmul(k1, 9017.0 / 3168.0)), vmul(k2, -355.0 / 33.0)),
                                           vmul(k3, 46732.0 / 5247.0)), vmul(k4, 49.0 / 176.0)),
-                                vmul(k5, -5103.0 / 18656.0)), zero, zero), h);
+                                vmul(k5, -5103.0 / 18656.0)), bottom, top), h);
   const vec3 m5s = vadd(vadd(vadd(vadd(vadd(m_t, vmul(k1, 35.0 / 384.0)), vmul(k3, 500.0 / 1113.0)),
                                   vmul(k4, 125.0 / 192.0)), vmul(k5, -2187.0 / 6784.0)), vmul(k6, 11.0 / 84.0));
-  const vec3 k7 = vmul(llg(L, time + 1.0 * h, 0, 0, m5s, zero, zero), h);
+  const vec3 k7 = vmul(llg(L, time + 1.0 * h, 0, 0, m5s, bottom, top), h);
   vec3 m_t5 = m5s;
   const vec3 m_t4 = vadd(vadd(vadd(vadd(vadd(vadd(m_t, vmul(k1, 5179.0 / 57600.0)), vmul(k3, 7571.0 / 16695.0)),
                                         vmul(k4, 393.0 / 640.0)), vmul(k5, -92097.0 / 339200.0)),
@@ -549,11 +549,21 @@ static int dp_step(olayer* L, double time, double* h_io, int* nan) {
 /* The adaptive branch of Junction::runSimulation (:1995-2044): the clock
  * advances by the step proposed for the next step; a log entry, labelled
  * with the new time, after each accepted step that reaches the next
- * multiple of the write frequency.  At most n_steps attempts. */
+ * multiple of the write frequency.  At most n_steps attempts.
+ * Several layers (runMultiLayerSolver, :1682-1707): neighbours are the
+ * magnetisations at the start of the attempt, the layers step in order and
+ * each takes the step size the previous one proposed (dormand_prince_step
+ * writes its proposal into the shared step), and a layer that accepts keeps
+ * its new m even when another rejects.  After a rejection the reference sets
+ * the step to the smallest Layer::hopt, which is never assigned and stays -1
+ * (:259), so the attempt is forced through with a step of -1 s and its clock
+ * runs backwards; the run stops there with CMTJ_ERR_DP_REJECTED. */
 static void dp_run(const cmtj_junction* j, olayer* Ls, double* log, cmtj_status* st) {
   const double T = j->total_time, wf = j->write_freq, h0 = j->dt;
+  const int nl = j->n_layers;
   const size_t E = j->n_entries;
-  const size_t tch = (size_t)oracle_n_channels(1, j->mr_mode, j->log_mask) - 1;
+  const size_t tch = (size_t)oracle_n_channels(nl, j->mr_mode, j->log_mask) - 1;
+  const vec3 zero = v3(0, 0, 0);
   double t = 0.0, h = h0, next_write = 0.0;
   uint32_t it = 0, k = 0, accepted = 0;
   while (t < T) {
@@ -568,11 +578,22 @@ static void dp_run(const cmtj_junction* j, olayer* Ls, double* log, cmtj_status*
       h = h0;
       ++rejections;
     }
-    int nan = 0;
-    int acc = dp_step(&Ls[0], t, &h, &nan);
-    if (nan) {
-      st->code = CMTJ_ERR_NAN;
-      st->nan_layer = 0;
+    vec3 copies[CMTJ_MAX_LAYERS + 2];
+    copies[0] = zero;
+    copies[nl + 1] = zero;
+    for (int l = 0; l < nl; ++l) copies[l + 1] = Ls[l].mag;
+    int acc = 1, nan = 0;
+    for (int l = 0; l < nl; ++l) {
+      if (!dp_step(&Ls[l], t, &h, copies[l], copies[l + 2], &nan)) acc = 0;
+      if (nan) {
+        st->code = CMTJ_ERR_NAN;
+        st->nan_layer = l;
+        break;
+      }
+    }
+    if (nan) break;
+    if (!acc && nl > 1) {
+      st->code = CMTJ_ERR_DP_REJECTED;
       break;
     }
     if (!acc && (h < 1e-15 || rejections >= 10)) acc = 1;
@@ -607,11 +628,7 @@ int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_cons
   st->entries_done = 0;
   st->nan_layer = -1;
   if (j->solver == CMTJ_DORMAND_PRINCE) {
-    if (j->n_layers != 1) {
-      st->code = CMTJ_ERR_INVALID;
-    } else {
-      dp_run(j, Ls, log, st);
-    }
+    dp_run(j, Ls, log, st);
     store_mag(j, Ls, mag_out);
     return 0;
   }

@@ -316,9 +316,9 @@ def test_run_errors():
     j0.setLayerTemperatureDriver("all", constantDriver(300))
     with pytest.raises(RuntimeError, match="Cell surface cannot be 0"):
         j0.runSimulation(1e-11, 1e-13, 1e-12)
-    with pytest.raises(NotImplementedError, match="single-layer"):
-        OracleJunction([_layer(), _layer("b")]).runSimulation(1e-11, 1e-13, 1e-12,
-                                                              solverMode=SolverMode.DormandPrince)
+    with pytest.raises(NotImplementedError, match="1 to 4 layers"):
+        OracleJunction([_layer(f"l{i}") for i in range(5)]).runSimulation(1e-11, 1e-13, 1e-12,
+                                                                         solverMode=SolverMode.DormandPrince)
     jt = OracleJunction([_layer(thickness=0.0)])
     with pytest.raises(RuntimeError, match="divide vector by zero"):
         jt.runSimulation(1e-11, 1e-13, 1e-12)

@@ -112,12 +112,10 @@ def test_tighter_tolerance_stays_closer_to_the_exact_trajectory():
 
 
 def test_dp_restrictions():
-    two = Junction([Layer("a", mag=CVector(0, 0, 1), anis=CVector(1, 0, 0), Ms=1.0, thickness=1e-9,
-                          cellSurface=1e-16, demagTensor=[CVector(0, 0, 0)] * 3),
-                    Layer("b", mag=CVector(0, 0, 1), anis=CVector(1, 0, 0), Ms=1.0, thickness=1e-9,
-                          cellSurface=1e-16, demagTensor=[CVector(0, 0, 0)] * 3)])
-    with pytest.raises(NotImplementedError, match="single-layer"):
-        two.runSimulation(1e-11, 1e-13, 1e-12, solverMode=SolverMode.DormandPrince)
+    five = Junction([Layer(f"l{i}", mag=CVector(0, 0, 1), anis=CVector(1, 0, 0), Ms=1.0, thickness=1e-9,
+                           cellSurface=1e-16, demagTensor=[CVector(0, 0, 0)] * 3) for i in range(5)])
+    with pytest.raises(NotImplementedError, match="1 to 4 layers"):
+        five.runSimulation(1e-11, 1e-13, 1e-12, solverMode=SolverMode.DormandPrince)
     j = _film(OracleJunction, (0, 0, 1), (0, 0, 1e5))
     p = AdaptiveIntegrationParams()
     p.use_pid_control = True
@@ -127,3 +125,97 @@ def test_dp_restrictions():
     with pytest.raises(NotImplementedError, match="energy"):
         _film(OracleJunction, (0, 0, 1), (0, 0, 1e5)).runSimulation(1e-11, 1e-13, 1e-12, calculateEnergies=True,
                                                                       solverMode=SolverMode.DormandPrince)
+
+
+# ---- several layers (runMultiLayerSolver, core/junction.hpp:1682-1707) -------
+#
+# A layer held exactly along its field has zero torque, so its error ratio is
+# 0, the standard controller's factor is infinite and falls back to
+# min_factor (:967-969): that layer multiplies the shared step by min_factor
+# on every attempt.  Put it first with min_factor c = 2^-1 or 2^2: the
+# precessing second layer then steps with c H_n and proposes H_{n+1}; its own
+# steps s_n = c H_n follow s_{n+1} = c s_n factor(s_n), which is a one-layer
+# run with safety, min_factor and max_factor scaled by c and the first step
+# c h0, while the two-layer clock adds H_{n+1} = s_{n+1} / c.  Scaling by a
+# power of two is exact, so up to the first rejection the two-layer run is
+# that one-layer run bit for bit, with its clock, total time and write
+# interval divided by c.
+
+
+def _pair(J, c, order=("fixed", "free"), tol=1e-8, fixed_min=None):
+    fixed = Layer("fixed", mag=CVector(0, 0, 1), anis=CVector(1, 0, 0), Ms=1.2, thickness=2e-9, cellSurface=1e-16,
+                  demagTensor=[CVector(0, 0, 0)] * 3, damping=0.02)
+    free = Layer("free", mag=CVector(0.6, 0.0, 0.8), anis=CVector(1, 0, 0), Ms=1.0, thickness=1e-9,
+                 cellSurface=1e-16, demagTensor=[CVector(0, 0, 0)] * 3, damping=0.02)
+    pf = AdaptiveIntegrationParams()
+    pf.min_factor = c if fixed_min is None else fixed_min
+    pf.max_factor = max(5.0, c)
+    fixed.setAdaptiveParams(pf)
+    pv = AdaptiveIntegrationParams()
+    pv.abs_tol = pv.rel_tol = tol
+    free.setAdaptiveParams(pv)
+    by = {"fixed": fixed, "free": free}
+    j = J([by[order[0]], by[order[1]]])
+    j.setLayerExternalFieldDriver("fixed", AxialDriver(0, 0, 2e5))
+    j.setLayerExternalFieldDriver("free", AxialDriver(3e4, 0, 1e5))
+    return j
+
+
+def _single_equivalent(J, c, tol=1e-8):
+    j = _film(J, (0.6, 0.0, 0.8), (3e4, 0, 1e5), tol=(tol, tol))
+    p = j.getLayer("free").adaptiveParams
+    p.safety_factor *= c
+    p.min_factor *= c
+    p.max_factor *= c
+    j.getLayer("free").setAdaptiveParams(p)
+    return j
+
+
+def _two_vs_one(J2, c, T=0.5e-9, h0=2e-13, wf=2e-13, **kw):
+    """(two-layer log, one-layer log, status message or None) of the pair."""
+    two = _pair(J2, c, **kw)
+    err = None
+    try:
+        two.runSimulation(T, h0, wf, solverMode=SolverMode.DormandPrince)
+    except RuntimeError as e:
+        err = str(e)
+    one = _single_equivalent(OracleJunction, c)
+    try:  # with c > 1 the one-layer controller grows the step after a rejection and never recovers
+        one.runSimulation(T * c, h0 * c, wf * c, solverMode=SolverMode.DormandPrince)
+    except RuntimeError:
+        pass
+    return two.getLog(), one.getLog(), err
+
+
+def _equivalent(l2, l1, c) -> bool:
+    n = len(l2["time"])
+    if n == 0 or len(l1["time"]) < n:
+        return False
+    same_m = all(np.array_equal(np.asarray(l2[f"free_m{x}"]), np.asarray(l1[f"free_m{x}"])[:n]) for x in "xyz")
+    same_t = np.array_equal(np.asarray(l2["time"]) * c, np.asarray(l1["time"])[:n])
+    fixed = np.all(np.asarray(l2["fixed_mz"]) == 1.0)
+    return bool(same_m and same_t and fixed)
+
+
+@pytest.mark.parametrize("c", [0.5, 0.25])
+def test_layers_share_one_step_size_in_order(c):
+    l2, l1, err = _two_vs_one(OracleJunction, c)
+    assert err is None, err
+    assert len(l2["time"]) > 40 and len(l2["time"]) == len(l1["time"])
+    assert _equivalent(l2, l1, c)
+    # planted mistakes in the descriptor: the first layer passes the step on
+    # unchanged, or the precessing layer steps first
+    l2b, l1b, _ = _two_vs_one(OracleJunction, c, fixed_min=1.0)
+    assert not _equivalent(l2b, l1b, c)
+    l2c, l1c, _ = _two_vs_one(OracleJunction, c, order=("free", "fixed"))
+    assert not _equivalent(l2c, l1c, c)
+
+
+def test_a_rejected_layer_step_stops_the_run():
+    # the first layer doubles the step every attempt until the second rejects
+    l2, l1, err = _two_vs_one(OracleJunction, 2.0, T=1e-11, tol=1e-10, h0=1e-14, wf=1e-14)
+    assert err is not None and "hopt = -1" in err
+    n = len(l2["time"])
+    print(f"{n} entries before the rejection")
+    assert 2 < n <= len(l1["time"])
+    assert _equivalent(l2, l1, 2.0)  # identical up to the rejection

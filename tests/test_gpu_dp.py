@@ -1,5 +1,5 @@
-"""Dormand-Prince on the GPU (k_dp: one adaptive lane per single-layer
-instance) against the oracle: the same accepted steps, entry times and
+"""Dormand-Prince on the GPU (k_dp / k_dp_ml: one adaptive lane per
+instance of one or several layers) against the oracle: the same accepted steps, entry times and
 magnetisations for an SOT-driven switch with second-order anisotropy, a sine
 drive, an exact equilibrium (the step-size collapse of the reference's
 controller) and a batch of mixed instances; at a tight tolerance, both on
@@ -133,3 +133,51 @@ def run_and_split(b, js):
     for q in range(len(js)):
         out.append(r.instance_log(q))
     return out
+
+
+# ---- several layers (k_dp_ml) ------------------------------------------------
+
+
+def _ml_case(name, J):
+    import parity_cases
+    from test_dp_cpu import _pair
+
+    if name == "shared_step":  # a torque-free first layer halves the shared step every attempt
+        return _pair(J, 0.5), (0.5e-9, 2e-13, 2e-13)
+    base = "bilayer_iec" if name.startswith("bilayer") else "trilayer"
+    j, _ = getattr(parity_cases, base)(J)
+    if name != "bilayer_rejects":  # controllers that keep the layers' steps within reach of each other
+        for l in j.layers:
+            p = AdaptiveIntegrationParams()
+            p.abs_tol = p.rel_tol = 1e-6 if base == "bilayer_iec" else 1e-8
+            p.safety_factor, p.max_factor = 0.3, 1.2
+            l.setAdaptiveParams(p)
+    return j, (1e-9, 1e-14, 1e-12)
+
+
+@pytest.mark.parametrize("name", ["shared_step", "bilayer_iec", "trilayer", "bilayer_rejects"])
+def test_multilayer_dp_matches_oracle(name):
+    """Two and three coupled layers: the same accepted attempts, entry times,
+    magnetisations and resistance as the oracle; a run whose layer rejects a
+    step ends with the same error and the same log up to it on both."""
+    out = []
+    for J in (Junction, OracleJunction):
+        j, args = _ml_case(name, J)
+        err = None
+        try:
+            j.runSimulation(*args, solverMode=SolverMode.DormandPrince)
+        except RuntimeError as e:
+            err = str(e)
+        out.append((j, err))
+    (g, eg), (o, eo) = out
+    assert eg == eo
+    worst = _compare(g, o)
+    print(f"[dp] {name}: {len(g.getLog()['time'])} entries ({'rejected' if eg else 'finished'}), "
+          f"max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+    assert (eg is not None) == (name == "bilayer_rejects")
+    if name != "bilayer_rejects":
+        assert len(g.getLog()["time"]) > (40 if name == "shared_step" else 500)
+    for lid in g.getLayerIds():
+        np.testing.assert_allclose(np.array(g.getLayerMagnetisation(lid).tolist()),
+                                   np.array(o.getLayerMagnetisation(lid).tolist()), atol=1e-8)
