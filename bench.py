@@ -531,6 +531,8 @@ def _cpu_baseline(b, args):
     CPUs (at most the box's 16-CPU share), then one core on a smaller sample."""
     import tempfile
 
+    import numpy as np
+
     import oracle
 
     ncores, nproc, avail = _host_cpus()

@@ -227,6 +227,8 @@ EXPORTS = (
     "cmtj_plan_create_in",
     "cmtj_plan_execute",
     "cmtj_plan_execute_slice",
+    "cmtj_plan_graph_create",
+    "cmtj_plan_graph_launch",
     "cmtj_plan_query",
     "cmtj_plan_destroy",
     "cmtj_vsd_mix",
@@ -274,6 +276,8 @@ def _load():
     lib.cmtj_plan_execute.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P]
     lib.cmtj_plan_execute_slice.argtypes = [_P, _P, ctypes.c_size_t, _P, _P, _P, ctypes.c_uint32,
                                             ctypes.POINTER(_P), ctypes.c_int32]
+    lib.cmtj_plan_graph_create.argtypes = [_P, _P, ctypes.c_size_t, _P, _P]
+    lib.cmtj_plan_graph_launch.argtypes = [_P, _P]
     lib.cmtj_plan_query.argtypes = [_P, ctypes.POINTER(PlanInfo)]
     lib.cmtj_plan_destroy.argtypes = [_P]
     lib.cmtj_vsd_mix.argtypes = [_P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _P, ctypes.c_double, _P, _P,
@@ -526,6 +530,12 @@ class Plan:
                                           arr, len(streams)),
             "cmtj_plan_execute_slice",
         )
+
+    def graph_create(self, d_log: int | None, log_capacity: int, d_mag: int | None, d_status: int | None) -> None:
+        check(lib().cmtj_plan_graph_create(self._h, d_log, log_capacity, d_mag, d_status), "cmtj_plan_graph_create")
+
+    def graph_launch(self, stream: int | None) -> None:
+        check(lib().cmtj_plan_graph_launch(self._h, stream), "cmtj_plan_graph_launch")
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -106,7 +106,7 @@ class BatchResult:
         return self.log[: n * nch * int(ne[0])].view(n, nch, int(ne[0]))[:, c, :]
 
 
-_LANES = {"auto": 0, "one": 1, "per_layer": 2, "per_component": 3}
+_LANES = {"auto": 0, "one": 1, "per_layer": 2, "per_component": 3, "queue": 4}
 
 
 class JunctionBatch:
@@ -324,9 +324,11 @@ class JunctionBatch:
         one GPU lane), "per_layer" (one lane per layer), "per_component" (one
         lane per vector component of each layer: binary32 RK4 groups of the
         lean feature set; binary64 groups of every feature but energy
-        channels) or "auto" (per component where such a kernel covers
+        channels), "auto" (per component where such a kernel covers
         the group and the batch leaves SIMDs idle, else per layer for
-        multilayer instances)."""
+        multilayer instances) or "queue" (single-layer Dormand-Prince groups
+        on a persistent grid whose lanes take instances from a device work
+        queue; other groups as "auto")."""
         if lanes not in _LANES:
             raise ValueError(f"lanes must be one of {sorted(_LANES)}")
         self.lanes = lanes
@@ -366,6 +368,29 @@ class JunctionBatch:
         sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
         p.execute(log.data_ptr() if self.log_elements else None, log.numel(), mag.data_ptr(), st.data_ptr(), sp)
         return BatchResult(self, log, mag, st)
+
+    def capture(self, device="cuda") -> "JunctionBatch":
+        """Captures one execute of the batch, bound to its device buffers, as
+        a HIP graph (cmtj_plan_graph_create): one kernel node per structure
+        group, no edges between them.  replay() then runs it."""
+        p = self.plan()
+        log, mag, st = self._buffers(device)
+        p.graph_create(log.data_ptr() if self.log_elements else None, log.numel(), mag.data_ptr(), st.data_ptr())
+        self._graph_bufs = self._bufs
+        return self
+
+    def replay(self, stream=None) -> BatchResult:
+        """Launches the captured graph (asynchronous on `stream`, default the
+        current torch stream); the same results as execute()."""
+        import torch
+
+        if getattr(self, "_graph_bufs", None) is None or self._bufs is not self._graph_bufs:
+            raise RuntimeError("capture() the batch first")
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        sp = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        self.plan().graph_launch(sp)
+        return BatchResult(self, *self._graph_bufs)
 
     def run_sliced(self, slice_steps: int, n_streams: int = 1, device="cuda") -> BatchResult:
         """Runs every instance to the end in slices of at most slice_steps

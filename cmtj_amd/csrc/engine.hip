@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -282,7 +283,19 @@ struct Group {
   GroupArgs a;
   uint32_t blocks;
   uint32_t threads = 64;  // per workgroup
+  uint32_t* queue = nullptr;  // work counter of a queue launch (k_dp_q), else null
 };
+
+// Lanes of a work-queue launch: four waves per SIMD of the chip, or fewer
+// when CMTJ_QUEUE_LANES says so (a smaller grid makes lanes refill more).
+static uint32_t queue_lanes() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("CMTJ_QUEUE_LANES");
+    const long x = e ? std::strtol(e, nullptr, 10) : 0;
+    return x > 0 ? (uint32_t)((x + 63) / 64 * 64) : (uint32_t)(64 * 1024 * 4);
+  }();
+  return v;
+}
 
 struct cmtj_plan {
   void* arena = nullptr;
@@ -290,7 +303,19 @@ struct cmtj_plan {
   std::vector<Group> groups;
   cmtj_plan_info info;
   uint64_t log_extent = 0;
+  // a captured execute (cmtj_plan_graph_create): one kernel node per group
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<GroupArgs> graph_args;
 };
+
+static void drop_graph(cmtj_plan* p) {
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+  p->graph_args.clear();
+}
 
 typedef void* (*ArenaFn)(size_t bytes, void* ctx);
 
@@ -380,6 +405,8 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
   off = align_up(off + rows * sizeof(cmtj_layer));
   const size_t o_tables = off;
   off = align_up(off + std::max<size_t>(n_table, 1) * sizeof(double));
+  const size_t o_queue = off;  // one work counter per group (queue launches)
+  off = align_up(off + G * sizeof(uint32_t));
   std::vector<size_t> o_order(G), o_dyn(G);
   for (size_t g = 0; g < G; ++g) {
     o_order[g] = off;
@@ -423,7 +450,8 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
       kinds |= (ty == CMTJ_STEP) ? KIND_STEP : (ty == CMTJ_SINE) ? KIND_SINE : KIND_OTHER;
     }
     const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], kinds,
-                                  (int)recs_of[orders[g][0]].size(), loge, lanes, orders[g].size(),
+                                  (int)recs_of[orders[g][0]].size(), loge,
+                                  lanes == CMTJ_LANES_QUEUE ? (int)CMTJ_LANES_AUTO : lanes, orders[g].size(),
                                   (uint32_t)j0.log_mask | ((uint32_t)j0.mr_mode << 8) |
                                       (log_dtype == CMTJ_LOG_F32 ? (1u << 16) : 0u));
     if (!k) return fail(CMTJ_ERR_INVALID, "no kernel for this junction configuration");
@@ -447,11 +475,16 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     for (size_t q = 0; q < r0.size(); ++q) Gr.a.dmeta[q] = r0[q].meta;
     Gr.a.c = *c;
     Gr.blocks = (uint32_t)(((uint64_t)Gr.a.n * (uint64_t)k->lpi + 63) / 64);
+    if (lanes == CMTJ_LANES_QUEUE && j0.solver == CMTJ_DORMAND_PRINCE && j0.n_layers == 1) {
+      Gr.queue = (uint32_t*)(arena + o_queue) + g;
+      Gr.blocks = (std::min<uint32_t>(Gr.a.n, queue_lanes()) + 63) / 64;
+    }
     max_lpi = std::max(max_lpi, k->lpi);
     P->groups.push_back(Gr);
-    if (names.find(k->name) == std::string::npos) {
+    const char* kname = Gr.queue ? "k_dp_q<double,1,F_ALL>" : k->name;
+    if (names.find(kname) == std::string::npos) {
       if (!names.empty()) names += ";";
-      names += k->name;
+      names += kname;
     }
   }
   // host staging vectors die with this function: finish the copies first
@@ -483,6 +516,15 @@ static int launch_plan(cmtj_plan* P, void* d_log, double* d_mag, cmtj_status* d_
     a.slice = slice;
     void* args[] = {&a};
     const hipStream_t st = n_streams > 0 ? (hipStream_t)streams[q % (size_t)n_streams] : s;
+    if (G.queue) {
+      uint32_t* ctr = G.queue;
+      uint32_t one = 1;
+      void* cargs[] = {&ctr, &one};
+      HIP_TRY(hipLaunchKernel(clear_u32_kernel(), dim3(1), dim3(64), cargs, 0, st));
+      void* qargs[] = {&a, &ctr};
+      HIP_TRY(hipLaunchKernel(dp_queue_kernel(), dim3(G.blocks), dim3(G.threads), qargs, 0, st));
+      continue;
+    }
     HIP_TRY(hipLaunchKernel(G.k->fn, dim3(G.blocks), dim3(G.threads), args, 0, st));
   }
   return 0;
@@ -891,10 +933,10 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n, const cmtj_layer*
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
   }
-  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_COMPONENT) {
+  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_QUEUE) {
     fail(CMTJ_ERR_INVALID,
-         "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per layer) or 3 (one lane "
-         "per component)");
+         "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per layer), 3 (one lane "
+         "per component) or 4 (a work queue)");
     return nullptr;
   }
   cmtj_plan* P = new cmtj_plan();
@@ -953,10 +995,10 @@ cmtj_plan* cmtj_plan_create_in(const cmtj_junction* js, size_t n, const cmtj_lay
     fail(CMTJ_ERR_INVALID, "null pointer");
     return nullptr;
   }
-  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_PER_COMPONENT) {
+  if (lanes_per_instance < CMTJ_LANES_AUTO || lanes_per_instance > CMTJ_LANES_QUEUE) {
     fail(CMTJ_ERR_INVALID,
-         "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per layer) or 3 (one lane "
-         "per component)");
+         "lanes_per_instance must be 0 (auto), 1 (one lane per instance), 2 (one lane per layer), 3 (one lane "
+         "per component) or 4 (a work queue)");
     return nullptr;
   }
   void* dev = d_arena;
@@ -1051,8 +1093,73 @@ int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out) {
   return 0;
 }
 
+int cmtj_plan_graph_create(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_mag_out,
+                           cmtj_status* d_status) {
+  if (!p) return fail(CMTJ_ERR_INVALID, "null plan");
+  if (d_log && log_capacity < p->log_extent)
+    return fail(CMTJ_ERR_INVALID, "log buffer smaller than the plan's log extent (" +
+                                      std::to_string(p->log_extent) + " elements)");
+  drop_graph(p);
+  // the kernel arguments stay with the plan: the graph's nodes were built from them
+  p->graph_args.resize(p->groups.size());
+  HIP_TRY(hipGraphCreate(&p->graph, 0));
+  for (size_t q = 0; q < p->groups.size(); ++q) {
+    const Group& G = p->groups[q];
+    GroupArgs& a = p->graph_args[q];
+    a = G.a;
+    a.log = d_log;
+    a.mag_out = d_mag_out;
+    a.status = d_status;
+    a.state = nullptr;
+    a.slice = 0;
+    void* args[] = {&a};
+    uint32_t* ctr = G.queue;
+    uint32_t one = 1;
+    void* cargs[] = {&ctr, &one};
+    void* qargs[] = {&a, &ctr};
+    hipKernelNodeParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    kp.func = const_cast<void*>(G.queue ? dp_queue_kernel() : G.k->fn);
+    kp.gridDim = dim3(G.blocks);
+    kp.blockDim = dim3(G.threads);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = G.queue ? qargs : args;
+    kp.extra = nullptr;
+    hipGraphNode_t clear = nullptr, node;
+    hipError_t e = hipSuccess;
+    if (G.queue) {  // the counter is cleared by a node the kernel node depends on
+      hipKernelNodeParams cp;
+      std::memset(&cp, 0, sizeof(cp));
+      cp.func = const_cast<void*>(clear_u32_kernel());
+      cp.gridDim = dim3(1);
+      cp.blockDim = dim3(64);
+      cp.kernelParams = cargs;
+      e = hipGraphAddKernelNode(&clear, p->graph, nullptr, 0, &cp);
+    }
+    if (e == hipSuccess) e = hipGraphAddKernelNode(&node, p->graph, G.queue ? &clear : nullptr, G.queue ? 1 : 0, &kp);
+    if (e != hipSuccess) {
+      drop_graph(p);
+      return fail(CMTJ_ERR_HIP, std::string("hipGraphAddKernelNode: ") + hipGetErrorString(e));
+    }
+  }
+  const hipError_t e = hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(p);
+    return fail(CMTJ_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int cmtj_plan_graph_launch(cmtj_plan* p, void* stream) {
+  if (!p) return fail(CMTJ_ERR_INVALID, "null plan");
+  if (!p->graph_exec) return fail(CMTJ_ERR_INVALID, "the plan has no graph (cmtj_plan_graph_create)");
+  HIP_TRY(hipGraphLaunch(p->graph_exec, (hipStream_t)stream));
+  return 0;
+}
+
 int cmtj_plan_destroy(cmtj_plan* p) {
   if (!p) return 0;
+  drop_graph(p);
   if (p->owns && p->arena) (void)hipFree(p->arena);
   delete p;
   return 0;

@@ -9,4 +9,6 @@ static const KEntry kD[] = {
     {CMTJ_FP64, 4, 1, CMTJ_DORMAND_PRINCE, F_ALL, false, (const void*)&k_dp_ml<4, F_ALL>, "k_dp_ml<double,4,F_ALL>"},
 };
 KTable kernels_dp() { return {kD, sizeof(kD) / sizeof(kD[0])}; }
+const void* dp_queue_kernel() { return (const void*)&k_dp_q<F_ALL>; }
+const void* clear_u32_kernel() { return (const void*)&k_clear_u32; }
 }  // namespace cmtj

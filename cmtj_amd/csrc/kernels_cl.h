@@ -29,6 +29,8 @@
 // results agree with them to rounding, not bitwise.
 #pragma once
 
+#include <type_traits>
+
 #include "kernels_ll.h"
 
 namespace cmtj {
@@ -325,7 +327,13 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   bot = hasB ? bot : 0.0;
   top = hasT ? top : 0.0;
 
-  for (; i < iend; ++i) {
+  // One step of the loop.  DRV: the step's stage times may meet a
+  // time-dependent record; without it the driver code is left out (kernels
+  // whose records are all step drivers run the steps outside the records'
+  // windows that way: those records are exactly 0 there).  Returns true when
+  // the run stops (a NaN).
+  auto step = [&](auto drv_tag) __attribute__((always_inline)) -> bool {
+    constexpr bool DRV = decltype(drv_tag)::value;
     const double t = (double)i * dt;
     double mn;
     if constexpr (F64) {
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // below takes the faster single-precision ones)
       auto drive = [&](double ts, int stage) -> CLDrive {
         CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL, K.bT, K.bK2, K.bJ2t, K.bJ2b, K.bFL2, K.bDL2, K.bDt, K.bDb};
-        if (!(CMTJ_HAS(F_DYNH) || CMTJ_HAS(F_DYN))) return d;
+        if (!DRV || !(CMTJ_HAS(F_DYNH) || CMTJ_HAS(F_DYN))) return d;
         if constexpr ((FM & F_KSTEP) != 0) {
           // step records are 0 outside [min t_start, max t_stop] (inclusive)
           if (!(ts >= win_lo && ts <= win_hi)) return d;
@@ -498,7 +506,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       double H = K.hb;
       bool inwin = true;
       if constexpr ((FM & F_KSTEP) != 0) inwin = t >= win_lo && t <= win_hi;
-      if (CMTJ_HAS(F_DYNH) && inwin) {
+      if (DRV && CMTJ_HAS(F_DYNH) && inwin) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
           if (ND >= 0 || r < g.ndyn) {
@@ -541,7 +549,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       };
       auto drive_delta = [&](double ts, int stage) -> float {
         float dd = 0.f;
-        if (!CMTJ_HAS(F_DYNH)) return dd;
+        if (!DRV || !CMTJ_HAS(F_DYNH)) return dd;
         if constexpr ((FM & F_KSTEP) != 0) {
           if (!(ts >= win_lo && t <= win_hi)) return dd;
         }
@@ -591,7 +599,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         nan_layer = __builtin_ctz(gb) >> 2;
         if (ly <= nan_layer) m = mn;
         code = CMTJ_ERR_NAN;
-        break;
+        return true;
       }
     }
     m = mn;
@@ -630,6 +638,25 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       ++k;
       next_log += W;
     }
+    return false;
+  };
+  bool stopped = false;
+  if constexpr ((FM & F_KSTEP) != 0 && !(FM & F_TABLE)) {
+    // steps [ia, ib) may meet a record's window [win_lo, win_hi]; every
+    // stage time of a step before ia is below win_lo, of a step from ib on
+    // above win_hi (one step of margin on each side)
+    uint32_t ia = iend, ib = iend;
+    if (win_lo <= win_hi) {
+      const double a = floor(win_lo / dt) - 2.0, b = floor(win_hi / dt) + 2.0;
+      ia = a <= (double)i ? i : (a >= (double)iend ? iend : (uint32_t)a);
+      ib = b <= (double)ia ? ia : (b >= (double)iend ? iend : (uint32_t)b);
+    }
+    // a stopping step is not counted (steps done = i)
+    while (!stopped && i < ia) stopped = step(std::false_type{}) || (++i, false);
+    while (!stopped && i < ib) stopped = step(std::true_type{}) || (++i, false);
+    while (!stopped && i < iend) stopped = step(std::false_type{}) || (++i, false);
+  } else {
+    while (!stopped && i < iend) stopped = step(std::true_type{}) || (++i, false);
   }
   if (g.mag_out && live) g.mag_out[((size_t)jid * CMTJ_MAX_LAYERS + ly) * 3 + c] = m;
   if (g.status && q == 0) {

@@ -337,4 +337,195 @@ __global__ __launch_bounds__(64) void k_dp_ml(GroupArgs g) {
   }
 }
 
+// Single-layer Dormand-Prince from a device work queue: a persistent grid
+// smaller than the batch, in which each lane runs an instance to its end and
+// then claims the next position of the group's order (`queue`, zeroed before
+// the launch) until the order is exhausted.  Adaptive runs take step counts
+// no one knows in advance, so lanes of a wave finish at different times; the
+// loop is flat (one attempt per pass, a lane whose instance ended claims and
+// loads the next one in the same pass), so a lane never waits for the other
+// lanes' instances to end.  Per instance the arithmetic is k_dp's, so the
+// results are k_dp's bit for bit.
+template <uint32_t FM>
+__global__ __launch_bounds__(64) void k_dp_q(GroupArgs g, uint32_t* queue) {
+  const uint32_t fl = g.flags;
+  const int bits = g.lbits[0];
+  const bool logging = (g.log != nullptr) && g.log_mask != 0;
+  const uint32_t esize = g.log_f32 ? 4u : 8u;
+  const uint64_t tch = (uint64_t)(log_channels(1, g.mr_mode, g.log_mask) - 1);  // the time channel is last
+  const V3<double> zero = mk<double>(0, 0, 0);
+  // the current instance
+  uint32_t pos = 0, jid = 0;
+  const cmtj_junction* J = nullptr;
+  LP<double> P;
+  DV<double> base[1];
+  V3<double> m[1];
+  DynRec dr[DREG];
+  FieldWeights<double, 1> fw;
+  ERec<double> er[1];
+  MRCoef mcoef[1];
+  double T = 0, wf = 1, h0 = 0, abs_tol = 0, rel_tol = 0, max_factor = 1, min_factor = 1, safety = 1;
+  uint32_t cap = 0, NE = 0;
+  char* lp = nullptr;
+  uint64_t cs = 0;
+  double t = 0.0, h = 0.0, next_write = 0.0;
+  uint32_t it = 0, k = 0, accepted_steps = 0;
+  int code = CMTJ_OK, nan_layer = -1;
+
+  auto load = [&](uint32_t q) {
+    pos = q;
+    jid = g.order[q];
+    J = g.js + jid;
+    const cmtj_layer& Ly = g.layers[J->layer_offset];
+    load_layer<double>(Ly, g.c, P, base[0], m[0]);
+#pragma unroll
+    for (int r = 0; r < DREG; ++r) {
+      if (r < g.ndyn)
+        dr[r] = g.dyn[(size_t)r * g.n + q];
+      else
+        dr[r] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
+    }
+    field_weights<double, 1>(fw, g, 0, dr);
+    T = J->total_time;
+    wf = J->write_freq;
+    h0 = J->dt;
+    abs_tol = Ly.dp_abs_tol;
+    rel_tol = Ly.dp_rel_tol;
+    max_factor = Ly.dp_max_factor;
+    min_factor = Ly.dp_min_factor;
+    safety = Ly.dp_safety;
+    cap = J->n_steps;
+    NE = J->n_entries;
+    mcoef[0] = mr_coef(g, J, 0, logging);
+    lp = logging ? (char*)g.log + J->log_offset * esize : nullptr;
+    cs = (uint64_t)NE * esize;
+    t = 0.0;
+    h = h0;
+    next_write = 0.0;
+    it = k = accepted_steps = 0;
+    code = CMTJ_OK;
+    nan_layer = -1;
+  };
+  auto store = [&]() {
+    if (g.mag_out) {
+      double* o = g.mag_out + (size_t)jid * CMTJ_MAX_LAYERS * 3;
+      o[0] = m[0].x;
+      o[1] = m[0].y;
+      o[2] = m[0].z;
+    }
+    if (g.status) {
+      cmtj_status s;
+      s.code = code;
+      s.steps_done = accepted_steps;
+      s.entries_done = k;
+      s.nan_layer = nan_layer;
+      g.status[jid] = s;
+    }
+  };
+  // dm/dt at (ts, mm) (calculateLLGWithFieldTorque, :851-864)
+  auto f = [&](double ts, V3<double> mm) -> V3<double> {
+    DV<double> dv[1];
+    eval_drivers<double, 1, FM, false>(dv, base, dr, fw, g, pos, fl, ts, 0, 0, 1);
+    return rhs<double, FM, false>(P, bits, fl, mm, dv[0], zero, zero, er[0]);
+  };
+  // one pass of the adaptive loop (:1995-2044); true when the run ended
+  auto attempt = [&]() -> bool {
+    if (!(t < T)) return true;
+    if (it >= cap) {
+      code = CMTJ_ERR_LIMIT;
+      return true;
+    }
+    ++it;
+    if (t + h > T) h = T - t;
+    int rejections = 0;
+    if (isnan(h) || isinf(h) || h <= 0) {
+      h = h0;
+      ++rejections;
+    }
+    const V3<double> mt = m[0];
+    const V3<double> k1 = f(t, mt) * h;
+    const V3<double> k2 = f(t + (1.0 / 5.0) * h, mt + k1 * (1.0 / 5.0)) * h;
+    const V3<double> k3 = f(t + (3.0 / 10.0) * h, mt + k1 * (3.0 / 40.0) + k2 * (9.0 / 40.0)) * h;
+    const V3<double> k4 =
+        f(t + (4.0 / 5.0) * h, mt + k1 * (44.0 / 45.0) + k2 * (-56.0 / 15.0) + k3 * (32.0 / 9.0)) * h;
+    const V3<double> k5 = f(t + (8.0 / 9.0) * h, mt + k1 * (19372.0 / 6561.0) + k2 * (-25360.0 / 2187.0) +
+                                                      k3 * (64448.0 / 6561.0) + k4 * (-212.0 / 729.0)) *
+                          h;
+    const V3<double> k6 = f(t + 1.0 * h, mt + k1 * (9017.0 / 3168.0) + k2 * (-355.0 / 33.0) +
+                                             k3 * (46732.0 / 5247.0) + k4 * (49.0 / 176.0) +
+                                             k5 * (-5103.0 / 18656.0)) *
+                          h;
+    const V3<double> k7 = f(t + 1.0 * h, mt + k1 * (35.0 / 384.0) + k3 * (500.0 / 1113.0) +
+                                             k4 * (125.0 / 192.0) + k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0)) *
+                          h;
+    V3<double> m5 = mt + k1 * (35.0 / 384.0) + k3 * (500.0 / 1113.0) + k4 * (125.0 / 192.0) +
+                    k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0);
+    const V3<double> m4 = mt + k1 * (5179.0 / 57600.0) + k3 * (7571.0 / 16695.0) + k4 * (393.0 / 640.0) +
+                          k5 * (-92097.0 / 339200.0) + k6 * (187.0 / 2100.0) + k7 * (1.0 / 40.0);
+    const double err = vlen(m5 - m4);
+    const double tol = abs_tol + rel_tol * vlen(mt);
+    double ratio = err / tol;
+    if (tol < 1e-15 || isnan(ratio) || isinf(ratio)) ratio = 10.0;
+    double factor = safety * pow(1.0 / ratio, 0.2);
+    if (isnan(factor) || isinf(factor)) factor = min_factor;
+    factor = fmin(factor, max_factor);
+    factor = fmax(factor, min_factor);
+    double nh = h * factor;
+    if (isnan(nh) || isinf(nh) || nh <= 0) nh = h * min_factor;
+    bool acc;
+    if (ratio <= 1.0) {
+      m5 = normalize_exact(m5);
+      m[0] = m5;
+      if (isnan(m5.x)) {  // the reference throws "NAN magnetisation"
+        code = CMTJ_ERR_NAN;
+        nan_layer = 0;
+        return true;
+      }
+      h = nh;
+      acc = true;
+    } else {
+      h = nh;
+      acc = false;
+    }
+    if (!acc && (h < 1e-15 || rejections >= 10)) acc = true;  // forced acceptance (:2017-2025)
+    if (acc) {
+      t += h;
+      ++accepted_steps;
+      if (t >= next_write) {
+        if (k >= NE) {
+          code = CMTJ_ERR_LIMIT;
+          return true;
+        }
+        if (logging) {
+          char* p = lp + (uint64_t)k * esize;
+          write_entry<double, 1, FM, false>(g, J, m, er, p, cs, J->Rp, J->Rap, mcoef);
+          log_put(g.log_f32, p + tch * cs, t);
+        }
+        ++k;
+        next_write = (double)((uint32_t)(t / wf) + 1u) * wf;
+      }
+    }
+    return false;
+  };
+
+  uint32_t q = atomicAdd(queue, 1u);
+  bool have = q < g.n;
+  if (have) load(q);
+  while (__ballot(have)) {
+    if (have && attempt()) {
+      store();
+      q = atomicAdd(queue, 1u);
+      have = q < g.n;
+      if (have) load(q);
+    }
+  }
+}
+
+// Clears the work counters of a launch (a kernel, not a memset: the plan's
+// launches may be captured into a graph).
+__global__ void k_clear_u32(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 }  // namespace cmtj

@@ -54,7 +54,9 @@ KTable kernels_exact();
 KTable kernels_cl();
 KTable kernels_cl64();
 KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stacks only
-KTable kernels_dp();     // Dormand-Prince (single-layer, FP64)
+KTable kernels_dp();     // Dormand-Prince (FP64)
+const void* dp_queue_kernel();   // single-layer Dormand-Prince from a device work queue
+const void* clear_u32_kernel();  // clears a launch's work counters
 KTable kernels_group();   // dipole-coupled groups, one wave each: chosen by cmtj_run_groups only
 KTable kernels_group4();  // the same on four waves per group
 

@@ -58,7 +58,9 @@ def _on_axis(direction, driver) -> AxialDriver:
 
 def _disturbed(junction, n: int, disturbance: float, rng) -> dict:
     """Per-point initial magnetisation of every layer: the present one plus a
-    normal disturbance, normalised (as setLayerMagnetisation does)."""
+    normal disturbance, normalised (as setLayerMagnetisation does); with no
+    disturbance the present one as stored (the reference then leaves the
+    magnetisation alone, procedures.py:130-139)."""
     out = {}
     for lid in junction.getLayerIds():
         m = junction.getLayerMagnetisation(lid)
@@ -66,7 +68,8 @@ def _disturbed(junction, n: int, disturbance: float, rng) -> dict:
         v = np.repeat(base[None, :], n, axis=0)
         if disturbance:
             v = v + rng.normal(0.0, disturbance, size=(n, 3))
-        out[lid] = v / np.linalg.norm(v, axis=1, keepdims=True)
+            v = v / np.linalg.norm(v, axis=1, keepdims=True)
+        out[lid] = v
     return out
 
 
@@ -107,8 +110,9 @@ def PIMM_procedure(junction, Hvecs: np.ndarray, int_step: float, resistance_para
         b = JunctionBatch.replicate(junction, n, simulation_duration, int_step, int_step, SolverMode.RK4,
                                     precision=precision, log="m", log_dtype="f64")
         b.set_external_field("all", Hvecs)
-        for lid, v in _disturbed(junction, n, disturbance, rng).items():
-            b.set_magnetisation(lid, v)
+        if disturbance:  # otherwise every point starts from the stored m, as in the reference
+            for lid, v in _disturbed(junction, n, disturbance, rng).items():
+                b.set_magnetisation(lid, v)
         r = b.run()
         ne = int(b.js[0]["n_entries"])
         traj = r.log[: n * 3 * nl * ne].view(n, nl, 3, ne)  # device, [point, layer, xyz, entry]
@@ -221,8 +225,9 @@ def VSD_procedure(junction, Hvecs: np.ndarray, frequencies: np.ndarray, int_step
         fq = np.zeros((n, 3))
         fq[:] = np.tile(frequencies, nH)[:, None]
         b.set_driver("all", "oersted", "frequency", fq)
-        for lid, v in _disturbed(junction, n, disturbance, rng).items():
-            b.set_magnetisation(lid, v)
+        if disturbance:  # otherwise every point starts from the stored m, as in the reference
+            for lid, v in _disturbed(junction, n, disturbance, rng).items():
+                b.set_magnetisation(lid, v)
         r = b.run()
         ne = int(b.js[0]["n_entries"])
         if Rtype == "Rz":

