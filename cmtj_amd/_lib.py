@@ -219,6 +219,7 @@ EXPORTS = (
     "cmtj_constants_reset",
     "cmtj_driver_eval",
     "cmtj_run",
+    "cmtj_run_noalloc",
     "cmtj_plan_create",
     "cmtj_plan_arena_size",
     "cmtj_group_key",
@@ -256,6 +257,7 @@ def _load():
     lib.cmtj_constants_set.argtypes = [_P]
     lib.cmtj_driver_eval.argtypes = [_P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     lib.cmtj_run.argtypes = [_P, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P]
+    lib.cmtj_run_noalloc.argtypes = [_P, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, _P]
     lib.cmtj_plan_create.argtypes = [
         _P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t,
         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -365,8 +367,11 @@ def driver_eval(rec: np.ndarray, t: float) -> float:
     return out.value
 
 
-def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: np.ndarray | None):
-    """One junction through cmtj_run; returns (log [ch, entries], mag [L,3], status)."""
+def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: np.ndarray | None,
+                 noalloc: bool = False, stream: int | None = None):
+    """One junction through cmtj_run (or cmtj_run_noalloc on `stream`, a
+    hipStream_t handle, None = default stream); returns (log [ch, entries],
+    mag [L,3], status)."""
     j = np.ascontiguousarray(j, dtype=JUNCTION_DTYPE).reshape(())
     layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
     nch = n_channels(int(j["n_layers"]), int(j["mr_mode"]), int(j["log_mask"]))
@@ -375,11 +380,12 @@ def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: 
     mag = np.zeros((int(j["n_layers"]), 3), dtype=np.float64)
     st = np.zeros((), dtype=STATUS_DTYPE)
     tabs = None if tables is None or len(tables) == 0 else np.ascontiguousarray(tables, dtype=np.float64)
-    rc = lib().cmtj_run(
-        ptr(j), ptr(layers), ptr(consts), ptr(tabs), 0 if tabs is None else tabs.size,
-        ptr(log) if log.size else None, log.size, ptr(mag), ptr(st),
-    )
-    check(rc, "cmtj_run")
+    args = (ptr(j), ptr(layers), ptr(consts), ptr(tabs), 0 if tabs is None else tabs.size,
+            ptr(log) if log.size else None, log.size, ptr(mag), ptr(st))
+    if noalloc:
+        check(lib().cmtj_run_noalloc(*args, stream), "cmtj_run_noalloc")
+    else:
+        check(lib().cmtj_run(*args), "cmtj_run")
     return log, mag, st
 
 

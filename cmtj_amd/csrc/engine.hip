@@ -744,6 +744,83 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_consta
   return 0;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the allocation-free drop-in: descriptors, log staging, final m and status
+// of one junction live in one file-scope device array.  Host code never
+// passes the array's name to a kernel or a copy: it uses the address
+// hipGetSymbolAddress returns for the current device, taken once per device.
+
+static __device__ __align__(256) char scratch[CMTJ_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Address of this device's copy of `scratch`; the caller holds g_static_mu.
+static int static_scratch(char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CMTJ_ERR_INVALID, "device ordinal out of range");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(CMTJ_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = (char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" {
+
+int cmtj_run_noalloc(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
+                     const double* tables, size_t n_table, double* log_host, size_t log_cap,
+                     double* mag_out_host, cmtj_status* st, void* stream) {
+  if (!j || !layers || !c || !mag_out_host || !st) return fail(CMTJ_ERR_INVALID, "null pointer");
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  char* base = nullptr;
+  if (const int rs = static_scratch(&base)) return rs;
+  const hipStream_t s = (hipStream_t)stream;
+  cmtj_junction jj = *j;
+  jj.layer_offset = 0;
+  jj.log_offset = 0;
+  const size_t nch = (size_t)n_channels(jj.n_layers, jj.mr_mode, jj.log_mask);
+  const size_t nlog = nch * jj.n_entries;
+  if (nlog > 0 && (!log_host || log_cap < nlog)) return fail(CMTJ_ERR_INVALID, "log buffer too small");
+  // scratch layout: [plan arena][log nlog doubles][final m][status]
+  cmtj_plan P;
+  P.owns = false;
+  size_t arena = 0;
+  int r = build_plan(&P, &jj, 1, layers, (size_t)jj.n_layers, c, tables, n_table, CMTJ_FP64, CMTJ_LOG_F64,
+                     CMTJ_LANES_AUTO, nullptr, nullptr, s, &arena);
+  if (r) return r;
+  const size_t o_log = align_up(arena);
+  const size_t o_mag = align_up(o_log + nlog * sizeof(double));
+  const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS);
+  const size_t need = o_st + sizeof(cmtj_status);
+  if (need > (size_t)CMTJ_STATIC_SCRATCH_BYTES)
+    return fail(CMTJ_ERR_INVALID, "run needs " + std::to_string(need) + " bytes of device scratch, more than the " +
+                                      std::to_string((size_t)CMTJ_STATIC_SCRATCH_BYTES) +
+                                      " of cmtj_run_noalloc (use cmtj_run)");
+  FixedArena fa{base, arena};
+  r = build_plan(&P, &jj, 1, layers, (size_t)jj.n_layers, c, tables, n_table, CMTJ_FP64, CMTJ_LOG_F64,
+                 CMTJ_LANES_AUTO, fixed_arena, &fa, s);
+  if (r) return r;
+  r = launch_plan(&P, nlog ? base + o_log : nullptr, (double*)(base + o_mag), (cmtj_status*)(base + o_st), s);
+  if (r) return r;
+  if (nlog) HIP_TRY(hipMemcpyAsync(log_host, base + o_log, nlog * sizeof(double), hipMemcpyDeviceToHost, s));
+  double mag[3 * CMTJ_MAX_LAYERS];
+  HIP_TRY(hipMemcpyAsync(mag, base + o_mag, sizeof(mag), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(st, base + o_st, sizeof(cmtj_status), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  std::memcpy(mag_out_host, mag, sizeof(double) * 3 * (size_t)jj.n_layers);
+  return 0;
+}
+
 int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_junction* js, const cmtj_layer* layers,
                     size_t n_layer_rows, const cmtj_constants* c, double* log_host, size_t log_cap,
                     double* mag_out_host, cmtj_status* st_host) {

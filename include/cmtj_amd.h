@@ -286,6 +286,28 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
              double* log_host, size_t log_cap, double* mag_out_host,
              cmtj_status* st);
 
+/* Device scratch of cmtj_run_noalloc, per GPU and per process: 64 MiB. */
+#define CMTJ_STATIC_SCRATCH_BYTES (64u * 1024u * 1024u)
+
+/* As cmtj_run (same arguments, results bit for bit equal), but allocates no
+ * device memory: no hipMalloc, hipFree or stream creation.  The plan
+ * descriptors, the log staging, the final magnetisation and the status live
+ * in one file-scope device array of CMTJ_STATIC_SCRATCH_BYTES that each GPU
+ * holds until the process ends; a run whose descriptors and log need more is
+ * rejected with CMTJ_ERR_INVALID.  Work is queued on `stream` (a
+ * hipStream_t, NULL = default stream) and synchronised before return.  Every
+ * call on one GPU reuses that GPU's array and leaves the last run's contents
+ * in it (nothing in it is read before the run writes it): calls are
+ * serialised by a process-wide lock, and calls that overlap on one GPU from
+ * outside that lock (another library copy in the process) race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code around HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). */
+int cmtj_run_noalloc(const cmtj_junction* j, const cmtj_layer* layers,
+                     const cmtj_constants* c, const double* tables, size_t n_table,
+                     double* log_host, size_t log_cap, double* mag_out_host,
+                     cmtj_status* st, void* stream);
+
 /* ---- electrically coupled stacks (FP64, host buffers) ------------------ */
 
 /* A SeriesStack / ParallelStack (core/stack.hpp:414-480): junctions that
