@@ -375,6 +375,39 @@ def _bench_group(side: int = 8, total_time: float = 2e-9, n_sweep: int = 1024) -
                     "back); engine_s: cmtj_run_groups on the packed sweep"}
 
 
+def _bench_sweep_forms(n_points: int = 65536) -> dict:
+    """The batching forms on an uneven PIMM sweep larger than the chip
+    (65,536 points, durations U(2, 20) ns, FP32, m_x and R logged every 10
+    steps): the default launch, then
+    the autotuner's measured candidates and its pick, each timed with HIP
+    events over one whole pass."""
+    import tempfile
+
+    from cmtj_amd import autotune
+
+    import numpy as np
+
+    os.environ.setdefault("CMTJ_AMD_TUNE_CACHE", os.path.join(tempfile.mkdtemp(), "tune.json"))
+    b = _pimm("fp32", n_points)
+    # the sweep's durations, logged every 10 steps (every step would be 86 GB)
+    b.set_times(np.random.default_rng(0).uniform(2e-9, 20e-9, n_points), 1e-13, 1e-12)
+    b.set_lanes("auto")
+    _, ms_auto = _timed_run(b)
+    name_auto = b.plan().info.kernel_name.decode()
+    pick = autotune.tune(b, force=True, verbose=True)
+    b.set_lanes(pick["lanes"])
+    b.plan()
+    if pick["graph"] and b.plan().info.n_kernels > 1:
+        b.capture()
+    _, ms_pick = _timed_run(b)
+    return {"workload": f"pimm_{n_points}_uneven", "junction_steps": b.total_steps,
+            "auto": {"kernel": name_auto, "ms": ms_auto, "junction_steps_per_s": b.total_steps / (ms_auto * 1e-3)},
+            "autotune": {"lanes": pick["lanes"], "graph": pick["graph"], "kernel": pick["kernels"], "ms": ms_pick,
+                         "junction_steps_per_s": b.total_steps / (ms_pick * 1e-3),
+                         "trials": [{k: t[k] for k in ("lanes", "graph", "ms_probe", "kernels")}
+                                    for t in pick["trials"]]}}
+
+
 def _bench_dp(n_fields: int = 64, n_currents: int = 64, total_time: float = 3e-9) -> dict:
     """Dormand-Prince stability-diagram sweep: single-layer SOT junctions
     (perpendicular, second-order anisotropy, 1 ns field- and damping-like
@@ -534,6 +567,7 @@ def _cpu_baseline(b, args):
     import numpy as np
 
     import oracle
+    from cmtj_amd import _lib
 
     ncores, nproc, avail = _host_cpus()
     try:
@@ -587,7 +621,8 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-qps", type=float, default=20.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving", "stack", "dp", "group"],
+    ap.add_argument("--workload", default="pimm", choices=["pimm", "rh", "cims", "sto", "serving", "stack", "dp", "group",
+                                                            "forms"],
                     help="pimm: the headline sweep (default); rh, cims, sto: the other sweep workloads, "
                          "one JSON line each")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -610,7 +645,7 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.workload != "pimm":
         fn = {"rh": _bench_rh, "cims": _bench_cims, "sto": _bench_sto, "stack": _bench_stack, "dp": _bench_dp,
-              "group": _bench_group,
+              "group": _bench_group, "forms": _bench_sweep_forms,
               "serving": lambda: _serving(args.serving_qps, args.serving_seconds, lanes=args.serving_lanes)}[args.workload]
         print(json.dumps(fn()))
         return 0

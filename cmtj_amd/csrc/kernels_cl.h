@@ -323,6 +323,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     iend = (code != CMTJ_OK) ? i : (uint32_t)min((uint64_t)nsteps, (uint64_t)i + g.slice);
   }
   uint64_t next_log = (uint64_t)k * W;
+  // lanes whose NaN stops a run: component x of every existing layer
+  const unsigned long long nan_lanes = __ballot(live && c == 0);
   double bot = dpp_d<DPP_ROW_SHR4>(m), top = dpp_d<DPP_ROW_SHL4>(m);
   bot = hasB ? bot : 0.0;
   top = hasT ? top : 0.0;
@@ -592,7 +594,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     // the reference updates layers in turn and stops at the first NaN
     // layer; layers up to and including it take the new value (RK4 only:
     // the Euler-Heun step of the reference has no such test)
-    const unsigned long long bad = (SOLVER == CMTJ_RK4) ? __ballot(live && c == 0 && isnan(mn)) : 0ull;
+    const unsigned long long bad = (SOLVER == CMTJ_RK4) ? (__ballot(isnan(mn)) & nan_lanes) : 0ull;
     if (bad) {  // wave-uniform test first: the per-instance work only on the rare path
       const uint32_t gb = (uint32_t)((bad >> gshift) & ((1ull << LPI) - 1));
       if (gb) {
