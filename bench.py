@@ -237,8 +237,8 @@ def _bench_stack(n_members: int = 8, total_time: float = 10e-9) -> dict:
         members.append(j)
 
     class _OracleStack(SeriesStack):
-        def _execute(self, stack, js, layers, consts):
-            return oracle.run_stack(stack, js, layers, consts)
+        def _execute(self, stack, js, layers, consts, tables=None):
+            return oracle.run_stack(stack, js, layers, consts, tables)
 
     out = {}
     for name, cls in (("gpu", SeriesStack), ("cpu_oracle", _OracleStack)):
@@ -312,8 +312,8 @@ def _bench_group(side: int = 8, total_time: float = 2e-9, n_sweep: int = 1024) -
         return js
 
     class _OracleGroup(GroupInteraction):
-        def _execute(self, g, js, layers, consts):
-            return oracle.run_group(g, js, layers, consts)
+        def _execute(self, g, js, layers, consts, pair_terms=None):
+            return oracle.run_group(g, js, layers, consts, pair_terms)
 
     base = elements(2e4)
     out = {}
@@ -489,7 +489,11 @@ def _bench_cims(n_currents: int = 256, n_seeds: int = 4096) -> dict:
     ro = b64.subset(np.searchsorted(idx64, idxo)).run()  # the GPU FP64 of the oracle's sample
     dmo = float(np.max(np.abs(ro.final_m.cpu().numpy()[:, :2] - omag[:, :2])))
     Pn = P.cpu().numpy()
+    jcur = np.logspace(10, 12, n_currents)
+    k50 = int(np.argmax(Pn >= 0.5)) if np.any(Pn >= 0.5) else -1
     return {"workload": "cims_pbit_256x4096", "instances": len(b.js), "junction_steps": b.total_steps,
+            "j50_A_per_m2": float(jcur[k50]) if k50 >= 0 else None,
+            "currents_with_0_lt_P_lt_1": int(np.sum((Pn > 0) & (Pn < 1))),
             "fp32_junction_steps_per_s": b.total_steps / (ms * 1e-3), "fp32_ms": ms,
             "fp64_sample_junction_steps_per_s": b64.total_steps / (ms64 * 1e-3),
             "switching_probability_first_mid_last": [float(Pn[0]), float(Pn[n_currents // 2]), float(Pn[-1])],
@@ -517,6 +521,11 @@ def _bench_sto(n_currents: int = 512) -> dict:
     b64 = workloads.sto(n_currents=n_currents, precision="fp64", log="full")
     r32, ms32 = _timed_run(b32)
     r64, ms64 = _timed_run(b64)
+    # the same FP32 request with the lane mapping left to the engine
+    b32a = workloads.sto(n_currents=n_currents, precision="fp32", log="full")
+    _, ms32a = _timed_run(b32a)
+    auto_info = b32a.plan().info
+    del b32a
     W = int(b64.js[0]["write_every"])
     dtl = W * float(b64.js[0]["dt"])
     f32, w32, _ = analysis.spectrum_peaks(r32.channel("free_mx"), dtl)
@@ -536,6 +545,9 @@ def _bench_sto(n_currents: int = 512) -> dict:
     dw = (w32 - w64).abs()
     return {"workload": "sto_512_currents_1e6_steps", "junction_steps": b64.total_steps,
             "fp32_junction_steps_per_s": b32.total_steps / (ms32 * 1e-3), "fp32_ms": ms32,
+            "fp32_kernel": b32.plan().info.kernel_name.decode(),
+            "fp32_auto": {"junction_steps_per_s": b32.total_steps / (ms32a * 1e-3), "ms": ms32a,
+                          "kernel": auto_info.kernel_name.decode(), "fp64_groups": int(auto_info.fp64_groups)},
             "fp64_junction_steps_per_s": b64.total_steps / (ms64 * 1e-3), "fp64_ms": ms64,
             "oscillating_points": int(osc.sum()),
             "frequency_GHz_first_mid_last": [float(f64[0]) / 1e9, float(f64[n_currents // 2]) / 1e9,

@@ -14,11 +14,11 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libcmtj_amd.so"
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_LAYERS = 8
 MAX_STACK = 64
 MAX_GROUP = 64
-INTERACTION_NULL, INTERACTION_KANAO, INTERACTION_NOMURA = 0, 1, 2
+INTERACTION_NULL, INTERACTION_KANAO, INTERACTION_NOMURA, INTERACTION_TENSOR = 0, 1, 2, 3
 NSLOT = 30
 ENERGY_CHANNELS = 23
 
@@ -97,6 +97,13 @@ LAYER_DTYPE = np.dtype(
         ("dp_max_factor", "<f8"),
         ("dp_min_factor", "<f8"),
         ("dp_safety", "<f8"),
+        ("dp_use_pid", "<i4"),
+        ("dp_pad", "<i4"),
+        ("dp_kp", "<f8"),
+        ("dp_ki", "<f8"),
+        ("dp_kd", "<f8"),
+        ("dp_integral_error", "<f8"),
+        ("dp_prev_error_ratio", "<f8"),
     ],
     align=True,
 )
@@ -145,6 +152,8 @@ STACK_DTYPE = np.dtype(
         ("kcl", "<i4"),
         ("n_junctions", "<i4"),
         ("two_layer", "<i4"),
+        ("delayed", "<i4"),
+        ("pad_", "<i4"),
         ("phase_offset", "<f8"),
         ("pol", "<f8", (3,)),
         ("current", DRIVER_DTYPE),
@@ -205,7 +214,7 @@ class PlanInfo(ctypes.Structure):
         ("max_layers", ctypes.c_int32),
         ("n_kernels", ctypes.c_int32),
         ("lanes_per_instance", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("fp64_groups", ctypes.c_int32),
         ("kernel_name", ctypes.c_char * 128),
     ]
 
@@ -284,8 +293,10 @@ def _load():
     lib.cmtj_plan_destroy.argtypes = [_P]
     lib.cmtj_vsd_mix.argtypes = [_P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _P, ctypes.c_double, _P, _P,
                                  ctypes.c_int32, _P, _P]
-    lib.cmtj_run_stacks.argtypes = [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P]
-    lib.cmtj_run_groups.argtypes = [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P]
+    lib.cmtj_run_stacks.argtypes = [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P,
+                                    ctypes.c_size_t, _P, _P]
+    lib.cmtj_run_groups.argtypes = [_P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P, _P, ctypes.c_size_t, _P,
+                                    ctypes.c_size_t, _P, _P]
     info = _AbiInfo()
     lib.cmtj_abi(ctypes.byref(info))
     checks = {
@@ -368,28 +379,35 @@ def driver_eval(rec: np.ndarray, t: float) -> float:
 
 
 def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: np.ndarray | None,
-                 noalloc: bool = False, stream: int | None = None):
+                 noalloc: bool = False, stream: int | None = None, pid_out: np.ndarray | None = None):
     """One junction through cmtj_run (or cmtj_run_noalloc on `stream`, a
     hipStream_t handle, None = default stream); returns (log [ch, entries],
-    mag [L,3], status)."""
+    mag [L,3], status).  pid_out (float64 [L, 2]): receives the final PID
+    controller state (integral, prev) of every layer of a Dormand-Prince run
+    whose layers use it (left alone otherwise)."""
     j = np.ascontiguousarray(j, dtype=JUNCTION_DTYPE).reshape(())
     layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
     nch = n_channels(int(j["n_layers"]), int(j["mr_mode"]), int(j["log_mask"]))
     ne = int(j["n_entries"])
+    nl = int(j["n_layers"])
     log = np.empty((nch, ne), dtype=np.float64)
-    mag = np.zeros((int(j["n_layers"]), 3), dtype=np.float64)
+    out = np.zeros(5 * nl, dtype=np.float64)  # m [L,3], then the PID state [L,2]
+    mag = out[: 3 * nl].reshape(nl, 3)
     st = np.zeros((), dtype=STATUS_DTYPE)
     tabs = None if tables is None or len(tables) == 0 else np.ascontiguousarray(tables, dtype=np.float64)
     args = (ptr(j), ptr(layers), ptr(consts), ptr(tabs), 0 if tabs is None else tabs.size,
-            ptr(log) if log.size else None, log.size, ptr(mag), ptr(st))
+            ptr(log) if log.size else None, log.size, ptr(out), ptr(st))
     if noalloc:
         check(lib().cmtj_run_noalloc(*args, stream), "cmtj_run_noalloc")
     else:
         check(lib().cmtj_run(*args), "cmtj_run")
+    if pid_out is not None and int(j["solver"]) == DORMAND_PRINCE and np.any(layers["dp_use_pid"][:nl]):
+        pid_out[...] = out[3 * nl:].reshape(nl, 2)
     return log, mag, st
 
 
-def run_stacks(stacks: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray):
+def run_stacks(stacks: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray,
+               tables: np.ndarray | None = None):
     """Stacks through cmtj_run_stacks (js[s * members + j]: member j of stack
     s); js[q]["log_offset"] must place each member's block of n_channels + 2
     channels.  Returns (log, mag [n, 8, 3], status [n])."""
@@ -404,15 +422,19 @@ def run_stacks(stacks: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: n
     log = np.zeros(need, dtype=np.float64)
     mag = np.zeros((n, MAX_LAYERS, 3), dtype=np.float64)
     st = np.zeros(n, dtype=STATUS_DTYPE)
+    tabs = None if tables is None or len(tables) == 0 else np.ascontiguousarray(tables, dtype=np.float64)
     check(lib().cmtj_run_stacks(ptr(stacks), stacks.size, ptr(js), ptr(layers), layers.size, ptr(consts),
+                                ptr(tabs), 0 if tabs is None else tabs.size,
                                 ptr(log) if log.size else None, log.size, ptr(mag), ptr(st)), "cmtj_run_stacks")
     return log, mag, st
 
 
-def run_groups(groups: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray):
+def run_groups(groups: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: np.ndarray,
+               pair_terms: np.ndarray | None = None):
     """Dipole-coupled groups through cmtj_run_groups (js[g * elements + e]:
     element e of group g); js[q]["log_offset"] must place each element's
-    block of n_channels channels.  Returns (log, mag [n, 8, 3], status [n])."""
+    block of n_channels channels; pair_terms [groups][n][12][n] for tensor
+    interactions.  Returns (log, mag [n, 8, 3], status [n])."""
     groups = np.ascontiguousarray(groups, dtype=GROUP_DTYPE).reshape(-1)
     js = np.ascontiguousarray(js, dtype=JUNCTION_DTYPE)
     layers = np.ascontiguousarray(layers, dtype=LAYER_DTYPE)
@@ -424,7 +446,9 @@ def run_groups(groups: np.ndarray, js: np.ndarray, layers: np.ndarray, consts: n
     log = np.zeros(need, dtype=np.float64)
     mag = np.zeros((n, MAX_LAYERS, 3), dtype=np.float64)
     st = np.zeros(n, dtype=STATUS_DTYPE)
+    pt = None if pair_terms is None else np.ascontiguousarray(pair_terms, dtype=np.float64).reshape(-1)
     check(lib().cmtj_run_groups(ptr(groups), groups.size, ptr(js), ptr(layers), layers.size, ptr(consts),
+                                ptr(pt), 0 if pt is None else pt.size,
                                 ptr(log) if log.size else None, log.size, ptr(mag), ptr(st)), "cmtj_run_groups")
     return log, mag, st
 

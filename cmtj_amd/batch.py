@@ -369,6 +369,13 @@ class JunctionBatch:
         p.execute(log.data_ptr() if self.log_elements else None, log.numel(), mag.data_ptr(), st.data_ptr(), sp)
         return BatchResult(self, log, mag, st)
 
+    @property
+    def fp64_groups(self) -> int:
+        """Structure groups of an FP32 batch that run binary64 kernels (spin
+        torques or scalar drives on one lane per component, where no
+        binary32 kernel covers them): their arithmetic is the drop-in's."""
+        return int(self.plan().info.fp64_groups)
+
     def capture(self, device="cuda") -> "JunctionBatch":
         """Captures one execute of the batch, bound to its device buffers, as
         a HIP graph (cmtj_plan_graph_create): one kernel node per structure

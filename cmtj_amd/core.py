@@ -831,6 +831,12 @@ class Layer:
         rec["dp_max_factor"] = ap.max_factor
         rec["dp_min_factor"] = ap.min_factor
         rec["dp_safety"] = ap.safety_factor
+        rec["dp_use_pid"] = int(bool(ap.use_pid_control))
+        rec["dp_kp"] = ap.kp
+        rec["dp_ki"] = ap.ki
+        rec["dp_kd"] = ap.kd
+        rec["dp_integral_error"] = ap.integral_error
+        rec["dp_prev_error_ratio"] = ap.prev_error_ratio
         rec["ref_type"] = int(self.referenceType)
         rec["include_stt"] = int(self.includeSTT)
         rec["include_sot"] = int(self.includeSOT)
@@ -1155,8 +1161,6 @@ class Junction:
         if local == SolverMode.DormandPrince:
             if self.layerNo > 4:
                 raise NotImplementedError("DormandPrince runs junctions of 1 to 4 layers on the GPU engine")
-            if any(l.adaptiveParams.use_pid_control for l in self.layers):
-                raise NotImplementedError("the PID step controller of DormandPrince is not provided")
         return int(local)
 
     def _descriptor(self, totalTime: float, timeStep: float, writeFrequency: float, calculateEnergies: bool,
@@ -1266,9 +1270,9 @@ class Junction:
         for i, l in enumerate(self.layers):
             l.mag = CVector(*mag[i])
 
-    def _execute(self, j, L, consts, tables):
+    def _execute(self, j, L, consts, tables, pid_out=None):
         """One packed run on the GPU through cmtj_run."""
-        return _lib.run_junction(j, L, consts, tables)
+        return _lib.run_junction(j, L, consts, tables, pid_out=pid_out)
 
     def runSimulation(
         self,
@@ -1285,9 +1289,20 @@ class Junction:
         j, L, tables, n_steps, W = self._descriptor(totalTime, timeStep, writeFrequency, calculateEnergies,
                                                     solverMode)
         consts = _lib.constants_get()
-        log, mag, st = self._execute(j, L, consts, tables)
+        pid = None
+        if int(j["solver"]) == _lib.DORMAND_PRINCE and np.any(L["dp_use_pid"]):
+            # the PID controller's integral and last error stay in the layer
+            # from run to run (AdaptiveIntegrationParams, core/junction.hpp:953-961)
+            pid = np.stack([L["dp_integral_error"], L["dp_prev_error_ratio"]], axis=1)
+            log, mag, st = self._execute(j, L, consts, tables, pid_out=pid)
+        else:
+            log, mag, st = self._execute(j, L, consts, tables)
         self._runs += 1
         self._apply(j, log, mag, st, timeStep, W)
+        if pid is not None:
+            for l, (integral, prev) in zip(self.layers, pid):
+                l.adaptiveParams.integral_error = float(integral)
+                l.adaptiveParams.prev_error_ratio = float(prev)
         if int(st["code"]) == _lib.ERR_NAN:
             raise RuntimeError("NAN magnetisation")
         if int(st["code"]) == _lib.ERR_LIMIT:

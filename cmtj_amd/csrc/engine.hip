@@ -47,9 +47,9 @@ static std::mutex g_consts_mu;
 #include "registry.h"
 
 
-// kinds: bit 0 = a record of another kind than step/sine, bit 1 = a step
-// record, bit 2 = a sine record.
-constexpr uint32_t KIND_OTHER = 1, KIND_STEP = 2, KIND_SINE = 4;
+// kinds: bit 0 = a record of another kind than step/sine/trapezoid, bit 1 =
+// a step record, bit 2 = a sine record, bit 3 = a trapezoid record.
+constexpr uint32_t KIND_OTHER = 1, KIND_STEP = 2, KIND_SINE = 4, KIND_TRAP = 8;
 
 // Log layout the LOGF = 1 kernels compile in: log_mask | mr_mode << 8 | f32 << 16.
 constexpr uint32_t kLogPimm = (CMTJ_LOG_MX | CMTJ_LOG_MR) | (CMTJ_MR_CLASSIC << 8) | (1u << 16);
@@ -61,6 +61,7 @@ static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds, int ndyn, ui
   if ((k.fm & F_EXACT) && (feat != flags || ndyn > DREG)) return false;  // exact kernels keep every record in registers
   if ((k.fm & F_KSTEP) && (kinds & ~KIND_STEP)) return false;
   if ((k.fm & F_KSINE) && (kinds & ~KIND_SINE)) return false;
+  if ((k.fm & F_KTRAP) && (kinds & ~KIND_TRAP)) return false;
   if (k.nd1 && k.nd1 != ndyn + 1) return false;
   return true;
 }
@@ -69,7 +70,7 @@ static bool covers(const KEntry& k, uint32_t flags, uint32_t kinds, int ndyn, ui
 // driver-kind specialisations, then a compiled-in record count, then fewer
 // compiled-in features.
 static int rank(const KEntry& k) {
-  return k.lm * 1000 + ((k.fm & F_EXACT) ? 0 : 200) + ((k.fm & (F_KSTEP | F_KSINE)) ? 0 : 100) +
+  return k.lm * 1000 + ((k.fm & F_EXACT) ? 0 : 200) + ((k.fm & (F_KSTEP | F_KSINE | F_KTRAP)) ? 0 : 100) +
          (k.nd1 ? 0 : 50) + (k.logf ? 0 : 25) + __builtin_popcount(k.fm & F_ALL);
 }
 
@@ -447,7 +448,8 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
     uint32_t kinds = 0;
     for (const Rec& r : recs_of[orders[g][0]]) {
       const int ty = r.meta & 0xff;
-      kinds |= (ty == CMTJ_STEP) ? KIND_STEP : (ty == CMTJ_SINE) ? KIND_SINE : KIND_OTHER;
+      kinds |= (ty == CMTJ_STEP) ? KIND_STEP : (ty == CMTJ_SINE) ? KIND_SINE : (ty == CMTJ_TRAPEZOID) ? KIND_TRAP
+                                                                                               : KIND_OTHER;
     }
     const KEntry* k = pick_kernel(prec, j0.n_layers, j0.solver, flags[g], kinds,
                                   (int)recs_of[orders[g][0]].size(), loge,
@@ -480,6 +482,7 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
       Gr.blocks = (std::min<uint32_t>(Gr.a.n, queue_lanes()) + 63) / 64;
     }
     max_lpi = std::max(max_lpi, k->lpi);
+    if (prec == CMTJ_FP32 && k->prec == CMTJ_FP64) ++P->info.fp64_groups;
     P->groups.push_back(Gr);
     const char* kname = Gr.queue ? "k_dp_q<double,1,F_ALL>" : k->name;
     if (names.find(kname) == std::string::npos) {
@@ -710,6 +713,19 @@ static int plain_bind() {
   return 0;
 }
 
+// Doubles of mag_out_host that a one-junction run fills: the final m of
+// every layer, then the PID state of every layer when a Dormand-Prince
+// layer runs the PID controller (the kernels' mag_out block layout; at most
+// 5 * 4 of its 3 * CMTJ_MAX_LAYERS doubles, since build_plan caps
+// Dormand-Prince junctions at 4 layers).
+static size_t run_mag_doubles(const cmtj_junction& j, const cmtj_layer* layers) {
+  const size_t nl = (size_t)j.n_layers;
+  if (j.solver == CMTJ_DORMAND_PRINCE)
+    for (size_t l = 0; l < nl; ++l)
+      if (layers[l].dp_use_pid) return 5 * nl;
+  return 3 * nl;
+}
+
 int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_constants* c,
              const double* tables, size_t n_table, double* log_host, size_t log_cap,
              double* mag_out_host, cmtj_status* st) {
@@ -740,7 +756,7 @@ int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_consta
   HIP_TRY(hipMemcpyAsync(mag, out + o_mag, sizeof(mag), hipMemcpyDeviceToHost, g_plain.stream));
   HIP_TRY(hipMemcpyAsync(st, out + o_st, sizeof(cmtj_status), hipMemcpyDeviceToHost, g_plain.stream));
   HIP_TRY(hipStreamSynchronize(g_plain.stream));
-  std::memcpy(mag_out_host, mag, sizeof(double) * 3 * (size_t)jj.n_layers);
+  std::memcpy(mag_out_host, mag, sizeof(double) * run_mag_doubles(jj, layers));
   return 0;
 }
 
@@ -817,13 +833,13 @@ int cmtj_run_noalloc(const cmtj_junction* j, const cmtj_layer* layers, const cmt
   HIP_TRY(hipMemcpyAsync(mag, base + o_mag, sizeof(mag), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(st, base + o_st, sizeof(cmtj_status), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  std::memcpy(mag_out_host, mag, sizeof(double) * 3 * (size_t)jj.n_layers);
+  std::memcpy(mag_out_host, mag, sizeof(double) * run_mag_doubles(jj, layers));
   return 0;
 }
 
 int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_junction* js, const cmtj_layer* layers,
-                    size_t n_layer_rows, const cmtj_constants* c, double* log_host, size_t log_cap,
-                    double* mag_out_host, cmtj_status* st_host) {
+                    size_t n_layer_rows, const cmtj_constants* c, const double* tables, size_t n_table,
+                    double* log_host, size_t log_cap, double* mag_out_host, cmtj_status* st_host) {
   if (!stacks || !js || !layers || !c || !mag_out_host || !st_host) return fail(CMTJ_ERR_INVALID, "null pointer");
   if (n_stacks == 0) return 0;
   if (n_stacks > (1u << 24)) return fail(CMTJ_ERR_INVALID, "too many stacks");
@@ -833,8 +849,18 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
   for (size_t q = 0; q < n_stacks; ++q) {
     const cmtj_stack& s = stacks[q];
     if (s.n_junctions != n) return fail(CMTJ_ERR_INVALID, "every stack of a batch has the same member count");
-    if (s.current.type == CMTJ_TABLE || s.current.type < 0 || s.current.type > CMTJ_GAUSSSTEP)
-      return fail(CMTJ_ERR_INVALID, "the stack current needs a closed-form driver");
+    if (s.current.type == CMTJ_TABLE) {
+      // a callback current, sampled once per step (KCL only)
+      if (!s.kcl)
+        return fail(CMTJ_ERR_INVALID, "a non-KCL stack needs a closed-form current (operator* drops a callback)");
+      if (s.current.table_index < 0 || !tables ||
+          (size_t)s.current.table_index + (size_t)js[0].n_steps > n_table)
+        return fail(CMTJ_ERR_INVALID, "stack current table out of range");
+    } else if (s.current.type < 0 || s.current.type > CMTJ_GAUSSSTEP) {
+      return fail(CMTJ_ERR_INVALID, "unknown stack current driver kind");
+    }
+    if (!s.kcl && !s.delayed && !s.two_layer)
+      return fail(CMTJ_ERR_INVALID, "non-delayed coupling is only supported for 2 layer stacks");
   }
   const size_t nj = n_stacks * (size_t)n;
   const cmtj_junction& j0 = js[0];
@@ -863,8 +889,8 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
   if (const int rb = plain_bind()) return rb;
   cmtj_plan P;
   P.owns = false;
-  int r = build_plan(&P, js, nj, layers, n_layer_rows, c, nullptr, 0, CMTJ_FP64, CMTJ_LOG_F64, CMTJ_LANES_ONE,
-                     plain_arena, &g_plain, g_plain.stream);
+  int r = build_plan(&P, js, nj, layers, n_layer_rows, c, tables, n_table, CMTJ_FP64, CMTJ_LOG_F64,
+                     CMTJ_LANES_ONE, plain_arena, &g_plain, g_plain.stream);
   if (r) return r;
   if (P.groups.size() != 1)
     return fail(CMTJ_ERR_INVALID,
@@ -904,8 +930,8 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
 }
 
 int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_junction* js, const cmtj_layer* layers,
-                    size_t n_layer_rows, const cmtj_constants* c, double* log_host, size_t log_cap,
-                    double* mag_out_host, cmtj_status* st_host) {
+                    size_t n_layer_rows, const cmtj_constants* c, const double* pair_terms, size_t n_pair_terms,
+                    double* log_host, size_t log_cap, double* mag_out_host, cmtj_status* st_host) {
   if (!groups || !js || !layers || !c || !mag_out_host || !st_host) return fail(CMTJ_ERR_INVALID, "null pointer");
   if (n_groups == 0) return 0;
   if (n_groups > (1u << 24)) return fail(CMTJ_ERR_INVALID, "too many groups");
@@ -921,8 +947,13 @@ int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_juncti
   for (size_t q = 0; q < n_groups; ++q) {
     const cmtj_group& G = groups[q];
     if (G.n_elements != n) return fail(CMTJ_ERR_INVALID, "every group of a batch has the same element count");
-    if (G.interaction < CMTJ_INTERACTION_NULL || G.interaction > CMTJ_INTERACTION_NOMURA)
+    if (G.interaction < CMTJ_INTERACTION_NULL || G.interaction > CMTJ_INTERACTION_TENSOR)
       return fail(CMTJ_ERR_INVALID, "unknown interaction");
+    if (G.interaction == CMTJ_INTERACTION_TENSOR) {
+      if (!pair_terms || n_pair_terms < n_groups * (size_t)n * 12 * (size_t)n)
+        return fail(CMTJ_ERR_INVALID, "pair terms missing or too short for the groups");
+      continue;  // the callable was evaluated (and checked) by the caller
+    }
     if (G.interaction == CMTJ_INTERACTION_NULL || j0.n_steps == 0) continue;
     // computeDipoleInteraction / getDipoleTensorFromRelPositions throw on
     // the first step (core/reservoir.hpp:83-86, :39-42)
@@ -982,17 +1013,24 @@ int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_juncti
   const size_t o_mag = align_up(need * sizeof(double));
   const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj);
   const size_t o_grp = align_up(o_st + sizeof(cmtj_status) * nj);
-  const size_t out_bytes = o_grp + sizeof(cmtj_group) * n_groups;
+  bool any_tensor = false;
+  for (size_t q = 0; q < n_groups; ++q) any_tensor |= groups[q].interaction == CMTJ_INTERACTION_TENSOR;
+  const size_t n_pt = any_tensor ? n_groups * (size_t)n * 12 * (size_t)n : 0;
+  const size_t o_pt = align_up(o_grp + sizeof(cmtj_group) * n_groups);
+  const size_t out_bytes = o_pt + sizeof(double) * n_pt;
   char* out = (char*)grow(g_plain.out, g_plain.out_bytes, out_bytes);
   if (!out) return CMTJ_ERR_HIP;
   HIP_TRY(hipMemcpyAsync(out + o_grp, groups, sizeof(cmtj_group) * n_groups, hipMemcpyHostToDevice,
                          g_plain.stream));
+  if (n_pt)
+    HIP_TRY(hipMemcpyAsync(out + o_pt, pair_terms, sizeof(double) * n_pt, hipMemcpyHostToDevice, g_plain.stream));
   Group& G = P.groups[0];
   G.k = ke;
   G.blocks = (uint32_t)n_groups;  // group g on workgroup g, element e on lane e of its first wave
   G.threads = 64 * gw;
   G.a.group = (const cmtj_group*)(out + o_grp);
   G.a.group_n = (uint32_t)n_groups;
+  G.a.pair = n_pt ? (const double*)(out + o_pt) : nullptr;
   r = launch_plan(&P, need ? out : nullptr, (double*)(out + o_mag), (cmtj_status*)(out + o_st), g_plain.stream);
   if (r) return r;
   if (need) HIP_TRY(hipMemcpyAsync(log_host, out, need * sizeof(double), hipMemcpyDeviceToHost, g_plain.stream));

@@ -45,6 +45,7 @@ struct GroupArgs {
   uint32_t stack_n;         // stacks of the launch; member j of stack s is instance s * members + j
   uint32_t stack_pad;       // lanes per stack: a power of two >= members, <= 64 (a stack never straddles a wave)
   const cmtj_group* group;  // dipole-coupled groups (k_junction<..., GRP>): group g on workgroup g
+  const double* pair;       // CMTJ_INTERACTION_TENSOR terms [group][j][12][e], or null
   uint32_t group_n;
   uint32_t slice;     // steps per launch when state is set
   uint32_t n;       // lanes of this group
@@ -555,6 +556,8 @@ __device__ __forceinline__ R wave_value(int type, const DynRec& r, const GroupAr
   } else if constexpr ((FM & F_KSINE) != 0) {
     if (sizeof(R) == 4 || FT) return (R)wave_part_f_trans(CMTJ_SINE, r.A, r.p1, r.p2, t);
     return (R)wave_part_trans(CMTJ_SINE, r.A, r.p1, r.p2, t);
+  } else if constexpr ((FM & F_KTRAP) != 0) {  // the only kind of this group
+    return (R)wave_trapezoid(r.A, r.p1, r.p2, r.p3, t);
   }
   if (type == CMTJ_TABLE) return (R)g.tables[(size_t)r.tidx + (size_t)step * sps + stage];
   if (sizeof(R) == 4) return (R)wave_part_f(type, r.A, r.p1, r.p2, r.p3, t);
@@ -887,6 +890,7 @@ struct StackLane {
   double cj;      // coupling of pair (j, j+1); 0 for the last member
   double phase;   // phase shift of member j (non-KCL)
   int two_layer, series, kcl;
+  bool nd;        // non-KCL, non-delayed: the members step one after another
 };
 
 __device__ __forceinline__ StackLane stack_lane(const cmtj_stack* sk, int j, int n) {
@@ -900,13 +904,14 @@ __device__ __forceinline__ StackLane stack_lane(const cmtj_stack* sk, int j, int
   s.series = sk->series;
   s.kcl = sk->kcl;
   s.phase = s.series ? sk->phase_offset * j : sk->phase_offset;
+  s.nd = !s.kcl && !sk->delayed;
   return s;
 }
 
 template <int LM>
 __device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n, int base, int ti, int bi,
-                                              const V3<double> (&m)[LM], double t, double dt, double (&I)[3],
-                                              double& Ilog, double& Clog) {
+                                              const V3<double> (&m)[LM], double t, double dt, const double* tab,
+                                              uint32_t step, double (&I)[3], double& Ilog, double& Clog) {
   V3<double> mt = m[0], mb = m[0];
 #pragma unroll
   for (int l = 0; l < LM; ++l) {
@@ -919,8 +924,10 @@ __device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n,
   const double dn = __shfl(d, base + ((j + 1 < n) ? j + 1 : j), 64);
   const double Cj = (j + 1 < n) ? sk.cj * (sk.series ? d + dn : d - dn) : 0.0;
   if (sk.kcl) {
-    // totalCurrent = I0 + sum_j C_j I0, the same constant for every member
-    const double I0 = driver_value(sk.current, t);
+    // totalCurrent = I0 + sum_j C_j I0, the same constant for every member;
+    // a callback current is tabulated once per step
+    const double I0 = (sk.current.type == CMTJ_TABLE) ? tab[(size_t)sk.current.table_index + step]
+                                                      : driver_value(sk.current, t);
     double total = I0;
     for (int q = 0; q + 1 < n; ++q) total += __shfl(Cj, base + q, 64) * I0;
     I[0] = I[1] = I[2] = total;
@@ -944,6 +951,45 @@ __device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n,
     I[2] = driver_value(dl, t + dt);
     Ilog = I[0];
   }
+}
+
+// Non-delayed non-KCL coupling (runSimulationNonKCL with delayed == false,
+// core/stack.hpp:334-343): within a step the members step in order, and
+// member q's coupling to member q-1 is read from q-1's magnetisations after
+// its step and q's before it.  Called by every lane of the stack at sub-step
+// q = 0..n-1 (member q steps at sub-step q) with the stack's running product
+// eff, 1 at q = 0: eff *= 1 + c_{q-1} (d_{q-1} +- d_q), d = m_top . m_bottom.
+// Member q's current is the stack driver times eff with its phase shifted;
+// the stack log's C_{q-1}_q is eff after the update.
+template <int LM>
+__device__ __forceinline__ void stack_current_nd(const StackLane& sk, int j, int n, int base, int ti, int bi,
+                                                 const V3<double> (&m)[LM], double t, double dt, int q,
+                                                 double& eff, double (&I)[3], double& Ilog, double& Clog) {
+  V3<double> mt = m[0], mb = m[0];
+#pragma unroll
+  for (int l = 0; l < LM; ++l) {
+    mt = (l == ti) ? m[l] : mt;
+    mb = (l == bi) ? m[l] : mb;
+  }
+  const double d = mt.x * mb.x + mt.y * mb.y + mt.z * mb.z;
+  if (q > 0) {
+    const double dp = __shfl(d, base + q - 1, 64), dq = __shfl(d, base + q, 64);
+    const double cp = __shfl(sk.cj, base + q - 1, 64);  // coupling of pair (q-1, q)
+    eff *= (1 + cp * (sk.series ? dp + dq : dp - dq));
+    if (j == q - 1) Clog = eff;
+  }
+  if (j == q) {
+    cmtj_driver dl = sk.current;
+    dl.constant *= eff;
+    dl.amplitude *= eff;
+    dl.phase += sk.phase;
+    I[0] = driver_value(dl, t);
+    I[1] = driver_value(dl, t + 0.5 * dt);
+    I[2] = driver_value(dl, t + dt);
+    Ilog = I[0];
+    if (q == n - 1) Clog = eff;
+  }
+  (void)n;
 }
 
 // A dipole-coupled group in LDS: the top-layer magnetisations of the step,
@@ -991,6 +1037,26 @@ __device__ __forceinline__ V3<double> group_field(const GroupLds<GW>& L, int j0,
   return H;
 }
 
+// Field on element e of the sources j0 <= j < j1 from per-pair affine
+// terms (CMTJ_INTERACTION_TENSOR): f_j = T_ej m_j + b_ej, with the group's
+// terms pt laid out [j][12][e] (the lanes of one source and component read
+// consecutive words), the sum over j ascending as the reference adds the
+// callable's vectors (core/reservoir.hpp:135-142), the diagonal skipped.
+template <int GW>
+__device__ __forceinline__ V3<double> group_field_tensor(const GroupLds<GW>& L, const double* pt, int n, int j0,
+                                                         int j1, int e) {
+  V3<double> H = mk<double>(0, 0, 0);
+  for (int j = j0; j < j1; ++j) {
+    if (j == e) continue;
+    const double* p = pt + (size_t)j * 12 * (size_t)n + (size_t)e;
+    const double mx = L.m[j][0], my = L.m[j][1], mz = L.m[j][2];
+    H.x += fma(p[0], mx, fma(p[(size_t)n], my, fma(p[2 * (size_t)n], mz, p[9 * (size_t)n])));
+    H.y += fma(p[3 * (size_t)n], mx, fma(p[4 * (size_t)n], my, fma(p[5 * (size_t)n], mz, p[10 * (size_t)n])));
+    H.z += fma(p[6 * (size_t)n], mx, fma(p[7 * (size_t)n], my, fma(p[8 * (size_t)n], mz, p[11 * (size_t)n])));
+  }
+  return H;
+}
+
 // Sources of wave w of a group of n elements on GW waves.
 template <int GW>
 __device__ __forceinline__ void group_share(int n, int w, int& j0, int& j1) {
@@ -1005,7 +1071,7 @@ __device__ __forceinline__ void group_share(int n, int w, int& j0, int& j1) {
 // sets ctl to 0 between the first two barriers of the round after its last
 // step, so every wave passes the same barriers and leaves together.
 template <int GW>
-__device__ __noinline__ void group_helper(GroupLds<GW>& L, int n, int w) {
+__device__ __noinline__ void group_helper(GroupLds<GW>& L, int n, int w, const double* pt) {
   const int e = (int)(threadIdx.x & 63);
   int j0, j1;
   group_share<GW>(n, w, j0, j1);
@@ -1015,7 +1081,7 @@ __device__ __noinline__ void group_helper(GroupLds<GW>& L, int n, int w) {
     __syncthreads();  // m and ctl are written
     if (L.ctl == 0) break;
     if (e < n) {
-      const V3<double> h = group_field<GW>(L, j0, j1, e);
+      const V3<double> h = pt ? group_field_tensor<GW>(L, pt, n, j0, j1, e) : group_field<GW>(L, j0, j1, e);
       L.part[w - 1][e][0] = h.x;
       L.part[w - 1][e][1] = h.y;
       L.part[w - 1][e][2] = h.z;
@@ -1043,6 +1109,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
   [[maybe_unused]] const cmtj_group* gp = nullptr;
   [[maybe_unused]] int g_n = 0, g_e = 0, g_top = 0, g_kind = 0;
   [[maybe_unused]] GroupLds<GW>* gl = nullptr;
+  [[maybe_unused]] const double* g_pt = nullptr;  // this group's pair terms (tensor interaction)
   if constexpr (GRP) {
     // group blockIdx.x on a workgroup of GW waves, element e on lane e of
     // wave 0; lanes of wave 0 past the group's elements leave at once (a
@@ -1052,9 +1119,11 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     if (blockIdx.x >= g.group_n) return;
     gp = g.group + blockIdx.x;
     g_n = gp->n_elements;
+    if (gp->interaction == CMTJ_INTERACTION_TENSOR && g.pair)
+      g_pt = g.pair + (size_t)blockIdx.x * 12 * (size_t)g_n * (size_t)g_n;
     if constexpr (GW > 1) {
       if (threadIdx.x >= 64) {
-        group_helper<GW>(lds, g_n, (int)(threadIdx.x >> 6));
+        group_helper<GW>(lds, g_n, (int)(threadIdx.x >> 6), g_pt);
         return;
       }
     }
@@ -1197,7 +1266,8 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       gl->m[g_e][2] = mt.z;
       if (g_e == 0) gl->ctl = 1;
       __syncthreads();  // m and ctl are written
-      if (g_kind != CMTJ_INTERACTION_NULL) gH = group_field<GW>(*gl, 0, g_j1, g_e);
+      if (g_kind != CMTJ_INTERACTION_NULL)
+        gH = g_pt ? group_field_tensor<GW>(*gl, g_pt, g_n, 0, g_j1, g_e) : group_field<GW>(*gl, 0, g_j1, g_e);
       if constexpr (GW > 1) {
         __syncthreads();  // the other waves' partial fields are written
         if (g_kind != CMTJ_INTERACTION_NULL)
@@ -1207,7 +1277,19 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
     }
     [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
     [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
-    if constexpr (STK) stack_current<LM>(sl, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, sI, sIlog, sClog);
+    // sub-steps of the step: one, or one per member of a non-delayed stack
+    // (member q commits at sub-step q; every member computes each sub-step)
+    int reps = 1;
+    [[maybe_unused]] double s_eff = 1.0;
+    if constexpr (STK) {
+      if (sl.nd)
+        reps = s_n;
+      else
+        stack_current<LM>(sl, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, g.tables, i, sI, sIlog, sClog);
+    }
+    // one sub-step: the whole step, or member sq's turn in a non-delayed
+    // stack (act: this lane commits); true when the run stops here
+    auto sub_step = [&](bool act) __attribute__((always_inline)) -> bool {
     if constexpr (ANCHORED) {
       // RK4 of the reference (core/junction.hpp:1017-1041, neighbours frozen
       // at the start of the step) in anchored binary32: k1 and the anchor
@@ -1265,14 +1347,14 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       }
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        if (l < nl && nan_layer < 0) {
+        if (l < nl && nan_layer < 0 && act) {
           m[l] = mn[l];
           if (isnan(mn[l].x)) nan_layer = l;
         }
       }
       if (nan_layer >= 0) {
         code = CMTJ_ERR_NAN;
-        break;
+        return true;
       }
     } else if (SOLVER == CMTJ_RK4) {
       // rk4_step (core/junction.hpp:1017-1041) for every layer, neighbours
@@ -1337,7 +1419,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       // the reference updates layers in turn; the first NaN stops the run
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        if (l < nl && nan_layer < 0) {
+        if (l < nl && nan_layer < 0 && act) {
           m[l] = mn[l];
           if (isnan(mn[l].x)) nan_layer = l;
         }
@@ -1347,22 +1429,23 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
         // are one wave, its lanes leave the loop together
         if (__any(nan_layer >= 0)) {
           if (nan_layer >= 0) code = CMTJ_ERR_NAN;
-          break;
+          return true;
         }
       } else if constexpr (STK) {
         // a member's NaN ends its stack's run (the reference throws): every
-        // member of that stack stops at this step, the lanes of the stack
-        // leave the loop together, other stacks go on
+        // member of that stack stops at this step (a non-delayed stack's later
+        // members keep their start-of-step m, as in the reference), the lanes
+        // of the stack leave the loop together, other stacks go on
         const unsigned long long bad = __ballot(nan_layer >= 0);
         const unsigned long long mine =
             ((g.stack_pad >= 64) ? ~0ull : ((1ull << g.stack_pad) - 1)) << (unsigned)s_base;
         if (bad & mine) {
           if (nan_layer >= 0) code = CMTJ_ERR_NAN;
-          break;
+          return true;
         }
       } else if (nan_layer >= 0) {
         code = CMTJ_ERR_NAN;
-        break;
+        return true;
       }
     } else if (SOLVER == CMTJ_EULER_HEUN) {
       // eulerHeunSolverStep, core/junction.hpp:1709-1754
@@ -1393,7 +1476,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       }
 #pragma unroll
       for (int l = 0; l < LM; ++l)
-        if (l < nl) m[l] = normalize_m<R>(mn[l]);
+        if (l < nl && act) m[l] = normalize_m<R>(mn[l]);
     } else {
       // heunSolverStep, core/junction.hpp:1756-1815
       V3<R> mr_[LM], fn[LM], gn[LM], dW[LM], mN[LM];
@@ -1428,9 +1511,21 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
             const V3<R> g2 = stoch<R>(P[l], mN[l], dW[l]);
             acc = acc + cvt<double>((gn[l] + g2) * R(0.5) * sdt);
           }
-          m[l] = normalize_m<R>(acc);
+          if (act) m[l] = normalize_m<R>(acc);
         }
       }
+    }
+      return false;
+    };
+    if constexpr (STK) {
+      bool halt = false;
+      for (int sq = 0; sq < reps && !halt; ++sq) {
+        if (sl.nd) stack_current_nd<LM>(sl, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, sq, s_eff, sI, sIlog, sClog);
+        halt = sub_step(!sl.nd || s_j == sq);
+      }
+      if (halt) break;
+    } else {
+      if (sub_step(true)) break;
     }
     if ((uint64_t)i == next_log) {
       if (logging) write_entry<R, LM, FM, LOGE>(g, J, m, er, lp + (uint64_t)k * esize, cs, Rp, Rap, mcoef);

@@ -22,6 +22,76 @@ namespace cmtj {
 
 __device__ __forceinline__ double vlen(V3<double> v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
 
+// Step-size control of one layer: AdaptiveIntegrationParams
+// (core/junction.hpp:125-139) as dormand_prince_step applies it (:942-982).
+// The PID controller's integral and previous error live in the layer and
+// advance on every attempt, accepted or rejected.
+struct DpCtl {
+  double atol, rtol, fmaxv, fminv, safety, kp, ki, kd, integral, prev;
+  bool pid;
+  __device__ void load(const cmtj_layer& Ly) {
+    atol = Ly.dp_abs_tol;
+    rtol = Ly.dp_rel_tol;
+    fmaxv = Ly.dp_max_factor;
+    fminv = Ly.dp_min_factor;
+    safety = Ly.dp_safety;
+    pid = Ly.dp_use_pid != 0;
+    kp = Ly.dp_kp;
+    ki = Ly.dp_ki;
+    kd = Ly.dp_kd;
+    integral = Ly.dp_integral_error;
+    prev = Ly.dp_prev_error_ratio;
+  }
+  __device__ void unit() {
+    atol = rtol = fmaxv = fminv = safety = 1.0;
+    kp = ki = kd = integral = prev = 0.0;
+    pid = false;
+  }
+  // error ratio of an attempt from |m5 - m4| and |m_t| (:942-950)
+  __device__ double ratio(double err, double mlen) const {
+    const double tol = atol + rtol * mlen;
+    double r = err / tol;
+    if (tol < 1e-15 || isnan(r) || isinf(r)) r = 10.0;
+    return r;
+  }
+  // the step proposed after an attempt of step h with error ratio r (:952-982)
+  __device__ double propose(double h, double r) {
+    double factor;
+    if (pid) {
+      // the exponent's roundings spelled out (two fma, one product), so
+      // every kernel that inlines this rounds it alike
+      const double e = log(r);
+      integral += e;
+      factor = exp(fma(kp, e, fma(ki, integral, kd * (e - prev))));
+      prev = e;
+    } else {
+      factor = safety * pow(1.0 / r, 0.2);
+    }
+    if (isnan(factor) || isinf(factor)) factor = fminv;
+    factor = fmin(factor, fmaxv);
+    factor = fmax(factor, fminv);
+    double nh = h * factor;
+    if (isnan(nh) || isinf(nh) || nh <= 0) nh = h * fminv;
+    return nh;
+  }
+};
+
+// The final controller state of the layers after their magnetisations in
+// an instance's mag_out block (only when a layer runs the PID controller).
+template <int LM>
+__device__ __forceinline__ void store_pid(double* o, int nl, const DpCtl* c) {
+  bool any = false;
+#pragma unroll
+  for (int l = 0; l < LM; ++l) any |= (l < nl) && c[l].pid;
+  if (!any) return;
+#pragma unroll
+  for (int l = 0; l < LM; ++l)
+    if (l < nl) {
+      o[3 * nl + 2 * l] = c[l].integral;
+      o[3 * nl + 2 * l + 1] = c[l].prev;
+    }
+}
+
 template <uint32_t FM>
 __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
   const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
@@ -54,8 +124,8 @@ __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
     return rhs<double, FM, false>(P, bits, fl, mm, dv[0], zero, zero, er[0]);
   };
   const double T = J->total_time, wf = J->write_freq, h0 = J->dt;
-  const double abs_tol = Ly.dp_abs_tol, rel_tol = Ly.dp_rel_tol;
-  const double max_factor = Ly.dp_max_factor, min_factor = Ly.dp_min_factor, safety = Ly.dp_safety;
+  DpCtl ctl[1];
+  ctl[0].load(Ly);
   const uint32_t cap = J->n_steps, NE = J->n_entries;
   const bool logging = (g.log != nullptr) && g.log_mask != 0;
   const MRCoef mcoef[1] = {mr_coef(g, J, 0, logging)};
@@ -100,16 +170,8 @@ __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
                     k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0);
     const V3<double> m4 = mt + k1 * (5179.0 / 57600.0) + k3 * (7571.0 / 16695.0) + k4 * (393.0 / 640.0) +
                           k5 * (-92097.0 / 339200.0) + k6 * (187.0 / 2100.0) + k7 * (1.0 / 40.0);
-    const double err = vlen(m5 - m4);
-    const double tol = abs_tol + rel_tol * vlen(mt);
-    double ratio = err / tol;
-    if (tol < 1e-15 || isnan(ratio) || isinf(ratio)) ratio = 10.0;
-    double factor = safety * pow(1.0 / ratio, 0.2);
-    if (isnan(factor) || isinf(factor)) factor = min_factor;
-    factor = fmin(factor, max_factor);
-    factor = fmax(factor, min_factor);
-    double nh = h * factor;
-    if (isnan(nh) || isinf(nh) || nh <= 0) nh = h * min_factor;
+    const double ratio = ctl[0].ratio(vlen(m5 - m4), vlen(mt));
+    const double nh = ctl[0].propose(h, ratio);
     bool acc;
     if (ratio <= 1.0) {
       m5 = normalize_exact(m5);
@@ -150,6 +212,7 @@ __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
     o[0] = m[0].x;
     o[1] = m[0].y;
     o[2] = m[0].z;
+    store_pid<1>(o, 1, ctl);
   }
   if (g.status) {
     cmtj_status s;
@@ -186,22 +249,18 @@ __global__ __launch_bounds__(64) void k_dp_ml(GroupArgs g) {
   LP<double> P[LM];
   DV<double> base[LM];
   V3<double> m[LM];
-  double atol[LM], rtol[LM], fmaxv[LM], fminv[LM], safety[LM];
+  DpCtl ctl[LM];
 #pragma unroll
   for (int l = 0; l < LM; ++l) {
     if (l < nl) {
       const cmtj_layer& Ly = g.layers[J->layer_offset + l];
       load_layer<double>(Ly, g.c, P[l], base[l], m[l]);
-      atol[l] = Ly.dp_abs_tol;
-      rtol[l] = Ly.dp_rel_tol;
-      fmaxv[l] = Ly.dp_max_factor;
-      fminv[l] = Ly.dp_min_factor;
-      safety[l] = Ly.dp_safety;
+      ctl[l].load(Ly);
     } else {
       dv_zero(base[l]);
       m[l] = mk<double>(0, 0, 0);
       P[l] = P[0];
-      atol[l] = rtol[l] = fmaxv[l] = fminv[l] = safety[l] = 1.0;
+      ctl[l].unit();
     }
   }
   DynRec dr[DREG];
@@ -273,17 +332,8 @@ __global__ __launch_bounds__(64) void k_dp_ml(GroupArgs g) {
                         k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0);
         const V3<double> m4 = mt + k1 * (5179.0 / 57600.0) + k3 * (7571.0 / 16695.0) + k4 * (393.0 / 640.0) +
                               k5 * (-92097.0 / 339200.0) + k6 * (187.0 / 2100.0) + k7 * (1.0 / 40.0);
-        const double err = vlen(m5 - m4);
-        const double tol = atol[l] + rtol[l] * vlen(mt);
-        double ratio = err / tol;
-        if (tol < 1e-15 || isnan(ratio) || isinf(ratio)) ratio = 10.0;
-        double factor = safety[l] * pow(1.0 / ratio, 0.2);
-        if (isnan(factor) || isinf(factor)) factor = fminv[l];
-        factor = fmin(factor, fmaxv[l]);
-        factor = fmax(factor, fminv[l]);
-        double nh = h * factor;
-        if (isnan(nh) || isinf(nh) || nh <= 0) nh = h * fminv[l];
-        h = nh;  // the next layer steps with this layer's proposal
+        const double ratio = ctl[l].ratio(vlen(m5 - m4), vlen(mt));
+        h = ctl[l].propose(h, ratio);  // the next layer steps with this layer's proposal
         if (ratio <= 1.0) {
           m5 = normalize_exact(m5);
           m[l] = m5;
@@ -326,6 +376,7 @@ __global__ __launch_bounds__(64) void k_dp_ml(GroupArgs g) {
         o[3 * l + 1] = m[l].y;
         o[3 * l + 2] = m[l].z;
       }
+    store_pid<LM>(o, nl, ctl);
   }
   if (g.status) {
     cmtj_status s;
@@ -364,7 +415,9 @@ __global__ __launch_bounds__(64) void k_dp_q(GroupArgs g, uint32_t* queue) {
   FieldWeights<double, 1> fw;
   ERec<double> er[1];
   MRCoef mcoef[1];
-  double T = 0, wf = 1, h0 = 0, abs_tol = 0, rel_tol = 0, max_factor = 1, min_factor = 1, safety = 1;
+  double T = 0, wf = 1, h0 = 0;
+  DpCtl ctl[1];
+  ctl[0].unit();
   uint32_t cap = 0, NE = 0;
   char* lp = nullptr;
   uint64_t cs = 0;
@@ -389,11 +442,7 @@ __global__ __launch_bounds__(64) void k_dp_q(GroupArgs g, uint32_t* queue) {
     T = J->total_time;
     wf = J->write_freq;
     h0 = J->dt;
-    abs_tol = Ly.dp_abs_tol;
-    rel_tol = Ly.dp_rel_tol;
-    max_factor = Ly.dp_max_factor;
-    min_factor = Ly.dp_min_factor;
-    safety = Ly.dp_safety;
+    ctl[0].load(Ly);
     cap = J->n_steps;
     NE = J->n_entries;
     mcoef[0] = mr_coef(g, J, 0, logging);
@@ -412,6 +461,7 @@ __global__ __launch_bounds__(64) void k_dp_q(GroupArgs g, uint32_t* queue) {
       o[0] = m[0].x;
       o[1] = m[0].y;
       o[2] = m[0].z;
+      store_pid<1>(o, 1, ctl);
     }
     if (g.status) {
       cmtj_status s;
@@ -462,16 +512,8 @@ __global__ __launch_bounds__(64) void k_dp_q(GroupArgs g, uint32_t* queue) {
                     k5 * (-2187.0 / 6784.0) + k6 * (11.0 / 84.0);
     const V3<double> m4 = mt + k1 * (5179.0 / 57600.0) + k3 * (7571.0 / 16695.0) + k4 * (393.0 / 640.0) +
                           k5 * (-92097.0 / 339200.0) + k6 * (187.0 / 2100.0) + k7 * (1.0 / 40.0);
-    const double err = vlen(m5 - m4);
-    const double tol = abs_tol + rel_tol * vlen(mt);
-    double ratio = err / tol;
-    if (tol < 1e-15 || isnan(ratio) || isinf(ratio)) ratio = 10.0;
-    double factor = safety * pow(1.0 / ratio, 0.2);
-    if (isnan(factor) || isinf(factor)) factor = min_factor;
-    factor = fmin(factor, max_factor);
-    factor = fmax(factor, min_factor);
-    double nh = h * factor;
-    if (isnan(nh) || isinf(nh) || nh <= 0) nh = h * min_factor;
+    const double ratio = ctl[0].ratio(vlen(m5 - m4), vlen(mt));
+    const double nh = ctl[0].propose(h, ratio);
     bool acc;
     if (ratio <= 1.0) {
       m5 = normalize_exact(m5);

@@ -41,6 +41,7 @@ enum Feature : uint32_t {
   F_EXACT = 1u << 16,     // the group's features are exactly the kernel's: feature tests are compile-time
   F_KSTEP = 1u << 17,     // every time-dependent record is a step driver
   F_KSINE = 1u << 18,     // every time-dependent record is a sine driver
+  F_KTRAP = 1u << 19,     // every time-dependent record is a trapezoid driver
 };
 
 template <class R>
@@ -109,6 +110,15 @@ static __host__ __device__ __noinline__ double wave_part_trans(int type, double 
   }
 }
 
+// trapezoidDriver (drivers.hpp:90-108): p1 start, p2 edge, p3 steady
+CMTJ_HD double wave_trapezoid(double A, double p1, double p2, double p3, double t) {
+  if (t < p1) return 0.0;
+  if (t <= p1 + p2) return (A / p2) * (t - p1);
+  if (t <= p1 + p2 + p3) return A;
+  if (t <= p1 + 2 * p2 + p3) return A - (A / p2) * (t - (p1 + p2 + p3));
+  return 0.0;
+}
+
 CMTJ_HD double wave_part(int type, double A, double p1, double p2, double p3, double t) {
   switch (type) {
     case CMTJ_PULSE: {  // p1 period, p2 cycle (pulseTrain, drivers.hpp:79-88)
@@ -119,13 +129,8 @@ CMTJ_HD double wave_part(int type, double A, double p1, double p2, double p3, do
     }
     case CMTJ_STEP:  // p1 start, p2 stop, both inclusive (stepUpdate, :73-78)
       return (t >= p1 && t <= p2) ? A : 0.0;
-    case CMTJ_TRAPEZOID: {  // p1 start, p2 edge, p3 steady (:90-108)
-      if (t < p1) return 0.0;
-      if (t <= p1 + p2) return (A / p2) * (t - p1);
-      if (t <= p1 + p2 + p3) return A;
-      if (t <= p1 + 2 * p2 + p3) return A - (A / p2) * (t - (p1 + p2 + p3));
-      return 0.0;
-    }
+    case CMTJ_TRAPEZOID:
+      return wave_trapezoid(A, p1, p2, p3, t);
     case CMTJ_SINE:
     case CMTJ_POSINE:
     case CMTJ_HALFSINE:

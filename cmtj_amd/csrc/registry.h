@@ -38,7 +38,7 @@ constexpr uint32_t FM_CPL = F_ANIS | F_IEC | F_DYN | F_DYNH | F_STT | F_SOT | F_
 constexpr uint32_t FM_PIMM = F_ANIS | F_IEC | F_DYNH | F_EXACT | F_KSTEP;          // 2-layer MTJ, Hoe step
 constexpr uint32_t FM_RH = F_ANIS | F_EXACT | F_KSTEP;                             // R(H) loop, static drive
 constexpr uint32_t FM_VSD = F_ANIS | F_DYNH | F_EXACT | F_KSINE;                   // sine Oersted drive
-constexpr uint32_t FM_CIMS = F_ANIS | F_DYN | F_SOT | F_THERMAL | F_EXACT;  // SOT pulse of any shape at T
+constexpr uint32_t FM_CIMS = F_ANIS | F_DYN | F_SOT | F_THERMAL | F_EXACT | F_KTRAP;  // SOT trapezoid pulse at T
 constexpr uint32_t FM_STO = F_ANIS | F_STT | F_EXACT | F_KSTEP;                    // DC spin-torque oscillator
 constexpr uint32_t FM_SOTK = F_ANIS | F_DYN | F_SOT | F_EXACT | F_KSTEP;            // SOT switching by a current step
 

@@ -113,12 +113,57 @@ class GroupInteraction:
 
     # --- run -------------------------------------------------------------------
     def _kind(self) -> int:
-        kind = _KINDS.get(self.interactionFunc)
-        if kind is None:
-            raise NotImplementedError("the GPU engine evaluates the interaction on the device: only "
-                                      "nullDipoleInteraction, computeDipoleInteraction and "
-                                      "computeDipoleInteractionNoumra are provided, not a Python callable")
-        return kind
+        # the module's three functions run on the device from the positions;
+        # any other callable is tabulated (_pair_terms)
+        return _KINDS.get(self.interactionFunc, _lib.INTERACTION_TENSOR)
+
+    def _pair_terms(self) -> np.ndarray:
+        """The interaction callable as per-pair affine terms: for every
+        ordered pair (i, j), f(r_i, r_j, layer_i, layer_j) = T_ij m_j + b_ij,
+        read off calls with m_j = +-x, +-y, +-z (T's columns are the odd
+        parts, b the even part) on copies of the top layers, and checked at
+        random unit m_j and m_i: a callable that is not affine in m_j, or
+        that reads m_i, raises NotImplementedError (the device cannot call
+        Python; the reference calls the function per pair and step,
+        core/reservoir.hpp:135-142).  Returns [n][12][n] ([j][k][i]: T_ij
+        row-major for k < 9, b_ij for k >= 9), zero on the diagonal."""
+        f = self.interactionFunc
+        n = self.noElements
+        tops = [copy.deepcopy(j.getLayer(self.topId)) for j in self.junctionList]
+        rng = np.random.default_rng(0x5EED)
+        out = np.zeros((n, 12, n), dtype=np.float64)
+        eye = np.eye(3)
+        for i in range(n):
+            for j in range(n):
+                if i == j:
+                    continue
+                li, lj = tops[i], tops[j]
+
+                def call(mj, mi=None):
+                    lj.mag = CVector(*[float(x) for x in mj])
+                    if mi is not None:
+                        li.mag = CVector(*[float(x) for x in mi])
+                    return _vec(f(self.coordinateMatrix[i], self.coordinateMatrix[j], li, lj))
+
+                fp = [call(eye[k]) for k in range(3)]
+                fm = [call(-eye[k]) for k in range(3)]
+                T = np.stack([(fp[k] - fm[k]) * 0.5 for k in range(3)], axis=1)
+                b = sum((fp[k] + fm[k]) * 0.5 for k in range(3)) / 3.0
+                for _ in range(2):
+                    u = rng.normal(size=3)
+                    w = rng.normal(size=3)
+                    u /= np.linalg.norm(u)
+                    w /= np.linalg.norm(w)
+                    got = call(u, w)
+                    want = T @ u + b
+                    scale = float(np.abs(T).sum() + np.abs(b).sum() + np.abs(got).sum())
+                    if not np.all(np.abs(got - want) <= 1e-12 * scale + 1e-300):
+                        raise NotImplementedError(
+                            "the interaction function is not affine in the source layer's magnetisation "
+                            "(or reads the target's): the GPU engine applies f = T(r_i, r_j) m_j + b per pair")
+                out[j, :9, i] = T.ravel()
+                out[j, 9:, i] = b
+        return out
 
     def _pack(self, totalTime: float, timeStep: float, writeFrequency: float):
         """Descriptors of one run: (group record, element records with layer
@@ -158,10 +203,11 @@ class GroupInteraction:
             nch = _lib.n_channels(int(jd["n_layers"]), int(jd["mr_mode"]), int(jd["log_mask"]))
             nchs.append(nch)
             off += nch * int(jd["n_entries"])
-        return g, js, np.concatenate(layers), nchs, n_steps, W, off
+        pt = self._pair_terms() if (kind == _lib.INTERACTION_TENSOR and n_steps > 0) else None
+        return g, js, np.concatenate(layers), nchs, n_steps, W, off, pt
 
-    def _execute(self, g, js, layers, consts):
-        return _lib.run_groups(g, js, layers, consts)
+    def _execute(self, g, js, layers, consts, pair_terms=None):
+        return _lib.run_groups(g, js, layers, consts, pair_terms)
 
     def _finish(self, js, log, mag, status, nchs, W: int, timeStep: float) -> None:
         """Elements take over their logs and final state."""
@@ -183,10 +229,10 @@ class GroupInteraction:
         """GroupInteraction::runSimulation (:181-221): each step the group
         field from the top layers at the step's start, then one step of every
         element; every element logs each write_every steps."""
-        g, js, layers, nchs, n_steps, W, _ = self._pack(totalTime, timeStep, writeFrequency)
+        g, js, layers, nchs, n_steps, W, _, pt = self._pack(totalTime, timeStep, writeFrequency)
         if n_steps == 0:
             return
-        log, mag, status = self._execute(g, js, layers, _lib.constants_get())
+        log, mag, status = self._execute(g, js, layers, _lib.constants_get(), pt)
         self._finish(js, log, mag, status, nchs, W, timeStep)
 
 
@@ -203,9 +249,16 @@ def run_groups(groups, totalTime: float, timeStep: float = 1e-13, writeFrequency
     if n_steps == 0:
         return
     recs = np.stack([p[0] for p in packs])
+    n = groups[0].noElements
+    pts = None
+    if any(p[7] is not None for p in packs):  # every group's slot of the pair terms, zeros for the others
+        pts = np.zeros((len(packs), n, 12, n), dtype=np.float64)
+        for k, p in enumerate(packs):
+            if p[7] is not None:
+                pts[k] = p[7]
     js_all, layers_all = [], []
     rows = off = 0
-    for g_, js, layers, nchs, _ns, _w, size in packs:
+    for g_, js, layers, nchs, _ns, _w, size, _pt in packs:
         js = js.copy()
         js["layer_offset"] += rows
         js["log_offset"] += off
@@ -214,10 +267,10 @@ def run_groups(groups, totalTime: float, timeStep: float = 1e-13, writeFrequency
         js_all.append(js)
         layers_all.append(layers)
     js_all = np.concatenate(js_all)
-    log, mag, status = _lib.run_groups(recs, js_all, np.concatenate(layers_all), _lib.constants_get())
+    log, mag, status = groups[0]._execute(recs, js_all, np.concatenate(layers_all), _lib.constants_get(), pts)
     nan = False
     base = 0
-    for grp, (g_, js, layers, nchs, _ns, _w, size) in zip(groups, packs):
+    for grp, (g_, js, layers, nchs, _ns, _w, size, _pt) in zip(groups, packs):
         n = grp.noElements
         sl = slice(base, base + n)
         jsq = js_all[sl]

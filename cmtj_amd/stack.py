@@ -140,17 +140,20 @@ class _Stack:
                 f.write("".join(f"{v[i]};" for v in log.values()) + "\n")
 
     # --- run -------------------------------------------------------------------
-    def _execute(self, stack, js, layers, consts):
-        return _lib.run_stacks(stack, js, layers, consts)
+    def _execute(self, stack, js, layers, consts, tables=None):
+        return _lib.run_stacks(stack, js, layers, consts, tables)
 
     def _pack(self, totalTime: float, timeStep: float, writeFrequency: float):
         """Descriptors of one run: (stack record, member records with layer
-        and log offsets from 0, layer rows, channels per member, steps, W)."""
-        if not self.useKCL and not self.delayed:
-            raise NotImplementedError("non-delayed coupling (each member seeing the previous member's updated "
-                                      "state within a step) is not provided by the GPU engine")
-        if self.currentDriver.update == UpdateType.custom:
-            raise NotImplementedError("a callback stack current is not provided by the GPU engine")
+        and log offsets from 0, layer rows, channels per member, steps, W,
+        log size, callback tables or None)."""
+        callback = self.currentDriver.update == UpdateType.custom
+        if callback and not self.useKCL:
+            # runSimulationNonKCL scales the driver with ScalarDriver::operator*
+            # (core/drivers.hpp:313-325), which builds a driver without the
+            # callback: the reference then fails in the callback call
+            raise RuntimeError("Error in Python callback: non-KCL coupling scales the stack current, "
+                               "which drops a callback driver")
         # initializeSimulation, :36-62 (the time-step check first)
         packed = [j._descriptor(totalTime, timeStep, writeFrequency, False, SolverMode.RK4)
                   for j in self.junctionList]
@@ -159,8 +162,16 @@ class _Stack:
             raise RuntimeError("Junctions have different solver modes!"
                                " Set the same solver mode for all junctions explicitly."
                                " Do not mix stochastic and deterministic solvers!")
-        if any(p[2] is not None and len(p[2]) for p in packed):
-            raise NotImplementedError("callback drivers on stack members are not provided by the GPU engine")
+        # the members' callback samples in one table, each member's indices
+        # moved past the previous members' samples
+        tabs, toff = [], 0
+        for p in packed:
+            if p[2] is not None and len(p[2]):
+                drv = p[1]["drv"]
+                sel = drv["type"] == _lib.TABLE
+                drv["table_index"][sel] += toff
+                tabs.append(np.asarray(p[2], dtype=np.float64))
+                toff += len(p[2])
         two = self.isTwoLayerMemberStack()
         n_steps, W = packed[0][3], packed[0][4]
         n = self.stackSize
@@ -169,12 +180,28 @@ class _Stack:
         st = np.zeros((), dtype=_lib.STACK_DTYPE)
         st["series"] = 1 if self._series else 0
         st["kcl"] = 1 if self.useKCL else 0
+        st["delayed"] = 1 if self.delayed else 0
         st["n_junctions"] = n
         st["two_layer"] = 1 if two else 0
         st["phase_offset"] = self.phaseOffset
         if not two and n_steps > 0:
             st["pol"] = self.getPolarisationVector().tolist()
-        st["current"] = self.currentDriver._record()
+        if callback:
+            # KCL reads the stack driver once per step, at the step's start
+            # time (runSimulationKCL, :376): one sample per step
+            from .core import _call_custom
+
+            t = np.arange(n_steps, dtype=np.float64) * timeStep
+            vals = np.fromiter((_call_custom(self.currentDriver._callback, float(x)) for x in t), dtype=np.float64,
+                               count=n_steps)
+            rec = np.zeros((), dtype=_lib.DRIVER_DTYPE)
+            rec["type"] = _lib.TABLE
+            rec["table_index"] = toff
+            st["current"] = rec
+            tabs.append(vals)
+            toff += n_steps
+        else:
+            st["current"] = self.currentDriver._record()
         for q in range(n - 1):
             st["coupling"][q] = self.getCoupling(q)
         for q, j in enumerate(self.junctionList):
@@ -195,7 +222,7 @@ class _Stack:
             nch = _lib.n_channels(int(jd["n_layers"]), int(jd["mr_mode"]), int(jd["log_mask"]))
             nchs.append(nch)
             off += (nch + 2) * int(jd["n_entries"])
-        return st, js, np.concatenate(layers), nchs, n_steps, W, off
+        return st, js, np.concatenate(layers), nchs, n_steps, W, off, (np.concatenate(tabs) if tabs else None)
 
     def _finish(self, js, log, mag, status, nchs, W: int, timeStep: float) -> None:
         """Members take over their logs and final state; the stack log."""
@@ -217,10 +244,10 @@ class _Stack:
     def runSimulation(self, totalTime: float, timeStep: float = 1e-13, writeFrequency: float = 1e-11) -> None:
         """Stack<T>::runSimulation (:296-300): KCL or non-KCL coupling, every
         member stepping with RK4 (Euler-Heun when stochastic)."""
-        st, js, layers, nchs, n_steps, W, _ = self._pack(totalTime, timeStep, writeFrequency)
+        st, js, layers, nchs, n_steps, W, _, tables = self._pack(totalTime, timeStep, writeFrequency)
         if n_steps == 0:
             return
-        log, mag, status = self._execute(st, js, layers, _lib.constants_get())
+        log, mag, status = self._execute(st, js, layers, _lib.constants_get(), tables)
         self._finish(js, log, mag, status, nchs, W, timeStep)
 
     def _stack_resistance(self, R: list) -> np.ndarray:
@@ -294,18 +321,26 @@ def run_stacks(stacks, totalTime: float, timeStep: float = 1e-13, writeFrequency
     if parts[0][4] == 0:
         return
     recs = np.stack([p[0] for p in parts])
-    js, layers = [], []
-    rows = off = 0
-    for st, jq, L, _n, _s, _w, size in parts:
+    js, layers, tabs = [], [], []
+    rows = off = toff = 0
+    for k, (st, jq, L, _n, _s, _w, size, tab) in enumerate(parts):
         jq = jq.copy()
         jq["layer_offset"] += rows
         jq["log_offset"] += off
         js.append(jq)
+        if tab is not None and len(tab):  # this stack's table indices follow the previous stacks' samples
+            drv = L["drv"]
+            drv["table_index"][drv["type"] == _lib.TABLE] += toff
+            if int(recs[k]["current"]["type"]) == _lib.TABLE:
+                recs[k]["current"]["table_index"] += toff
+            tabs.append(tab)
+            toff += len(tab)
         layers.append(L)
         rows += len(L)
         off += size
     js = np.concatenate(js)
-    log, mag, status = stacks[0]._execute(recs, js, np.concatenate(layers), _lib.constants_get())
+    log, mag, status = stacks[0]._execute(recs, js, np.concatenate(layers), _lib.constants_get(),
+                                          np.concatenate(tabs) if tabs else None)
     failed = False
     for k, (s, p) in enumerate(zip(stacks, parts)):
         n = s.stackSize

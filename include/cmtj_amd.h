@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define CMTJ_ABI_VERSION 5
+#define CMTJ_ABI_VERSION 6
 #define CMTJ_MAX_LAYERS 8
 #define CMTJ_MAX_STACK 64 /* junctions of one coupled stack (one wave) */
 #define CMTJ_MAX_GROUP 64 /* elements of one dipole-coupled group (one workgroup) */
@@ -150,8 +150,17 @@ typedef struct cmtj_layer {
   int32_t temperature_set; /* thermal (stochastic) field active */
   cmtj_driver drv[CMTJ_NSLOT];
   /* Dormand-Prince step control (AdaptiveIntegrationParams,
-   * core/junction.hpp:125-139; the standard controller) */
+   * core/junction.hpp:125-139).  dp_use_pid selects the PID controller of
+   * dormand_prince_step (:953-961): every attempt sets e = log(error ratio),
+   * integral += e, factor = exp(kp e + ki integral + kd (e - prev)), prev = e.
+   * integral and prev are state that the reference keeps in the layer from
+   * run to run; a run starts from the values given here and reports the
+   * final ones after the magnetisations (cmtj_run, cmtj_plan_execute). */
   double dp_abs_tol, dp_rel_tol, dp_max_factor, dp_min_factor, dp_safety;
+  int32_t dp_use_pid;
+  int32_t dp_pad;
+  double dp_kp, dp_ki, dp_kd;
+  double dp_integral_error, dp_prev_error_ratio;
 } cmtj_layer;
 
 /* One junction instance of a run.  n_steps and write_every are the FP64
@@ -279,6 +288,9 @@ int cmtj_driver_eval(const cmtj_driver* d, double t, double* out);
  * (core/junction.hpp:1948-1981) with the log of logLayerParams (:1580-1636)
  * written to `log_host` as [channel][entry] doubles (capacity log_cap
  * elements), the final magnetisation of every layer to mag_out_host[3*L].
+ * A Dormand-Prince run with dp_use_pid set in any layer also writes the
+ * final PID state (integral, prev) of layer l to mag_out_host[3*L + 2*l]
+ * and [3*L + 2*l + 1]: give it 5*L doubles.
  * tables/n_table: samples of CMTJ_TABLE drivers (may be NULL/0).
  * Returns 0 on success (st->code tells NaN failures), nonzero on error. */
 int cmtj_run(const cmtj_junction* j, const cmtj_layer* layers,
@@ -324,12 +336,19 @@ int cmtj_run_noalloc(const cmtj_junction* j, const cmtj_layer* layers,
  * (series) or phase_offset (parallel), evaluated at its stage times. */
 typedef struct cmtj_stack {
   int32_t series;      /* 1: SeriesStack, 0: ParallelStack */
-  int32_t kcl;         /* 1: KCL, 0: non-KCL (delayed coupling) */
+  int32_t kcl;         /* 1: KCL, 0: non-KCL */
   int32_t n_junctions; /* 2..CMTJ_MAX_STACK */
   int32_t two_layer;   /* 1: p_j = bottom layer of junction j, 0: p_j = pol */
+  int32_t delayed;     /* non-KCL only.  1: every pair coupling from the
+                          magnetisations at the start of the step; 0
+                          (setDelayed(false), two-layer members): members
+                          step in order, member j's coupling to j-1 from
+                          j-1's magnetisations after its step (:334-343) */
+  int32_t pad_;
   double phase_offset;
   double pol[3];
-  cmtj_driver current; /* setCoupledCurrentDriver (:196-202); not CMTJ_TABLE */
+  cmtj_driver current; /* setCoupledCurrentDriver (:196-202); CMTJ_TABLE
+                          (a Python callback) under KCL only */
   double coupling[CMTJ_MAX_STACK]; /* pair (j, j+1), j < n_junctions - 1 */
   int32_t top[CMTJ_MAX_STACK];     /* layer index of topId in junction j */
   int32_t bottom[CMTJ_MAX_STACK];  /* layer index of bottomId (two-layer members) */
@@ -348,9 +367,16 @@ typedef struct cmtj_stack {
  * mag_out_host: [n_stacks * members][8][3], st_host: [n_stacks * members].
  * A NaN in a member stops every member of its stack at that step.  One stack
  * replaces Stack<T>::runSimulation (:296-300); a batch replaces a loop of
- * them over a parameter sweep. */
+ * them over a parameter sweep.  tables/n_table (may be NULL/0): samples of
+ * the members' CMTJ_TABLE drivers, as for cmtj_run, and of a KCL stack's
+ * CMTJ_TABLE current: tables[current.table_index + i] is its value at step
+ * i's start time i*dt (KCL reads the driver once per step, :376).  A
+ * non-KCL stack needs a closed-form current: the reference scales it with
+ * ScalarDriver::operator* (core/drivers.hpp:313-325), which drops a
+ * callback. */
 int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_junction* js,
                     const cmtj_layer* layers, size_t n_layer_rows, const cmtj_constants* c,
+                    const double* tables, size_t n_table,
                     double* log_host, size_t log_cap, double* mag_out_host, cmtj_status* st_host);
 
 /* ---- dipole-coupled groups (FP64, host buffers) ------------------------- */
@@ -358,7 +384,12 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
 enum cmtj_interaction {
   CMTJ_INTERACTION_NULL = 0,   /* nullDipoleInteraction (core/reservoir.hpp:63-68) */
   CMTJ_INTERACTION_KANAO = 1,  /* computeDipoleInteraction (:79-96) */
-  CMTJ_INTERACTION_NOMURA = 2  /* computeDipoleInteractionNoumra (:106-119) */
+  CMTJ_INTERACTION_NOMURA = 2, /* computeDipoleInteractionNoumra (:106-119) */
+  /* any interaction affine in the source's magnetisation,
+   * f(r_i, r_j, layer_i, layer_j) = T_ij m_j + b_ij (the reference's
+   * interactionFunction given as a Python callable, :58-62, :181): T and b
+   * of every ordered pair come with the run (cmtj_run_groups pair_terms) */
+  CMTJ_INTERACTION_TENSOR = 3
 };
 
 /* A GroupInteraction (core/reservoir.hpp:121-230): junctions at fixed
@@ -381,9 +412,17 @@ typedef struct cmtj_group {
  * are replaced by the group field.  js[q].log_offset places element q's
  * cmtj_run-style log block; mag_out_host [n][8][3], st_host [n].  A NaN in
  * an element stops its group at that step.  One group replaces
- * GroupInteraction::runSimulation (:181-221); a batch, a loop of them. */
+ * GroupInteraction::runSimulation (:181-221); a batch, a loop of them.
+ * pair_terms (n_pair_terms doubles; may be NULL/0 unless a group's
+ * interaction is CMTJ_INTERACTION_TENSOR): [n_groups][n][12][n] doubles,
+ * element [g][j][k][i] = for k < 9 the row-major T_ij[k], for k >= 9
+ * b_ij[k - 9], of the field on element i from element j of group g; each
+ * step adds f_j = T_ij m_j + b_ij (evaluated as fma(T0, mx, fma(T1, my,
+ * fma(T2, mz, b0))) per row) to the field, j ascending; the diagonal j = i
+ * is skipped. */
 int cmtj_run_groups(const cmtj_group* groups, size_t n_groups, const cmtj_junction* js,
                     const cmtj_layer* layers, size_t n_layer_rows, const cmtj_constants* c,
+                    const double* pair_terms, size_t n_pair_terms,
                     double* log_host, size_t log_cap, double* mag_out_host, cmtj_status* st_host);
 
 /* ---- batched plans: many uneven instances, device buffers -------------- */
@@ -411,7 +450,9 @@ typedef struct cmtj_plan_info {
   int32_t max_layers;       /* largest layer count in the batch */
   int32_t n_kernels;        /* kernel launches per execute */
   int32_t lanes_per_instance;
-  int32_t reserved;
+  int32_t fp64_groups;      /* CMTJ_FP32 plans: groups that run a binary64 kernel
+                               (spin torques or scalar drives on one lane per
+                               component, where no binary32 kernel covers them) */
   char kernel_name[128];
 } cmtj_plan_info;
 
@@ -440,7 +481,9 @@ cmtj_plan* cmtj_plan_create(const cmtj_junction* js, size_t n,
  * d_log: device log buffer (f64 or f32 per the plan) of log_capacity
  * elements (checked against the plan's log extent), d_mag_out: device
  * [n][CMTJ_MAX_LAYERS][3] doubles in the caller's junction order (may be
- * NULL), d_status: device [n] cmtj_status (may be NULL).  stream: a
+ * NULL; an L-layer Dormand-Prince instance with dp_use_pid set in any
+ * layer puts the final PID state (integral, prev) of layer l at elements
+ * 3L + 2l and 3L + 2l + 1 of its block), d_status: device [n] cmtj_status (may be NULL).  stream: a
  * hipStream_t (NULL = default stream).  Asynchronous; allocates nothing. */
 int cmtj_plan_execute(cmtj_plan* p, void* d_log, size_t log_capacity,
                       double* d_mag_out, cmtj_status* d_status, void* stream);

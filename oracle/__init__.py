@@ -54,8 +54,8 @@ def _bind(path):
     lib.oracle_thermal_normals.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
                                            ctypes.c_uint32, P]
     lib.oracle_n_channels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    lib.oracle_run_stack.argtypes = [P, P, P, P, P, P, P]
-    lib.oracle_run_group.argtypes = [P, P, P, P, P, P, P]
+    lib.oracle_run_stack.argtypes = [P, P, P, P, P, P, P, P]
+    lib.oracle_run_group.argtypes = [P, P, P, P, P, P, P, P]
     return (lib, abi)
 
 
@@ -84,18 +84,24 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
-def run(j, layers, consts, tables=None):
-    """One junction; returns (log [ch, entries], mag [L, 3], status) like cmtj_run."""
+def run(j, layers, consts, tables=None, pid_out=None):
+    """One junction; returns (log [ch, entries], mag [L, 3], status) like
+    cmtj_run; pid_out ([L, 2]) receives the final PID controller state of a
+    Dormand-Prince run whose layers use it."""
     lib, abi = _load()
     j = np.ascontiguousarray(j, dtype=abi.JUNCTION_DTYPE).reshape(())
     layers = np.ascontiguousarray(layers, dtype=abi.LAYER_DTYPE)
     nch = lib.oracle_n_channels(int(j["n_layers"]), int(j["mr_mode"]), int(j["log_mask"]))
+    nl = int(j["n_layers"])
     log = np.zeros((nch, int(j["n_entries"])), dtype=np.float64)
-    mag = np.zeros((int(j["n_layers"]), 3), dtype=np.float64)
+    out = np.zeros(5 * nl, dtype=np.float64)
+    mag = out[: 3 * nl].reshape(nl, 3)
     st = np.zeros((), dtype=abi.STATUS_DTYPE)
     tabs = None if tables is None else np.ascontiguousarray(tables, dtype=np.float64)
     lib.oracle_run(_p(j), _p(layers), _p(np.ascontiguousarray(consts)), _p(tabs),
-                   0 if tabs is None else tabs.size, _p(log), _p(mag), _p(st))
+                   0 if tabs is None else tabs.size, _p(log), _p(out), _p(st))
+    if pid_out is not None and int(j["solver"]) == abi.DORMAND_PRINCE and np.any(layers["dp_use_pid"][:nl]):
+        pid_out[...] = out[3 * nl:].reshape(nl, 2)
     return log, mag, st
 
 
@@ -120,7 +126,7 @@ def run_batch(js, layers, consts, tables=None, log_elements=0, n_threads=None, l
     return log, mag, st
 
 
-def run_stack(stacks, js, layers, consts):
+def run_stack(stacks, js, layers, consts, tables=None):
     """Coupled stacks, same layout as cmtj_run_stacks (js[s * members + j]:
     member j of stack s), one after another: returns (log, mag [n, 8, 3],
     status [n])."""
@@ -136,20 +142,22 @@ def run_stack(stacks, js, layers, consts):
     mag = np.zeros((js.size, abi.MAX_LAYERS, 3), dtype=np.float64)
     st = np.zeros(js.size, dtype=abi.STATUS_DTYPE)
     consts = np.ascontiguousarray(consts)
+    tabs = None if tables is None or len(tables) == 0 else np.ascontiguousarray(tables, dtype=np.float64)
     n = int(stacks[0]["n_junctions"]) if stacks.size else 0
     for s in range(stacks.size):
         sub = js[s * n:(s + 1) * n]
-        rc = lib.oracle_run_stack(_p(stacks[s:s + 1]), _p(sub), _p(layers), _p(consts), _p(log),
+        rc = lib.oracle_run_stack(_p(stacks[s:s + 1]), _p(sub), _p(layers), _p(consts), _p(tabs), _p(log),
                                   _p(mag[s * n:(s + 1) * n]), _p(st[s * n:(s + 1) * n]))
         if rc:
             raise ValueError("oracle_run_stack: bad stack descriptor")
     return log[:need], mag, st
 
 
-def run_group(groups, js, layers, consts):
+def run_group(groups, js, layers, consts, pair_terms=None):
     """Dipole-coupled groups, same layout as cmtj_run_groups (js[g * elements
-    + e]: element e of group g), one after another: returns (log, mag [n, 8,
-    3], status [n])."""
+    + e]: element e of group g; pair_terms [groups][n][12][n] for tensor
+    interactions), one after another: returns (log, mag [n, 8, 3], status
+    [n])."""
     lib, abi = _load()
     groups = np.ascontiguousarray(groups, dtype=abi.GROUP_DTYPE).reshape(-1)
     js = np.ascontiguousarray(js, dtype=abi.JUNCTION_DTYPE)
@@ -163,9 +171,11 @@ def run_group(groups, js, layers, consts):
     st = np.zeros(js.size, dtype=abi.STATUS_DTYPE)
     consts = np.ascontiguousarray(consts)
     n = int(groups[0]["n_elements"]) if groups.size else 0
+    pt = None if pair_terms is None else np.ascontiguousarray(pair_terms, dtype=np.float64).reshape(-1)
     for g in range(groups.size):
-        rc = lib.oracle_run_group(_p(groups[g:g + 1]), _p(js[g * n:(g + 1) * n]), _p(layers), _p(consts), _p(log),
-                                  _p(mag[g * n:(g + 1) * n]), _p(st[g * n:(g + 1) * n]))
+        ptg = None if pt is None else pt[g * 12 * n * n:(g + 1) * 12 * n * n]
+        rc = lib.oracle_run_group(_p(groups[g:g + 1]), _p(js[g * n:(g + 1) * n]), _p(layers), _p(consts), _p(ptg),
+                                  _p(log), _p(mag[g * n:(g + 1) * n]), _p(st[g * n:(g + 1) * n]))
         if rc == 2:
             raise RuntimeError("Points are too close for stable dipole calculation")
         if rc:

@@ -173,6 +173,9 @@ typedef struct {
   int sps; /* table samples per step */
   vec3 mag;
   double cellVolume;
+  /* PID controller state of Dormand-Prince (AdaptiveIntegrationParams
+   * integral_error, prev_error_ratio; kept by the layer across runs) */
+  double pid_integral, pid_prev;
   /* values of the last field/torque evaluation (logged with energies) */
   double K_log, K2_log, Jbottom_log, Jtop_log, I_log;
   vec3 Hext, HIEC, HAnis, Hdemag, Hfl_v, Hdl_v;
@@ -420,6 +423,8 @@ static void load_layers(const cmtj_junction* j, const cmtj_layer* layers, const 
     Ls[l].sps = sps;
     Ls[l].mag = vload(layers[l].mag);
     Ls[l].cellVolume = layers[l].cell_surface * layers[l].thickness;
+    Ls[l].pid_integral = layers[l].dp_integral_error;
+    Ls[l].pid_prev = layers[l].dp_prev_error_ratio;
   }
 }
 
@@ -496,10 +501,12 @@ static void store_mag(const cmtj_junction* j, const olayer* Ls, double* mag_out)
   }
 }
 
-/* Layer::dormand_prince_step (core/junction.hpp:866-1002) with the standard
- * controller and the neighbours bottom/top (zero vectors at the stack ends,
- * :1688-1691).  Returns 1 if accepted; *h is replaced by the proposed next
- * step either way; *nan set on a NaN. */
+/* Layer::dormand_prince_step (core/junction.hpp:866-1002) with the
+ * neighbours bottom/top (zero vectors at the stack ends, :1688-1691) and the
+ * layer's step controller: the standard one, safety * (1/ratio)^(1/5)
+ * (:962-965), or the PID one (:953-961), whose state advances on every
+ * attempt, accepted or not.  Returns 1 if accepted; *h is replaced by the
+ * proposed next step either way; *nan set on a NaN. */
 static int dp_step(olayer* L, double time, double* h_io, vec3 bottom, vec3 top, int* nan) {
   const double h = *h_io;
   const vec3 m_t = L->mag;
@@ -530,7 +537,15 @@ static int dp_step(olayer* L, double time, double* h_io, vec3 bottom, vec3 top, 
   const double tol = L->p->dp_abs_tol + L->p->dp_rel_tol * mlen;
   double error_ratio = error_norm / tol;
   if (tol < 1e-15 || isnan(error_ratio) || isinf(error_ratio)) error_ratio = 10.0;
-  double factor = L->p->dp_safety * pow(1.0 / error_ratio, 0.2);
+  double factor;
+  if (L->p->dp_use_pid) {
+    const double e_n = log(error_ratio);
+    L->pid_integral += e_n;
+    factor = exp(fma(L->p->dp_kp, e_n, fma(L->p->dp_ki, L->pid_integral, L->p->dp_kd * (e_n - L->pid_prev))));
+    L->pid_prev = e_n;
+  } else {
+    factor = L->p->dp_safety * pow(1.0 / error_ratio, 0.2);
+  }
   if (isnan(factor) || isinf(factor)) factor = L->p->dp_min_factor;
   factor = fmin(factor, L->p->dp_max_factor);
   factor = fmax(factor, L->p->dp_min_factor);
@@ -630,6 +645,13 @@ int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_cons
   if (j->solver == CMTJ_DORMAND_PRINCE) {
     dp_run(j, Ls, log, st);
     store_mag(j, Ls, mag_out);
+    int pid = 0;
+    for (int l = 0; l < j->n_layers; ++l) pid |= layers[l].dp_use_pid;
+    if (pid) /* the layers' final controller state after the magnetisations */
+      for (int l = 0; l < j->n_layers; ++l) {
+        mag_out[3 * j->n_layers + 2 * l] = Ls[l].pid_integral;
+        mag_out[3 * j->n_layers + 2 * l + 1] = Ls[l].pid_prev;
+      }
     return 0;
   }
   uint32_t k = 0;
@@ -648,8 +670,9 @@ int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_cons
 
 /* ---- electrically coupled stacks (core/stack.hpp) ----------------------- */
 
-/* Stack<T>::runSimulationKCL (:372-410) and runSimulationNonKCL with delayed
- * coupling (:305-370) of SeriesStack / ParallelStack (:414-480).  Member q
+/* Stack<T>::runSimulationKCL (:372-410) and runSimulationNonKCL with
+ * delayed or non-delayed coupling (:305-370) of SeriesStack / ParallelStack
+ * (:414-480).  Member q
  * logs into log + js[q].log_offset: its junction channels, then the stack's
  * I_q and C_q_{q+1} (logStackData, :206-217).  Per step the coupling uses
  * the members' magnetisations at the start of the step (storeMagnetisations,
@@ -657,7 +680,8 @@ int oracle_run(const cmtj_junction* j, const cmtj_layer* layers, const cmtj_cons
  * local driver (executeJunctionStep, :78-91: setLayerCurrentDriver("all")),
  * then the member takes one step of its solver. */
 int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_layer* layers,
-                     const cmtj_constants* c, double* log, double* mag_out, cmtj_status* st) {
+                     const cmtj_constants* c, const double* tables, double* log, double* mag_out,
+                     cmtj_status* st) {
   const int n = s->n_junctions;
   if (n < 2 || n > CMTJ_MAX_STACK) return 1;
   cmtj_layer* own = (cmtj_layer*)malloc(sizeof(cmtj_layer) * CMTJ_MAX_LAYERS * (size_t)n);
@@ -670,7 +694,7 @@ int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_la
   for (int q = 0; q < n; ++q) {
     const cmtj_junction* j = &js[q];
     memcpy(&own[q * CMTJ_MAX_LAYERS], layers + j->layer_offset, sizeof(cmtj_layer) * (size_t)j->n_layers);
-    load_layers(j, &own[q * CMTJ_MAX_LAYERS], c, NULL, &Ls[q * CMTJ_MAX_LAYERS]);
+    load_layers(j, &own[q * CMTJ_MAX_LAYERS], c, tables, &Ls[q * CMTJ_MAX_LAYERS]);
     st[q].code = CMTJ_OK;
     st[q].steps_done = 0;
     st[q].entries_done = 0;
@@ -693,7 +717,8 @@ int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_la
     for (int q = 0; q + 1 < n; ++q) C[q] = s->coupling[q] * (s->series ? d[q] + d[q + 1] : d[q] - d[q + 1]);
     cmtj_driver local[CMTJ_MAX_STACK];
     if (s->kcl) {
-      const double I0 = oracle_driver_value(&s->current, t, i, 0, 1, NULL);
+      /* a callback current: one sample per step (getCurrentScalarValue(t), :376) */
+      const double I0 = oracle_driver_value(&s->current, t, i, 0, 1, tables);
       double total = I0;
       for (int q = 0; q + 1 < n; ++q) {
         total += C[q] * I0;
@@ -717,8 +742,23 @@ int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js, const cmtj_la
         local[q].phase += s->series ? s->phase_offset * q : s->phase_offset; /* phaseShift */
       }
     }
+    double eff_nd = 1.0;
     for (int q = 0; q < n && !stop; ++q) {
       const cmtj_junction* j = &js[q];
+      if (!s->kcl && !s->delayed && q > 0) {
+        /* non-delayed coupling (:334-343): member q-1 has taken this step,
+         * member q has not; two-layer members, p = the bottom layer */
+        const olayer* A = &Ls[(q - 1) * CMTJ_MAX_LAYERS];
+        const olayer* B = &Ls[q * CMTJ_MAX_LAYERS];
+        const double da = vdot(A[s->top[q - 1]].mag, A[s->bottom[q - 1]].mag);
+        const double db = vdot(B[s->top[q]].mag, B[s->bottom[q]].mag);
+        eff_nd *= (1 + s->coupling[q - 1] * (s->series ? da + db : da - db));
+        Clog[q - 1] = eff_nd;
+        local[q] = s->current;
+        local[q].constant = s->current.constant * eff_nd;
+        local[q].amplitude = s->current.amplitude * eff_nd;
+        local[q].phase += s->series ? s->phase_offset * q : s->phase_offset;
+      }
       for (int l = 0; l < j->n_layers; ++l) own[q * CMTJ_MAX_LAYERS + l].drv[CMTJ_SLOT_CURRENT] = local[q];
       if (junction_step(j, &Ls[q * CMTJ_MAX_LAYERS], i, &st[q])) stop = 1;
       Ilog[q] = oracle_driver_value(&local[q], t, i, 0, 1, NULL);
@@ -791,7 +831,8 @@ static int dipole_nomura(vec3 r1, vec3 r2, const cmtj_layer* layer2, vec3 m1, do
  * [n][CMTJ_MAX_LAYERS][3], st [n].  Returns 0, 1 for a bad descriptor, 2
  * when two points are too close. */
 int oracle_run_group(const cmtj_group* g, const cmtj_junction* js, const cmtj_layer* layers,
-                     const cmtj_constants* c, double* log, double* mag_out, cmtj_status* st) {
+                     const cmtj_constants* c, const double* pair_terms, double* log, double* mag_out,
+                     cmtj_status* st) {
   const int n = g->n_elements;
   if (n < 1 || n > CMTJ_MAX_GROUP) return 1;
   cmtj_layer* own = (cmtj_layer*)malloc(sizeof(cmtj_layer) * CMTJ_MAX_LAYERS * (size_t)n);
@@ -827,6 +868,14 @@ int oracle_run_group(const cmtj_group* g, const cmtj_junction* js, const cmtj_la
           rc = dipole_kanao(ra, vload(g->pos[b]), lb, mb, c->mu0, &f);
         else if (g->interaction == CMTJ_INTERACTION_NOMURA)
           rc = dipole_nomura(ra, vload(g->pos[b]), lb, mb, c->mu0, &f);
+        else if (g->interaction == CMTJ_INTERACTION_TENSOR) {
+          /* an affine callable, f = T_ab m_b + b_ab; terms [b][12][a] */
+          const double* p = pair_terms + (size_t)b * 12 * (size_t)n + (size_t)a;
+          const size_t sN = (size_t)n;
+          f.x = fma(p[0], mb.x, fma(p[sN], mb.y, fma(p[2 * sN], mb.z, p[9 * sN])));
+          f.y = fma(p[3 * sN], mb.x, fma(p[4 * sN], mb.y, fma(p[5 * sN], mb.z, p[10 * sN])));
+          f.z = fma(p[6 * sN], mb.x, fma(p[7 * sN], mb.y, fma(p[8 * sN], mb.z, p[11 * sN])));
+        }
         H[a] = vadd(H[a], f);
       }
     }

@@ -39,11 +39,12 @@ int oracle_run_batch(const cmtj_junction* js, size_t n,
                      const double* tables, size_t n_table, double* log,
                      double* mag_out, cmtj_status* st, int n_threads);
 
-/* A coupled stack (core/stack.hpp), same descriptors and log layout as
- * cmtj_run_stacks for one stack; mag_out [n][CMTJ_MAX_LAYERS][3], st [n].  Returns 0. */
+/* A coupled stack (core/stack.hpp), same descriptors, tables and log layout
+ * as cmtj_run_stacks for one stack; mag_out [n][CMTJ_MAX_LAYERS][3], st [n].
+ * Returns 0. */
 int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js,
                      const cmtj_layer* layers, const cmtj_constants* c,
-                     double* log, double* mag_out, cmtj_status* st);
+                     const double* tables, double* log, double* mag_out, cmtj_status* st);
 
 /* A dipole-coupled group (core/reservoir.hpp GroupInteraction), same
  * descriptors and log layout as cmtj_run_groups for one group; mag_out
@@ -51,7 +52,7 @@ int oracle_run_stack(const cmtj_stack* s, const cmtj_junction* js,
  * (two points too close for the interaction). */
 int oracle_run_group(const cmtj_group* g, const cmtj_junction* js,
                      const cmtj_layer* layers, const cmtj_constants* c,
-                     double* log, double* mag_out, cmtj_status* st);
+                     const double* pair_terms, double* log, double* mag_out, cmtj_status* st);
 
 /* Channels of an instance for a log mask (same rule as the engine). */
 int oracle_n_channels(int n_layers, int mr_mode, int log_mask);

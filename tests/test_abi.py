@@ -32,9 +32,9 @@ def test_every_declared_symbol_is_exported():
 
 def test_struct_layout_matches_binding():
     _lib.lib()  # raises ImportError on any mismatch
-    assert _lib.LAYER_DTYPE.itemsize == 3096
+    assert _lib.LAYER_DTYPE.itemsize == 3144
     assert _lib.JUNCTION_DTYPE.itemsize == 544
-    assert _lib.STACK_DTYPE.itemsize == 1160
+    assert _lib.STACK_DTYPE.itemsize == 1168
     assert _lib.GROUP_DTYPE.itemsize == 1800
 
 

@@ -116,12 +116,6 @@ def test_dp_restrictions():
                            cellSurface=1e-16, demagTensor=[CVector(0, 0, 0)] * 3) for i in range(5)])
     with pytest.raises(NotImplementedError, match="1 to 4 layers"):
         five.runSimulation(1e-11, 1e-13, 1e-12, solverMode=SolverMode.DormandPrince)
-    j = _film(OracleJunction, (0, 0, 1), (0, 0, 1e5))
-    p = AdaptiveIntegrationParams()
-    p.use_pid_control = True
-    j.getLayer("free").setAdaptiveParams(p)
-    with pytest.raises(NotImplementedError, match="PID"):
-        j.runSimulation(1e-11, 1e-13, 1e-12, solverMode=SolverMode.DormandPrince)
     with pytest.raises(NotImplementedError, match="energy"):
         _film(OracleJunction, (0, 0, 1), (0, 0, 1e5)).runSimulation(1e-11, 1e-13, 1e-12, calculateEnergies=True,
                                                                       solverMode=SolverMode.DormandPrince)
@@ -219,3 +213,128 @@ def test_a_rejected_layer_step_stops_the_run():
     print(f"{n} entries before the rejection")
     assert 2 < n <= len(l1["time"])
     assert _equivalent(l2, l1, 2.0)  # identical up to the rejection
+
+
+# ---- the PID step controller (dormand_prince_step, core/junction.hpp:953-961)
+#
+# factor = exp(kp e + ki I + kd (e - prev)) with e = log(error ratio),
+# I = integral + e, then prev = e; the state starts from the layer's
+# integral_error and prev_error_ratio and stays in the layer after the run.
+# A run whose first proposed step already carries the clock past totalTime
+# takes exactly one attempt, and its one log entry is labelled with that
+# proposal (the clock advances by the proposed step, :997 then :2029): the
+# first attempt's error ratio r1 is the same under both controllers (same
+# step h0 from the same m), so a standard-controller run tells r1 through
+# h0 safety r1^(-1/5), and the PID run's entry, final integral and final
+# prev then follow in closed form.
+
+
+def _pid_film(params, field=(2e4, 0, 1e5)):
+    j = _film(OracleJunction, (0.6, 0, 0.8), field, alpha=0.02)
+    j.getLayer("free").setAdaptiveParams(params)
+    return j
+
+
+def _params(**kw):
+    p = AdaptiveIntegrationParams()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+H0 = 2.0 ** -43  # 1.14e-13 s; sums of it are exact
+
+
+def _first_error_log():
+    """log(r1) of the first attempt, from a standard-controller run with the
+    factor unclamped."""
+    j = _pid_film(_params(max_factor=1e9))
+    j.runSimulation(1.0001 * H0, H0, H0, solverMode=SolverMode.DormandPrince)
+    t1 = j.getLog()["time"]
+    assert len(t1) == 1 and t1[0] > 1.0001 * H0
+    return -5.0 * math.log(t1[0] / (0.9 * H0))
+
+
+@pytest.mark.parametrize("kp,ki,kd,I0,P0", [
+    (-0.03, 0.0, 0.0, 0.0, 1.0),
+    (-0.02, -0.01, 0.0, -3.0, 1.0),
+    (-0.02, -0.01, 0.015, -3.0, -10.0),
+    (0.0, 0.0, -0.02, 0.0, 1.0),  # the reference's initial prev_error_ratio 1.0 enters through kd
+])
+def test_pid_first_step_closed_form(kp, ki, kd, I0, P0):
+    e1 = _first_error_log()
+    assert e1 < -1  # an accepted first attempt
+    x = kp * e1 + ki * (I0 + e1) + kd * (e1 - P0)
+    assert math.log(1.0001) < x < math.log(5.0)  # unclamped, and one attempt ends the run
+    p = _params(use_pid_control=True, kp=kp, ki=ki, kd=kd, integral_error=I0, prev_error_ratio=P0)
+    j = _pid_film(p)
+    j.runSimulation(1.0001 * H0, H0, H0, solverMode=SolverMode.DormandPrince)
+    t = j.getLog()["time"]
+    assert len(t) == 1
+    np.testing.assert_allclose(t[0], H0 * math.exp(x), rtol=1e-11)
+    ap = j.getLayer("free").adaptiveParams
+    np.testing.assert_allclose(ap.integral_error, I0 + e1, rtol=1e-11)
+    np.testing.assert_allclose(ap.prev_error_ratio, e1, rtol=1e-11)
+    # the caller's parameters are the layer's own copy: untouched
+    assert p.integral_error == I0 and p.prev_error_ratio == P0
+
+
+def test_pid_integral_sums_every_attempt():
+    """kp only: every attempt's factor exp(kp e_n) is the ratio of the step
+    it proposes (the clock increment) to the step it took, so the log gives
+    each e_n back; the layer's final integral is I0 + sum e_n and its prev
+    the last e_n.  (kp = -1/5 would be the standard controller with safety 1,
+    whose error ratios settle at the acceptance threshold.)"""
+    kp, I0 = -0.01, 2.5
+    j = _pid_film(_params(use_pid_control=True, kp=kp, ki=0.0, kd=0.0, integral_error=I0, max_factor=10.0))
+    T = 60 * H0
+    j.runSimulation(T, H0, H0, solverMode=SolverMode.DormandPrince)
+    t = np.asarray(j.getLog()["time"])
+    p = np.diff(np.concatenate([[0.0], t]))          # proposals = clock increments
+    used = np.minimum(np.concatenate([[H0], p[:-1]]), T - np.concatenate([[0.0], t[:-1]]))
+    e = np.log(p / used) / kp
+    assert len(t) > 10 and np.all(e < 0)              # every attempt accepted, one entry each
+    assert np.all(p / used < 10.0) and np.all(p / used > 0.1)  # no clamping
+    ap = j.getLayer("free").adaptiveParams
+    np.testing.assert_allclose(ap.integral_error, I0 + e.sum(), rtol=1e-9)
+    np.testing.assert_allclose(ap.prev_error_ratio, e[-1], rtol=1e-9)
+
+
+def test_pid_zero_gains_keep_the_first_step():
+    """kp = ki = kd = 0: factor exp(0) = 1, every accepted step is h0, and the
+    clock is the running sum of h0."""
+    j = _pid_film(_params(use_pid_control=True, kp=0.0, ki=0.0, kd=0.0))
+    n = 40
+    j.runSimulation(n * H0, H0, H0, solverMode=SolverMode.DormandPrince)
+    t = np.asarray(j.getLog()["time"])
+    assert len(t) == n
+    np.testing.assert_array_equal(t, H0 * np.arange(1, n + 1))
+    ap = j.getLayer("free").adaptiveParams
+    assert ap.prev_error_ratio < 0 and ap.integral_error < n * ap.prev_error_ratio * 0.5
+
+
+def test_pid_state_carries_over_to_the_next_run():
+    """Two runs in a row equal a second junction started from the first
+    run's final m and controller state."""
+    p = _params(use_pid_control=True, kp=-0.15, ki=-0.002, kd=0.01)
+    a = _pid_film(p)
+    a.runSimulation(0.1e-9, H0, 1e-11, solverMode=SolverMode.DormandPrince)
+    ap = a.getLayer("free").adaptiveParams
+    I1, P1 = ap.integral_error, ap.prev_error_ratio
+    m1 = a.getLayer("free").mag
+    n1 = len(a.getLog()["time"])
+    a.runSimulation(0.1e-9, H0, 1e-11, solverMode=SolverMode.DormandPrince)
+    b = _film(OracleJunction, (m1.x, m1.y, m1.z), (2e4, 0, 1e5), alpha=0.02)
+    b.getLayer("free").setAdaptiveParams(_params(use_pid_control=True, kp=-0.15, ki=-0.002, kd=0.01,
+                                                 integral_error=I1, prev_error_ratio=P1))
+    b.runSimulation(0.1e-9, H0, 1e-11, solverMode=SolverMode.DormandPrince)
+    la, lb = a.getLog(), b.getLog()
+    for k in ("free_mx", "free_my", "free_mz"):
+        np.testing.assert_array_equal(np.asarray(la[k])[n1:], lb[k])
+    np.testing.assert_array_equal(np.asarray(la["time"])[n1:], np.asarray(lb["time"]) + 0.0)
+    assert a.getLayer("free").adaptiveParams.integral_error == b.getLayer("free").adaptiveParams.integral_error
+    # a plain second run from the default state differs
+    c = _film(OracleJunction, (m1.x, m1.y, m1.z), (2e4, 0, 1e5), alpha=0.02)
+    c.getLayer("free").setAdaptiveParams(_params(use_pid_control=True, kp=-0.15, ki=-0.002, kd=0.01))
+    c.runSimulation(0.1e-9, H0, 1e-11, solverMode=SolverMode.DormandPrince)
+    assert not np.array_equal(c.getLog()["time"], lb["time"])

@@ -311,3 +311,14 @@ def test_zero_and_short_runs():
     olog, omag, ost = _oracle_blocks(b)
     assert list(ost["steps_done"]) == [0, 5, 30] and list(ost["entries_done"]) == [0, 1, 3]
     np.testing.assert_allclose(r.final_m.cpu().numpy()[:, :2], omag[:, :2], atol=1e-12)
+
+
+def test_fp32_plans_report_binary64_groups():
+    """An FP32 batch says which of its groups run binary64 kernels: the STT
+    oscillator on one lane per component has no binary32 kernel, the PIMM
+    sweep does."""
+    sto = workloads.sto(n_currents=8, total_time=0.05e-9, precision="fp32")
+    pimm = workloads.pimm(n_points=8, t_min=0.05e-9, t_max=0.05e-9, precision="fp32")
+    assert sto.fp64_groups == 1 and "double" in sto.plan().info.kernel_name.decode()
+    assert pimm.fp64_groups == 0
+    assert sto.set_lanes("per_layer").fp64_groups == 0

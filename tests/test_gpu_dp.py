@@ -236,3 +236,182 @@ def test_work_queue_equals_one_lane_per_instance(monkeypatch):
         la, lb = ra.instance_log(i), rb.instance_log(i)
         for k in la:
             np.testing.assert_array_equal(la[k], lb[k])
+
+
+# ---- the PID step controller (core/junction.hpp:953-961) ------------------------
+
+PID = dict(kp=-0.15, ki=0.0, kd=0.01)
+# an integral term makes the trajectory far more sensitive to rounding
+# (ki > 0 shrinks the step until |m5 - m4| is rounding noise): the oracle's own
+# response to a 1e-14 change of m0 sets that case's bar
+PID_I = dict(kp=-0.15, ki=-0.0003, kd=0.01)
+
+
+def _with_pid(j, tol=None, **gains):
+    for l in j.layers:
+        p = AdaptiveIntegrationParams()
+        if tol is not None:
+            p.abs_tol = p.rel_tol = tol
+        p.use_pid_control = True
+        for k, v in gains.items():
+            setattr(p, k, v)
+        l.setAdaptiveParams(p)
+    return j
+
+
+def _pid_state(j):
+    return np.array([[l.adaptiveParams.integral_error, l.adaptiveParams.prev_error_ratio] for l in j.layers])
+
+
+def _state_difference(a, b) -> float:
+    """Largest relative difference of two controller states; an infinite
+    entry (log of a zero error ratio) must be the same infinity."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    assert np.array_equal(np.isinf(a), np.isinf(b)) and np.array_equal(a[np.isinf(a)], b[np.isinf(b)])
+    f = np.isfinite(a)
+    return float(np.max(np.abs(a[f] - b[f]) / np.maximum(np.abs(b[f]), 1e-300))) if f.any() else 0.0
+
+
+# The integral sums log(error ratio) over every attempt, including attempts
+# whose error estimate |m5 - m4| is near the rounding level of m (a step
+# clipped to the end of the run, a step just after a reset): there one
+# rounding of m4 moves log(ratio) by up to ~1e-2, without changing the
+# trajectory (the factor is clamped).  So the state is compared against 100
+# times the oracle's own response to a 1e-14 change of the initial m (2e-7 to
+# 2e-4 relative in the cases below), as the logs are.
+
+
+def _pid_runs(name, gains, J, eps=0.0):
+    j, args = _case(name, J)
+    _with_pid(j, **gains)
+    l = j.layers[0]
+    l.mag = CVector(l.mag.x, l.mag.y + eps, l.mag.z)
+    if not gains:
+        args = (3e-12, 1e-13, 1e-13)
+    states = []
+    for _ in range(2):
+        j.runSimulation(*args, solverMode=SolverMode.DormandPrince)
+        states.append(_pid_state(j))
+    return j, states
+
+
+@pytest.mark.parametrize("name,gains", [("sot_switch", PID), ("sine", PID), ("sot_switch", PID_I), ("sine", {})])
+def test_pid_dp_matches_oracle(name, gains):
+    """One layer (k_dp) with the PID controller, twice in a row: the same
+    entries and magnetisations as the oracle within 1e-8 (or 100 times the
+    oracle's response to a 1e-14 change of the initial m), and the same
+    controller state after each run (the second run starts from it).  The
+    reference's default gains (kp = +0.2) shrink the step after every
+    accepted attempt down to nothing and reset it, so that case runs 3 ps."""
+    (g, sg), (o, so) = _pid_runs(name, gains, Junction), _pid_runs(name, gains, OracleJunction)
+    p, sp = _pid_runs(name, gains, OracleJunction, 1e-14)
+    bar = max(1e-8, 100 * _compare(p, o))
+    worst = _compare(g, o)
+    print(f"[dp] pid {name} {gains or 'default gains'}: {len(g.getLog()['time'])} entries, max relative "
+          f"difference {worst:.3e}, bar {bar:.1e}, state {sg[-1].ravel()}", flush=True)
+    assert worst <= bar
+    assert len(g.getLog()["time"]) > 20
+    for a, b, c in zip(sg, so, sp):
+        d, sbar = _state_difference(a, b), max(1e-9, 100 * _state_difference(c, b))
+        print(f"[dp]   state {a.ravel()} vs oracle {b.ravel()}: {d:.2e} (bar {sbar:.1e})", flush=True)
+        assert d <= sbar
+    assert not np.array_equal(sg[0], sg[1])
+
+
+PAIR_PID = dict(kp=-0.15, ki=0.0, kd=0.01)
+
+
+def _pid_pair(J, eps=0.0):
+    """The torque-free layer first, both layers on the PID controller: the
+    torque-free layer's error ratio is 0, log 0 = -inf makes its factor
+    infinite or undefined, so it passes min_factor = 1/2 of the step on (its
+    state goes to -inf); the precessing layer's controller sets the rest.
+    (With ki > 0 the step here shrinks until |m5 - m4| is rounding noise.)"""
+    import copy
+
+    from test_dp_cpu import _pair
+
+    j = _pair(J, 0.5, tol=1e-6)
+    for l in j.layers:
+        p = copy.copy(l.adaptiveParams)
+        p.use_pid_control = True
+        for k, v in PAIR_PID.items():
+            setattr(p, k, v)
+        l.setAdaptiveParams(p)
+    l = j.getLayer("free")
+    l.mag = CVector(l.mag.x, l.mag.y + eps, l.mag.z)
+    j.runSimulation(1e-9, 2e-13, 2e-13, solverMode=SolverMode.DormandPrince)
+    return j
+
+
+def test_pid_multilayer_dp_matches_oracle():
+    """Two layers (k_dp_ml), each with its own PID state, within 1e-8 or 100
+    times the oracle's response to a 1e-14 change of the initial state."""
+    g, o, s = _pid_pair(Junction), _pid_pair(OracleJunction), _pid_pair(OracleJunction, 1e-14)
+    bar = max(1e-8, 100 * _compare(s, o))
+    worst = _compare(g, o)
+    sg, so = _pid_state(g), _pid_state(o)
+    print(f"[dp] pid pair: {len(g.getLog()['time'])} entries, max relative difference {worst:.3e}, bar {bar:.1e}, "
+          f"state {sg.ravel()}", flush=True)
+    assert worst <= bar
+    assert len(g.getLog()["time"]) > 50
+    assert np.all(np.isneginf(sg[0])) and np.all(np.isneginf(so[0]))
+    assert _state_difference(sg, so) <= max(1e-9, 100 * _state_difference(_pid_state(s), so))
+
+
+def test_pid_batch_and_work_queue(monkeypatch):
+    """A batch of PID instances: the PID state after the magnetisations in
+    each instance's final-m block, against the oracle; the work-queue kernel
+    gives k_dp's statuses (the same accepted steps and entries) and its
+    magnetisations and logs within 1e-10.  (With the standard controller
+    the two are bit-equal; with PID they were measured up to 5e-15 apart in
+    m on 42 of 288 components: the two kernels' builds round some step of
+    the controller apart, which this test does not isolate.)"""
+    monkeypatch.setenv("CMTJ_QUEUE_LANES", "64")
+    js, T, gains = [], [], []
+    for q in range(96):
+        name = ("sot_switch", "sine")[q % 2]
+        j, args = _case(name, Junction)
+        gains.append(dict(PID, kp=PID["kp"] * (1 + 0.2 * (q % 5) / 4)))
+        _with_pid(j, **gains[-1])
+        js.append(j)
+        T.append(args[0] * (0.3 + 0.7 * ((q * 37) % 101) / 100.0))
+    mk = lambda lanes: JunctionBatch.from_junctions(js, T, 1e-13, 1e-12, SolverMode.DormandPrince,  # noqa: E731
+                                                     precision="fp64", log="full", log_dtype="f64").set_lanes(lanes)
+    a, b = mk("auto"), mk("queue")
+    assert "k_dp_q" in b.plan().info.kernel_name.decode()
+    ra, rb = a.run(), b.run()
+    fa, fb = ra.final_m.cpu().numpy(), rb.final_m.cpu().numpy()
+    assert ra.status().tobytes() == rb.status().tobytes()
+    np.testing.assert_allclose(fa[:, 0], fb[:, 0], rtol=0, atol=1e-10)  # m of the one layer
+    for i in range(len(js)):
+        la, lb = ra.instance_log(i), rb.instance_log(i)
+        for k in la:
+            np.testing.assert_allclose(la[k], lb[k], rtol=1e-10, atol=1e-10 if k != "time" else 0)
+    worst = 0.0
+    for q in range(0, len(js), 7):
+        name = ("sot_switch", "sine")[q % 2]
+        runs = []
+        for eps in (0.0, 1e-14):
+            o, _ = _case(name, OracleJunction)
+            _with_pid(o, **gains[q])
+            l = o.layers[0]
+            l.mag = CVector(l.mag.x, l.mag.y + eps, l.mag.z)
+            o.runSimulation(T[q], 1e-13, 1e-12, solverMode=SolverMode.DormandPrince)
+            runs.append(o)
+        o, p = runs
+        so = _pid_state(o)[0]
+        sbar = max(1e-9, 100 * _state_difference(_pid_state(p)[0], so))
+        for f in (fa, fb):
+            d = _state_difference(f[q].reshape(-1)[3:5], so)
+            worst = max(worst, d / sbar)
+            assert d <= sbar, (q, d, sbar)
+        ol, gl = o.getLog(), ra.instance_log(q)
+        assert len(ol["time"]) == len(gl["time"])
+        lbar = max(1e-8, 100 * _compare(p, o))
+        for k in ol:
+            x, y = np.asarray(gl[k]), np.asarray(ol[k])
+            d = float(np.max(np.abs(x - y))) / max(float(np.max(np.abs(y))), 1e-300) if y.size else 0.0
+            worst = max(worst, d / lbar)
+            assert d <= lbar, (q, k, d, lbar)
+    print(f"[dp] pid batch: {len(js)} instances, worst difference {worst:.2f} of its bar", flush=True)

@@ -183,3 +183,58 @@ def test_four_wave_group_nan_releases_every_wave():
             b.runSimulation(0.1e-9, 1e-13, 1e-12)
         assert _compare(g[k], b, 12) <= 1e-8
     assert 0 < len(g[2].getLog(0)["time"]) < len(g[1].getLog(0)["time"])
+
+
+def _biased_kanao(r1, r2, layer1, layer2):
+    """A Python interaction: the Kanao field plus a constant bias (affine,
+    not the module's function object)."""
+    h = reservoir.computeDipoleInteraction(r1, r2, layer1, layer2)
+    return CVector(h.x + 35.0, h.y - 20.0, h.z + 5.0)
+
+
+@pytest.mark.parametrize("n", [4, 64])
+def test_callable_interaction_matches_oracle(n):
+    """A Python callable runs as per-pair affine terms read on the host; 64
+    elements put the group on four waves (the helper waves sum their share
+    of the sources from the same terms)."""
+    js = _elements(Junction, n)
+    pos = [CVector(*p) for p in POS[:4]] if n == 4 else _grid(n)
+    g, o = GroupInteraction(pos, js), OracleGroupInteraction(pos, js)
+    for x in (g, o):
+        x.setInteractionFunction(_biased_kanao)
+        x.runSimulation(0.1e-9, 1e-13, 1e-12)
+    worst = _compare(g, o, n)
+    print(f"[group] callable n={n}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_callable_and_builtin_groups_in_one_launch():
+    """Groups with a Python callable and groups with the built-in functions
+    side by side in one launch; a callable computing the Kanao field runs as
+    the built-in one within rounding."""
+    from test_group_cpu import _my_kanao
+
+    def build(k, G):
+        js = _elements(Junction, 5)
+        grp = G(_grid(5, pitch=50e-9 + 2e-9 * k), js)
+        grp.setInteractionFunction([_my_kanao, _FN["kanao"], _biased_kanao, _FN["nomura"]][k % 4])
+        return grp
+
+    K = 12
+    g = [build(k, GroupInteraction) for k in range(K)]
+    o = [build(k, OracleGroupInteraction) for k in range(K)]
+    run_groups(g, 0.2e-9, 1e-13, 1e-12)
+    worst = 0.0
+    for k, b in enumerate(o):
+        b.runSimulation(0.2e-9, 1e-13, 1e-12)
+        worst = max(worst, _compare(g[k], b, 5))
+    print(f"[group] {K} groups, callables and built-ins: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+    a = GroupInteraction(_grid(5), _elements(Junction, 5))
+    b = GroupInteraction(_grid(5), _elements(Junction, 5))
+    a.setInteractionFunction(_my_kanao)
+    for x in (a, b):
+        x.runSimulation(0.2e-9, 1e-13, 1e-12)
+    d = _compare(a, b, 5)
+    print(f"[group] Kanao as a callable vs built in: {d:.3e}", flush=True)
+    assert d <= 1e-10

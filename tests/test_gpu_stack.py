@@ -126,3 +126,77 @@ def test_batched_stacks_match_oracle():
     print(f"[stack] {K} stacks in one launch: max relative difference {worst:.3e}", flush=True)
     assert worst <= 1e-8
     assert 0 < len(g[7].getLog()["time"]) < len(g[6].getLog()["time"])
+
+
+@pytest.mark.parametrize("series", [True, False])
+@pytest.mark.parametrize("n", [3, 5])
+def test_non_delayed_stack_matches_oracle(series, n):
+    """setDelayed(False): the members step one after another within a step,
+    each coupled to its predecessor's new state (the wave runs one sub-step
+    per member)."""
+    js = _members(_two, n=n)
+    drv = ScalarDriver.getSineDriver(8e11, 5e11, 3e9, 0.4)
+    gcls, ocls = (SeriesStack, OracleSeriesStack) if series else (ParallelStack, OracleParallelStack)
+    stacks = [cls(js, useKCL=False, phaseOffset=0.3) for cls in (gcls, ocls)]
+    for s in stacks:
+        s.setDelayed(False)
+        s.setCouplingStrength([0.25, -0.4, 0.3, -0.2][: n - 1])
+        s.setCoupledCurrentDriver(drv)
+        s.setExternalFieldDriver(AxialDriver(constantDriver(1e4), constantDriver(0), constantDriver(2e4)))
+        s.runSimulation(0.5e-9, 1e-13, 1e-12)
+    worst = _compare(*stacks)
+    print(f"[stack] non-delayed n={n} series={series}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_mixed_delay_rules_in_one_launch():
+    """Delayed and non-delayed stacks (and KCL ones) side by side in one wave:
+    each as the oracle runs it alone."""
+    from cmtj_amd.stack import run_stacks
+
+    def build(k, cls_s, cls_p):
+        js = _members(_two, n=3)
+        for j in js:
+            j.setLayerExternalFieldDriver("all", AxialDriver(constantDriver(1e3 * (k % 5)), constantDriver(0),
+                                                             constantDriver(1e5)))
+        s = (cls_s if k % 2 else cls_p)(js, useKCL=(k % 3 == 0), phaseOffset=0.1 * k)
+        if k % 3 == 1:
+            s.setDelayed(False)
+        s.setCouplingStrength([0.1 + 0.01 * k, -0.2])
+        s.setCoupledCurrentDriver(ScalarDriver.getSineDriver(5e11, 2e11 * (k % 4), 2e9, 0.0))
+        return s
+
+    K = 24
+    g = [build(k, SeriesStack, ParallelStack) for k in range(K)]
+    o = [build(k, OracleSeriesStack, OracleParallelStack) for k in range(K)]
+    run_stacks(g, 0.3e-9, 1e-13, 1e-12)
+    worst = 0.0
+    for k, b in enumerate(o):
+        b.runSimulation(0.3e-9, 1e-13, 1e-12)
+        worst = max(worst, _compare(g[k], b))
+    print(f"[stack] {K} stacks, three coupling rules, one launch: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+@pytest.mark.parametrize("series", [True, False])
+def test_callback_currents_and_drivers_match_oracle(series):
+    """A KCL stack whose current is a Python callback (tabulated once per
+    step) and whose members carry a callback anisotropy (tabulated at the
+    RK4 stage times), against the oracle on the same tables."""
+    from test_stack_cpu import _pulse
+
+    def ramp(t):
+        return 1.05e3 + 4e13 * t
+
+    js = _members(_single, n=4)
+    for j in js:
+        j.setLayerAnisotropyDriver("free", ScalarDriver.getCustomDriver(ramp))
+    gcls, ocls = (SeriesStack, OracleSeriesStack) if series else (ParallelStack, OracleParallelStack)
+    stacks = [cls(js, useKCL=True) for cls in (gcls, ocls)]
+    for s in stacks:
+        s.setCouplingStrength([0.25, -0.4, 0.3])
+        s.setCoupledCurrentDriver(ScalarDriver.getCustomDriver(_pulse))
+        s.runSimulation(0.3e-9, 1e-13, 1e-12)
+    worst = _compare(*stacks)
+    print(f"[stack] callbacks series={series}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8

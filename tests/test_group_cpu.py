@@ -125,12 +125,17 @@ def test_too_close_points():
         g.runSimulation(1e-12, 1e-13, 1e-13)
 
 
-def test_python_callable_interaction_is_not_provided():
+def test_non_affine_callables_are_refused():
+    """The device applies f = T(r_i, r_j) m_j + b per pair: a callable that
+    is not affine in the source's m, or that reads the target's m, is
+    refused before the run."""
     js = [create_test_junction()[0] for _ in range(2)]
-    g = reservoir.GroupInteraction([CVector(0, 0, 0), CVector(1e-7, 0, 0)], js)
-    g.setInteractionFunction(lambda r1, r2, a, b: CVector(0, 0, 0))
-    with pytest.raises(NotImplementedError, match="Python callable"):
-        g.runSimulation(1e-12, 1e-13, 1e-13)
+    for fn in (lambda r1, r2, a, b: CVector(b.mag.x * b.mag.x, 0, 0),
+               lambda r1, r2, a, b: CVector(a.mag.x * 1e3, 0, 0)):
+        g = reservoir.GroupInteraction([CVector(0, 0, 0), CVector(1e-7, 0, 0)], js)
+        g.setInteractionFunction(fn)
+        with pytest.raises(NotImplementedError, match="not affine"):
+            g.runSimulation(1e-12, 1e-13, 1e-13)
 
 
 # --- the oracle's group runner, step by step --------------------------------
@@ -160,20 +165,29 @@ def _elements(J, n=3, two=False):
 POS = [(0, 0, 0), (60e-9, 0, 0), (20e-9, 45e-9, 5e-9), (-30e-9, 10e-9, 0)]
 
 
-@pytest.mark.parametrize("kind", ["kanao", "nomura"])
+def _my_kanao(r1, r2, layer1, layer2):
+    """A Python interaction (not the module's function object): the Kanao
+    field, so the run takes the per-pair tensor path."""
+    return reservoir.computeDipoleInteraction(r1, r2, layer1, layer2)
+
+
+@pytest.mark.parametrize("kind", ["kanao", "nomura", "callable"])
 @pytest.mark.parametrize("two", [False, True])
 def test_oracle_group_composes_single_steps(kind, two):
     """The group run equals, step for step, single-junction runs of one step
     each whose top layer carries, as its reserved interaction field, the
     textbook dipole field of the other elements at the step's start (Kanao:
-    4 pi times it)."""
+    4 pi times it; a Python callable computing it: the same, through the
+    per-pair tensors)."""
     n, dt, steps = 3, 1e-13, 12
     group = OracleGroupInteraction([CVector(*p) for p in POS[:n]], _elements(OracleJunction, n, two))
     if kind == "nomura":
         group.setInteractionFunction(reservoir.computeDipoleInteractionNoumra)
+    elif kind == "callable":
+        group.setInteractionFunction(_my_kanao)
     group.runSimulation(steps * dt, dt, dt)
     solo = _elements(OracleJunction, n, two)
-    scale = 4 * math.pi if kind == "kanao" else 1.0
+    scale = 1.0 if kind == "nomura" else 4 * math.pi
     want = [[] for _ in range(n)]
     for _ in range(steps):
         ms = [np.array(j.getLayerMagnetisation("free").tolist()) for j in solo]
@@ -242,3 +256,42 @@ def test_mixed_solvers_are_rejected():
     g = OracleGroupInteraction([CVector(*p) for p in POS[:2]], js)
     with pytest.raises(RuntimeError, match="different solver modes"):
         g.runSimulation(1e-12, 1e-13, 1e-13)
+
+
+def test_callable_tensors_are_the_textbook_dipole_tensor():
+    """The per-pair terms read off a Python Kanao callable are
+    (Ms_j V_j / mu0) (3 r r^T / |r|^5 - I / |r|^3), r = r_i - r_j, with
+    b = 0, for every ordered pair."""
+    js = _elements(OracleJunction, 4)
+    g = OracleGroupInteraction([CVector(*p) for p in POS[:4]], js)
+    g.setInteractionFunction(_my_kanao)
+    pt = g._pair_terms()
+    mu0 = PhysicalConstants.magnetic_permeability()
+    for i in range(4):
+        assert np.all(pt[i, :, i] == 0)
+        for j in range(4):
+            if i == j:
+                continue
+            r = np.array(POS[i], float) - np.array(POS[j], float)
+            lj = js[j].getLayer("free")
+            d = np.linalg.norm(r)
+            want = (lj.Ms / mu0) * lj.thickness * lj.cellSurface * (3 * np.outer(r, r) / d ** 5 - np.eye(3) / d ** 3)
+            np.testing.assert_allclose(pt[j, :9, i].reshape(3, 3), want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+            assert np.all(pt[j, 9:, i] == 0)
+
+
+def test_callable_constant_field_is_an_offset():
+    """A callable that returns a constant vector c is all offset (T = 0): each
+    element gets (n - 1) c, as a junction alone whose reserved field is
+    (n - 1) c -- bit for bit."""
+    c = (120.0, -40.0, 15.0)
+    js = _elements(OracleJunction, 3, True)
+    g = OracleGroupInteraction([CVector(*p) for p in POS[:3]], js)
+    g.setInteractionFunction(lambda r1, r2, a, b: CVector(*c))
+    g.runSimulation(3e-11, 1e-13, 1e-12)
+    for q, j in enumerate(_elements(OracleJunction, 3, True)):
+        j.setLayerReservedInteractionField("free", AxialDriver(*(2 * x for x in c)))
+        j.runSimulation(3e-11, 1e-13, 1e-12)
+        a, b = g.getLog(q), j.getLog()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])

@@ -263,3 +263,131 @@ def test_batched_stacks_equal_separate_runs_oracle():
         for q in range(3):
             for k in a.getLog(q):
                 np.testing.assert_array_equal(np.asarray(a.getLog(q)[k]), np.asarray(b.getLog(q)[k]))
+
+
+@pytest.mark.parametrize("series", [True, False])
+def test_non_delayed_currents_follow_from_logged_magnetisations(series):
+    """setDelayed(False) (core/stack.hpp:334-343): member q's coupling to
+    member q-1 reads q-1's magnetisations after this step (its log entry of
+    the step, W = 1) and q's before it (q's previous entry).  Recomputed from
+    the logs, every I_q and C_{q-1}_q matches; the delayed rule (both from
+    the step start) does not."""
+    I0 = 1.5e12
+    members = [_two(OracleJunction, stt=True), _two(OracleJunction, stt=True, mag=(0.4, 0.1, 0.9)),
+               _two(OracleJunction, stt=True, mag=(0.1, 0.5, 0.8))]
+    cls = OracleSeriesStack if series else OracleParallelStack
+    s = cls(members, useKCL=False)
+    s.setDelayed(False)
+    s.setCouplingStrength([0.2, -0.35])
+    s.setCoupledCurrentDriver(ScalarDriver.getConstantDriver(I0))
+    init = [{l.id: np.array(l.mag.tolist()) for l in j.layers} for j in s.junctionList]
+    s.runSimulation(2e-10, 1e-13, 1e-13)
+    after, before = [], []
+    for q, j in enumerate(s.junctionList):
+        lg = j.getLog()
+        post = [np.stack([np.asarray(lg[f"{lid}_m{c}"]) for c in "xyz"], axis=1) for lid in ("free", "bottom")]
+        pre = _m_start(lg, ["free", "bottom"], init[q])
+        after.append(np.sum(post[0] * post[1], axis=1))
+        before.append(np.sum(pre[0] * pre[1], axis=1))
+    c = [0.2, -0.35]
+    sgn = 1.0 if series else -1.0
+    eff1 = 1 + c[0] * (after[0] + sgn * before[1])
+    eff2 = eff1 * (1 + c[1] * (after[1] + sgn * before[2]))
+    log = s.getLog()
+    np.testing.assert_allclose(log["I_0"], I0, rtol=1e-15)
+    np.testing.assert_allclose(log["I_1"], I0 * eff1, rtol=1e-12)
+    np.testing.assert_allclose(log["I_2"], I0 * eff2, rtol=1e-12)
+    np.testing.assert_allclose(log["C_0_1"], eff1, rtol=1e-12)
+    np.testing.assert_allclose(log["C_1_2"], eff2, rtol=1e-12)
+    delayed1 = 1 + c[0] * (before[0] + sgn * before[1])
+    assert np.max(np.abs(delayed1 - eff1)) > 1e-6  # the two rules differ on this run
+
+
+def test_non_delayed_needs_two_layer_members_in_the_engine():
+    s = SeriesStack([_two(), _two()], useKCL=False)
+    s.setDelayed(False)
+    st, js, layers, *_ = s._pack(1e-11, 1e-13, 1e-12)
+    assert int(st["delayed"]) == 0 and int(st["kcl"]) == 0 and int(st["two_layer"]) == 1
+
+
+# --- callback drivers in stacks ------------------------------------------------
+
+def _pulse(t):
+    """A callback stack current: a smooth rise and a plateau (A/m^2)."""
+    import math
+    return 1.5e12 * (1 - math.exp(-t / 4e-11)) + 2e11
+
+
+def test_kcl_callback_current_follows_the_callback():
+    """KCL reads the stack driver once per step at the step's start
+    (runSimulationKCL, core/stack.hpp:376): every logged I_q equals
+    callback(t) (1 + sum of the pair couplings recomputed from the logged
+    magnetisations), and a constant callback runs exactly as the constant
+    driver."""
+    members = [_single(OracleJunction, stt=True), _single(OracleJunction, stt=True, mag=(0.4, 0.1, 0.9)),
+               _single(OracleJunction, stt=True, mag=(0.1, 0.5, 0.8))]
+    s = OracleSeriesStack(members, useKCL=True)
+    s.setCouplingStrength([0.2, -0.35])
+    s.setCoupledCurrentDriver(ScalarDriver.getCustomDriver(_pulse))
+    init = [{l.id: np.array(l.mag.tolist()) for l in j.layers} for j in s.junctionList]
+    s.runSimulation(2e-10, 1e-13, 1e-13)
+    d = []
+    for q, j in enumerate(s.junctionList):
+        ms = _m_start(j.getLog(), ["free"], init[q])
+        d.append(ms[0][:, 0])  # pol = x
+    C = [c * (d[q] + d[q + 1]) for q, c in enumerate([0.2, -0.35])]
+    t = np.asarray(s.getLog()["time"])
+    I0 = np.array([_pulse(x) for x in t])
+    for q in range(3):
+        np.testing.assert_allclose(s.getLog()[f"I_{q}"], I0 * (1 + C[0] + C[1]), rtol=1e-12)
+    a = OracleSeriesStack([_single(OracleJunction, stt=True), _single(OracleJunction, stt=True, mag=(0.4, 0.1, 0.9))])
+    b = OracleSeriesStack([_single(OracleJunction, stt=True), _single(OracleJunction, stt=True, mag=(0.4, 0.1, 0.9))])
+    a.setCoupledCurrentDriver(ScalarDriver.getCustomDriver(lambda t: 7e11))
+    b.setCoupledCurrentDriver(ScalarDriver.getConstantDriver(7e11))
+    for x in (a, b):
+        x.setCouplingStrength(0.3)
+        x.runSimulation(1e-10, 1e-13, 1e-12)
+    for k in a.getLog():
+        np.testing.assert_array_equal(np.asarray(a.getLog()[k]), np.asarray(b.getLog()[k]))
+    for q in range(2):
+        for k in a.getLog(q):
+            np.testing.assert_array_equal(np.asarray(a.getLog(q)[k]), np.asarray(b.getLog(q)[k]))
+
+
+def test_non_kcl_callback_current_fails_as_in_the_reference():
+    s = OracleSeriesStack([_single(OracleJunction), _single(OracleJunction)], useKCL=False)
+    s.setCoupledCurrentDriver(ScalarDriver.getCustomDriver(_pulse))
+    with pytest.raises(RuntimeError, match="Error in Python callback"):
+        s.runSimulation(1e-11, 1e-13, 1e-12)
+
+
+def test_member_callback_drivers_in_a_stack():
+    """A member's own callback driver (an anisotropy that ramps) is sampled
+    at its stage times as for a junction alone: with coupling 0 each member
+    evolves as the same junction run alone."""
+    def ramp(t):
+        return 1.05e3 + 4e13 * t
+
+    members = [_single(OracleJunction, stt=True), _single(OracleJunction, stt=True, mag=(0.2, 0.3, 0.9))]
+    for j in members:
+        j.setLayerAnisotropyDriver("free", ScalarDriver.getCustomDriver(ramp))
+    s = OracleParallelStack(members, useKCL=False, phaseOffset=0.0)
+    s.setCouplingStrength(0.0)
+    drv = ScalarDriver.getSineDriver(5e11, 3e11, 4e9, 0.2)
+    s.setCoupledCurrentDriver(drv)
+    s.runSimulation(2e-10, 1e-13, 1e-12)
+    for q, mag in enumerate([(0, 0, 1), (0.2, 0.3, 0.9)]):
+        base = _single(OracleJunction, stt=True, mag=mag)
+        base.setLayerAnisotropyDriver("free", ScalarDriver.getCustomDriver(ramp))
+        base.setLayerCurrentDriver("all", drv)
+        base.runSimulation(2e-10, 1e-13, 1e-12)
+        a, b = base.getLog(), s.getLog(q)
+        for k in a:
+            np.testing.assert_array_equal(np.asarray(a[k]), np.asarray(b[k]))
+    # the ramp reaches the members: a constant anisotropy gives another path
+    c = OracleParallelStack([_single(OracleJunction, stt=True), _single(OracleJunction, stt=True, mag=(0.2, 0.3, 0.9))],
+                            useKCL=False, phaseOffset=0.0)
+    c.setCouplingStrength(0.0)
+    c.setCoupledCurrentDriver(drv)
+    c.runSimulation(2e-10, 1e-13, 1e-12)
+    assert np.max(np.abs(np.asarray(c.getLog(1)["free_mx"]) - np.asarray(s.getLog(1)["free_mx"]))) > 1e-6
