@@ -378,6 +378,16 @@ def driver_eval(rec: np.ndarray, t: float) -> float:
     return out.value
 
 
+def cat_records(arrs, dtype) -> np.ndarray:
+    """Concatenation of structured records (0-d or 1-d arrays of `dtype`)
+    through their bytes: numpy's structured-dtype promotion costs tens of
+    microseconds per array."""
+    parts = [np.ascontiguousarray(a, dtype=dtype).reshape(-1).view(np.uint8) for a in arrs]
+    if not parts:
+        return np.zeros(0, dtype=dtype)
+    return np.concatenate(parts).view(dtype)
+
+
 def run_junction(j: np.ndarray, layers: np.ndarray, consts: np.ndarray, tables: np.ndarray | None,
                  noalloc: bool = False, stream: int | None = None, pid_out: np.ndarray | None = None):
     """One junction through cmtj_run (or cmtj_run_noalloc on `stream`, a

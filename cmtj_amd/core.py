@@ -807,42 +807,9 @@ class Layer:
         return f"Layer(id={self.id!r}, Ms={self.Ms}, thickness={self.thickness})"
 
     # descriptor ------------------------------------------------------------
-    def _fill(self, rec: np.ndarray, tables: "_Tables", run_info) -> None:
-        rec["mag"] = (self.mag.x, self.mag.y, self.mag.z)
-        rec["anis"] = (self.anis.x, self.anis.y, self.anis.z)
-        rec["Ms"] = self.Ms
-        rec["thickness"] = self.thickness
-        rec["cell_surface"] = self.cellSurface
-        rec["damping"] = self.damping
-        rec["demag"] = [c for v in self.demagTensor for c in (v.x, v.y, v.z)]
-        rec["dipole_top"] = [c for v in self.dipoleTop for c in (v.x, v.y, v.z)]
-        rec["dipole_bottom"] = [c for v in self.dipoleBottom for c in (v.x, v.y, v.z)]
-        rec["ref"] = self.referenceLayer.tolist()
-        rec["ref2"] = self.secondaryReferenceLayer.tolist()
-        rec["kappa"] = self.kappa
-        rec["beta"] = self.beta
-        rec["slonczewski"] = self.SlonczewskiSpacerLayerParameter
-        rec["polarisation"] = self.spinPolarisation
-        rec["fl_torque"] = self.fieldLikeTorque
-        rec["dl_torque"] = self.dampingLikeTorque
-        ap = self.adaptiveParams
-        rec["dp_abs_tol"] = ap.abs_tol
-        rec["dp_rel_tol"] = ap.rel_tol
-        rec["dp_max_factor"] = ap.max_factor
-        rec["dp_min_factor"] = ap.min_factor
-        rec["dp_safety"] = ap.safety_factor
-        rec["dp_use_pid"] = int(bool(ap.use_pid_control))
-        rec["dp_kp"] = ap.kp
-        rec["dp_ki"] = ap.ki
-        rec["dp_kd"] = ap.kd
-        rec["dp_integral_error"] = ap.integral_error
-        rec["dp_prev_error_ratio"] = ap.prev_error_ratio
-        rec["ref_type"] = int(self.referenceType)
-        rec["include_stt"] = int(self.includeSTT)
-        rec["include_sot"] = int(self.includeSOT)
-        rec["dynamic_sot"] = int(self.dynamicSOT)
-        rec["alternative_stt"] = int(self.alternativeSTT)
-        rec["temperature_set"] = int(self.temperatureSet)
+    def _fill(self, tables: "_Tables", run_info) -> tuple:
+        """The layer's cmtj_layer record as one tuple in the dtype's field
+        order (assigned with one numpy call instead of one per field)."""
         rows = [None] * _lib.NSLOT
         for name, slot in _SCALAR_SLOTS.items():
             d = self._scalar[name]
@@ -850,7 +817,24 @@ class Layer:
         for name, slot in _AXIAL_SLOTS.items():
             for q, d in enumerate(self._axial[name].drivers):
                 rows[slot + q] = _driver_row(d, tables.add(d, run_info) if d.update == UpdateType.custom else -1)
-        rec["drv"] = np.array(rows, dtype=_lib.DRIVER_DTYPE)
+        ap = self.adaptiveParams
+        return (
+            (self.mag.x, self.mag.y, self.mag.z),
+            (self.anis.x, self.anis.y, self.anis.z),
+            self.Ms, self.thickness, self.cellSurface, self.damping,
+            tuple(c for v in self.demagTensor for c in (v.x, v.y, v.z)),
+            tuple(c for v in self.dipoleTop for c in (v.x, v.y, v.z)),
+            tuple(c for v in self.dipoleBottom for c in (v.x, v.y, v.z)),
+            tuple(self.referenceLayer.tolist()),
+            tuple(self.secondaryReferenceLayer.tolist()),
+            self.kappa, self.beta, self.SlonczewskiSpacerLayerParameter, self.spinPolarisation,
+            self.fieldLikeTorque, self.dampingLikeTorque,
+            int(self.referenceType), int(self.includeSTT), int(self.includeSOT), int(self.dynamicSOT),
+            int(self.alternativeSTT), int(self.temperatureSet),
+            rows,
+            ap.abs_tol, ap.rel_tol, ap.max_factor, ap.min_factor, ap.safety_factor,
+            int(bool(ap.use_pid_control)), 0, ap.kp, ap.ki, ap.kd, ap.integral_error, ap.prev_error_ratio,
+        )
 
 
 class _Tables:
@@ -928,6 +912,22 @@ class Junction:
         self.logLength = 0
         self._seed = int.from_bytes(os.urandom(8), "little")
         self._runs = 0
+
+    def __deepcopy__(self, memo) -> "Junction":
+        """A copy that owns its layers, drivers and log, as the reference's
+        containers copy junctions by value; layer by layer instead of a
+        generic deep copy, which dominates building many stacks or groups."""
+        J = self.__class__.__new__(self.__class__)
+        memo[id(self)] = J
+        for k, v in self.__dict__.items():
+            if k == "layers":
+                v = [copy.copy(l) for l in v]
+            elif k == "_log":  # chunks are never written in place
+                v = {key: list(chunks) for key, chunks in v.items()}
+            else:
+                v = copy.deepcopy(v, memo)
+            J.__dict__[k] = v
+        return J
 
     def _init_plain(self, layers) -> None:
         # core/junction.hpp:1164-1180
@@ -1188,7 +1188,7 @@ class Junction:
         tables = _Tables()
         L = np.zeros(self.layerNo, dtype=_lib.LAYER_DTYPE)
         for i, l in enumerate(self.layers):
-            l._fill(L[i], tables, (n_steps, float(timeStep), solver))
+            L[i] = l._fill(tables, (n_steps, float(timeStep), solver))
         j = np.zeros((), dtype=_lib.JUNCTION_DTYPE)
         j["n_layers"] = self.layerNo
         j["mr_mode"] = self.MR_mode

@@ -6,7 +6,15 @@
 #include "registry.h"
 namespace cmtj {
 constexpr uint32_t FM_STACK = F_ALL;
+constexpr uint32_t FM_STT1 = F_ANIS | F_STT | F_EXACT;          // an STT free layer against a fixed polariser
+constexpr uint32_t FM_STT2 = F_ANIS | F_IEC | F_STT | F_EXACT;  // the same with a coupled second layer
 static const KEntry kS[] = {
+    // exact feature sets of the common STT members (feature tests compiled out)
+    {CMTJ_FP64, 1, 1, CMTJ_RK4, FM_STT1, false,
+     (const void*)&k_junction<double, 1, CMTJ_RK4, FM_STT1, false, true>, "k_junction<double,1,RK4,anis+STT,stack>"},
+    {CMTJ_FP64, 2, 1, CMTJ_RK4, FM_STT2, false,
+     (const void*)&k_junction<double, 2, CMTJ_RK4, FM_STT2, false, true>,
+     "k_junction<double,2,RK4,anis+IEC+STT,stack>"},
     // the usual member: anisotropy, IEC, spin torques, drives, temperature
     {CMTJ_FP64, 1, 1, CMTJ_RK4, FM_CPL, false,
      (const void*)&k_junction<double, 1, CMTJ_RK4, FM_CPL, false, true>, "k_junction<double,1,RK4,coupled,stack>"},

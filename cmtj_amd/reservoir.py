@@ -191,20 +191,19 @@ class GroupInteraction:
             if n_steps > 0:
                 g["top"][q] = j.getLayerIds().index(j.getLayer(self.topId).id)
             g["pos"][q] = [c.x, c.y, c.z]
-        js = np.zeros(n, dtype=_lib.JUNCTION_DTYPE)
+        js = _lib.cat_records([p[0] for p in packed], _lib.JUNCTION_DTYPE)
         layers, nchs = [], []
         rows = off = 0
         for q, (jd, L, _t, _ns, _w) in enumerate(packed):
-            js[q] = jd
-            js[q]["layer_offset"] = rows
-            js[q]["log_offset"] = off
+            js["layer_offset"][q] = rows
+            js["log_offset"][q] = off
             rows += len(L)
             layers.append(L)
             nch = _lib.n_channels(int(jd["n_layers"]), int(jd["mr_mode"]), int(jd["log_mask"]))
             nchs.append(nch)
             off += nch * int(jd["n_entries"])
         pt = self._pair_terms() if (kind == _lib.INTERACTION_TENSOR and n_steps > 0) else None
-        return g, js, np.concatenate(layers), nchs, n_steps, W, off, pt
+        return g, js, _lib.cat_records(layers, _lib.LAYER_DTYPE), nchs, n_steps, W, off, pt
 
     def _execute(self, g, js, layers, consts, pair_terms=None):
         return _lib.run_groups(g, js, layers, consts, pair_terms)
@@ -248,7 +247,7 @@ def run_groups(groups, totalTime: float, timeStep: float = 1e-13, writeFrequency
     n_steps, W = packs[0][4], packs[0][5]
     if n_steps == 0:
         return
-    recs = np.stack([p[0] for p in packs])
+    recs = _lib.cat_records([p[0] for p in packs], _lib.GROUP_DTYPE)
     n = groups[0].noElements
     pts = None
     if any(p[7] is not None for p in packs):  # every group's slot of the pair terms, zeros for the others
@@ -266,8 +265,9 @@ def run_groups(groups, totalTime: float, timeStep: float = 1e-13, writeFrequency
         off += size
         js_all.append(js)
         layers_all.append(layers)
-    js_all = np.concatenate(js_all)
-    log, mag, status = groups[0]._execute(recs, js_all, np.concatenate(layers_all), _lib.constants_get(), pts)
+    js_all = _lib.cat_records(js_all, _lib.JUNCTION_DTYPE)
+    log, mag, status = groups[0]._execute(recs, js_all, _lib.cat_records(layers_all, _lib.LAYER_DTYPE),
+                                          _lib.constants_get(), pts)
     nan = False
     base = 0
     for grp, (g_, js, layers, nchs, _ns, _w, size, _pt) in zip(groups, packs):

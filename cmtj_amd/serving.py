@@ -35,10 +35,7 @@ from .batch import _LANES, _LOG_MODES, JunctionBatch, _channel_names, _Meta
 from .core import SolverMode
 
 
-def _cat(arrs, dtype) -> np.ndarray:
-    """Concatenation of structured records through their bytes (numpy's
-    structured-dtype promotion is slow per array)."""
-    return np.concatenate([np.ascontiguousarray(a).view(np.uint8).reshape(-1) for a in arrs]).view(dtype)
+_cat = _lib.cat_records
 
 
 class LogArena:

@@ -210,19 +210,19 @@ class _Stack:
                 st["top"][q] = ids.index(j.getLayer(self.topId).id)
                 if two:
                     st["bottom"][q] = ids.index(j.getLayer(self.bottomId).id)
-        js = np.zeros(n, dtype=_lib.JUNCTION_DTYPE)
+        js = _lib.cat_records([p[0] for p in packed], _lib.JUNCTION_DTYPE)
         layers, nchs = [], []
         rows = off = 0
         for q, (jd, L, _t, _ns, _w) in enumerate(packed):
-            js[q] = jd
-            js[q]["layer_offset"] = rows
-            js[q]["log_offset"] = off
+            js["layer_offset"][q] = rows
+            js["log_offset"][q] = off
             rows += len(L)
             layers.append(L)
             nch = _lib.n_channels(int(jd["n_layers"]), int(jd["mr_mode"]), int(jd["log_mask"]))
             nchs.append(nch)
             off += (nch + 2) * int(jd["n_entries"])
-        return st, js, np.concatenate(layers), nchs, n_steps, W, off, (np.concatenate(tabs) if tabs else None)
+        return (st, js, _lib.cat_records(layers, _lib.LAYER_DTYPE), nchs, n_steps, W, off,
+                (np.concatenate(tabs) if tabs else None))
 
     def _finish(self, js, log, mag, status, nchs, W: int, timeStep: float) -> None:
         """Members take over their logs and final state; the stack log."""
@@ -320,7 +320,7 @@ def run_stacks(stacks, totalTime: float, timeStep: float = 1e-13, writeFrequency
         raise ValueError("every stack of a batch has the same member count")
     if parts[0][4] == 0:
         return
-    recs = np.stack([p[0] for p in parts])
+    recs = _lib.cat_records([p[0] for p in parts], _lib.STACK_DTYPE)
     js, layers, tabs = [], [], []
     rows = off = toff = 0
     for k, (st, jq, L, _n, _s, _w, size, tab) in enumerate(parts):
@@ -338,8 +338,8 @@ def run_stacks(stacks, totalTime: float, timeStep: float = 1e-13, writeFrequency
         layers.append(L)
         rows += len(L)
         off += size
-    js = np.concatenate(js)
-    log, mag, status = stacks[0]._execute(recs, js, np.concatenate(layers), _lib.constants_get(),
+    js = _lib.cat_records(js, _lib.JUNCTION_DTYPE)
+    log, mag, status = stacks[0]._execute(recs, js, _lib.cat_records(layers, _lib.LAYER_DTYPE), _lib.constants_get(),
                                           np.concatenate(tabs) if tabs else None)
     failed = False
     for k, (s, p) in enumerate(zip(stacks, parts)):
