@@ -223,6 +223,7 @@ class PlanInfo(ctypes.Structure):
 EXPORTS = (
     "cmtj_abi",
     "cmtj_last_error",
+    "cmtj_last_kernels",
     "cmtj_constants_get",
     "cmtj_constants_set",
     "cmtj_constants_reset",
@@ -262,6 +263,7 @@ def _load():
     lib = ctypes.CDLL(str(LIB_PATH), mode=ctypes.RTLD_GLOBAL)
     lib.cmtj_abi.argtypes = [ctypes.POINTER(_AbiInfo)]
     lib.cmtj_last_error.restype = ctypes.c_char_p
+    lib.cmtj_last_kernels.restype = ctypes.c_char_p
     lib.cmtj_constants_get.argtypes = [_P]
     lib.cmtj_constants_set.argtypes = [_P]
     lib.cmtj_driver_eval.argtypes = [_P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
@@ -329,6 +331,11 @@ def lib():
 
 def last_error() -> str:
     return (lib().cmtj_last_error() or b"").decode()
+
+
+def last_kernels() -> list[str]:
+    """Kernels the calling thread's last engine run launched."""
+    return [k for k in (lib().cmtj_last_kernels() or b"").decode().split(";") if k]
 
 
 def ptr(a: np.ndarray | None) -> int | None:

@@ -19,6 +19,7 @@ using namespace cmtj;
 // errors
 
 static thread_local std::string g_err;
+static thread_local std::string g_last_kernels;  // kernels of the calling thread's last launch
 
 static int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -509,8 +510,10 @@ static int build_plan(cmtj_plan* P, const cmtj_junction* js, size_t n, const cmt
 static int launch_plan(cmtj_plan* P, void* d_log, double* d_mag, cmtj_status* d_status, hipStream_t s,
                        cmtj_state* d_state = nullptr, uint32_t slice = 0, void* const* streams = nullptr,
                        int n_streams = 0) {
+  g_last_kernels.clear();
   for (size_t q = 0; q < P->groups.size(); ++q) {
     Group& G = P->groups[q];
+    g_last_kernels += (G.queue ? std::string("k_dp_q<double,1,F_ALL>") : std::string(G.k->name)) + ";";
     GroupArgs a = G.a;
     a.log = d_log;
     a.mag_out = d_mag;
@@ -667,6 +670,7 @@ int cmtj_abi(cmtj_abi_info* out) {
 }
 
 const char* cmtj_last_error(void) { return g_err.c_str(); }
+const char* cmtj_last_kernels(void) { return g_last_kernels.c_str(); }
 
 int cmtj_constants_get(cmtj_constants* out) {
   if (!out) return fail(CMTJ_ERR_INVALID, "null pointer");
@@ -905,6 +909,31 @@ int cmtj_run_stacks(const cmtj_stack* stacks, size_t n_stacks, const cmtj_juncti
       ke = &e;
   }
   if (!ke) return fail(CMTJ_ERR_INVALID, "no stack kernel for this solver");
+  // one lane per vector component (4 per layer) when every stack fits a
+  // wave, none couples without delay (its members step one after another:
+  // the one-lane form's sub-steps), and the stacks leave SIMDs free: a step
+  // of a stack then takes the component kernel's latency; a sweep larger
+  // than the chip keeps one lane per member (fewer instructions per step)
+  bool any_nd = false;
+  for (size_t q = 0; q < n_stacks; ++q) any_nd |= (!stacks[q].kcl && !stacks[q].delayed);
+  if (!any_nd) {
+    const KTable kc = kernels_stack_cl();
+    const KEntry* kce = nullptr;
+    for (size_t q = 0; q < kc.n; ++q) {
+      const KEntry& e = kc.e[q];
+      if (e.solver == j0.solver && e.lm >= P.groups[0].a.nl &&
+          covers(e, P.groups[0].a.flags, 0, P.groups[0].a.ndyn) && (!kce || rank(e) < rank(*kce)))
+        kce = &e;
+    }
+    if (kce) {
+      uint32_t pc = 1;
+      while (pc < (uint32_t)(n * kce->lpi)) pc <<= 1;
+      if (pc <= 64 && (uint64_t)n_stacks * pc <= kSimds * 64) {
+        ke = kce;
+        pad = pc;
+      }
+    }
+  }
   const size_t o_mag = align_up(need * sizeof(double));
   const size_t o_st = align_up(o_mag + sizeof(double) * 3 * CMTJ_MAX_LAYERS * nj);
   const size_t o_stk = align_up(o_st + sizeof(cmtj_status) * nj);

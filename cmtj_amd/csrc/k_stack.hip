@@ -31,4 +31,20 @@ static const KEntry kS[] = {
      "k_junction<double,2,EulerHeun,stack>"},
 };
 KTable kernels_stack() { return {kS, sizeof(kS) / sizeof(kS[0])}; }
+
+// The same stacks with one lane per vector component (4 lanes per layer):
+// the exact STT members, the general coupled set (callback drivers read at
+// the RK4 stage times) and the stochastic set for Euler-Heun.
+#define CMTJ_KSCL(L, SOLVER, FM, NAME)                                                                   \
+  {CMTJ_FP64, L, 4 * (L), SOLVER, FM, false,                                                            \
+   (const void*)&k_junction_cl<L, FM, -1, true, SOLVER, 0, true>, "k_junction_cl<double," #L "," NAME ",stack>", 0}
+static const KEntry kSC[] = {
+    CMTJ_KSCL(1, CMTJ_RK4, FM_STT1, "RK4,anis+STT"),
+    CMTJ_KSCL(2, CMTJ_RK4, FM_STT2, "RK4,anis+IEC+STT"),
+    CMTJ_KSCL(1, CMTJ_RK4, FM_STTC | F_TABLE, "RK4,FM_STTC|F_TABLE"),
+    CMTJ_KSCL(2, CMTJ_RK4, FM_STTC | F_TABLE, "RK4,FM_STTC|F_TABLE"),
+    CMTJ_KSCL(1, CMTJ_EULER_HEUN, FM_STTC | F_THERMAL, "EulerHeun,FM_STTC|F_THERMAL"),
+    CMTJ_KSCL(2, CMTJ_EULER_HEUN, FM_STTC | F_THERMAL, "EulerHeun,FM_STTC|F_THERMAL"),
+};
+KTable kernels_stack_cl() { return {kSC, sizeof(kSC) / sizeof(kSC[0])}; }
 }  // namespace cmtj

@@ -908,10 +908,11 @@ __device__ __forceinline__ StackLane stack_lane(const cmtj_stack* sk, int j, int
   return s;
 }
 
+// m_top . p of a member whose layers are in this lane's registers (p: its
+// bottom layer, or the stack's polarisation vector), c_dot of
+// core/junction.hpp:82-84.
 template <int LM>
-__device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n, int base, int ti, int bi,
-                                              const V3<double> (&m)[LM], double t, double dt, const double* tab,
-                                              uint32_t step, double (&I)[3], double& Ilog, double& Clog) {
+__device__ __forceinline__ double stack_d(const StackLane& sk, int ti, int bi, const V3<double> (&m)[LM]) {
   V3<double> mt = m[0], mb = m[0];
 #pragma unroll
   for (int l = 0; l < LM; ++l) {
@@ -919,9 +920,15 @@ __device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n,
     mb = (l == bi) ? m[l] : mb;
   }
   const V3<double> p = sk.two_layer ? mb : mk<double>(sk.pol[0], sk.pol[1], sk.pol[2]);
-  const double d = mt.x * p.x + mt.y * p.y + mt.z * p.z;  // c_dot, core/junction.hpp:82-84
-  // base: the wave lane of member 0 of this stack
-  const double dn = __shfl(d, base + ((j + 1 < n) ? j + 1 : j), 64);
+  return mt.x * p.x + mt.y * p.y + mt.z * p.z;
+}
+
+// d: this member's m_top . p; member q of the stack holds its copy in wave
+// lane base + q stride (stride: lanes per member).
+__device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n, int base, int stride, double d,
+                                              double t, double dt, const double* tab, uint32_t step,
+                                              double (&I)[3], double& Ilog, double& Clog) {
+  const double dn = __shfl(d, base + ((j + 1 < n) ? j + 1 : j) * stride, 64);
   const double Cj = (j + 1 < n) ? sk.cj * (sk.series ? d + dn : d - dn) : 0.0;
   if (sk.kcl) {
     // totalCurrent = I0 + sum_j C_j I0, the same constant for every member;
@@ -929,7 +936,10 @@ __device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n,
     const double I0 = (sk.current.type == CMTJ_TABLE) ? tab[(size_t)sk.current.table_index + step]
                                                       : driver_value(sk.current, t);
     double total = I0;
-    for (int q = 0; q + 1 < n; ++q) total += __shfl(Cj, base + q, 64) * I0;
+    // unrolled so the lane reads of several pairs are in flight together;
+    // the sum keeps the reference's order
+#pragma unroll 4
+    for (int q = 0; q + 1 < n; ++q) total += __shfl(Cj, base + q * stride, 64) * I0;
     I[0] = I[1] = I[2] = total;
     Ilog = total;
     Clog = Cj;
@@ -937,8 +947,9 @@ __device__ __forceinline__ void stack_current(const StackLane& sk, int j, int n,
     // effectiveCoupling *= (1 + C_{j-1}) along the chain; the member's driver
     // is current * effectiveCoupling with its phase shifted
     double eff = 1.0;
+#pragma unroll 4
     for (int q = 0; q + 1 < n; ++q) {
-      const double cq = __shfl(Cj, base + q, 64);
+      const double cq = __shfl(Cj, base + q * stride, 64);
       if (q < j) eff *= (1 + cq);
     }
     Clog = eff * (1 + Cj);
@@ -1285,7 +1296,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
       if (sl.nd)
         reps = s_n;
       else
-        stack_current<LM>(sl, s_j, s_n, s_base, s_ti, s_bi, m, t, dt, g.tables, i, sI, sIlog, sClog);
+        stack_current(sl, s_j, s_n, s_base, 1, stack_d<LM>(sl, s_ti, s_bi, m), t, dt, g.tables, i, sI, sIlog, sClog);
     }
     // one sub-step: the whole step, or member sq's turn in a non-delayed
     // stack (act: this lane commits); true when the run stops here

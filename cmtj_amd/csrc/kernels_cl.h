@@ -149,8 +149,17 @@ __device__ __forceinline__ void scalar_add(CLDrive& d, int kind, double v) {
 // (the drop-in's arithmetic) instead of the anchored binary32 increments.
 // LOGF: 0 reads the log layout from the group; 1 compiles in m_x of every
 // layer, the classic resistance and binary32 entries (the PIMM sweep's log).
-template <int L, uint32_t FM, int ND, bool F64 = false, int SOLVER = CMTJ_RK4, int LOGF = 0>
+// STK: the instances are members of electrically coupled stacks
+// (Stack<T>::runSimulationKCL / runSimulationNonKCL with delayed coupling,
+// core/stack.hpp:305-410): stack s on lanes [s pad, s pad + members x 4L)
+// of one wave, member j on the 4L lanes from 4L j; the pair couplings come
+// from the members' magnetisations at the start of the step and the stack
+// current replaces the members' current at every stage, as in k_junction's
+// stack form (binary64 only).
+template <int L, uint32_t FM, int ND, bool F64 = false, int SOLVER = CMTJ_RK4, int LOGF = 0, bool STK = false>
 __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
+  static_assert(!STK || (F64 && LOGF == 0 && !(FM & F_KSTEP)),
+                "stacks: binary64, the group's log layout, every step through the same code");
   static_assert(L == 1 || L == 2 || L == 4, "4L lanes per instance must divide a 16-lane DPP row");
   static_assert(F64 || !(FM & (F_STT | F_SOT | F_SOT2 | F_DYN | F_THERMAL | F_ANIS2 | F_IEC2 | F_DEMAG_OFF |
                                 F_IDMI | F_DIPOLE)),
@@ -161,9 +170,28 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   static_assert(!(FM & F_TABLE) || (F64 && SOLVER == CMTJ_RK4), "tabulated drivers: binary64 RK4 only");
   constexpr int LPI = 4 * L;
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t inst = gl / LPI;
-  if (inst >= g.n) return;
-  const int q = (int)(gl % LPI);
+  uint32_t inst;
+  int q;
+  [[maybe_unused]] const cmtj_stack* sk = nullptr;
+  [[maybe_unused]] int s_j = 0, s_n = 0, s_base = 0, s_lane0 = 0;
+  if constexpr (STK) {
+    // lanes of a stack past its members leave at once; a stack never
+    // straddles a wave (pad is a power of two of at most 64)
+    const uint32_t pad = g.stack_pad, sid = gl / pad;
+    if (sid >= g.stack_n) return;
+    sk = g.stack + sid;
+    s_n = sk->n_junctions;
+    s_j = (int)((gl % pad) / LPI);
+    if (s_j >= s_n) return;
+    q = (int)(gl % LPI);
+    inst = sid * (uint32_t)s_n + (uint32_t)s_j;
+    s_base = (int)(threadIdx.x & ~(pad - 1));
+    s_lane0 = s_base + s_j * LPI;
+  } else {
+    inst = gl / LPI;
+    if (inst >= g.n) return;
+    q = (int)(gl % LPI);
+  }
   const int ly = q >> 2, c = q & 3;
   const uint32_t fl = g.flags;
   const int nl = g.nl;
@@ -328,6 +356,19 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   double bot = dpp_d<DPP_ROW_SHR4>(m), top = dpp_d<DPP_ROW_SHL4>(m);
   bot = hasB ? bot : 0.0;
   top = hasT ? top : 0.0;
+  // stacks: the member's constants and this step's stack current (stage
+  // times t, t + dt/2, t + dt) with the stack log's I_j and C_j_{j+1}
+  [[maybe_unused]] StackLane sl;
+  [[maybe_unused]] int s_ti = 0, s_bi = 0;
+  [[maybe_unused]] double sI[3] = {0.0, 0.0, 0.0};
+  [[maybe_unused]] double sIlog = 0.0, sClog = 0.0;
+  [[maybe_unused]] unsigned long long s_mask = 0ull;  // the stack's lanes
+  if constexpr (STK) {
+    sl = stack_lane(sk, s_j, s_n);
+    s_ti = sk->top[s_j];
+    s_bi = sk->bottom[s_j];
+    s_mask = ((g.stack_pad >= 64) ? ~0ull : ((1ull << g.stack_pad) - 1)) << (unsigned)s_base;
+  }
 
   // One step of the loop.  DRV: the step's stage times may meet a
   // time-dependent record; without it the driver code is left out (kernels
@@ -338,6 +379,24 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     constexpr bool DRV = decltype(drv_tag)::value;
     const double t = (double)i * dt;
     double mn;
+    if constexpr (STK) {
+      // pair couplings from the members' m at the start of the step
+      // (storeMagnetisations, core/stack.hpp:65-76): d = m_top . p summed
+      // over the member's top-layer quad (in the reference's order), p its
+      // bottom layer's m or the polarisation vector
+      // (members have one or two layers: the bottom layer is the top layer's
+      // neighbour quad, whose start-of-step m this lane holds as top / bot)
+      const double pol_c = (c == 0) ? sl.pol[0] : (c == 1) ? sl.pol[1] : (c == 2) ? sl.pol[2] : 0.0;
+      double pc = pol_c;
+      if constexpr (L > 1) pc = sl.two_layer ? ((s_bi > s_ti) ? top : bot) : pol_c;
+      double dq = qsum(m * pc);  // d in the top layer's quad
+      if constexpr (L > 1) {
+        // and in the member's other quads, by row shifts of 4 lanes
+        const double below = dpp_d<DPP_ROW_SHR4>(dq), above = dpp_d<DPP_ROW_SHL4>(dq);
+        dq = (ly == s_ti) ? dq : (ly == s_ti + 1) ? below : above;
+      }
+      stack_current(sl, s_j, s_n, s_base, LPI, dq, t, dt, g.tables, i, sI, sIlog, sClog);
+    }
     if constexpr (F64) {
       // rk4_step (core/junction.hpp:1017-1041) in binary64: every stage
       // evaluates the same terms (calculateHeffDipoleInjection, :590-622;
@@ -348,7 +407,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // drives at ts: this component's field slots and the layer's scalar
       // slots, transcendental waveforms in full binary64 (the binary32 path
       // below takes the faster single-precision ones)
-      auto drive = [&](double ts, int stage) -> CLDrive {
+      auto drive0 = [&](double ts, int stage) -> CLDrive {
         CLDrive d{K.hb, K.bK, K.bJt, K.bJb, K.bI, K.bFL, K.bDL, K.bT, K.bK2, K.bJ2t, K.bJ2b, K.bFL2, K.bDL2, K.bDt, K.bDb};
         if (!DRV || !(CMTJ_HAS(F_DYNH) || CMTJ_HAS(F_DYN))) return d;
         if constexpr ((FM & F_KSTEP) != 0) {
@@ -369,6 +428,13 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           d.H = d.H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
           if constexpr ((FM & F_DYN) != 0) scalar_add(d, lact ? scalar_slot(g.dmeta[r], ly, c) : 0, v);
         }
+        return d;
+      };
+      // a stack member's current is the stack's at every stage
+      // (executeJunctionStep, core/stack.hpp:78-91)
+      auto drive = [&](double ts, int stage) -> CLDrive {
+        CLDrive d = drive0(ts, stage);
+        if constexpr (STK) d.I = (stage == 0) ? sI[0] : (stage == 1) ? sI[1] : sI[2];
         return d;
       };
       // dm/dt dt at (ms, drives d) with neighbour components tp (top) and
@@ -603,6 +669,14 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         code = CMTJ_ERR_NAN;
         return true;
       }
+      if constexpr (STK) {
+        // a member's NaN ends its stack's run (the reference throws): the
+        // other members take this step and stop with it
+        if (bad & s_mask) {
+          m = mn;
+          return true;
+        }
+      }
     }
     m = mn;
     bot = dpp_d<DPP_ROW_SHR4>(m);
@@ -635,6 +709,13 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         if (q == 0) {
           log_put(f32log, p + orr, 1 / rx);
           log_put(f32log, p + orr + cs, 1 / ry);
+        }
+      }
+      if constexpr (STK) {
+        if (logging && q == 0) {  // the member's stack channels follow its junction channels
+          const uint64_t nch = (uint64_t)log_channels(nl, g.mr_mode, g.log_mask);
+          log_put(f32log, p + nch * cs, sIlog);
+          log_put(f32log, p + (nch + 1) * cs, sClog);
         }
       }
       ++k;

@@ -54,6 +54,7 @@ KTable kernels_exact();
 KTable kernels_cl();
 KTable kernels_cl64();
 KTable kernels_stack();  // coupled stacks: chosen by cmtj_run_stacks only
+KTable kernels_stack_cl();  // coupled stacks on one lane per component (binary64)
 KTable kernels_dp();     // Dormand-Prince (FP64)
 const void* dp_queue_kernel();   // single-layer Dormand-Prince from a device work queue
 const void* clear_u32_kernel();  // clears a launch's work counters

@@ -269,6 +269,13 @@ int cmtj_abi(cmtj_abi_info* out);
 /* Message of the last failing call on this thread ("" if none). */
 const char* cmtj_last_error(void);
 
+/* Names of the kernels the calling thread's last run launched ("name;" per
+ * structure group; empty before any).  No counterpart in the reference: it
+ * tells a caller which kernel form (one lane per instance, per layer or per
+ * component) a cmtj_run / cmtj_run_stacks / cmtj_run_groups / plan execute
+ * chose. */
+const char* cmtj_last_kernels(void);
+
 /* Process-wide constants store behind the Python PhysicalConstants class.
  * Replaces the static members and setters/getters of
  * core/constants.hpp:6-53 (bound at python/cmtj.cpp:496-511). */

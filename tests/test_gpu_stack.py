@@ -6,7 +6,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from cmtj_amd import AxialDriver, ScalarDriver, constantDriver, stepDriver
+from cmtj_amd import AxialDriver, ScalarDriver, _lib, constantDriver, stepDriver
 from cmtj_amd.stack import ParallelStack, SeriesStack
 from oracle_backend import OracleParallelStack, OracleSeriesStack
 from test_stack_cpu import _single, _two
@@ -51,9 +51,12 @@ def test_stack_matches_oracle(member, kcl, series):
         s.setCoupledCurrentDriver(drv)
         s.setExternalFieldDriver(AxialDriver(constantDriver(1e4), constantDriver(0), constantDriver(2e4)))
         s.runSimulation(0.5e-9, 1e-13, 1e-12)
+    kern = _lib.last_kernels()  # the GPU stack ran first
     worst = _compare(*stacks)
-    print(f"[stack] {member.__name__} kcl={kcl} series={series}: max relative difference {worst:.3e}", flush=True)
+    print(f"[stack] {member.__name__} kcl={kcl} series={series}: {kern}, max relative difference {worst:.3e}",
+          flush=True)
     assert worst <= 1e-8
+    assert len(kern) == 1 and kern[0].startswith("k_junction_cl<double")  # one lane per component
 
 
 def test_stochastic_stack_matches_oracle_pathwise():
@@ -144,6 +147,9 @@ def test_non_delayed_stack_matches_oracle(series, n):
         s.setCoupledCurrentDriver(drv)
         s.setExternalFieldDriver(AxialDriver(constantDriver(1e4), constantDriver(0), constantDriver(2e4)))
         s.runSimulation(0.5e-9, 1e-13, 1e-12)
+        if s is stacks[0]:
+            kern = _lib.last_kernels()
+            assert len(kern) == 1 and kern[0].startswith("k_junction<double")  # sub-steps: one lane per member
     worst = _compare(*stacks)
     print(f"[stack] non-delayed n={n} series={series}: max relative difference {worst:.3e}", flush=True)
     assert worst <= 1e-8
@@ -199,4 +205,35 @@ def test_callback_currents_and_drivers_match_oracle(series):
         s.runSimulation(0.3e-9, 1e-13, 1e-12)
     worst = _compare(*stacks)
     print(f"[stack] callbacks series={series}: max relative difference {worst:.3e}", flush=True)
+    assert worst <= 1e-8
+
+
+def test_large_sweep_keeps_one_lane_per_member_and_matches_component_lanes():
+    """A sweep larger than the chip's SIMDs at one wave each keeps one lane
+    per member; the same stacks run few at a time take the component-lane
+    kernel: both against the oracle."""
+    from cmtj_amd.stack import run_stacks
+
+    def build(k, cls_s):
+        js = _members(_single, n=8)
+        s = cls_s(js)
+        s.setCouplingStrength(0.02 + 0.001 * (k % 17))
+        s.setCoupledCurrentDriver(constantDriver(5e11 + 1e9 * (k % 5)))
+        return s
+
+    big = [build(k, SeriesStack) for k in range(2100)]  # 2100 x 32 lanes > 1024 waves
+    run_stacks(big, 2e-11, 1e-13, 1e-12)
+    kb = _lib.last_kernels()
+    few = [build(k, SeriesStack) for k in range(3)]
+    run_stacks(few, 2e-11, 1e-13, 1e-12)
+    kf = _lib.last_kernels()
+    print(f"[stack] 2100 stacks: {kb}; 3 stacks: {kf}", flush=True)
+    assert kb[0].startswith("k_junction<double") and kf[0].startswith("k_junction_cl<double")
+    worst = 0.0
+    for k in (0, 1, 2, 999, 2099):
+        o = build(k, OracleSeriesStack)
+        o.runSimulation(2e-11, 1e-13, 1e-12)
+        worst = max(worst, _compare(big[k], o))
+        if k < 3:
+            worst = max(worst, _compare(few[k], o))
     assert worst <= 1e-8
