@@ -21,9 +21,12 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# The serving leg runs one stream per structure group; with HIP's default of 4
-# hardware queues a fifth group would queue behind another group's slices.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# The serving leg runs one stream per structure group; with 4 hardware queues
+# (HIP's default, and what some environments export) a fifth group queues
+# behind another group's slices.  HIP reads the count once, at its first call,
+# which here comes before the serving leg: raise it now.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 FLOP_PER_STEP = 666.0  # algorithmic FLOP per junction-step of this 2-layer RK4 workload
 PEAK_FP32 = 157.3      # MI355X published FP32 vector peak, TFLOP/s

@@ -22,8 +22,10 @@ what else is in flight (with the lane mapping fixed, as the server does).
 from __future__ import annotations
 
 import bisect
+import os
 import threading
 import time
+import warnings
 from collections import deque
 from concurrent.futures import Future
 from dataclasses import dataclass, field
@@ -36,6 +38,30 @@ from .core import SolverMode
 
 
 _cat = _lib.cat_records
+
+
+def _raise_hw_queues(torch, want: int) -> bool:
+    """HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues
+    (4 unless set), read once when HIP initialises; kernels of two streams
+    that share a queue run one after the other, however idle the chip is.
+    The server runs one stream per structure group, so it raises the count
+    to `want` (at most 32) while HIP is not yet initialised in this process,
+    and warns when that is too late.  Returns whether the count now holds
+    `want` queues."""
+    want = min(int(want), 32)
+    try:
+        have = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        have = 4
+    if have >= want:
+        return True
+    if torch.cuda.is_initialized():
+        warnings.warn(f"GPU_MAX_HW_QUEUES is {have} and HIP is already initialised: groups beyond {have} share "
+                      f"hardware queues and wait for each other's slices (set GPU_MAX_HW_QUEUES={want} before "
+                      f"the first GPU call)", RuntimeWarning, stacklevel=3)
+        return False
+    os.environ["GPU_MAX_HW_QUEUES"] = str(want)
+    return True
 
 
 class LogArena:
@@ -136,11 +162,15 @@ class Server:
     group's kernel then depends on its structure only, never on how many
     requests share it, so a result is the same alone and in a mix);
     n_streams: streams created up front (more are created when more groups
-    are active)."""
+    are active); hw_queues: hardware queues asked of HIP through
+    GPU_MAX_HW_QUEUES when the server starts before HIP is initialised
+    (None leaves the environment alone).  With HIP's default of 4, a fifth
+    concurrent group shares a queue with another and both take twice as
+    long."""
 
     def __init__(self, precision: str = "fp32", log_elements: int = 1 << 28, max_active: int = 65536,
                  slice_ms: float = 20.0, slice_steps: int = 5000, lanes: str = "per_component", n_streams: int = 8,
-                 log_dtype=None, device: str = "cuda") -> None:
+                 log_dtype=None, device: str = "cuda", hw_queues: int | None = 16) -> None:
         if lanes not in ("per_component", "per_layer", "one"):
             raise ValueError("a server fixes the lane mapping: 'per_component', 'per_layer' or 'one'")
         self.precision = _lib.FP32 if precision in ("fp32", _lib.FP32) else _lib.FP64
@@ -153,6 +183,7 @@ class Server:
         self.slice_ms = float(slice_ms)
         self.slice_steps = int(slice_steps)  # first slice of a new group
         self.n_streams = max(1, int(n_streams))
+        self.hw_queues = None if hw_queues is None else max(1, int(hw_queues))
         self.device = device
         self._pending: deque = deque()
         self._groups: dict = {}
@@ -202,6 +233,8 @@ class Server:
         if self._thread is None:
             import torch
 
+            if self.hw_queues is not None:
+                _raise_hw_queues(torch, self.hw_queues)
             d = torch.device(self.device)
             self.device = torch.device(d.type, d.index if d.index is not None else torch.cuda.current_device())
             self._thread = threading.Thread(target=self._loop, name="cmtj-server", daemon=True)

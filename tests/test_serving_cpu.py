@@ -2,6 +2,8 @@
 validation (no GPU needed)."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -94,3 +96,37 @@ def test_each_submission_takes_its_own_noise_stream():
     assert min(b.js["stream"].tolist()) > max(p[1] for p in pairs)
     r = JunctionBatch.replicate(j, 4, 1e-10, 1e-13, 1e-13)
     assert r.js["stream"].tolist() == list(range(5, 9)) and j._runs == 9
+
+
+class _FakeCuda:
+    def __init__(self, initialised: bool) -> None:
+        self._init = initialised
+
+    def is_initialized(self) -> bool:
+        return self._init
+
+
+class _FakeTorch:
+    def __init__(self, initialised: bool) -> None:
+        self.cuda = _FakeCuda(initialised)
+
+
+def test_server_raises_hardware_queues_before_hip_starts(monkeypatch):
+    """Streams beyond GPU_MAX_HW_QUEUES share hardware queues (their kernels
+    serialise), so the server raises the count while HIP is uninitialised,
+    keeps a larger one, caps at 32 and only warns once HIP has started."""
+    from cmtj_amd.serving import _raise_hw_queues
+
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "4")
+    assert _raise_hw_queues(_FakeTorch(False), 16)
+    assert os.environ["GPU_MAX_HW_QUEUES"] == "16"
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "24")
+    assert _raise_hw_queues(_FakeTorch(False), 16)
+    assert os.environ["GPU_MAX_HW_QUEUES"] == "24"
+    monkeypatch.delenv("GPU_MAX_HW_QUEUES")
+    assert _raise_hw_queues(_FakeTorch(False), 100)
+    assert os.environ["GPU_MAX_HW_QUEUES"] == "32"
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "4")
+    with pytest.warns(RuntimeWarning, match="share hardware queues"):
+        assert not _raise_hw_queues(_FakeTorch(True), 16)
+    assert os.environ["GPU_MAX_HW_QUEUES"] == "4"
