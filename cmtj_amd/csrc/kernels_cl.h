@@ -49,6 +49,44 @@ template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
+// v of the lane CTRL names, written only to the lanes of the banks (groups
+// of 4 lanes in a 16-lane row) that BANK enables; the other lanes keep old.
+template <int CTRL, int BANK>
+__device__ __forceinline__ double dpp_keep(double old, double v) {
+  const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)(o & 0xffffffffull), (int)(unsigned)(u & 0xffffffffull),
+                                             CTRL, 0xF, BANK, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), CTRL, 0xF, BANK,
+                                             false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// Banks whose quad has a layer below (UP = false) or above (UP = true) in
+// the same instance, for instances of L = 1, 2 or 4 layers (one quad per
+// layer, instances aligned to 4L lanes): the row shifts by 4 lanes then
+// never read another instance's lanes, and the lanes of the other banks
+// keep the zero they start with.
+template <int L, bool UP>
+constexpr int neighbour_banks() {
+  int m = 0;
+  for (int b = 0; b < 4; ++b) {
+    const int ly = b % L;
+    if (UP ? (ly + 1 < L) : (ly > 0)) m |= 1 << b;
+  }
+  return m;
+}
+// The smallest nonzero n over the active lanes of the wave (0 if none):
+// wave-uniform, read lane by lane through the ballot.
+__device__ __forceinline__ uint32_t wave_min_nonzero(uint32_t n) {
+  unsigned long long act = __ballot(n != 0u);
+  uint32_t best = 0u;
+  while (act) {
+    const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)n, (int)__builtin_ctzll(act));
+    best = (best == 0u || v < best) ? v : best;
+    act &= act - 1ull;
+  }
+  return best;
+}
 template <int CTRL>
 __device__ __forceinline__ double dpp_x(double v) {
   return dpp_d<CTRL>(v);
@@ -215,7 +253,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   [[maybe_unused]] const double sdt = sqrt(dt);
   constexpr int sps = 3;
   const bool hasB = lact && ly > 0, hasT = lact && ly + 1 < nl;
-  const uint32_t gshift = threadIdx.x & ~(uint32_t)(LPI - 1);
+  const uint32_t gshift = (threadIdx.x & 63u) & ~(uint32_t)(LPI - 1);  // the instance's first lane in the wave
 
   // per-lane constants (loadLayer parameters of this layer, this component)
   CLConst K;
@@ -353,6 +391,12 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   uint64_t next_log = (uint64_t)k * W;
   // lanes whose NaN stops a run: component x of every existing layer
   const unsigned long long nan_lanes = __ballot(live && c == 0);
+  // neighbour layers' m (start of step): lanes without a neighbour in the
+  // instance hold 0 from here on, and the per-step row shifts (dpp_keep)
+  // leave them alone.  Lanes of layers the junction does not have (nl < L)
+  // hold m = 0 throughout (no field term moves a zero vector), so the
+  // existing layer next to them reads 0 from them.
+  constexpr int BANK_B = neighbour_banks<L, false>(), BANK_T = neighbour_banks<L, true>();
   double bot = dpp_d<DPP_ROW_SHR4>(m), top = dpp_d<DPP_ROW_SHL4>(m);
   bot = hasB ? bot : 0.0;
   top = hasT ? top : 0.0;
@@ -373,10 +417,15 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   // One step of the loop.  DRV: the step's stage times may meet a
   // time-dependent record; without it the driver code is left out (kernels
   // whose records are all step drivers run the steps outside the records'
-  // windows that way: those records are exactly 0 there).  Returns true when
-  // the run stops (a NaN).
-  auto step = [&](auto drv_tag) __attribute__((always_inline)) -> bool {
+  // windows that way: those records are exactly 0 there).  LA: every lane
+  // of the wave writes a log entry at every step.  `hitb` (wave-uniform)
+  // keeps the step's NaN ballot: when it is not 0, only the lanes whose run
+  // stops on it have taken the step (their m as the reference leaves it);
+  // the others left their state alone and take the step again.
+  unsigned long long hitb = 0ull;
+  auto step = [&](auto drv_tag, auto la_tag) __attribute__((always_inline)) {
     constexpr bool DRV = decltype(drv_tag)::value;
+    constexpr bool LA = decltype(la_tag)::value;
     const double t = (double)i * dt;
     double mn;
     if constexpr (STK) {
@@ -531,9 +580,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           g1 = K.therm ? stoch_c(m, dW) : 0.0;
         }
         const double mN = m + k1 + g1 * sdt;
-        double bN = dpp_d<DPP_ROW_SHR4>(mN), tN = dpp_d<DPP_ROW_SHL4>(mN);
-        bN = hasB ? bN : 0.0;
-        tN = hasT ? tN : 0.0;
+        const double bN = dpp_keep<DPP_ROW_SHR4, BANK_B>(bot, mN), tN = dpp_keep<DPP_ROW_SHL4, BANK_T>(top, mN);
         const double k2 = f(mN, drive(t + dt, 1), tN, bN);
         mn = m + (k1 + k2) * 0.5;
         if (CMTJ_HAS(F_THERMAL)) {
@@ -657,73 +704,124 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       y = y * (1.5 - (0.5 * s) * (y * y));
       mn = (s != 0.0) ? mn * y : mn;
     }
+    // the end of a step that goes on: new m, neighbours, log entry
+    auto finish = [&]() __attribute__((always_inline)) {
+      m = mn;
+      bot = dpp_keep<DPP_ROW_SHR4, BANK_B>(bot, m);
+      top = dpp_keep<DPP_ROW_SHL4, BANK_T>(top, m);
+      if (LA || (uint64_t)i == next_log) {
+        char* const p = lp + (uint64_t)k * esize;
+        if (wm) log_put(f32log, p + om, m);
+        if (classic) {
+          // calculateMagnetoresistance (:1865-1867): layer 0 against layer 1,
+          // one layer against the fixed reference vector (:1623-1629)
+          const double cosT = qsum(m * (nl == 1 ? K.ref : top));
+          if (q == 0) log_put(f32log, p + orr, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
+        } else if (strip) {
+          // stripMagnetoResistance (:1835-1857), parallel sum of the layers
+          const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
+          double rx = 0.0, ry = 0.0;
+          if (lact && c == 0) {
+            const double Rx = sRx0 + sAMRX * (mx * mx) + sSMRX * (my * my);
+            const double Ry = sRy0 + sAHE2 * mz + sXY * mx * my;
+            rx = 1. / Rx;
+            ry = 1. / Ry;
+          }
+#pragma unroll
+          for (int o = 4; o < LPI; o <<= 1) {
+            rx += __shfl_xor(rx, o, 64);
+            ry += __shfl_xor(ry, o, 64);
+          }
+          if (q == 0) {
+            log_put(f32log, p + orr, 1 / rx);
+            log_put(f32log, p + orr + cs, 1 / ry);
+          }
+        }
+        if constexpr (STK) {
+          if (logging && q == 0) {  // the member's stack channels follow its junction channels
+            const uint64_t nch = (uint64_t)log_channels(nl, g.mr_mode, g.log_mask);
+            log_put(f32log, p + nch * cs, sIlog);
+            log_put(f32log, p + (nch + 1) * cs, sClog);
+          }
+        }
+        ++k;
+        if constexpr (!LA) next_log += W;
+      }
+    };
     // the reference updates layers in turn and stops at the first NaN
     // layer; layers up to and including it take the new value (RK4 only:
     // the Euler-Heun step of the reference has no such test)
     const unsigned long long bad = (SOLVER == CMTJ_RK4) ? (__ballot(isnan(mn)) & nan_lanes) : 0ull;
-    if (bad) {  // wave-uniform test first: the per-instance work only on the rare path
-      const uint32_t gb = (uint32_t)((bad >> gshift) & ((1ull << LPI) - 1));
-      if (gb) {
-        nan_layer = __builtin_ctz(gb) >> 2;
-        if (ly <= nan_layer) m = mn;
-        code = CMTJ_ERR_NAN;
-        return true;
-      }
-      if constexpr (STK) {
-        // a member's NaN ends its stack's run (the reference throws): the
-        // other members take this step and stop with it
-        if (bad & s_mask) {
-          m = mn;
-          return true;
-        }
-      }
+    hitb = bad;
+    if (__builtin_expect(bad == 0ull, 1)) {  // the common case, wave-uniform: no NaN in the wave
+      finish();
+      return;
     }
-    m = mn;
-    bot = dpp_d<DPP_ROW_SHR4>(m);
-    top = dpp_d<DPP_ROW_SHL4>(m);
-    bot = hasB ? bot : 0.0;
-    top = hasT ? top : 0.0;
-    if ((uint64_t)i == next_log) {
-      char* const p = lp + (uint64_t)k * esize;
-      if (wm) log_put(f32log, p + om, m);
-      if (classic) {
-        // calculateMagnetoresistance (:1865-1867): layer 0 against layer 1,
-        // one layer against the fixed reference vector (:1623-1629)
-        const double cosT = qsum(m * (nl == 1 ? K.ref : top));
-        if (q == 0) log_put(f32log, p + orr, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
-      } else if (strip) {
-        // stripMagnetoResistance (:1835-1857), parallel sum of the layers
-        const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
-        double rx = 0.0, ry = 0.0;
-        if (lact && c == 0) {
-          const double Rx = sRx0 + sAMRX * (mx * mx) + sSMRX * (my * my);
-          const double Ry = sRy0 + sAHE2 * mz + sXY * mx * my;
-          rx = 1. / Rx;
-          ry = 1. / Ry;
-        }
-#pragma unroll
-        for (int o = 4; o < LPI; o <<= 1) {
-          rx += __shfl_xor(rx, o, 64);
-          ry += __shfl_xor(ry, o, 64);
-        }
-        if (q == 0) {
-          log_put(f32log, p + orr, 1 / rx);
-          log_put(f32log, p + orr + cs, 1 / ry);
-        }
-      }
-      if constexpr (STK) {
-        if (logging && q == 0) {  // the member's stack channels follow its junction channels
-          const uint64_t nch = (uint64_t)log_channels(nl, g.mr_mode, g.log_mask);
-          log_put(f32log, p + nch * cs, sIlog);
-          log_put(f32log, p + (nch + 1) * cs, sClog);
-        }
-      }
-      ++k;
-      next_log += W;
+    // a NaN in the wave (rare): layers up to and including the instance's
+    // first NaN layer take the new value; a member's NaN ends its stack's
+    // run (the reference throws), the other members take this step and stop
+    // with it
+    const uint32_t gb = (uint32_t)((bad >> gshift) & ((1ull << LPI) - 1));
+    if (gb) {
+      if (ly <= (int)(__builtin_ctz(gb) >> 2)) m = mn;
+    } else if constexpr (STK) {
+      if (bad & s_mask) m = mn;
     }
-    return false;
   };
   bool stopped = false;
+  // Every lane logs every step when each has W = 1 and its next entry due
+  // now (then entries and steps advance together).
+  const bool log_all = __ballot(!(W == 1u && next_log == (uint64_t)i)) == 0ull;
+  // Steps of one phase (the lane's steps below lim) in wave-uniform passes:
+  // each pass runs the fewest steps any lane of the wave still has in the
+  // phase on a scalar counter, so the loop needs no per-lane exit test;
+  // lanes done with the phase sit out the passes that follow.  A NaN in any
+  // lane ends the pass at that step, which no lane counts: the lanes of
+  // the instances (stacks) it stops end their run there (steps done = i),
+  // the others take the step again in the next pass (the same inputs: the
+  // same result).
+  auto phase = [&](uint32_t lim, auto drv_tag, auto la_tag) __attribute__((always_inline)) {
+    for (;;) {
+      const uint32_t n = (!stopped && i < lim) ? lim - i : 0u;
+      const uint32_t seg = wave_min_nonzero(n);
+      if (seg == 0u) break;
+      if (n != 0u) {
+        const uint32_t i0 = i;
+        uint32_t end = seg;  // wave-uniform
+        for (uint32_t u = 0; u < end; ++u) {
+          i = i0 + u;
+          step(drv_tag, la_tag);
+          end = hitb ? u : end;
+        }
+        i = i0 + end;
+        if (hitb) {
+          const uint32_t gb = (uint32_t)((hitb >> gshift) & ((1ull << LPI) - 1));
+          bool stop = gb != 0u;
+          if constexpr (STK) stop = stop || (hitb & s_mask) != 0ull;
+          if (gb) {
+            nan_layer = __builtin_ctz(gb) >> 2;
+            code = CMTJ_ERR_NAN;
+          }
+          // every lane whose NaN is in the ballot belongs to an instance
+          // that stops here, so the pass makes progress; were none to stop,
+          // the lanes would take the same step again for ever: end their
+          // runs with the error instead
+          if (__ballot(stop) == 0ull) {
+            code = CMTJ_ERR_NAN;
+            stop = true;
+          }
+          stopped = stop;
+          hitb = 0ull;
+        }
+      }
+    }
+  };
+  auto run = [&](uint32_t lim, auto drv_tag) __attribute__((always_inline)) {
+    if (log_all)
+      phase(lim, drv_tag, std::true_type{});
+    else
+      phase(lim, drv_tag, std::false_type{});
+  };
   if constexpr ((FM & F_KSTEP) != 0 && !(FM & F_TABLE)) {
     // steps [ia, ib) may meet a record's window [win_lo, win_hi]; every
     // stage time of a step before ia is below win_lo, of a step from ib on
@@ -734,12 +832,11 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       ia = a <= (double)i ? i : (a >= (double)iend ? iend : (uint32_t)a);
       ib = b <= (double)ia ? ia : (b >= (double)iend ? iend : (uint32_t)b);
     }
-    // a stopping step is not counted (steps done = i)
-    while (!stopped && i < ia) stopped = step(std::false_type{}) || (++i, false);
-    while (!stopped && i < ib) stopped = step(std::true_type{}) || (++i, false);
-    while (!stopped && i < iend) stopped = step(std::false_type{}) || (++i, false);
+    run(ia, std::false_type{});
+    run(ib, std::true_type{});
+    run(iend, std::false_type{});
   } else {
-    while (!stopped && i < iend) stopped = step(std::true_type{}) || (++i, false);
+    run(iend, std::true_type{});
   }
   if (g.mag_out && live) g.mag_out[((size_t)jid * CMTJ_MAX_LAYERS + ly) * 3 + c] = m;
   if (g.status && q == 0) {

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
   const MRCoef mcoef = mr_coef(g, J, lyr, logging && act);
   const int sps = (SOLVER == CMTJ_RK4) ? 3 : (SOLVER == CMTJ_HEUN ? 2 : 1);
   const bool hasB = act && ly > 0, hasT = act && ly + 1 < nl;
-  const uint32_t gshift = threadIdx.x & ~(uint32_t)(L - 1);  // first lane of the instance in the wave
+  const uint32_t gshift = (threadIdx.x & 63u) & ~(uint32_t)(L - 1);  // first lane of the instance in the wave
 
   int bits = 0;
 #pragma unroll
