@@ -364,6 +364,13 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   const uint64_t orr = (uint64_t)(nl * nm) * cs;
   const bool classic = (lmask & CMTJ_LOG_MR) && mrm == CMTJ_MR_CLASSIC;
   const bool strip = (lmask & CMTJ_LOG_MR) && mrm == CMTJ_MR_STRIP;
+  // compiled-in layout: the lane's one entry per step (R from the pad lane
+  // of layer 0), and the vector a single layer's R is taken against (a
+  // layer with no layer above holds top = 0 exactly)
+  const bool rlane = (LOGF == 1) && classic && q == 3;
+  const bool wlog = wm || rlane;
+  const uint64_t olog = rlane ? orr : om;
+  const double refR = (nl == 1) ? K.ref : 0.0;
   // STRIP coefficients of the lane's layer, read once (the descriptors may
   // sit in host-mapped memory); 0.5 AHE and AMR_Y + SMR_Y are the reference's
   // own first products, so the entries round as before
@@ -580,7 +587,11 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           g1 = K.therm ? stoch_c(m, dW) : 0.0;
         }
         const double mN = m + k1 + g1 * sdt;
-        const double bN = dpp_keep<DPP_ROW_SHR4, BANK_B>(bot, mN), tN = dpp_keep<DPP_ROW_SHL4, BANK_T>(top, mN);
+        double bN = 0.0, tN = 0.0;
+        if constexpr (L > 1) {
+          bN = dpp_keep<DPP_ROW_SHR4, BANK_B>(bot, mN);
+          tN = dpp_keep<DPP_ROW_SHL4, BANK_T>(top, mN);
+        }
         const double k2 = f(mN, drive(t + dt, 1), tN, bN);
         mn = m + (k1 + k2) * 0.5;
         if (CMTJ_HAS(F_THERMAL)) {
@@ -697,44 +708,58 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       mn = m + k1 + (double)(acc * sixth_inv);
       // CVector::normalize (core/cvector.hpp:185-192) over the quad: binary32
       // reciprocal square root and one Newton step in binary64 (|m| = 1 to
-      // ~2e-14, far inside the increments' binary32 rounding); a zero vector
-      // stays as it is
+      // ~2e-14, far inside the increments' binary32 rounding).  The 1e-30
+      // added to the binary32 s leaves any s near 1 unchanged and a NaN a
+      // NaN, and keeps y finite for a zero vector, which so stays as it is
       const double s = qsum(mn * mn);
-      double y = (double)__frsqrt_rn((float)s);
+      double y = (double)__frsqrt_rn((float)s + 1e-30f);
       y = y * (1.5 - (0.5 * s) * (y * y));
-      mn = (s != 0.0) ? mn * y : mn;
+      mn = mn * y;
     }
     // the end of a step that goes on: new m, neighbours, log entry
     auto finish = [&]() __attribute__((always_inline)) {
       m = mn;
-      bot = dpp_keep<DPP_ROW_SHR4, BANK_B>(bot, m);
-      top = dpp_keep<DPP_ROW_SHL4, BANK_T>(top, m);
+      if constexpr (L > 1) {
+        bot = dpp_keep<DPP_ROW_SHR4, BANK_B>(bot, m);
+        top = dpp_keep<DPP_ROW_SHL4, BANK_T>(top, m);
+      }
       if (LA || (uint64_t)i == next_log) {
         char* const p = lp + (uint64_t)k * esize;
-        if (wm) log_put(f32log, p + om, m);
-        if (classic) {
-          // calculateMagnetoresistance (:1865-1867): layer 0 against layer 1,
-          // one layer against the fixed reference vector (:1623-1629)
-          const double cosT = qsum(m * (nl == 1 ? K.ref : top));
-          if (q == 0) log_put(f32log, p + orr, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
-        } else if (strip) {
-          // stripMagnetoResistance (:1835-1857), parallel sum of the layers
-          const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
-          double rx = 0.0, ry = 0.0;
-          if (lact && c == 0) {
-            const double Rx = sRx0 + sAMRX * (mx * mx) + sSMRX * (my * my);
-            const double Ry = sRy0 + sAHE2 * mz + sXY * mx * my;
-            rx = 1. / Rx;
-            ry = 1. / Ry;
-          }
+        if constexpr (LOGF == 1) {
+          // the compiled-in layout (m_x of every layer, then the classic R;
+          // binary32 entries): one store per writing lane, m_x from the x
+          // lane of each layer, R from the pad lane of layer 0
+          // (calculateMagnetoresistance, :1865-1867, and against the fixed
+          // reference vector for one layer, :1623-1629)
+          const double cosT = qsum(m * (top + refR));
+          const double rv = Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT));
+          if (wlog) log_put(true, p + olog, rlane ? rv : m);
+        } else {
+          if (wm) log_put(f32log, p + om, m);
+          if (classic) {
+            // calculateMagnetoresistance (:1865-1867): layer 0 against layer 1,
+            // one layer against the fixed reference vector (:1623-1629)
+            const double cosT = qsum(m * (nl == 1 ? K.ref : top));
+            if (q == 0) log_put(f32log, p + orr, Rp + (((Rap - Rp) / 2.0) * (1.0 - cosT)));
+          } else if (strip) {
+            // stripMagnetoResistance (:1835-1857), parallel sum of the layers
+            const double mx = dpp_d<QP_BC0>(m), my = dpp_d<QP_BC1>(m), mz = dpp_d<QP_BC2>(m);
+            double rx = 0.0, ry = 0.0;
+            if (lact && c == 0) {
+              const double Rx = sRx0 + sAMRX * (mx * mx) + sSMRX * (my * my);
+              const double Ry = sRy0 + sAHE2 * mz + sXY * mx * my;
+              rx = 1. / Rx;
+              ry = 1. / Ry;
+            }
 #pragma unroll
-          for (int o = 4; o < LPI; o <<= 1) {
-            rx += __shfl_xor(rx, o, 64);
-            ry += __shfl_xor(ry, o, 64);
-          }
-          if (q == 0) {
-            log_put(f32log, p + orr, 1 / rx);
-            log_put(f32log, p + orr + cs, 1 / ry);
+            for (int o = 4; o < LPI; o <<= 1) {
+              rx += __shfl_xor(rx, o, 64);
+              ry += __shfl_xor(ry, o, 64);
+            }
+            if (q == 0) {
+              log_put(f32log, p + orr, 1 / rx);
+              log_put(f32log, p + orr + cs, 1 / ry);
+            }
           }
         }
         if constexpr (STK) {
@@ -753,20 +778,19 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     // the Euler-Heun step of the reference has no such test)
     const unsigned long long bad = (SOLVER == CMTJ_RK4) ? (__ballot(isnan(mn)) & nan_lanes) : 0ull;
     hitb = bad;
-    if (__builtin_expect(bad == 0ull, 1)) {  // the common case, wave-uniform: no NaN in the wave
-      finish();
+    if (__builtin_expect(bad != 0ull, 0)) {
+      // a NaN in the wave (rare): layers up to and including the instance's
+      // first NaN layer take the new value; a member's NaN ends its stack's
+      // run (the reference throws), the other members take this step and
+      // stop with it (selects, no per-lane branch: the whole test stays
+      // wave-uniform)
+      const uint32_t gb = (uint32_t)((bad >> gshift) & ((1ull << LPI) - 1));
+      bool take = (gb != 0u) & (ly <= (int)(__builtin_ctz(gb | 0x80000000u) >> 2));
+      if constexpr (STK) take = take | ((gb == 0u) & ((bad & s_mask) != 0ull));
+      m = take ? mn : m;
       return;
     }
-    // a NaN in the wave (rare): layers up to and including the instance's
-    // first NaN layer take the new value; a member's NaN ends its stack's
-    // run (the reference throws), the other members take this step and stop
-    // with it
-    const uint32_t gb = (uint32_t)((bad >> gshift) & ((1ull << LPI) - 1));
-    if (gb) {
-      if (ly <= (int)(__builtin_ctz(gb) >> 2)) m = mn;
-    } else if constexpr (STK) {
-      if (bad & s_mask) m = mn;
-    }
+    finish();  // the common case, wave-uniform: no NaN in the wave
   };
   bool stopped = false;
   // Every lane logs every step when each has W = 1 and its next entry due
