@@ -409,3 +409,61 @@ def test_component_lanes_sliced_run_equals_uninterrupted():
     np.testing.assert_array_equal(ra.final_m.cpu().numpy(), rb.final_m.cpu().numpy())
     sa, sb = ra.status(), rb.status()
     assert (sa["steps_done"] == sb["steps_done"]).all() and (sa["entries_done"] == sb["entries_done"]).all()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("w", [1, 3])
+def test_component_lanes_nan_in_step_passes(precision, w):
+    """The step loop runs in wave-uniform passes and a NaN ends a pass: the
+    lanes of the instance it stops keep the reference's state, the other
+    lanes take that step again.  Eight bilayers share one wave, with uneven
+    lengths; NaNs come at different steps and layers, one at the first step
+    of a pass (the step where another instance's run ends) and one at an
+    instance's last step.  Every instance's status, final m and log entries
+    (every step, w = 1, and every third step) match the oracle."""
+    dt = 1e-13
+    n_steps = [400, 350, 420, 200, 380, 410, 300, 390]
+    nan_at = {1: (0, 50), 2: (1, 51), 5: (0, 200), 6: (1, 299)}  # instance: (layer, step)
+    js = []
+    for q in range(8):
+        j = _chain(2, mr="classic")
+        lay, s = nan_at.get(q, (0, 10**6))
+        amp = float("inf") if q in nan_at else 0.0
+        # the same driver kinds on every instance (one structure group, one
+        # wave); the x field turns infinite a quarter step into step s, so
+        # the step's midpoint stages see it
+        j.setLayerExternalFieldDriver(f"L{lay}", AxialDriver(stepDriver(200e3, amp, (s + 0.25) * dt, 1e-6),
+                                                             constantDriver(20e3), constantDriver(5e3)))
+        j.setLayerExternalFieldDriver(f"L{1 - lay}", AxialDriver(stepDriver(200e3, 0.0, 1.0, 2.0),
+                                                                 constantDriver(20e3), constantDriver(5e3)))
+        js.append(j)
+    T = np.array(n_steps, dtype=np.float64) * dt
+    b = JunctionBatch.from_junctions(js, T, dt, w * dt, SolverMode.RK4, precision=precision, log="full",
+                                     log_dtype="f64").set_lanes("per_component")
+    info = b.plan().info
+    print(f"[cl] nan passes {precision} w={w}: {info.kernel_name.decode()}, launches {info.n_kernels}", flush=True)
+    assert "k_junction_cl" in info.kernel_name.decode() and info.n_kernels == 1
+    r = b.run()
+    st = r.status()
+    olog, omag, ost = oracle.run_batch(b.js, b.layers, _lib.constants_get(), b.tables, b.log_elements)
+    tol = FP32_TOL if precision == "fp32" else 1e-8
+    glog = r.log.cpu().numpy()
+    gm = r.final_m.cpu().numpy()
+    for i in range(8):
+        for f in ("code", "steps_done", "entries_done", "nan_layer"):
+            assert int(st[i][f]) == int(ost[i][f]), (i, f, int(st[i][f]), int(ost[i][f]))
+        if i in nan_at:
+            assert int(st[i]["code"]) == _lib.ERR_NAN and int(st[i]["nan_layer"]) == nan_at[i][0]
+            assert int(st[i]["steps_done"]) == nan_at[i][1]
+        else:
+            # (the step count truncates totalTime / timeStep in binary64, as
+            # the reference does: 380 * dt / dt gives 379)
+            assert int(st[i]["code"]) == _lib.OK and n_steps[i] - 1 <= int(st[i]["steps_done"]) <= n_steps[i]
+        ne = int(ost[i]["entries_done"])
+        g, o = _channels(b, glog, i), _channels(b, olog, i)
+        for k in o:
+            d = float(np.max(np.abs(g[k][:ne] - o[k][:ne]) / np.maximum(1.0, np.abs(o[k][:ne]))))
+            assert d <= tol, (i, k, d)
+    assert np.array_equal(np.isnan(gm[:, :2]), np.isnan(omag[:, :2]))
+    ok = np.isfinite(omag[:, :2])
+    assert float(np.max(np.abs(gm[:, :2][ok] - omag[:, :2][ok]))) <= tol
