@@ -21,7 +21,8 @@
 // Integrator: the anchored binary32 RK4 of the other FP32 kernels (rk4_step,
 // core/junction.hpp:1017-1041): k1 and the state in binary64, stages 2-4 as
 // binary32 increments f(m0 + d, t_s) - f(m0, t) (rhs_delta in kernels.h,
-// same terms, component form).  Features: demagnetisation with a diagonal
+// same terms in component form, the cross products expanded about the
+// anchor, so they round differently).  Features: demagnetisation with a diagonal
 // tensor, first-order anisotropy, bilinear IEC with constant coupling,
 // time-dependent drivers that feed field slots only; log channels m and the
 // classic or strip magnetoresistance.  Sums over the quad associate
@@ -662,15 +663,22 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
 
       // ---- stages 2-4: binary32 increments about the anchor
       const float m0a = (float)m1, m0b = (float)m2;
+      // the anchor's H and m x H at components c + 1 and c + 2
       const float H0 = (float)H, Pf = (float)P0;
+      const float H0a = dpp_f<QP_ROT1>(H0), H0b = dpp_f<QP_ROT2>(H0);
+      const float Pfa = dpp_f<QP_ROT1>(Pf), Pfb = dpp_f<QP_ROT2>(Pf);
       auto delta = [&](float d, float ddH) -> float {
-        // f(m0 + d, ts) - f(m0, t), component c (rhs_delta, kernels.h)
+        // f(m0 + d, ts) - f(m0, t), component c (rhs_delta, kernels.h),
+        // with the cross products expanded about the anchor:
+        //   (m0 + d) x (H0 + dH) - m0 x H0 = (m0 + d) x dH + d x H0
+        // and the same for m x (m x H) with P0 = m0 x H0, so only the fresh
+        // dH and dP cross lanes, and d x H0, d x P0 fill the DPP waits
         const float d1 = dpp_f<QP_ROT1>(d), d2 = dpp_f<QP_ROT2>(d);
+        const float s1 = m0a + d1, s2 = m0b + d2;
         float dH = fmaf(-K.nmsf, d, ddH);
         if (CMTJ_HAS(F_ANIS)) dH = fmaf(K.aknf, fmaf(K.an2f, d2, fmaf(K.an1f, d1, K.anf * d)), dH);
-        const float Hs = H0 + dH;
-        const float dP = cross_c(m0a, m0b, dH) + cross_c(d1, d2, Hs);
-        const float dQ = cross_c(m0a, m0b, dP) + cross_c(d1, d2, Pf + dP);
+        const float dP = cross_c(s1, s2, dH) + fmaf(d1, H0b, -d2 * H0a);
+        const float dQ = cross_c(s1, s2, dP) + fmaf(d1, Pfb, -d2 * Pfa);
         return fmaf(dQ, K.alphaf, dP) * K.gdtf;
       };
       auto drive_delta = [&](double ts, int stage) -> float {
