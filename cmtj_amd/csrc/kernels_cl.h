@@ -76,6 +76,9 @@ constexpr int neighbour_banks() {
   }
   return m;
 }
+// Steps between two NaN tests of the component-lane step loop.
+constexpr int NAN_BLOCK = 64;
+
 // The smallest nonzero n over the active lanes of the wave (0 if none):
 // wave-uniform, read lane by lane through the ballot.
 __device__ __forceinline__ uint32_t wave_min_nonzero(uint32_t n) {
@@ -431,9 +434,14 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
   // stops on it have taken the step (their m as the reference leaves it);
   // the others left their state alone and take the step again.
   unsigned long long hitb = 0ull;
-  auto step = [&](auto drv_tag, auto la_tag) __attribute__((always_inline)) {
+  // CHK: the step tests its new m for a NaN (the ballot below); without it
+  // the step folds m into nanacc (a NaN stays a NaN in the sum), which the
+  // pass tests once per block of steps.
+  double nanacc = 0.0;
+  auto step = [&](auto drv_tag, auto la_tag, auto chk_tag) __attribute__((always_inline)) {
     constexpr bool DRV = decltype(drv_tag)::value;
     constexpr bool LA = decltype(la_tag)::value;
+    constexpr bool CHK = decltype(chk_tag)::value;
     const double t = (double)i * dt;
     double mn;
     if constexpr (STK) {
@@ -784,6 +792,11 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
     // the reference updates layers in turn and stops at the first NaN
     // layer; layers up to and including it take the new value (RK4 only:
     // the Euler-Heun step of the reference has no such test)
+    if constexpr (!CHK) {
+      if constexpr (SOLVER == CMTJ_RK4) nanacc = nanacc + mn;
+      finish();
+      return;
+    }
     const unsigned long long bad = (SOLVER == CMTJ_RK4) ? (__ballot(isnan(mn)) & nan_lanes) : 0ull;
     hitb = bad;
     if (__builtin_expect(bad != 0ull, 0)) {
@@ -820,10 +833,44 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       if (n != 0u) {
         const uint32_t i0 = i;
         uint32_t end = seg;  // wave-uniform
-        for (uint32_t u = 0; u < end; ++u) {
-          i = i0 + u;
-          step(drv_tag, la_tag);
-          end = hitb ? u : end;
+        if constexpr (SOLVER == CMTJ_RK4) {
+          // blocks of NAN_BLOCK steps without a per-step NaN test; a block
+          // whose sum shows a NaN is taken again from its start, step by
+          // step with the test, up to the NaN step (the same inputs: the
+          // same values; log entries a stopping instance wrote past that
+          // step lie beyond its entries_done)
+          for (uint32_t u0 = 0; u0 < end;) {
+            const uint32_t ue = min(u0 + (uint32_t)NAN_BLOCK, end);
+            const double m_b = m, bot_b = bot, top_b = top;
+            const uint32_t k_b = k;
+            const uint64_t nl_b = next_log;
+            nanacc = 0.0;
+            for (uint32_t u = u0; u < ue; ++u) {
+              i = i0 + u;
+              step(drv_tag, la_tag, std::false_type{});
+            }
+            if (__builtin_expect((__ballot(isnan(nanacc)) & nan_lanes) != 0ull, 0)) {
+              m = m_b;
+              bot = bot_b;
+              top = top_b;
+              k = k_b;
+              next_log = nl_b;
+              for (uint32_t u = u0; u < ue; ++u) {
+                i = i0 + u;
+                step(drv_tag, la_tag, std::true_type{});
+                if (hitb) {
+                  end = u;
+                  break;
+                }
+              }
+            }
+            u0 = ue;
+          }
+        } else {
+          for (uint32_t u = 0; u < end; ++u) {
+            i = i0 + u;
+            step(drv_tag, la_tag, std::false_type{});
+          }
         }
         i = i0 + end;
         if (hitb) {

@@ -226,7 +226,8 @@ enum cmtj_status_code {
 typedef struct cmtj_status {
   int32_t code;
   uint32_t steps_done;    /* steps completed (the failing step excluded) */
-  uint32_t entries_done;  /* log entries written */
+  uint32_t entries_done;  /* log entries written; an instance stopped by a
+                           * NaN leaves the entries after these unspecified */
   int32_t nan_layer;      /* first layer with a NaN, else -1 */
 } cmtj_status;
 
