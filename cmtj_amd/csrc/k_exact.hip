@@ -14,7 +14,7 @@ static const KEntry kE[] = {
     CMTJ_KE(double, CMTJ_FP64, 1, CMTJ_RK4, FM_RH, false),
     CMTJ_KE(float, CMTJ_FP32, 1, CMTJ_RK4, FM_VSD, false),
     CMTJ_KE(double, CMTJ_FP64, 1, CMTJ_RK4, FM_VSD, false),
-    CMTJ_KL(float, CMTJ_FP32, 2, CMTJ_EULER_HEUN, FM_CIMS),
+    CMTJ_KL_WPE(float, CMTJ_FP32, 2, CMTJ_EULER_HEUN, FM_CIMS, 4),  // the ensemble fills the chip
     CMTJ_KL(double, CMTJ_FP64, 2, CMTJ_EULER_HEUN, FM_CIMS),
     CMTJ_KE(float, CMTJ_FP32, 2, CMTJ_EULER_HEUN, FM_CIMS, false),
     CMTJ_KL(float, CMTJ_FP32, 2, CMTJ_RK4, FM_STO),

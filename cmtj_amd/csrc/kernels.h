@@ -28,6 +28,20 @@ struct DynRec {
   int32_t pad;
 };
 
+// A record as the kernels hold it.  Where every record is a trapezoid
+// (F_KTRAP: no table index to keep) the copy carries the ramp slope A / p2
+// in the 8 bytes of tidx and pad, so the division runs once per run instead
+// of at every stage time, with the same result.
+template <uint32_t FM>
+__device__ __forceinline__ DynRec load_rec(const DynRec* p) {
+  DynRec r = *p;
+  if constexpr ((FM & F_KTRAP) != 0) {
+    const double slope = r.A / r.p2;
+    __builtin_memcpy(&r.tidx, &slope, sizeof slope);
+  }
+  return r;
+}
+
 // Kernel arguments of one group.  A group shares its structure (layer count,
 // per-layer modes, driver kinds and slots, log layout): branches on these
 // are wave-uniform.  Numbers differ per lane.
@@ -528,6 +542,7 @@ __device__ __forceinline__ void add_field(V3<R>& H, int axis, bool on, R v) {
   H.z = fma(R(axis == 2 ? 1 : 0), vv, H.z);
 }
 
+template <uint32_t FM>
 __device__ __forceinline__ DynRec pick_rec(const DynRec (&dr)[DREG], const GroupArgs& g, int k, uint32_t lane) {
   DynRec r;
   if (k < DREG) {
@@ -535,7 +550,7 @@ __device__ __forceinline__ DynRec pick_rec(const DynRec (&dr)[DREG], const Group
     for (int q = 0; q < DREG; ++q)
       if (q == k) r = dr[q];
   } else {
-    r = g.dyn[(size_t)k * g.n + lane];
+    r = load_rec<FM>(g.dyn + (size_t)k * g.n + lane);
   }
   return r;
 }
@@ -556,8 +571,10 @@ __device__ __forceinline__ R wave_value(int type, const DynRec& r, const GroupAr
   } else if constexpr ((FM & F_KSINE) != 0) {
     if (sizeof(R) == 4 || FT) return (R)wave_part_f_trans(CMTJ_SINE, r.A, r.p1, r.p2, t);
     return (R)wave_part_trans(CMTJ_SINE, r.A, r.p1, r.p2, t);
-  } else if constexpr ((FM & F_KTRAP) != 0) {  // the only kind of this group
-    return (R)wave_trapezoid(r.A, r.p1, r.p2, r.p3, t);
+  } else if constexpr ((FM & F_KTRAP) != 0) {  // the only kind of this group (slope from load_rec)
+    double slope;
+    __builtin_memcpy(&slope, &r.tidx, sizeof slope);
+    return (R)wave_trapezoid_slope(r.A, slope, r.p1, r.p2, r.p3, t);
   }
   if (type == CMTJ_TABLE) return (R)g.tables[(size_t)r.tidx + (size_t)step * sps + stage];
   if (sizeof(R) == 4) return (R)wave_part_f(type, r.A, r.p1, r.p2, r.p3, t);
@@ -600,7 +617,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
       const int type = meta & 0xff;
       const int slot = (meta >> 8) & 0xff;
       const int lmask = (meta >> 16) & 0xff;
-      const DynRec r = pick_rec(dr, g, k, lane);
+      const DynRec r = pick_rec<FM>(dr, g, k, lane);
       const R v = wave_value<R, FT, FM>(type, r, g, t, step, stage, sps);
       keep_v0(v0, k, (double)v);
 #pragma unroll
@@ -620,7 +637,7 @@ __device__ __forceinline__ void eval_drivers(DV<R> (&dv)[LM], const DV<R> (&base
     }
     for (int k = DREG; !(FM & F_EXACT) && k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
-      const DynRec r = pick_rec(dr, g, k, lane);
+      const DynRec r = pick_rec<FM>(dr, g, k, lane);
       const R v = wave_value<R, FT, FM>(meta & 0xff, r, g, t, step, stage, sps);
 #pragma unroll
       for (int l = 0; l < LM; ++l)
@@ -661,7 +678,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
     for (int k = 0; k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
       const int type = meta & 0xff;
-      const DynRec r = pick_rec(dr, g, k, lane);
+      const DynRec r = pick_rec<FM>(dr, g, k, lane);
       const double v = wave_value<double, true, FM>(type, r, g, ts, step, stage, sps) -
                        value_t0<FM>(v0, k, type, r, g, t0, step, sps);
 #pragma unroll
@@ -680,7 +697,7 @@ __device__ __forceinline__ void eval_driver_deltas(DV<float> (&dd)[LM], const Dy
     }
     for (int k = DREG; !(FM & F_EXACT) && k < g.ndyn; ++k) {
       const int meta = g.dmeta[k];
-      const DynRec r = pick_rec(dr, g, k, lane);
+      const DynRec r = pick_rec<FM>(dr, g, k, lane);
       const double v = wave_value<double, true, FM>(meta & 0xff, r, g, ts, step, stage, sps) -
                        wave_value<double, true, FM>(meta & 0xff, r, g, t0, step, 0, sps);
 #pragma unroll
@@ -1203,7 +1220,7 @@ __device__ __forceinline__ void junction_body(const GroupArgs& g) {
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
     if (k < g.ndyn)
-      dr[k] = g.dyn[(size_t)k * g.n + lane];
+      dr[k] = load_rec<FM>(g.dyn + (size_t)k * g.n + lane);
     else
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
   }

@@ -344,7 +344,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
     if (k < NR && (ND >= 0 || k < g.ndyn)) {
-      dr[k] = g.dyn[(size_t)k * g.n + inst];
+      dr[k] = load_rec<FM>(g.dyn + (size_t)k * g.n + inst);
       win_lo = fmin(win_lo, dr[k].p1);
       win_hi = fmax(win_hi, dr[k].p2);
       sel[k] = live && feeds(g.dmeta[k], ly, c);
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           }
         }
         for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
-          const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+          const DynRec rr = load_rec<FM>(g.dyn + (size_t)r * g.n + inst);
           const double v = wave_value<double, false, FM>(g.dmeta[r] & 0xff, rr, g, ts, i, stage, sps);
           d.H = d.H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
           if constexpr ((FM & F_DYN) != 0) scalar_add(d, lact ? scalar_slot(g.dmeta[r], ly, c) : 0, v);
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           }
         }
         for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
-          const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+          const DynRec rr = load_rec<FM>(g.dyn + (size_t)r * g.n + inst);
           const double v = wave_value<double, true, FM>(g.dmeta[r] & 0xff, rr, g, t, i, 0, sps);
           H = H + ((live && feeds(g.dmeta[r], ly, c)) ? v : 0.0);
         }
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
           }
         }
         for (int r = DREG; ND < 0 && !(FM & F_EXACT) && r < g.ndyn; ++r) {
-          const DynRec rr = g.dyn[(size_t)r * g.n + inst];
+          const DynRec rr = load_rec<FM>(g.dyn + (size_t)r * g.n + inst);
           const int ty = g.dmeta[r] & 0xff;
           const float v = (float)(wave_value<double, true, FM>(ty, rr, g, ts, i, stage, sps) -
                                   wave_value<double, true, FM>(ty, rr, g, t, i, 0, sps));

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(64) void k_dp(GroupArgs g) {
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
     if (k < g.ndyn)
-      dr[k] = g.dyn[(size_t)k * g.n + lane];
+      dr[k] = load_rec<FM>(g.dyn + (size_t)k * g.n + lane);
     else
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
   }
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(64) void k_dp_ml(GroupArgs g) {
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
     if (k < g.ndyn)
-      dr[k] = g.dyn[(size_t)k * g.n + lane];
+      dr[k] = load_rec<FM>(g.dyn + (size_t)k * g.n + lane);
     else
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
   }
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(64) void k_dp_q(GroupArgs g, uint32_t* queue) {
 #pragma unroll
     for (int r = 0; r < DREG; ++r) {
       if (r < g.ndyn)
-        dr[r] = g.dyn[(size_t)r * g.n + q];
+        dr[r] = load_rec<FM>(g.dyn + (size_t)r * g.n + q);
       else
         dr[r] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
     }

@@ -128,8 +128,12 @@ __device__ __forceinline__ void write_entry_ll(const GroupArgs& g, const cmtj_ju
   }
 }
 
-template <class R, int L, int SOLVER, uint32_t FM, bool LOGE>
-__global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
+// WPE: waves per SIMD the register allocation must leave room for (1: no
+// limit).  A batch that fills the chip many times over hides latency with
+// more waves; 4 caps the kernel at 128 VGPRs (the CIMS ensemble's kernel
+// spills 216 B to scratch at that cap and still runs 1.22x faster).
+template <class R, int L, int SOLVER, uint32_t FM, bool LOGE, int WPE = 1>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_junction_ll(GroupArgs g) {
   static_assert(L == 2 || L == 4 || L == 8, "L lanes per instance must divide a 16-lane DPP row");
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t inst = gl / L;
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(64) void k_junction_ll(GroupArgs g) {
 #pragma unroll
   for (int k = 0; k < DREG; ++k) {
     if (k < g.ndyn)
-      dr[k] = g.dyn[(size_t)k * g.n + inst];
+      dr[k] = load_rec<FM>(g.dyn + (size_t)k * g.n + inst);
     else
       dr[k] = DynRec{0.0, 0.0, 0.0, 0.0, 0, 0};
   }

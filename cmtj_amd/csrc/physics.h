@@ -111,12 +111,17 @@ static __host__ __device__ __noinline__ double wave_part_trans(int type, double 
 }
 
 // trapezoidDriver (drivers.hpp:90-108): p1 start, p2 edge, p3 steady
-CMTJ_HD double wave_trapezoid(double A, double p1, double p2, double p3, double t) {
+// trapezoid with its ramp slope A / p2 given (computed once per run by the
+// kernels whose records are all trapezoids; the same value as the division)
+CMTJ_HD double wave_trapezoid_slope(double A, double slope, double p1, double p2, double p3, double t) {
   if (t < p1) return 0.0;
-  if (t <= p1 + p2) return (A / p2) * (t - p1);
+  if (t <= p1 + p2) return slope * (t - p1);
   if (t <= p1 + p2 + p3) return A;
-  if (t <= p1 + 2 * p2 + p3) return A - (A / p2) * (t - (p1 + p2 + p3));
+  if (t <= p1 + 2 * p2 + p3) return A - slope * (t - (p1 + p2 + p3));
   return 0.0;
+}
+CMTJ_HD double wave_trapezoid(double A, double p1, double p2, double p3, double t) {
+  return wave_trapezoid_slope(A, A / p2, p1, p2, p3, t);
 }
 
 CMTJ_HD double wave_part(int type, double A, double p1, double p2, double p3, double t) {

@@ -71,6 +71,11 @@ KTable kernels_group4();  // the same on four waves per group
   {PREC, L, L, S, FM, false, (const void*)&cmtj::k_junction_ll<R, L, S, FM, false>, \
    "k_junction_ll<" #R "," #L "," #S "," #FM ">"}
 
+// the same with the register allocation capped for WPE waves per SIMD
+#define CMTJ_KL_WPE(R, PREC, L, S, FM, WPE)                                              \
+  {PREC, L, L, S, FM, false, (const void*)&cmtj::k_junction_ll<R, L, S, FM, false, WPE>, \
+   "k_junction_ll<" #R "," #L "," #S "," #FM ",waves " #WPE ">"}
+
 #define CMTJ_KC(L, FM, ND)                                                                     \
   {CMTJ_FP32, L, 4 * (L), CMTJ_RK4, FM, false, (const void*)&cmtj::k_junction_cl<L, FM, ND>,  \
    "k_junction_cl<" #L "," #FM "," #ND ">", (ND) + 1}
