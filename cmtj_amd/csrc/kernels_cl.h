@@ -114,6 +114,21 @@ __device__ __forceinline__ T cross_c(T a1, T a2, T b) {
   return a1 * dpp_x<QP_ROT2>(b) - a2 * dpp_x<QP_ROT1>(b);
 }
 
+// (a x b) and a x (a x b) at this lane's component, given a's components
+// a0 = a_c, a1 = a_{c+1}, a2 = a_{c+2}.  The first product at components
+// c + 1 and c + 2 is formed here from the same operands, in the same order,
+// as in the lanes that own those components, so it equals their values and
+// need not cross lanes before the second product (for binary64 values a
+// lane move is two DPP moves and the wait states after them).
+template <class T>
+__device__ __forceinline__ void cross2_c(T a0, T a1, T a2, T b, T& p, T& q) {
+  const T b1 = dpp_x<QP_ROT1>(b), b2 = dpp_x<QP_ROT2>(b);
+  p = fma(a1, b2, -(a2 * b1));          // (a x b)_c
+  const T p1 = fma(a2, b, -(a0 * b2));  // (a x b)_{c+1}
+  const T p2 = fma(a0, b1, -(a1 * b));  // (a x b)_{c+2}
+  q = fma(a1, p2, -(a2 * p1));          // (a x (a x b))_c
+}
+
 template <class T>
 __device__ __forceinline__ T comp(V3<T> v, int c) {
   return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : T(0)));
@@ -537,15 +552,15 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
         }
         // m x H and m x (m x H) as cross products: the initial m of a run
         // need not have unit length
-        const double Pc = cross_c(s1, s2, Hs);
-        const double Qc = cross_c(s1, s2, Pc);
+        double Pc, Qc;
+        cross2_c(ms, s1, s2, Hs, Pc, Qc);
         double r = fma(Qc, K.alpha, Pc) * K.gdt;
         if constexpr ((FM & (F_STT | F_SOT)) != 0) {
           if (CMTJ_HAS(F_STT) || CMTJ_HAS(F_SOT)) {
             const double refc = (K.rt == CMTJ_REF_FIXED) ? K.ref : (K.rt == CMTJ_REF_TOP) ? tp
                               : (K.rt == CMTJ_REF_BOTTOM) ? bt : 0.0;
-            const double FLc = cross_c(s1, s2, refc);  // m x p
-            const double DLc = cross_c(s1, s2, FLc);   // m x (m x p)
+            double FLc, DLc;  // m x p, m x (m x p)
+            cross2_c(ms, s1, s2, refc, FLc, DLc);
             if (CMTJ_HAS(F_SOT)) {
               // SOT (:770-812): coefficients from the drivers, or static
               // ones scaling the current
@@ -555,8 +570,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
               if (CMTJ_HAS(F_SOT2)) {
                 // the secondary reference (dynamic SOT drivers only)
                 const double Hdl2 = K.dyn_sot ? d.DL2 : 0.0, Hfl2 = K.dyn_sot ? d.FL2 : 0.0;
-                const double F2 = cross_c(s1, s2, K.ref2);  // m x p2
-                const double D2 = cross_c(s1, s2, F2);      // m x (m x p2)
+                double F2, D2;  // m x p2, m x (m x p2)
+                cross2_c(ms, s1, s2, K.ref2, F2, D2);
                 br = br + F2 * (Hfl2 - K.alpha * Hdl2) + D2 * (Hdl2 + K.alpha * Hfl2);
               }
               r = K.sot ? br * K.gdt : r;
@@ -579,8 +594,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // component dW of the layer's three normals (shared Philox stream)
       [[maybe_unused]] auto stoch_c = [&](double ms, double dW) -> double {
         const double s1 = dpp_d<QP_ROT1>(ms), s2 = dpp_d<QP_ROT2>(ms);
-        const double c1 = cross_c(s1, s2, dW);  // m x dW
-        const double c2 = cross_c(s1, s2, c1);  // m x (m x dW)
+        double c1, c2;  // m x dW, m x (m x dW)
+        cross2_c(ms, s1, s2, dW, c1, c2);
         return fma(c2, K.alpha, c1) * K.gconv;
       };
       if constexpr (SOLVER == CMTJ_HEUN) {
@@ -665,8 +680,8 @@ __global__ __launch_bounds__(64) void k_junction_cl(GroupArgs g) {
       // m x H and m x (m x H) as cross products: the initial m of a run need
       // not have unit length (the reference normalises its constructor
       // argument, not the stored vector), so m (m . H) - H is no substitute
-      const double P0 = cross_c(m1, m2, H);
-      const double Q0 = cross_c(m1, m2, P0);
+      double P0, Q0;
+      cross2_c(m, m1, m2, H, P0, Q0);
       const double k1 = fma(Q0, K.alpha, P0) * K.gdt;
 
       // ---- stages 2-4: binary32 increments about the anchor
